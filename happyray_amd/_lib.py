@@ -1,0 +1,104 @@
+"""ctypes binding of libhappyray.so (include/happyray.h).
+
+The library is loaded from this package directory only.  When it is missing
+the import fails loudly: there is no CPU fallback behind this module.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhappyray.so")
+
+HR_OK = 0
+STATUS_NAMES = {0: "HR_OK", 1: "HR_ERR_ARG", 2: "HR_ERR_DEVICE", 3: "HR_ERR_OOM", 4: "HR_ERR_IO",
+                5: "HR_ERR_STATE", 6: "HR_ERR_CAPACITY", 7: "HR_ERR_PARSE", 8: "HR_ERR_BUDGET"}
+
+GRID_UNIFORM = 0
+GRID_UNIFORM_EXACT = 1
+GRID_TWO_LEVEL = 2
+MODE_AO = 3
+MODE_PATH = 2
+
+_f32p = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_u8p = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+
+class Options(C.Structure):
+    _fields_ = [("device", C.c_int), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32)]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [("kind", C.c_int), ("bmin", C.c_float * 3), ("bmax", C.c_float * 3),
+                ("res", C.c_int * 3), ("cell_size", C.c_float * 3), ("cell_size_rcp", C.c_float * 3),
+                ("num_cells", C.c_uint32), ("num_leaves", C.c_uint32), ("num_refs", C.c_uint32),
+                ("num_top_pairs", C.c_uint32), ("num_triangles", C.c_uint32)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("jitter", C.c_int), ("ao_samples", C.c_int), ("ao_dist", C.c_float),
+                ("pt_depth", C.c_int)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("render_ms", C.c_double), ("rays_traced", C.c_uint64)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "hr_last_error": (C.c_char_p, []),
+    "hr_create": (C.c_int, [C.POINTER(_vp), C.POINTER(Options)]),
+    "hr_destroy": (C.c_int, [_vp]),
+    "hr_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "hr_upload_mesh": (C.c_int, [_vp, _f32p, _f32p, C.c_float, C.c_uint32, _u32p, C.c_uint32]),
+    "hr_upload_materials": (C.c_int, [_vp, _f32p, C.c_uint32, _u32p, C.c_uint32]),
+    "hr_set_keyframes": (C.c_int, [_vp, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint32]),
+    "hr_set_frame": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float]),
+    "hr_set_grid": (C.c_int, [_vp, C.c_int, C.c_float, C.c_float]),
+    "hr_build": (C.c_int, [_vp]),
+    "hr_get_grid_info": (C.c_int, [_vp, C.POINTER(GridInfo)]),
+    "hr_get_grid": (C.c_int, [_vp, _u32p, _u32p, _u32p]),
+    "hr_set_camera": (C.c_int, [_vp, _f32p, _f32p, _f32p, C.c_float, C.c_int, C.c_int]),
+    "hr_set_lights": (C.c_int, [_vp, _f32p, _f32p, C.c_int]),
+    "hr_render": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(RenderParams), _f32p]),
+    "hr_get_primary_hits": (C.c_int, [_vp, _f32p, _u32p]),
+    "hr_get_ao_occlusion": (C.c_int, [_vp, _u8p]),
+    "hr_trace": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, _f32p, _u32p]),
+    "hr_occluded": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_float, _u8p]),
+    "hr_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp]),
+    "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
+    "hr_synchronize": (C.c_int, [_vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libhappyray.so from the package directory (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m happyray_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class HappyRayError(RuntimeError):
+    def __init__(self, status, where, message):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {message}")
+
+
+def check(status, where):
+    if status != HR_OK:
+        msg = load().hr_last_error()
+        raise HappyRayError(status, where, msg.decode() if msg else "")

@@ -1,0 +1,50 @@
+// hr_build.h -- host interface of the grid build kernels (hr_build.hip).
+#pragma once
+
+#include "hr_device.h"
+
+namespace hr {
+
+struct BuildArgs {
+    int kind;                   // 0 uniform, 1 uniform exact, 2 two-level
+    float lambda_top, lambda_leaf;
+    const uint32_t* idx;        // device: 3 * nt vertex indices
+    const float* vx;            // device: SoA vertices
+    const float* vy;
+    const float* vz;
+    uint32_t nt;
+};
+
+// Device buffers of one build.  Capacities are element counts; the host grows
+// them after an overflow and repeats the build.
+struct BuildBuffers {
+    GridParams* params;
+    BuildStatus* status;
+    float4* rec;                // 3 * nt triangle records
+    uint32_t* wave_counts;      // per-wave pair counts, then offsets
+    uint32_t* keys;             // unsorted keys / values (pairs, then leaf pairs)
+    uint32_t* vals;
+    uint32_t* keys_sorted;
+    uint32_t* top_vals_sorted;  // two-level: sorted top-level primitive ids
+    uint32_t* prims;            // final primitive references
+    uint2* cells;               // cells_cap
+    uint2* leaves;              // leaves_cap
+    uint32_t* leaf_counts;      // cells_cap
+    uint64_t cells_cap;
+    uint32_t pairs_cap;
+    uint32_t leaves_cap;
+    uint32_t leaf_pairs_cap;
+    uint32_t waves_cap;         // entries of wave_counts
+    ScanTemp scan;
+    SortTemp sort;
+};
+
+void launch_reset(BuildStatus* st, hipStream_t s);
+void launch_lerp_bounds(const float* k1, const float* k2, float c, uint32_t nv, float* out,
+                        BuildStatus* st, hipStream_t s);
+void launch_grid_setup(GridParams* gp, BuildStatus* st, uint32_t nprims, float lambda,
+                       uint64_t cells_cap, hipStream_t s);
+// everything after the bounds: setup, pairs, sort, ranges (and the leaf level)
+void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s);
+
+}  // namespace hr

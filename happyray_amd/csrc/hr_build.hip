@@ -1,0 +1,585 @@
+// hr_build.hip -- key-frame interpolation, bounds and the uniform /
+// two-level grid builds for gfx950.
+//
+// Reference call chain replaced (host-synchronous there, device-resident
+// here):
+//   ObjUploader::uploadObjFrameVertexData   src/Application/WFObjectUploader.h:48-119
+//   UGridSortBuilder::init / build          src/RT/Algorithm/UGridSortBuilder.h:64-407
+//   TLGridSortBuilder::init / build         src/RT/Algorithm/TLGridSortBuilder.h:66-452
+//   countPairs, writeKeysAndValues<exact>   src/RT/Algorithm/UniformGridBuildKernels.h:40-252,
+//                                           src/RT/Algorithm/UniformGridBuildKernels.cu:8-383
+//   prepareCellRanges                       src/RT/Algorithm/UniformGridBuildKernels.h:384-487
+//   countLeafLevelCells, prepareTopLevelCellRanges, countLeafLevelPairs,
+//   writeLeafLevelKeysAndValues, prepareLeafCellRanges
+//                                           src/RT/Algorithm/TLGridBuildKernels.h:45-834
+//
+// Design (MI355X):
+//  * the bounds reduction and the resolution are computed on the device and
+//    left in a GridParams block, so a build has no host round trip; sizes
+//    that depend on the data stay in device memory and every buffer is
+//    sized by a capacity that the host grows when a BuildStatus overflow bit
+//    comes back (then the build is repeated);
+//  * pairs are produced by wave64-cooperative enumeration: the 64 lanes of a
+//    wave expand the cell ranges of 64 consecutive primitives into one
+//    candidate list and test 64 candidates per step, so one large triangle
+//    does not serialise a lane.  Output slots come from a ballot prefix and a
+//    scan of per-wave counts, so pairs are written in primitive order without
+//    atomics and the stable radix sort yields (cell, primitive) order;
+//  * exact insertion tests candidates in the count pass and the write pass
+//    and writes only the pairs that pass (the reference writes failing pairs
+//    with key = #cells and sorts them to the end);
+//  * triangles are re-laid once per build as three float4 per triangle
+//    (vertex xyz), the record every later pass and the traversal read.
+#include "hr_build.h"
+
+namespace hr {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ bool stop(const BuildStatus* st) { return st->overflow != 0u; }
+
+// ---------------------------------------------------------------------------
+// reset, interpolation, bounds, resolution
+// ---------------------------------------------------------------------------
+
+__global__ void k_reset(BuildStatus* st)
+{
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < 3; ++i) { st->bounds_enc[i] = 0xFFFFFFFFu; st->bounds_enc[3 + i] = 0u; }
+        st->num_cells = st->num_pairs = st->num_leaves = st->num_leaf_pairs = 0u;
+        st->overflow = 0u;
+        st->n_pairs_eff = st->n_leaves_eff = st->n_leaf_pairs_eff = 0u;
+        st->pad[0] = st->pad[1] = 0u;
+    }
+}
+
+// v = v1 * (1 - c) + v2 * c (WFObjectUploader.h:82-88) and min / max of all
+// vertices (WFObjectUploader.h:90-95)
+__global__ __launch_bounds__(kBlock) void k_lerp_bounds(
+    const float* __restrict__ x1, const float* __restrict__ y1, const float* __restrict__ z1,
+    const float* __restrict__ x2, const float* __restrict__ y2, const float* __restrict__ z2,
+    float c, uint32_t nv, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
+    BuildStatus* __restrict__ st)
+{
+    __shared__ float red[6][kWavesPerBlock];
+    f3 lo = rep(kFltMax), hi = rep(-kFltMax);
+    const float ic = 1.f - c;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
+        const float x = x1[i] * ic + x2[i] * c;
+        const float y = y1[i] * ic + y2[i] * c;
+        const float z = z1[i] * ic + z2[i] * c;
+        ox[i] = x; oy[i] = y; oz[i] = z;
+        lo = vmin(mk(x, y, z), lo);
+        hi = vmax(mk(x, y, z), hi);
+    }
+    lo.x = wave_min(lo.x); lo.y = wave_min(lo.y); lo.z = wave_min(lo.z);
+    hi.x = wave_max(hi.x); hi.y = wave_max(hi.y); hi.z = wave_max(hi.z);
+    const int w = (int)threadIdx.x / kWave;
+    if (__lane_id() == 0) {
+        red[0][w] = lo.x; red[1][w] = lo.y; red[2][w] = lo.z;
+        red[3][w] = hi.x; red[4][w] = hi.y; red[5][w] = hi.z;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float v = red[threadIdx.x][0];
+        for (int i = 1; i < kWavesPerBlock; ++i)
+            v = threadIdx.x < 3 ? fminf(v, red[threadIdx.x][i]) : fmaxf(v, red[threadIdx.x][i]);
+        if (threadIdx.x < 3) atomicMin(&st->bounds_enc[threadIdx.x], enc_f(v));
+        else atomicMax(&st->bounds_enc[threadIdx.x], enc_f(v));
+    }
+}
+
+// Bounds padding (WFObjectUploader.h:112-114) and resolution from density
+// (UGridSortBuilder.h:78-92 / TLGridSortBuilder.h:81-98, checkResolution
+// UGridMemoryManager.cu:338-346); cell size and its reciprocal as the memory
+// managers compute them (UGridMemoryManager.h:107-115).
+__global__ void k_grid_setup(GridParams* __restrict__ gp, BuildStatus* __restrict__ st,
+                             uint32_t nprims, float lambda, uint64_t cells_cap)
+{
+    if (threadIdx.x != 0) return;
+    f3 lo = mk(dec_f(st->bounds_enc[0]), dec_f(st->bounds_enc[1]), dec_f(st->bounds_enc[2]));
+    f3 hi = mk(dec_f(st->bounds_enc[3]), dec_f(st->bounds_enc[4]), dec_f(st->bounds_enc[5]));
+    const float diag = len(hi - lo);
+    lo = lo - rep(0.001f * diag);
+    hi = hi + rep(0.001f * diag);
+    const f3 d = hi - lo;
+    const float volume = d.x * d.y * d.z;
+    const float magic = spec_powf(lambda * (float)nprims / volume, 0.3333333f);
+    const f3 dm = d * magic;
+    const float dv[3] = { dm.x, dm.y, dm.z };
+    int res[3];
+    bool bad = false;
+    for (int i = 0; i < 3; ++i) {
+        if (!(dv[i] >= 1.f && dv[i] < 2147483520.f)) bad = true;
+        else res[i] = (int)dv[i];
+    }
+    gp->fallback = bad ? 1u : 0u;
+    if (bad) res[0] = res[1] = res[2] = 32;
+    const f3 rf = mk((float)res[0], (float)res[1], (float)res[2]);
+    const f3 cs = d / rf;
+    const f3 csr = rf / d;
+    gp->bmin[0] = lo.x; gp->bmin[1] = lo.y; gp->bmin[2] = lo.z;
+    gp->bmax[0] = hi.x; gp->bmax[1] = hi.y; gp->bmax[2] = hi.z;
+    gp->res[0] = res[0]; gp->res[1] = res[1]; gp->res[2] = res[2];
+    gp->cs[0] = cs.x; gp->cs[1] = cs.y; gp->cs[2] = cs.z;
+    gp->csr[0] = csr.x; gp->csr[1] = csr.y; gp->csr[2] = csr.z;
+    const uint64_t nc = (uint64_t)res[0] * (uint64_t)res[1] * (uint64_t)res[2];
+    if (nc > cells_cap || nc > 0xFFFFFFF0ull) {
+        st->overflow |= kOverflowCells;
+        st->num_cells = nc > 0xFFFFFFF0ull ? 0xFFFFFFFFu : (uint32_t)nc;
+        gp->num_cells = 0u;
+    } else {
+        st->num_cells = (uint32_t)nc;
+        gp->num_cells = (uint32_t)nc;
+    }
+}
+
+// total > cap sets the overflow bit and zeroes the effective size
+__global__ void k_check(BuildStatus* st, uint32_t* total, uint32_t* eff, uint32_t cap, uint32_t bit)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t t = *total;
+    if (t > cap) st->overflow |= bit;
+    *eff = st->overflow ? 0u : t;
+}
+
+// ---------------------------------------------------------------------------
+// exact triangle / cell test (UniformGridBuildKernels.cu:8-48, :305-361)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool axis_sep(float a, float b, float fa, float fb, float p0a, float p0b,
+                                         float p1a, float p1b, float ha, float hb)
+{
+    const float q0 = a * p0a + b * p0b;
+    const float q1 = a * p1a + b * p1b;
+    const float mn = fminf(q0, q1);
+    const float mx = fmaxf(q0, q1);
+    const float rad = fa * ha + fb * hb;
+    return !(mn > rad + kEps || mx + kEps < -rad);
+}
+
+__device__ __forceinline__ bool tri_cell_overlap(f3 t0, f3 t1, f3 t2, f3 n, f3 c, f3 h)
+{
+    const f3 a = t0 - c, b = t1 - c, d = t2 - c;
+    const f3 e0 = b - a, e1 = d - b, e2 = a - d;
+    f3 lo;
+    lo.x = (n.x > 0.f) ? -h.x : h.x;
+    lo.y = (n.y > 0.f) ? -h.y : h.y;
+    lo.z = (n.z > 0.f) ? -h.z : h.z;
+    f3 up = -lo;
+    up = up - a;
+    lo = lo - a;
+    bool ok = dot(n, lo) <= 0.f && dot(n, up) > 0.f;
+    f3 f = vabs(e0);
+    ok = ok && axis_sep(e0.z, -e0.y, f.z, f.y, a.y, a.z, d.y, d.z, h.y, h.z);
+    ok = ok && axis_sep(-e0.z, e0.x, f.z, f.x, a.x, a.z, d.x, d.z, h.x, h.z);
+    ok = ok && axis_sep(e0.y, -e0.x, f.y, f.x, b.x, b.y, d.x, d.y, h.x, h.y);
+    f = vabs(e1);
+    ok = ok && axis_sep(e1.z, -e1.y, f.z, f.y, a.y, a.z, d.y, d.z, h.y, h.z);
+    ok = ok && axis_sep(-e1.z, e1.x, f.z, f.x, a.x, a.z, d.x, d.z, h.x, h.z);
+    ok = ok && axis_sep(e1.y, -e1.x, f.y, f.x, a.x, a.y, b.x, b.y, h.x, h.y);
+    f = vabs(e2);
+    ok = ok && axis_sep(e2.z, -e2.y, f.z, f.y, a.y, a.z, b.y, b.z, h.y, h.z);
+    ok = ok && axis_sep(-e2.z, e2.x, f.z, f.x, a.x, a.z, b.x, b.z, h.x, h.z);
+    ok = ok && axis_sep(e2.y, -e2.x, f.y, f.x, b.x, b.y, d.x, d.y, h.x, h.y);
+    return ok;
+}
+
+// cell-centre coordinate of index i along one axis, accumulated as the
+// reference's nested loops do (start - size, then + size per step)
+__device__ __forceinline__ float centre_along(float first, float size, int steps)
+{
+    float c = first - size;
+    for (int i = 0; i <= steps; ++i) c += size;
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// top level (uniform grid / two-level top): triangle records and counts
+// ---------------------------------------------------------------------------
+
+struct WaveItems {
+    int lo[3][kWave];
+    int dx[kWave], dy[kWave];
+    uint32_t excl[kWave];
+    uint32_t id[kWave];
+    // exact insertion
+    float v[9][kWave];
+    float n[3][kWave];
+    float mcc[3][kWave];
+    // leaf level
+    uint32_t lbeg[kWave];
+    uint32_t r0[kWave], r1[kWave];
+};
+
+template <bool kExact, bool kWrite>
+__global__ __launch_bounds__(kBlock) void k_top_pairs(
+    const uint32_t* __restrict__ idx, const float* __restrict__ vx, const float* __restrict__ vy,
+    const float* __restrict__ vz, uint32_t nt, const GridParams* __restrict__ gp,
+    const BuildStatus* __restrict__ st, float4* __restrict__ rec, uint32_t* __restrict__ wave_counts,
+    const uint32_t* __restrict__ wave_offsets, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+    uint32_t waves_cap)
+{
+    __shared__ WaveItems sh[kWavesPerBlock];
+    if (stop(st)) return;
+    const int wl = (int)threadIdx.x / kWave;
+    const int lane = (int)__lane_id();
+    const uint32_t wg = blockIdx.x * kWavesPerBlock + (uint32_t)wl;
+    const uint32_t p = wg * kWave + (uint32_t)lane;
+    const bool valid = p < nt;
+    if (wg * kWave >= nt) {         // whole wave idle
+        if (!kWrite && lane == 0 && wg < waves_cap) wave_counts[wg] = 0u;
+        return;
+    }
+
+    const f3 bmin = mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]);
+    const f3 cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
+    const f3 csr = mk(gp->csr[0], gp->csr[1], gp->csr[2]);
+    const int rx = gp->res[0], ry = gp->res[1], rz = gp->res[2];
+
+    f3 v0 = rep(0.f), v1 = rep(0.f), v2 = rep(0.f);
+    uint32_t cnt = 0;
+    int mn[3] = { 0, 0, 0 }, mx[3] = { 0, 0, 0 };
+    if (valid) {
+        if (!kWrite) {
+            const uint32_t i0 = idx[3 * (size_t)p], i1 = idx[3 * (size_t)p + 1], i2 = idx[3 * (size_t)p + 2];
+            v0 = mk(vx[i0], vy[i0], vz[i0]);
+            v1 = mk(vx[i1], vy[i1], vz[i1]);
+            v2 = mk(vx[i2], vy[i2], vz[i2]);
+            rec[3 * (size_t)p + 0] = make_float4(v0.x, v0.y, v0.z, 0.f);
+            rec[3 * (size_t)p + 1] = make_float4(v1.x, v1.y, v1.z, 0.f);
+            rec[3 * (size_t)p + 2] = make_float4(v2.x, v2.y, v2.z, 0.f);
+        } else {
+            load_tri(rec, p, v0, v1, v2);
+        }
+        f3 lo, hi;
+        tri_bounds(v0, v1, v2, lo, hi);
+        box_cells(lo, hi, bmin, csr, rx, ry, rz, mn, mx);
+        cnt = (uint32_t)((mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]));
+    }
+
+    const uint32_t kcell_z = (uint32_t)rx * (uint32_t)ry;
+    if (!kExact && !kWrite) {
+        const uint32_t s = wave_sum(cnt);
+        if (lane == 0) wave_counts[wg] = s;
+        return;
+    }
+
+    WaveItems& W = sh[wl];
+    const uint32_t inc = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(inc, kWave - 1, kWave);
+    W.excl[lane] = inc - cnt;
+    W.lo[0][lane] = mn[0]; W.lo[1][lane] = mn[1]; W.lo[2][lane] = mn[2];
+    W.dx[lane] = mx[0] - mn[0];
+    W.dy[lane] = mx[1] - mn[1];
+    W.id[lane] = p;
+    if (kExact) {
+        W.v[0][lane] = v0.x; W.v[1][lane] = v0.y; W.v[2][lane] = v0.z;
+        W.v[3][lane] = v1.x; W.v[4][lane] = v1.y; W.v[5][lane] = v1.z;
+        W.v[6][lane] = v2.x; W.v[7][lane] = v2.y; W.v[8][lane] = v2.z;
+        const f3 n = normalize(cross(v1 - v0, v2 - v0));
+        W.n[0][lane] = n.x; W.n[1][lane] = n.y; W.n[2][lane] = n.z;
+        const f3 half = cs * 0.505f;
+        f3 m = mk((float)mn[0], (float)mn[1], (float)mn[2]);
+        m = m * cs;
+        m = m + bmin + half;
+        W.mcc[0][lane] = m.x; W.mcc[1][lane] = m.y; W.mcc[2][lane] = m.z;
+    }
+    wave_sync();
+
+    const f3 half = cs * 0.505f;
+    const uint64_t lt = __lanemask_lt();
+    uint32_t run = kWrite ? wave_offsets[wg] : 0u;
+    for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+        const uint32_t j = j0 + (uint32_t)lane;
+        bool ok = false;
+        uint32_t key = 0, id = 0;
+        if (j < total) {
+            const int o = wave_owner(W.excl, j);
+            const uint32_t k = j - W.excl[o];
+            const uint32_t dxo = (uint32_t)W.dx[o], dyo = (uint32_t)W.dy[o];
+            const int ix = (int)(k % dxo), t = (int)(k / dxo);
+            const int iy = t % (int)dyo, iz = t / (int)dyo;
+            const int x = W.lo[0][o] + ix, y = W.lo[1][o] + iy, z = W.lo[2][o] + iz;
+            id = W.id[o];
+            key = (uint32_t)x + (uint32_t)y * (uint32_t)rx + (uint32_t)z * kcell_z;
+            if (kExact) {
+                const f3 c = mk(centre_along(W.mcc[0][o], cs.x, ix), centre_along(W.mcc[1][o], cs.y, iy),
+                                centre_along(W.mcc[2][o], cs.z, iz));
+                const f3 a = mk(W.v[0][o], W.v[1][o], W.v[2][o]);
+                const f3 b = mk(W.v[3][o], W.v[4][o], W.v[5][o]);
+                const f3 d = mk(W.v[6][o], W.v[7][o], W.v[8][o]);
+                ok = tri_cell_overlap(a, b, d, mk(W.n[0][o], W.n[1][o], W.n[2][o]), c, half);
+            } else {
+                ok = true;
+            }
+        }
+        const uint64_t bal = __ballot(ok);
+        if (kWrite && ok) {
+            const uint32_t pos = run + (uint32_t)__popcll(bal & lt);
+            keys[pos] = key;
+            vals[pos] = id;
+        }
+        run += (uint32_t)__popcll(bal);
+    }
+    if (!kWrite && lane == 0) wave_counts[wg] = run;
+}
+
+// prepareCellRanges (UniformGridBuildKernels.h:384-487) on sorted keys:
+// begin at the first pair of a key, end one past its last pair.  Keys are
+// always < #cells here (no invalid-key tail).
+__global__ void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_dev,
+                         uint2* __restrict__ cells, const BuildStatus* __restrict__ st)
+{
+    if (stop(st)) return;
+    const uint32_t n = *n_dev;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t k = keys[i];
+        if (i == 0 || keys[i - 1] != k) cells[k].x = i;
+        if (i + 1 == n || keys[i + 1] != k) cells[k].y = i + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// leaf level
+// ---------------------------------------------------------------------------
+
+// countLeafLevelCells<false> (TLGridBuildKernels.h:57-114): leaf resolution
+// per top cell and the packed TwoLevelGridCell (TwoLevelGrid.h:38-161)
+__global__ void k_leaf_res(const GridParams* __restrict__ gp, uint2* __restrict__ cells,
+                           uint32_t* __restrict__ counts, float lambda_leaf,
+                           const BuildStatus* __restrict__ st)
+{
+    if (stop(st)) return;
+    const uint32_t nc = gp->num_cells;
+    const f3 cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
+    const float vol = cs.x * cs.y * cs.z;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const uint2 r = cells[c];
+        const uint32_t n = r.y - r.x;
+        const float magic = spec_powf(lambda_leaf * (float)n / vol, 0.3333333f);
+        const f3 res = vmin(rep(255.f), vmax(rep(1.f), cs * magic));
+        const bool leaf = n <= 16u;
+        const uint32_t lx = leaf ? 1u : (uint32_t)res.x;
+        const uint32_t ly = leaf ? 1u : (uint32_t)res.y;
+        const uint32_t lz = leaf ? 1u : (uint32_t)res.z;
+        counts[c] = lx * ly * lz;
+        uint32_t y = leaf ? 0x10000000u : 0u;
+        y |= 0x40000000u;
+        if (n == 0u) y &= 0x3FFFFFFFu;
+        y |= (lx & 0xFFu) | ((ly & 0xFFu) << 8) | ((lz & 0xFFu) << 16);
+        cells[c].y = y;
+    }
+}
+
+// countLeafLevelPairs / writeLeafLevelKeysAndValues
+// (TLGridBuildKernels.h:304-409, :536-663) over the sorted top pairs
+template <bool kWrite>
+__global__ __launch_bounds__(kBlock) void k_leaf_pairs(
+    const uint32_t* __restrict__ top_keys, const uint32_t* __restrict__ top_vals,
+    const uint32_t* __restrict__ n_top, const uint2* __restrict__ cells,
+    const GridParams* __restrict__ gp, const BuildStatus* __restrict__ st,
+    const float4* __restrict__ rec, uint32_t* __restrict__ wave_counts,
+    const uint32_t* __restrict__ wave_offsets, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+    uint32_t waves_cap)
+{
+    __shared__ WaveItems sh[kWavesPerBlock];
+    if (stop(st)) return;
+    const uint32_t n = *n_top;
+    const int wl = (int)threadIdx.x / kWave;
+    const int lane = (int)__lane_id();
+    const uint32_t wg = blockIdx.x * kWavesPerBlock + (uint32_t)wl;
+    if (wg * kWave >= n) {
+        if (!kWrite && lane == 0 && wg < waves_cap) wave_counts[wg] = 0u;
+        return;
+    }
+    const uint32_t i = wg * kWave + (uint32_t)lane;
+    const bool valid = i < n;
+
+    const f3 bmin = mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]);
+    const f3 cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
+    const uint32_t gx = (uint32_t)gp->res[0], gy = (uint32_t)gp->res[1];
+
+    uint32_t cnt = 0, prim = 0, lb = 0, r0 = 1, r1 = 1;
+    int mn[3] = { 0, 0, 0 }, mx[3] = { 0, 0, 0 };
+    if (valid) {
+        const uint32_t key = top_keys[i];
+        prim = top_vals[i];
+        f3 v0, v1, v2, lo, hi;
+        load_tri(rec, prim, v0, v1, v2);
+        tri_bounds(v0, v1, v2, lo, hi);
+        const uint32_t tx = key % gx;
+        const uint32_t ty = (key % (gx * gy)) / gx;
+        const uint32_t tz = key / (gx * gy);
+        const uint2 cell = cells[key];
+        lb = cell.x;
+        const int c0 = (int)(cell.y & 0xFFu), c1 = (int)((cell.y >> 8) & 0xFFu),
+                  c2 = (int)((cell.y >> 16) & 0xFFu);
+        r0 = (uint32_t)c0;
+        r1 = (uint32_t)c1;
+        const f3 cres = mk((float)c0, (float)c1, (float)c2);
+        f3 org;
+        org.x = (float)tx * cs.x + bmin.x;
+        org.y = (float)ty * cs.y + bmin.y;
+        org.z = (float)tz * cs.z + bmin.z;
+        const f3 sub_rcp = cres / cs;
+        box_cells(lo, hi, org, sub_rcp, c0, c1, c2, mn, mx);
+        cnt = (uint32_t)((mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]));
+    }
+    if (!kWrite) {
+        const uint32_t s = wave_sum(cnt);
+        if (lane == 0) wave_counts[wg] = s;
+        return;
+    }
+
+    WaveItems& W = sh[wl];
+    const uint32_t inc = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(inc, kWave - 1, kWave);
+    W.excl[lane] = inc - cnt;
+    W.lo[0][lane] = mn[0]; W.lo[1][lane] = mn[1]; W.lo[2][lane] = mn[2];
+    W.dx[lane] = mx[0] - mn[0];
+    W.dy[lane] = mx[1] - mn[1];
+    W.id[lane] = prim;
+    W.lbeg[lane] = lb;
+    W.r0[lane] = r0;
+    W.r1[lane] = r1;
+    wave_sync();
+
+    const uint64_t lt = __lanemask_lt();
+    uint32_t run = wave_offsets[wg];
+    for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+        const uint32_t j = j0 + (uint32_t)lane;
+        const bool ok = j < total;
+        uint32_t key = 0, id = 0;
+        if (ok) {
+            const int o = wave_owner(W.excl, j);
+            const uint32_t k = j - W.excl[o];
+            const uint32_t dxo = (uint32_t)W.dx[o], dyo = (uint32_t)W.dy[o];
+            const uint32_t ix = k % dxo, t = k / dxo;
+            const uint32_t iy = t % dyo, iz = t / dyo;
+            const uint32_t x = (uint32_t)W.lo[0][o] + ix, y = (uint32_t)W.lo[1][o] + iy,
+                           z = (uint32_t)W.lo[2][o] + iz;
+            key = x + y * W.r0[o] + z * W.r0[o] * W.r1[o] + W.lbeg[o];
+            id = W.id[o];
+        }
+        const uint64_t bal = __ballot(ok);
+        if (ok) {
+            const uint32_t pos = run + (uint32_t)__popcll(bal & lt);
+            keys[pos] = key;
+            vals[pos] = id;
+        }
+        run += (uint32_t)__popcll(bal);
+    }
+}
+
+inline uint32_t cdiv(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
+
+inline int bits_for(uint64_t n)
+{
+    int b = 1;
+    while (b < 32 && (1ull << b) < n) ++b;
+    return b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+void launch_lerp_bounds(const float* k1, const float* k2, float c, uint32_t nv, float* out,
+                        BuildStatus* st, hipStream_t s)
+{
+    uint32_t blocks = cdiv(nv > 0 ? nv : 1, kBlock);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_lerp_bounds, dim3(blocks), dim3(kBlock), 0, s, k1, k1 + nv, k1 + 2 * (size_t)nv,
+                       k2, k2 + nv, k2 + 2 * (size_t)nv, c, nv, out, out + nv, out + 2 * (size_t)nv, st);
+}
+
+void launch_reset(BuildStatus* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, st);
+}
+
+void launch_grid_setup(GridParams* gp, BuildStatus* st, uint32_t nprims, float lambda,
+                       uint64_t cells_cap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, s, gp, st, nprims, lambda, cells_cap);
+}
+
+void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
+{
+    const uint32_t nt = a.nt;
+    const uint32_t waves = cdiv(nt > 0 ? nt : 1, kWave);
+    const uint32_t blocks = cdiv(waves, kWavesPerBlock);
+    BuildStatus* st = b.status;
+    const bool exact = a.kind == 1;
+    const bool two = a.kind == 2;
+
+    launch_grid_setup(b.params, st, nt, a.lambda_top, b.cells_cap, s);
+
+    // ---- top level / uniform: count, scan, write ----
+    if (exact)
+        hipLaunchKernelGGL((k_top_pairs<true, false>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
+                           a.vy, a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap);
+    else
+        hipLaunchKernelGGL((k_top_pairs<false, false>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
+                           a.vy, a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap);
+    scan_exclusive(b.wave_counts, b.wave_counts, waves, nullptr, &st->num_pairs, b.scan, s);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_pairs, &st->n_pairs_eff,
+                       b.pairs_cap, kOverflowPairs);
+    if (exact)
+        hipLaunchKernelGGL((k_top_pairs<true, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
+                           a.vy, a.vz, nt, b.params, st, b.rec, nullptr, b.wave_counts, b.keys, b.vals, b.waves_cap);
+    else
+        hipLaunchKernelGGL((k_top_pairs<false, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
+                           a.vy, a.vz, nt, b.params, st, b.rec, nullptr, b.wave_counts, b.keys, b.vals, b.waves_cap);
+
+    // ---- sort by cell key, then cell ranges ----
+    const int cell_bits = bits_for(b.cells_cap);
+    uint32_t* out_vals = two ? b.top_vals_sorted : b.prims;
+    radix_sort_pairs(b.keys, b.vals, b.keys_sorted, out_vals, b.pairs_cap, &st->n_pairs_eff, cell_bits,
+                     b.sort, s);
+    fill_u32((uint32_t*)b.cells, 0u, 2 * (size_t)b.cells_cap, &b.params->num_cells, 2, s);
+    const uint32_t rblocks = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256) > 4096 ? 4096
+                                 : cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256);
+    hipLaunchKernelGGL(k_ranges, dim3(rblocks), dim3(256), 0, s, b.keys_sorted, &st->n_pairs_eff, b.cells, st);
+    if (!two) return;
+
+    // ---- leaf resolution, leaf begin offsets ----
+    const uint32_t cblocks = cdiv(b.cells_cap, 256) > 4096 ? 4096 : cdiv(b.cells_cap, 256);
+    hipLaunchKernelGGL(k_leaf_res, dim3(cblocks), dim3(256), 0, s, b.params, b.cells, b.leaf_counts,
+                       a.lambda_leaf, st);
+    scan_exclusive_strided(b.leaf_counts, (uint32_t*)b.cells, 2, (uint32_t)b.cells_cap,
+                           &b.params->num_cells, &st->num_leaves, b.scan, s);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaves, &st->n_leaves_eff,
+                       b.leaves_cap, kOverflowLeaves);
+
+    // ---- leaf pairs: count, scan, write ----
+    const uint32_t lwaves = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, kWave);
+    const uint32_t lblocks = cdiv(lwaves, kWavesPerBlock);
+    hipLaunchKernelGGL((k_leaf_pairs<false>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
+                       b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, b.wave_counts,
+                       nullptr, nullptr, nullptr, b.waves_cap);
+    scan_exclusive(b.wave_counts, b.wave_counts, lwaves, nullptr, &st->num_leaf_pairs, b.scan, s);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaf_pairs, &st->n_leaf_pairs_eff,
+                       b.leaf_pairs_cap, kOverflowLeafPairs);
+    hipLaunchKernelGGL((k_leaf_pairs<true>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
+                       b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, nullptr,
+                       b.wave_counts, b.keys, b.vals, b.waves_cap);
+
+    // ---- leaf sort and ranges ----
+    const int leaf_bits = bits_for(b.leaves_cap);
+    radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.prims, b.leaf_pairs_cap, &st->n_leaf_pairs_eff,
+                     leaf_bits, b.sort, s);
+    fill_u32((uint32_t*)b.leaves, 0u, 2 * (size_t)b.leaves_cap, &st->n_leaves_eff, 2, s);
+    const uint32_t lrblocks = cdiv(b.leaf_pairs_cap > 0 ? b.leaf_pairs_cap : 1, 256) > 4096 ? 4096
+                                  : cdiv(b.leaf_pairs_cap > 0 ? b.leaf_pairs_cap : 1, 256);
+    hipLaunchKernelGGL(k_ranges, dim3(lrblocks), dim3(256), 0, s, b.keys_sorted, &st->n_leaf_pairs_eff,
+                       b.leaves, st);
+}
+
+}  // namespace hr

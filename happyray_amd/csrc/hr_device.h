@@ -1,0 +1,192 @@
+// hr_device.h -- device-resident structures and wave64 helpers shared by the
+// build, sort and trace kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hr_math.h"
+
+namespace hr {
+
+constexpr int kWave = 64;
+
+// Grid parameters as the builder leaves them in device memory
+// (UniformGrid / TwoLevelGrid, src/RT/Structure/UniformGrid.h:37-138,
+// src/RT/Structure/TwoLevelGrid.h:163-230).  Written by the setup kernel so
+// that no host round trip is needed between bounds and resolution.
+struct GridParams {
+    float bmin[3], bmax[3];
+    int res[3];
+    float cs[3], csr[3];        // cell size, reciprocal cell size
+    uint32_t num_cells;
+    uint32_t fallback;          // 1 when the 32^3 fallback was taken
+    uint32_t pad[2];
+};
+
+// Build counters and capacity flags.
+struct BuildStatus {
+    uint32_t bounds_enc[6];     // order-preserving encodings: min xyz, max xyz
+    uint32_t num_cells;         // required sizes (valid up to the first overflow)
+    uint32_t num_pairs;         // uniform pairs / two-level top pairs
+    uint32_t num_leaves;
+    uint32_t num_leaf_pairs;
+    uint32_t overflow;          // kOverflow* bits
+    uint32_t n_pairs_eff;       // sizes the later kernels use: 0 after an overflow
+    uint32_t n_leaves_eff;
+    uint32_t n_leaf_pairs_eff;
+    uint32_t pad[2];
+};
+
+constexpr uint32_t kOverflowCells = 1u;
+constexpr uint32_t kOverflowPairs = 2u;
+constexpr uint32_t kOverflowLeaves = 4u;
+constexpr uint32_t kOverflowLeafPairs = 8u;
+
+// order-preserving float <-> uint encoding for atomic min / max
+__device__ __forceinline__ uint32_t enc_f(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+HR_HD float dec_f(uint32_t e)
+{
+    const uint32_t b = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
+    return __builtin_bit_cast(float, b);
+}
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+
+// orders LDS traffic between the lanes of one wave (no workgroup barrier)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, kWave);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
+// largest o in [0, 64) with excl[o] <= j (excl[0] == 0 <= j)
+__device__ __forceinline__ int wave_owner(const uint32_t* excl, uint32_t j)
+{
+    int o = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (excl[o + step] <= j) o += step;
+    return o;
+}
+
+// triangle record: three float4 (vertex xyz, unused w) per triangle
+__device__ __forceinline__ void load_tri(const float4* __restrict__ rec, uint32_t p, f3& v0, f3& v1,
+                                         f3& v2)
+{
+    const float4 a = rec[3 * (size_t)p + 0];
+    const float4 b = rec[3 * (size_t)p + 1];
+    const float4 c = rec[3 * (size_t)p + 2];
+    v0 = mk(a.x, a.y, a.z);
+    v1 = mk(b.x, b.y, b.z);
+    v2 = mk(c.x, c.y, c.z);
+}
+
+// BBoxExtractor<Triangle> (src/RT/Primitive/BBox.hpp:96-110)
+__device__ __forceinline__ void tri_bounds(f3 v0, f3 v1, f3 v2, f3& lo, f3& hi)
+{
+    lo = rep(kFltMax);
+    hi = rep(-kFltMax);
+    lo = vmin(lo, v0); hi = vmax(hi, v0);
+    lo = vmin(lo, v1); hi = vmax(hi, v1);
+    lo = vmin(lo, v2); hi = vmax(hi, v2);
+}
+
+// clamped cell range of a box (countPairs, UniformGridBuildKernels.h:59-69);
+// an empty range is kept empty instead of going negative.
+__device__ __forceinline__ void box_cells(f3 lo, f3 hi, f3 origin, f3 rcp, int rx, int ry, int rz,
+                                          int mn[3], int mx[3])
+{
+    const f3 a = (lo - origin) * rcp;
+    const f3 b = (hi - origin) * rcp + rep(1.f);
+    const int a0 = (int)a.x, a1 = (int)a.y, a2 = (int)a.z;
+    const int b0 = (int)b.x, b1 = (int)b.y, b2 = (int)b.z;
+    mn[0] = a0 > 0 ? a0 : 0;
+    mn[1] = a1 > 0 ? a1 : 0;
+    mn[2] = a2 > 0 ? a2 : 0;
+    mx[0] = rx < b0 ? rx : b0;
+    mx[1] = ry < b1 ? ry : b1;
+    mx[2] = rz < b2 ? rz : b2;
+    mx[0] = mx[0] < mn[0] ? mn[0] : mx[0];
+    mx[1] = mx[1] < mn[1] ? mn[1] : mx[1];
+    mx[2] = mx[2] < mn[2] ? mn[2] : mx[2];
+}
+
+// ---------------------------------------------------------------------------
+// host launch helpers (hr_build.hip, hr_sort.hip, hr_trace.hip)
+// ---------------------------------------------------------------------------
+
+struct ScanTemp {
+    uint32_t* block_sums;   // capacity: ceil(n_cap / kScanTile) + 1
+    size_t cap;
+};
+
+constexpr uint32_t kScanTile = 2048;     // 256 threads x 8
+constexpr uint32_t kSortTile = 4096;     // 256 threads x 16
+
+// exclusive scan of n (host value, or *n_dev when n_dev != nullptr) uint32;
+// *total_dev (may be nullptr) receives the sum.
+void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_cap, const uint32_t* n_dev,
+                    uint32_t* total_dev, ScanTemp& tmp, hipStream_t s);
+// same, writing out[i * out_stride]
+void scan_exclusive_strided(const uint32_t* in, uint32_t* out, uint32_t out_stride, uint32_t n_cap,
+                            const uint32_t* n_dev, uint32_t* total_dev, ScanTemp& tmp,
+                            hipStream_t s);
+size_t scan_temp_words(uint32_t n_cap);
+
+// stable LSD radix sort of (key, value) pairs on the low `bits` key bits.
+// n = *n_dev (<= n_cap).  keys_in / vals_in are overwritten when the number
+// of passes exceeds 2.  The result lands in keys_out / vals_out.
+struct SortTemp {
+    uint32_t* keys_tmp;
+    uint32_t* vals_tmp;
+    uint32_t* hist;         // 256 * tiles(n_cap)
+    ScanTemp scan;
+};
+size_t sort_hist_words(uint32_t n_cap);
+void radix_sort_pairs(uint32_t* keys_in, uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                      uint32_t n_cap, const uint32_t* n_dev, int bits, SortTemp& tmp, hipStream_t s);
+
+void fill_u32(uint32_t* p, uint32_t value, size_t n_cap, const uint32_t* n_dev, uint32_t n_mul,
+              hipStream_t s);
+
+}  // namespace hr

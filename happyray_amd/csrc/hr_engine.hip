@@ -1,0 +1,769 @@
+// hr_engine.hip -- host side of the library: the context that replaces the
+// reference's static RTEngine (src/RT/RTEngine.h:36-68, src/RT/RTEngine.cu:91-474)
+// and the C ABI declared in include/happyray.h.
+//
+// The context owns one HIP stream (default priority) and grow-only device
+// buffers (MemoryManager::allocate*, src/RT/Structure/MemoryManager.h:73-139).
+// A build enqueues interpolation, bounds and the whole grid build without a
+// host round trip, then reads one status block; when a capacity was too
+// small the buffers grow to the reported size and the build runs again.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/happyray.h"
+#include "hr_build.h"
+#include "hr_trace.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+hr_status fail(hr_status s, const std::string& msg)
+{
+    g_last_error = msg;
+    return s;
+}
+
+#define HR_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(e_ == hipErrorOutOfMemory ? HR_ERR_OOM : HR_ERR_DEVICE,                 \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+#define HR_TRY(call)                                                                            \
+    do {                                                                                        \
+        hr_status s_ = (call);                                                                  \
+        if (s_ != HR_OK) return s_;                                                             \
+    } while (0)
+
+// grow-only device buffer
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hr_status reserve(size_t n)
+    {
+        if (n <= cap && p) return HR_OK;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc((void**)&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(HR_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                        hipGetErrorString(e));
+        }
+        cap = std::max<size_t>(n, 1);
+        return HR_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct hr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t seed_lo = 0, seed_hi = 0;
+
+    // geometry sources: the frame is (src1, src2, coeff)
+    uint32_t nv = 0, nt = 0;
+    DevBuf<float> key_a, key_b;       // hr_upload_mesh sources
+    DevBuf<float> keyframes;          // hr_set_keyframes
+    uint32_t nframes = 0;
+    const float* src1 = nullptr;
+    const float* src2 = nullptr;
+    float coeff = 1.f;
+    DevBuf<float> verts;
+    DevBuf<uint32_t> idx;
+
+    DevBuf<float> mats;
+    DevBuf<uint32_t> mat_idx;
+    bool has_mat_idx = false;
+
+    int kind = HR_GRID_TWO_LEVEL;
+    float lambda_top = 1.2f, lambda_leaf = 2.2f;   // src/RT/RTEngine.cu:79-80
+
+    // build
+    DevBuf<hr::GridParams> params;
+    DevBuf<hr::BuildStatus> status;
+    hr::BuildStatus* h_status = nullptr;  // pinned
+    DevBuf<float4> rec;
+    DevBuf<uint32_t> wave_counts, keys, vals, keys_sorted, top_vals_sorted, prims, leaf_counts;
+    DevBuf<uint2> cells, leaves;
+    DevBuf<uint32_t> scan_tmp, sort_keys_tmp, sort_vals_tmp, sort_hist;
+    uint64_t cells_cap = 0;
+    uint32_t pairs_cap = 0, leaves_cap = 0, leaf_pairs_cap = 0;
+    bool built = false;
+    hr_grid_info info{};
+    hr::GridParams h_params{};
+
+    // camera / lights
+    hr::CameraDev cam{};
+    bool has_camera = false;
+    DevBuf<float> lights, light_cdf;
+    int nlights = 0;
+
+    // render
+    DevBuf<float> hits, fb;
+    DevBuf<uint32_t> hit_prims;
+    DevBuf<uint8_t> occ;
+    uint32_t last_npix = 0;
+    int last_ao_samples = 0;
+
+    // ray batches
+    DevBuf<float> ray_org, ray_dir, ray_t;
+    DevBuf<uint32_t> ray_prim;
+    DevBuf<uint8_t> ray_occ;
+
+    // timing / counters
+    hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr, ev_r0 = nullptr, ev_r1 = nullptr;
+    bool build_timed = false, render_timed = false;
+    DevBuf<unsigned long long> ray_count;
+};
+
+namespace {
+
+hr_status set_device(hr_ctx* c)
+{
+    HR_HIP(hipSetDevice(c->device));
+    return HR_OK;
+}
+
+hr_status check_ctx(hr_ctx* c)
+{
+    if (!c) return fail(HR_ERR_ARG, "null context");
+    return set_device(c);
+}
+
+uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+hr_status upload(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return HR_OK;
+    HR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return HR_OK;
+}
+
+hr_status validate_indices(const uint32_t* idx, uint32_t nt, uint32_t nv)
+{
+    for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+        if (idx[i] >= nv)
+            return fail(HR_ERR_ARG, "vertex index " + std::to_string(idx[i]) + " at position " +
+                                        std::to_string(i) + " is out of range (nv = " +
+                                        std::to_string(nv) + ")");
+    return HR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// build
+// ---------------------------------------------------------------------------
+
+hr_status reserve_build(hr_ctx* c)
+{
+    const uint32_t nt = c->nt;
+    const bool two = c->kind == HR_GRID_TWO_LEVEL;
+    const uint32_t pair_max = std::max(c->pairs_cap, two ? c->leaf_pairs_cap : 0u);
+    HR_TRY(c->rec.reserve(3 * (size_t)std::max(nt, 1u)));
+    const uint32_t waves = cdiv(std::max<uint64_t>(std::max(nt, c->pairs_cap), 1), 64);
+    const uint32_t waves_cap = (waves + 3) / 4 * 4 + 4;
+    HR_TRY(c->wave_counts.reserve(waves_cap));
+    HR_TRY(c->keys.reserve(pair_max));
+    HR_TRY(c->vals.reserve(pair_max));
+    HR_TRY(c->keys_sorted.reserve(pair_max));
+    HR_TRY(c->prims.reserve(two ? c->leaf_pairs_cap : c->pairs_cap));
+    HR_TRY(c->cells.reserve(c->cells_cap));
+    if (two) {
+        HR_TRY(c->top_vals_sorted.reserve(c->pairs_cap));
+        HR_TRY(c->leaf_counts.reserve(c->cells_cap));
+        HR_TRY(c->leaves.reserve(c->leaves_cap));
+    }
+    const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)c->cells_cap,
+                                             hr::sort_hist_words(pair_max) });
+    HR_TRY(c->scan_tmp.reserve(hr::scan_temp_words((uint32_t)std::min<size_t>(scan_n, 0xFFFFFFFFu))));
+    HR_TRY(c->sort_keys_tmp.reserve(pair_max));
+    HR_TRY(c->sort_vals_tmp.reserve(pair_max));
+    HR_TRY(c->sort_hist.reserve(hr::sort_hist_words(pair_max)));
+    return HR_OK;
+}
+
+hr::BuildBuffers build_buffers(hr_ctx* c)
+{
+    hr::BuildBuffers b;
+    b.params = c->params.p;
+    b.status = c->status.p;
+    b.rec = c->rec.p;
+    b.wave_counts = c->wave_counts.p;
+    b.keys = c->keys.p;
+    b.vals = c->vals.p;
+    b.keys_sorted = c->keys_sorted.p;
+    b.top_vals_sorted = c->top_vals_sorted.p;
+    b.prims = c->prims.p;
+    b.cells = c->cells.p;
+    b.leaves = c->leaves.p;
+    b.leaf_counts = c->leaf_counts.p;
+    b.cells_cap = c->cells_cap;
+    b.pairs_cap = c->pairs_cap;
+    b.leaves_cap = c->leaves_cap;
+    b.leaf_pairs_cap = c->leaf_pairs_cap;
+    b.waves_cap = (uint32_t)c->wave_counts.cap;
+    b.scan.block_sums = c->scan_tmp.p;
+    b.scan.cap = c->scan_tmp.cap;
+    b.sort.keys_tmp = c->sort_keys_tmp.p;
+    b.sort.vals_tmp = c->sort_vals_tmp.p;
+    b.sort.hist = c->sort_hist.p;
+    b.sort.scan = b.scan;
+    return b;
+}
+
+void enqueue_build(hr_ctx* c)
+{
+    hr::launch_reset(c->status.p, c->stream);
+    hr::launch_lerp_bounds(c->src1, c->src2, c->coeff, c->nv, c->verts.p, c->status.p, c->stream);
+    hr::BuildArgs a;
+    a.kind = c->kind;
+    a.lambda_top = c->lambda_top;
+    a.lambda_leaf = c->lambda_leaf;
+    a.idx = c->idx.p;
+    a.vx = c->verts.p;
+    a.vy = c->verts.p + c->nv;
+    a.vz = c->verts.p + 2 * (size_t)c->nv;
+    a.nt = c->nt;
+    hr::BuildBuffers b = build_buffers(c);
+    hr::run_build(a, b, c->stream);
+}
+
+uint32_t grow(uint64_t cur, uint64_t need)
+{
+    uint64_t v = std::max<uint64_t>(need + need / 4 + 1024, cur + cur / 2);
+    return (uint32_t)std::min<uint64_t>(v, 0xFFFFFFF0ull);
+}
+
+hr_status do_build(hr_ctx* c)
+{
+    if (!c->src1 || c->nv == 0) return fail(HR_ERR_STATE, "no geometry uploaded");
+    const bool two = c->kind == HR_GRID_TWO_LEVEL;
+    // resolution product <= lambda * N (up to rounding) or the 32^3 fallback
+    const uint64_t cells_bound =
+        std::max<uint64_t>(32768, (uint64_t)((double)c->lambda_top * (double)c->nt * 1.001) + 64);
+    if (cells_bound > 0xFFFFFFF0ull) return fail(HR_ERR_CAPACITY, "grid density too high");
+    if (c->cells_cap < cells_bound) c->cells_cap = cells_bound;
+    if (c->pairs_cap == 0) c->pairs_cap = std::max<uint32_t>(4096, 4 * c->nt);
+    if (two) {
+        if (c->leaves_cap == 0) c->leaves_cap = std::max<uint32_t>(65536, 8 * c->nt);
+        if (c->leaf_pairs_cap == 0) c->leaf_pairs_cap = std::max<uint32_t>(65536, 8 * c->nt);
+    }
+    c->built = false;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        HR_TRY(reserve_build(c));
+        HR_HIP(hipEventRecord(c->ev_b0, c->stream));
+        enqueue_build(c);
+        HR_HIP(hipEventRecord(c->ev_b1, c->stream));
+        HR_HIP(hipMemcpyAsync(c->h_status, c->status.p, sizeof(hr::BuildStatus), hipMemcpyDeviceToHost,
+                              c->stream));
+        HR_HIP(hipStreamSynchronize(c->stream));
+        HR_HIP(hipGetLastError());
+        const hr::BuildStatus& st = *c->h_status;
+        if (st.overflow == 0u) {
+            c->build_timed = true;
+            HR_HIP(hipMemcpy(&c->h_params, c->params.p, sizeof(hr::GridParams), hipMemcpyDeviceToHost));
+            hr_grid_info& in = c->info;
+            in.kind = c->kind;
+            for (int i = 0; i < 3; ++i) {
+                in.bmin[i] = c->h_params.bmin[i];
+                in.bmax[i] = c->h_params.bmax[i];
+                in.res[i] = c->h_params.res[i];
+                in.cell_size[i] = c->h_params.cs[i];
+                in.cell_size_rcp[i] = c->h_params.csr[i];
+            }
+            in.num_cells = st.num_cells;
+            in.num_leaves = two ? st.num_leaves : 0u;
+            in.num_refs = two ? st.num_leaf_pairs : st.num_pairs;
+            in.num_top_pairs = two ? st.num_pairs : 0u;
+            in.num_triangles = c->nt;
+            c->built = true;
+            return HR_OK;
+        }
+        if (st.overflow & hr::kOverflowCells) {
+            if (st.num_cells == 0xFFFFFFFFu) return fail(HR_ERR_CAPACITY, "grid resolution too large");
+            c->cells_cap = grow(c->cells_cap, st.num_cells);
+        }
+        if (st.overflow & hr::kOverflowPairs) c->pairs_cap = grow(c->pairs_cap, st.num_pairs);
+        if (st.overflow & hr::kOverflowLeaves) c->leaves_cap = grow(c->leaves_cap, st.num_leaves);
+        if (st.overflow & hr::kOverflowLeafPairs)
+            c->leaf_pairs_cap = grow(c->leaf_pairs_cap, st.num_leaf_pairs);
+    }
+    return fail(HR_ERR_CAPACITY, "grid build did not fit after repeated growth");
+}
+
+hr::TraceScene trace_scene(hr_ctx* c)
+{
+    hr::TraceScene S;
+    S.gp = c->params.p;
+    S.cells = c->cells.p;
+    S.leaves = c->kind == HR_GRID_TWO_LEVEL ? c->leaves.p : nullptr;
+    S.prims = c->prims.p;
+    S.rec = c->rec.p;
+    S.nt = c->nt;
+    S.two_level = c->kind == HR_GRID_TWO_LEVEL ? 1 : 0;
+    return S;
+}
+
+// Camera::init (src/RT/Primitive/Camera.h:42-61), evaluated on the host
+void camera_init(hr::CameraDev& cam, const float pos[3], const float dir[3], const float up[3], float fov,
+                 int rx, int ry)
+{
+    using namespace hr;
+    const float aspect = (float)rx / (float)ry;
+    const f3 fwd = normalize(mk(dir[0], dir[1], dir[2]));
+    const f3 right = normalize(cross(fwd, mk(up[0], up[1], up[2])));
+    const f3 upa = normalize(cross(fwd, right));
+    const float ang = fov * (float)kPi / 180.f;
+    const f3 row = ((2.f * right) * tanf(ang / 2.f)) * aspect;
+    const f3 col = (2.f * upa) * tanf(ang / 2.f);
+    const f3 sx = row / (float)rx;
+    const f3 sy = col / (float)ry;
+    const f3 tl = fwd - row / 2.f - col / 2.f;
+    for (int i = 0; i < 3; ++i) cam.center[i] = pos[i];
+    cam.top_left[0] = tl.x; cam.top_left[1] = tl.y; cam.top_left[2] = tl.z;
+    cam.step_x[0] = sx.x; cam.step_x[1] = sx.y; cam.step_x[2] = sx.z;
+    cam.step_y[0] = sy.x; cam.step_y[1] = sy.y; cam.step_y[2] = sy.z;
+    cam.res_x = rx;
+    cam.res_y = ry;
+}
+
+hr_status ensure_default_material(hr_ctx* c)
+{
+    if (c->mats.p) return HR_OK;
+    const float m[12] = { 0.2f, 0.2f, 0.2f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f };
+    HR_TRY(c->mats.reserve(12));
+    HR_TRY(upload(c->mats.p, m, sizeof m, c->stream));
+    return HR_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* hr_last_error(void) { return g_last_error.c_str(); }
+
+hr_status hr_create(hr_ctx** out, const hr_options* opts)
+{
+    if (!out) return fail(HR_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(HR_ERR_DEVICE, "no HIP device available");
+    hr_ctx* c = new hr_ctx();
+    c->device = opts ? opts->device : 0;
+    c->seed_lo = opts ? opts->seed_lo : 0u;
+    c->seed_hi = opts ? opts->seed_hi : 0u;
+    if (c->device < 0 || c->device >= n) {
+        delete c;
+        return fail(HR_ERR_ARG, "device ordinal out of range");
+    }
+    hr_status s = HR_OK;
+    do {
+        if ((s = set_device(c)) != HR_OK) break;
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+        if ((s = c->params.reserve(1)) != HR_OK) break;
+        if ((s = c->status.reserve(1)) != HR_OK) break;
+        if ((s = c->ray_count.reserve(1)) != HR_OK) break;
+        e = hipHostMalloc((void**)&c->h_status, sizeof(hr::BuildStatus), hipHostMallocDefault);
+        if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
+        hipEvent_t* evs[4] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1 };
+        for (hipEvent_t* ev : evs) {
+            e = hipEventCreate(ev);
+            if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+        }
+        if (s != HR_OK) break;
+        e = hipMemsetAsync(c->ray_count.p, 0, sizeof(unsigned long long), c->stream);
+        if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+    } while (0);
+    if (s != HR_OK) {
+        hr_destroy(c);
+        return s;
+    }
+    *out = c;
+    return HR_OK;
+}
+
+hr_status hr_destroy(hr_ctx* c)
+{
+    if (!c) return HR_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf<float>* fbufs[] = { &c->key_a, &c->key_b, &c->keyframes, &c->verts, &c->mats, &c->lights,
+                               &c->light_cdf, &c->hits, &c->fb, &c->ray_org, &c->ray_dir, &c->ray_t };
+    for (auto* b : fbufs) b->release();
+    DevBuf<uint32_t>* ubufs[] = { &c->idx, &c->mat_idx, &c->wave_counts, &c->keys, &c->vals,
+                                  &c->keys_sorted, &c->top_vals_sorted, &c->prims, &c->leaf_counts,
+                                  &c->scan_tmp, &c->sort_keys_tmp, &c->sort_vals_tmp, &c->sort_hist,
+                                  &c->hit_prims, &c->ray_prim };
+    for (auto* b : ubufs) b->release();
+    c->occ.release();
+    c->ray_occ.release();
+    c->rec.release();
+    c->cells.release();
+    c->leaves.release();
+    c->params.release();
+    c->status.release();
+    c->ray_count.release();
+    if (c->h_status) (void)hipHostFree(c->h_status);
+    hipEvent_t evs[4] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1 };
+    for (hipEvent_t ev : evs)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HR_OK;
+}
+
+hr_status hr_get_stream(hr_ctx* c, void** stream)
+{
+    HR_TRY(check_ctx(c));
+    if (!stream) return fail(HR_ERR_ARG, "null output pointer");
+    *stream = (void*)c->stream;
+    return HR_OK;
+}
+
+hr_status hr_upload_mesh(hr_ctx* c, const float* xyz1, const float* xyz2, float coeff, uint32_t nv,
+                         const uint32_t* idx, uint32_t nt)
+{
+    HR_TRY(check_ctx(c));
+    if (!xyz1 || nv == 0) return fail(HR_ERR_ARG, "a mesh needs at least one vertex");
+    if (nt > 0 && !idx) return fail(HR_ERR_ARG, "null index buffer");
+    if (nt > 0x3FFFFFFFu) return fail(HR_ERR_ARG, "too many triangles");
+    HR_TRY(validate_indices(idx, nt, nv));
+    const size_t vb = 3 * (size_t)nv * sizeof(float);
+    HR_TRY(c->key_a.reserve(3 * (size_t)nv));
+    HR_TRY(upload(c->key_a.p, xyz1, vb, c->stream));
+    if (xyz2 && xyz2 != xyz1) {
+        HR_TRY(c->key_b.reserve(3 * (size_t)nv));
+        HR_TRY(upload(c->key_b.p, xyz2, vb, c->stream));
+        c->src2 = c->key_b.p;
+    } else {
+        c->src2 = c->key_a.p;
+    }
+    c->src1 = c->key_a.p;
+    c->coeff = coeff;
+    HR_TRY(c->verts.reserve(3 * (size_t)nv));
+    HR_TRY(c->idx.reserve(3 * (size_t)std::max(nt, 1u)));
+    HR_TRY(upload(c->idx.p, idx, 3 * (size_t)nt * sizeof(uint32_t), c->stream));
+    c->nv = nv;
+    c->nt = nt;
+    c->built = false;
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
+}
+
+hr_status hr_upload_materials(hr_ctx* c, const float* mats, uint32_t nmat, const uint32_t* mat_idx,
+                              uint32_t nt)
+{
+    HR_TRY(check_ctx(c));
+    if (!mats || nmat == 0) return fail(HR_ERR_ARG, "empty material table");
+    HR_TRY(c->mats.reserve(12 * (size_t)nmat));
+    HR_TRY(upload(c->mats.p, mats, 12 * (size_t)nmat * sizeof(float), c->stream));
+    c->has_mat_idx = false;
+    if (mat_idx) {
+        if (nt != c->nt) return fail(HR_ERR_ARG, "material index count differs from the triangle count");
+        for (uint32_t i = 0; i < nt; ++i)
+            if (mat_idx[i] >= nmat) return fail(HR_ERR_ARG, "material index out of range");
+        HR_TRY(c->mat_idx.reserve(std::max(nt, 1u)));
+        HR_TRY(upload(c->mat_idx.p, mat_idx, (size_t)nt * sizeof(uint32_t), c->stream));
+        c->has_mat_idx = true;
+    }
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
+}
+
+hr_status hr_set_keyframes(hr_ctx* c, const float* xyz, uint32_t nframes, uint32_t nv, const uint32_t* idx,
+                           uint32_t nt)
+{
+    HR_TRY(check_ctx(c));
+    if (!xyz || nframes == 0 || nv == 0) return fail(HR_ERR_ARG, "empty key frames");
+    if (nt > 0 && !idx) return fail(HR_ERR_ARG, "null index buffer");
+    HR_TRY(validate_indices(idx, nt, nv));
+    const size_t n = (size_t)nframes * 3 * nv;
+    HR_TRY(c->keyframes.reserve(n));
+    HR_TRY(upload(c->keyframes.p, xyz, n * sizeof(float), c->stream));
+    HR_TRY(c->verts.reserve(3 * (size_t)nv));
+    HR_TRY(c->idx.reserve(3 * (size_t)std::max(nt, 1u)));
+    HR_TRY(upload(c->idx.p, idx, 3 * (size_t)nt * sizeof(uint32_t), c->stream));
+    c->nframes = nframes;
+    c->nv = nv;
+    c->nt = nt;
+    c->src1 = c->src2 = c->keyframes.p;
+    c->coeff = 1.f;
+    c->built = false;
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
+}
+
+hr_status hr_set_frame(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff)
+{
+    HR_TRY(check_ctx(c));
+    if (c->nframes == 0) return fail(HR_ERR_STATE, "no key frames resident");
+    if (k1 >= c->nframes || k2 >= c->nframes) return fail(HR_ERR_ARG, "key frame index out of range");
+    c->src1 = c->keyframes.p + (size_t)k1 * 3 * c->nv;
+    c->src2 = c->keyframes.p + (size_t)k2 * 3 * c->nv;
+    c->coeff = coeff;
+    c->built = false;
+    return HR_OK;
+}
+
+hr_status hr_set_grid(hr_ctx* c, hr_grid_kind kind, float lambda_top, float lambda_leaf)
+{
+    HR_TRY(check_ctx(c));
+    if (kind != HR_GRID_UNIFORM && kind != HR_GRID_UNIFORM_EXACT && kind != HR_GRID_TWO_LEVEL)
+        return fail(HR_ERR_ARG, "unknown grid kind");
+    if (!(lambda_top > 0.f) || !(lambda_top < 1e6f)) return fail(HR_ERR_ARG, "lambda_top out of range");
+    if (kind == HR_GRID_TWO_LEVEL && (!(lambda_leaf > 0.f) || !(lambda_leaf < 1e6f)))
+        return fail(HR_ERR_ARG, "lambda_leaf out of range");
+    if (kind != c->kind) {
+        c->pairs_cap = c->leaves_cap = c->leaf_pairs_cap = 0;
+    }
+    c->kind = kind;
+    c->lambda_top = lambda_top;
+    c->lambda_leaf = lambda_leaf;
+    c->built = false;
+    return HR_OK;
+}
+
+hr_status hr_build(hr_ctx* c)
+{
+    HR_TRY(check_ctx(c));
+    return do_build(c);
+}
+
+hr_status hr_get_grid_info(hr_ctx* c, hr_grid_info* out)
+{
+    HR_TRY(check_ctx(c));
+    if (!out) return fail(HR_ERR_ARG, "null output pointer");
+    if (!c->built) return fail(HR_ERR_STATE, "no grid built");
+    *out = c->info;
+    return HR_OK;
+}
+
+hr_status hr_get_grid(hr_ctx* c, uint32_t* cells, uint32_t* leaves, uint32_t* prims)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->built) return fail(HR_ERR_STATE, "no grid built");
+    HR_HIP(hipStreamSynchronize(c->stream));
+    if (cells)
+        HR_HIP(hipMemcpy(cells, c->cells.p, 2 * (size_t)c->info.num_cells * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    if (leaves && c->kind == HR_GRID_TWO_LEVEL && c->info.num_leaves)
+        HR_HIP(hipMemcpy(leaves, c->leaves.p, 2 * (size_t)c->info.num_leaves * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    if (prims && c->info.num_refs)
+        HR_HIP(hipMemcpy(prims, c->prims.p, (size_t)c->info.num_refs * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    return HR_OK;
+}
+
+hr_status hr_set_camera(hr_ctx* c, const float pos[3], const float dir[3], const float up[3], float fov_deg,
+                        int res_x, int res_y)
+{
+    HR_TRY(check_ctx(c));
+    if (!pos || !dir || !up) return fail(HR_ERR_ARG, "null camera vector");
+    if (res_x <= 0 || res_y <= 0 || (int64_t)res_x * res_y > (1ll << 28))
+        return fail(HR_ERR_ARG, "invalid resolution");
+    camera_init(c->cam, pos, dir, up, fov_deg, res_x, res_y);
+    c->has_camera = true;
+    return HR_OK;
+}
+
+hr_status hr_set_lights(hr_ctx* c, const hr_area_light* lights, const float* weights, int n)
+{
+    HR_TRY(check_ctx(c));
+    if (n < 0 || (n > 0 && !lights)) return fail(HR_ERR_ARG, "invalid light list");
+    c->nlights = n;
+    if (n == 0) return HR_OK;
+    std::vector<float> cdf((size_t)n);
+    float acc = 0.f;
+    for (int i = 0; i < n; ++i) {
+        acc += weights ? weights[i] : 1.f;
+        cdf[(size_t)i] = acc;
+    }
+    HR_TRY(c->lights.reserve(15 * (size_t)n));
+    HR_TRY(c->light_cdf.reserve((size_t)n));
+    HR_TRY(upload(c->lights.p, lights, 15 * (size_t)n * sizeof(float), c->stream));
+    HR_TRY(upload(c->light_cdf.p, cdf.data(), (size_t)n * sizeof(float), c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
+}
+
+hr_status hr_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_params* p, float* rgb_out)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->built) return fail(HR_ERR_STATE, "render before build");
+    if (!c->has_camera) return fail(HR_ERR_STATE, "render before setCamera");
+    if (!p) return fail(HR_ERR_ARG, "null render parameters");
+    if (mode != HR_MODE_AO && mode != HR_MODE_PATH) return fail(HR_ERR_ARG, "unknown render mode");
+    if (image_id < 0) return fail(HR_ERR_ARG, "negative image id");
+    if (mode == HR_MODE_AO && (p->ao_samples < 1 || p->ao_samples > 256))
+        return fail(HR_ERR_ARG, "ao_samples must be in [1, 256]");
+    if (mode == HR_MODE_PATH && (p->pt_depth < 1 || p->pt_depth > 64))
+        return fail(HR_ERR_ARG, "pt_depth must be in [1, 64]");
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    HR_TRY(c->hits.reserve(7 * (size_t)npix));
+    HR_TRY(c->hit_prims.reserve(npix));
+    HR_TRY(c->fb.reserve(3 * (size_t)npix));
+    if (mode == HR_MODE_AO) HR_TRY(c->occ.reserve((size_t)npix * (size_t)p->ao_samples));
+    HR_TRY(ensure_default_material(c));
+    if (image_id == 0 || npix != c->last_npix) {
+        hr::fill_u32((uint32_t*)c->fb.p, 0u, 3 * (size_t)npix, nullptr, 1, c->stream);
+        c->last_npix = npix;
+    }
+    hr::RenderArgs ra;
+    ra.seed_lo = c->seed_lo;
+    ra.seed_hi = c->seed_hi;
+    ra.jitter = p->jitter;
+    ra.ao_samples = p->ao_samples;
+    ra.ao_dist = p->ao_dist;
+    ra.pt_depth = p->pt_depth;
+    ra.image = image_id;
+    const hr::TraceScene S = trace_scene(c);
+    hr::Shading sh;
+    sh.mats = c->mats.p;
+    sh.mat_idx = c->has_mat_idx ? c->mat_idx.p : nullptr;
+    sh.lights = c->lights.p;
+    sh.light_cdf = c->light_cdf.p;
+    sh.nlights = c->nlights;
+    HR_HIP(hipEventRecord(c->ev_r0, c->stream));
+    hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
+    if (mode == HR_MODE_AO) {
+        hr::launch_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->ray_count.p,
+                      c->stream);
+        c->last_ao_samples = p->ao_samples;
+    } else {
+        hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
+    }
+    HR_HIP(hipEventRecord(c->ev_r1, c->stream));
+    HR_HIP(hipGetLastError());
+    c->render_timed = true;
+    if (rgb_out) {
+        HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, 3 * (size_t)npix * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
+        HR_HIP(hipStreamSynchronize(c->stream));
+    }
+    return HR_OK;
+}
+
+hr_status hr_get_primary_hits(hr_ctx* c, float* planes7, uint32_t* prims)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->last_npix) return fail(HR_ERR_STATE, "nothing rendered");
+    HR_HIP(hipStreamSynchronize(c->stream));
+    if (planes7)
+        HR_HIP(hipMemcpy(planes7, c->hits.p, 7 * (size_t)c->last_npix * sizeof(float), hipMemcpyDeviceToHost));
+    if (prims)
+        HR_HIP(hipMemcpy(prims, c->hit_prims.p, (size_t)c->last_npix * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    return HR_OK;
+}
+
+hr_status hr_get_ao_occlusion(hr_ctx* c, uint8_t* occ)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->last_npix || !c->last_ao_samples) return fail(HR_ERR_STATE, "no AO render");
+    if (!occ) return fail(HR_ERR_ARG, "null output pointer");
+    HR_HIP(hipStreamSynchronize(c->stream));
+    HR_HIP(hipMemcpy(occ, c->occ.p, (size_t)c->last_npix * (size_t)c->last_ao_samples, hipMemcpyDeviceToHost));
+    return HR_OK;
+}
+
+static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uint32_t n, int anyhit,
+                              float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out, bool device)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->built) return fail(HR_ERR_STATE, "trace before build");
+    if (n == 0) return HR_OK;
+    if (!org || !dir) return fail(HR_ERR_ARG, "null ray buffer");
+    const hr::TraceScene S = trace_scene(c);
+    if (device) {
+        hr::launch_trace_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->ray_count.p,
+                               c->stream);
+        HR_HIP(hipGetLastError());
+        return HR_OK;
+    }
+    HR_TRY(c->ray_org.reserve(3 * (size_t)n));
+    HR_TRY(c->ray_dir.reserve(3 * (size_t)n));
+    HR_TRY(c->ray_t.reserve(n));
+    HR_TRY(c->ray_prim.reserve(n));
+    HR_TRY(c->ray_occ.reserve(n));
+    HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
+    HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
+    hr::launch_trace_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
+                           c->ray_occ.p, c->ray_count.p, c->stream);
+    HR_HIP(hipGetLastError());
+    if (t_out) HR_HIP(hipMemcpyAsync(t_out, c->ray_t.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (prim_out)
+        HR_HIP(hipMemcpyAsync(prim_out, c->ray_prim.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+    if (occ_out) HR_HIP(hipMemcpyAsync(occ_out, c->ray_occ.p, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
+}
+
+hr_status hr_trace(hr_ctx* c, const float* org, const float* dir, uint32_t n, float* t_out, uint32_t* prim_out)
+{
+    if (!t_out || !prim_out) return fail(HR_ERR_ARG, "null output buffer");
+    return trace_common(c, org, dir, n, 0, 0.f, t_out, prim_out, nullptr, false);
+}
+
+hr_status hr_occluded(hr_ctx* c, const float* org, const float* dir, uint32_t n, float tmax, uint8_t* occ_out)
+{
+    if (!occ_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (!(tmax > 0.f)) return fail(HR_ERR_ARG, "tmax must be positive");
+    return trace_common(c, org, dir, n, 1, tmax, nullptr, nullptr, occ_out, false);
+}
+
+hr_status hr_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, uint32_t n, float* d_t_out,
+                       uint32_t* d_prim_out)
+{
+    if (!d_t_out || !d_prim_out) return fail(HR_ERR_ARG, "null output buffer");
+    return trace_common(c, d_org, d_dir, n, 0, 0.f, d_t_out, d_prim_out, nullptr, true);
+}
+
+hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
+{
+    HR_TRY(check_ctx(c));
+    if (!out) return fail(HR_ERR_ARG, "null output pointer");
+    HR_HIP(hipStreamSynchronize(c->stream));
+    out->build_ms = 0.0;
+    out->render_ms = 0.0;
+    float ms = 0.f;
+    if (c->build_timed && hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1) == hipSuccess) out->build_ms = ms;
+    if (c->render_timed && hipEventElapsedTime(&ms, c->ev_r0, c->ev_r1) == hipSuccess) out->render_ms = ms;
+    unsigned long long rays = 0;
+    HR_HIP(hipMemcpy(&rays, c->ray_count.p, sizeof rays, hipMemcpyDeviceToHost));
+    out->rays_traced = rays;
+    return HR_OK;
+}
+
+hr_status hr_synchronize(hr_ctx* c)
+{
+    HR_TRY(check_ctx(c));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
+}
+
+}  // extern "C"

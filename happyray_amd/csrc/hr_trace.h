@@ -1,0 +1,55 @@
+// hr_trace.h -- host interface of the traversal and integrator kernels.
+#pragma once
+
+#include "hr_device.h"
+
+namespace hr {
+
+// Everything a traversal reads, by device pointer.
+struct TraceScene {
+    const GridParams* gp;
+    const uint2* cells;
+    const uint2* leaves;       // two-level only
+    const uint32_t* prims;
+    const float4* rec;         // triangle records, 3 float4 each
+    uint32_t nt;
+    int two_level;
+};
+
+// Camera::init result (src/RT/Primitive/Camera.h:35-72)
+struct CameraDev {
+    float center[3], top_left[3], step_x[3], step_y[3];
+    int res_x, res_y;
+};
+
+struct RenderArgs {
+    uint32_t seed_lo, seed_hi;
+    int jitter;
+    int ao_samples;
+    float ao_dist;
+    int pt_depth;
+    int image;
+};
+
+struct Shading {
+    const float* mats;          // 12 floats per PhongMaterial
+    const uint32_t* mat_idx;    // per triangle, nullptr -> material 0
+    const float* lights;        // 15 floats per AreaLightSource
+    const float* light_cdf;     // inclusive sum of light weights
+    int nlights;
+};
+
+// hit buffer: 7 SoA planes of npix floats (hit point xyz, dir xyz, t) + prims
+void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
+                    uint32_t* hit_prims, unsigned long long* ray_count, hipStream_t s);
+void launch_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+               const uint32_t* hit_prims, uint8_t* occ, const Shading& sh, float* fb,
+               unsigned long long* ray_count, hipStream_t s);
+void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+               const uint32_t* hit_prims, const Shading& sh, float* fb,
+               unsigned long long* ray_count, hipStream_t s);
+void launch_trace_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n,
+                        int anyhit, float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out,
+                        unsigned long long* ray_count, hipStream_t s);
+
+}  // namespace hr

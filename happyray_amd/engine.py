@@ -1,0 +1,212 @@
+"""Python mirror of the reference's RTEngine facade over the C ABI.
+
+RTEngine (src/RT/RTEngine.h:36-68) is a static, single-instance class; here
+each Engine is one hr_ctx.  Method names follow the reference:
+
+    init / cleanup              -> Engine() / close()
+    upload(frame1, frame2, c)   -> upload(verts1, verts2, coeff, idx)
+    setGridDensities(top, leaf) -> set_grid_densities(top, leaf)
+    buildAccStruct()            -> build_acc_struct()
+    setCamera(...)              -> set_camera(...)
+    setLights(...)              -> set_lights(...)
+    renderFrame(fb, id, mode)   -> render_frame(image_id, mode, ...)
+    getBoundingBoxDiagonalLength() -> get_bounding_box_diagonal_length()
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH,
+                   check)
+
+_f32p = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _f(a):
+    return a.ctypes.data_as(_f32p)
+
+
+def _u(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def _vec3(v):
+    return np.ascontiguousarray(np.asarray(v, np.float32).reshape(3))
+
+
+class Engine:
+    def __init__(self, device=0, seed=(0x1234, 0x5678)):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        opts = _lib.Options(device, seed[0] & 0xFFFFFFFF, seed[1] & 0xFFFFFFFF)
+        check(self._L.hr_create(C.byref(h), C.byref(opts)), "hr_create")
+        self._h = h
+        self._res = None
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.hr_destroy(self._h)
+            self._h = None
+
+    cleanup = close
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stream(self):
+        s = C.c_void_p()
+        check(self._L.hr_get_stream(self._h, C.byref(s)), "hr_get_stream")
+        return s.value
+
+    # -- geometry ---------------------------------------------------------
+    def upload(self, verts1, verts2, coeff, idx):
+        """verts: (3, nv) float32 SoA key frames; idx: (nt, 3) uint32."""
+        v1 = np.ascontiguousarray(verts1, np.float32)
+        v2 = v1 if verts2 is None else np.ascontiguousarray(verts2, np.float32)
+        ix = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+        self._nt = len(ix) // 3
+        check(self._L.hr_upload_mesh(self._h, _f(v1), _f(v2), float(coeff), v1.shape[1], _u(ix),
+                                     self._nt), "hr_upload_mesh")
+
+    def upload_scene(self, scene):
+        self.upload(scene.verts, None, 1.0, scene.idx)
+        self.upload_materials(scene.materials, scene.mat_idx)
+
+    def upload_materials(self, mats, mat_idx=None):
+        m = np.ascontiguousarray(mats, np.float32).reshape(-1, 12)
+        mi = None if mat_idx is None else np.ascontiguousarray(mat_idx, np.uint32)
+        check(self._L.hr_upload_materials(self._h, _f(m), len(m), None if mi is None else _u(mi),
+                                          0 if mi is None else len(mi)), "hr_upload_materials")
+
+    def set_keyframes(self, keys, idx):
+        """keys: (frames, 3, nv) float32."""
+        k = np.ascontiguousarray(keys, np.float32)
+        ix = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+        self._nt = len(ix) // 3
+        check(self._L.hr_set_keyframes(self._h, _f(k), k.shape[0], k.shape[2], _u(ix), self._nt),
+              "hr_set_keyframes")
+
+    def set_frame(self, k1, k2, coeff):
+        check(self._L.hr_set_frame(self._h, k1, k2, float(coeff)), "hr_set_frame")
+
+    # -- build ------------------------------------------------------------
+    def set_grid(self, kind, lambda_top, lambda_leaf=2.2):
+        check(self._L.hr_set_grid(self._h, kind, float(lambda_top), float(lambda_leaf)),
+              "hr_set_grid")
+        self._kind = kind
+
+    def set_grid_densities(self, top, leaf):
+        self.set_grid(getattr(self, "_kind", GRID_TWO_LEVEL), top, leaf)
+
+    def build_acc_struct(self):
+        check(self._L.hr_build(self._h), "hr_build")
+
+    build = build_acc_struct
+
+    def grid_info(self):
+        gi = _lib.GridInfo()
+        check(self._L.hr_get_grid_info(self._h, C.byref(gi)), "hr_get_grid_info")
+        return gi
+
+    def grid(self):
+        """Returns (info, cells (C, 2), leaves (L, 2), prims (P,)) copied to the host."""
+        gi = self.grid_info()
+        cells = np.zeros((gi.num_cells, 2), np.uint32)
+        leaves = np.zeros((gi.num_leaves, 2), np.uint32)
+        prims = np.zeros(gi.num_refs, np.uint32)
+        check(self._L.hr_get_grid(self._h, _u(cells), _u(leaves) if gi.num_leaves else None,
+                                  _u(prims) if gi.num_refs else None), "hr_get_grid")
+        return gi, cells, leaves, prims
+
+    def get_bounding_box_diagonal_length(self):
+        gi = self.grid_info()
+        d = np.array(gi.bmax[:], np.float32) - np.array(gi.bmin[:], np.float32)
+        return float(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])))
+
+    # -- camera / lights --------------------------------------------------
+    def set_camera(self, pos, dir, up, fov, res_x, res_y):
+        check(self._L.hr_set_camera(self._h, _f(_vec3(pos)), _f(_vec3(dir)), _f(_vec3(up)),
+                                    float(fov), int(res_x), int(res_y)), "hr_set_camera")
+        self._res = (int(res_x), int(res_y))
+
+    def set_camera_dict(self, cam):
+        self.set_camera(cam["pos"], cam["dir"], cam["up"], cam["fov"], *cam["res"])
+
+    def set_lights(self, lights, weights=None):
+        L = np.ascontiguousarray(lights, np.float32).reshape(-1, 15)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        check(self._L.hr_set_lights(self._h, _f(L), None if w is None else _f(w), len(L)),
+              "hr_set_lights")
+
+    # -- render -----------------------------------------------------------
+    def render_frame(self, image_id=0, mode=MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0,
+                     pt_depth=4, download=True):
+        p = _lib.RenderParams(int(jitter), int(ao_samples), float(ao_dist), int(pt_depth))
+        out = None
+        if download:
+            out = np.zeros((self._res[0] * self._res[1], 3), np.float32)
+        check(self._L.hr_render(self._h, mode, int(image_id), C.byref(p),
+                                None if out is None else _f(out)), "hr_render")
+        return out
+
+    def primary_hits(self):
+        n = self._res[0] * self._res[1]
+        planes = np.zeros((7, n), np.float32)
+        prims = np.zeros(n, np.uint32)
+        check(self._L.hr_get_primary_hits(self._h, _f(planes), _u(prims)), "hr_get_primary_hits")
+        return planes, prims
+
+    def ao_occlusion(self, ao_samples):
+        n = self._res[0] * self._res[1]
+        occ = np.zeros(n * ao_samples, np.uint8)
+        check(self._L.hr_get_ao_occlusion(self._h, occ.ctypes.data_as(_u8p)), "hr_get_ao_occlusion")
+        return occ
+
+    # -- ray batches ------------------------------------------------------
+    def trace(self, org, dirs):
+        """org, dirs: (3, n) float32. Closest hit: (t, prim)."""
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[1]
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.uint32)
+        check(self._L.hr_trace(self._h, _f(o), _f(d), n, _f(t), _u(prim)), "hr_trace")
+        return t, prim
+
+    def occluded(self, org, dirs, tmax):
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[1]
+        occ = np.zeros(n, np.uint8)
+        check(self._L.hr_occluded(self._h, _f(o), _f(d), n, float(tmax), occ.ctypes.data_as(_u8p)),
+              "hr_occluded")
+        return occ
+
+    def timing(self):
+        t = _lib.Timing()
+        check(self._L.hr_get_timing(self._h, C.byref(t)), "hr_get_timing")
+        return t
+
+    def synchronize(self):
+        check(self._L.hr_synchronize(self._h), "hr_synchronize")
+
+
+__all__ = ["Engine", "GRID_UNIFORM", "GRID_UNIFORM_EXACT", "GRID_TWO_LEVEL", "MODE_AO",
+           "MODE_PATH"]

@@ -1,0 +1,243 @@
+"""Seeded synthetic scenes for the hot path's workloads.
+
+The reference ships no scene data (its scenes.7z is a Git-LFS pointer), so the
+workloads are generated: every generator is a pure function of its seed.
+
+Geometry is returned as a :class:`Scene`: SoA float32 vertices (3, nv),
+uint32 index triplets (nt, 3), a PhongMaterial table (nm, 12) laid out as
+src/RT/Primitive/Material.hpp:37-84 (kd.xyz, ior+10 flag, ks.xyz, exponent,
+ke.xyz, pad) and one material index per triangle.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Scene:
+    verts: np.ndarray            # (3, nv) float32, SoA
+    idx: np.ndarray              # (nt, 3) uint32
+    materials: np.ndarray        # (nm, 12) float32
+    mat_idx: np.ndarray          # (nt,) uint32
+    camera: dict = field(default_factory=dict)
+    lights: np.ndarray = None    # (nl, 15) float32: position, normal, intensity, edge1, edge2
+    name: str = ""
+
+    @property
+    def nt(self):
+        return int(self.idx.shape[0])
+
+    @property
+    def nv(self):
+        return int(self.verts.shape[1])
+
+
+def material(kd, ks=(0.0, 0.0, 0.0), spec_exp=1.0, ke=(0.0, 0.0, 0.0), ior=None):
+    """One PhongMaterial row. ior set => refractive (w = ior + 10)."""
+    w = 0.0 if ior is None else float(ior) + 10.0
+    return np.array([kd[0], kd[1], kd[2], w, ks[0], ks[1], ks[2], spec_exp,
+                     ke[0], ke[1], ke[2], 0.0], np.float32)
+
+
+def area_light(position, edge1, edge2, intensity):
+    """AreaLightSource: the normal is normalize(edge1 x edge2) reversed so that
+    it faces along -(e1 x e2) when lighting downwards is wanted by the caller."""
+    p = np.asarray(position, np.float32)
+    e1 = np.asarray(edge1, np.float32)
+    e2 = np.asarray(edge2, np.float32)
+    n = np.cross(e1, e2).astype(np.float32)
+    n = (n / np.linalg.norm(n)).astype(np.float32)
+    return np.concatenate([p, n, np.asarray(intensity, np.float32), e1, e2]).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# primitives
+# ---------------------------------------------------------------------------
+
+def icosphere(subdiv):
+    """Unit icosphere: 10*4^s + 2 vertices, 20*4^s triangles."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+         (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = f
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts, np.float64), np.array(faces, np.uint32)
+
+
+def _assemble(parts, materials, name, camera, lights=None):
+    vs, ids, mis = [], [], []
+    off = 0
+    for verts, faces, mi in parts:
+        vs.append(np.asarray(verts, np.float32))
+        ids.append(np.asarray(faces, np.uint32) + off)
+        mis.append(np.full(len(faces), mi, np.uint32))
+        off += len(verts)
+    V = np.concatenate(vs).astype(np.float32)
+    return Scene(verts=np.ascontiguousarray(V.T), idx=np.concatenate(ids).astype(np.uint32),
+                 materials=np.stack(materials).astype(np.float32),
+                 mat_idx=np.concatenate(mis).astype(np.uint32), camera=camera,
+                 lights=None if lights is None else np.stack(lights).astype(np.float32),
+                 name=name)
+
+
+def quad_grid(origin, du, dv, nu, nv_):
+    """Regular nu x nv_ quad tessellation of the parallelogram origin + [0,1]du + [0,1]dv."""
+    o = np.asarray(origin, np.float64)
+    du = np.asarray(du, np.float64)
+    dv = np.asarray(dv, np.float64)
+    i, j = np.meshgrid(np.arange(nu + 1), np.arange(nv_ + 1), indexing="ij")
+    P = o + (i[..., None] / nu) * du + (j[..., None] / nv_) * dv
+    verts = P.reshape(-1, 3)
+    a = (i[:-1, :-1] * (nv_ + 1) + j[:-1, :-1]).reshape(-1)
+    b = a + (nv_ + 1)
+    c = b + 1
+    d = a + 1
+    faces = np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], 1).reshape(-1, 3)
+    return verts, faces.astype(np.uint32)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+def icosphere_cluster(seed=1, n_spheres=16, subdiv=4, res=256):
+    """16 subdivision-4 icospheres, centres U[-10,10]^3, radii U[0.5,2], and a
+    2-triangle ground quad at y=-12 spanning the spheres' x/z extent:
+    81,922 triangles, 40,996 vertices."""
+    rng = np.random.default_rng(seed)
+    sv, sf = icosphere(subdiv)
+    centers = rng.uniform(-10.0, 10.0, size=(n_spheres, 3))
+    radii = rng.uniform(0.5, 2.0, size=n_spheres)
+    kds = rng.uniform(0.05, 0.25, size=(n_spheres, 3))
+    parts, mats = [], []
+    for s in range(n_spheres):
+        parts.append((sv * radii[s] + centers[s], sf, s))
+        mats.append(material(kds[s]))
+    lo = (centers - radii[:, None]).min(0)
+    hi = (centers + radii[:, None]).max(0)
+    gv = np.array([[lo[0], -12.0, lo[2]], [hi[0], -12.0, lo[2]], [hi[0], -12.0, hi[2]],
+                   [lo[0], -12.0, hi[2]]])
+    parts.append((gv, np.array([[0, 2, 1], [0, 3, 2]], np.uint32), n_spheres))
+    mats.append(material((0.2, 0.2, 0.2)))
+    cam = dict(pos=(0.0, 6.0, 38.0), dir=(0.0, -0.25, -1.0), up=(0.0, 1.0, 0.0), fov=45.0,
+               res=(res, res))
+    light = area_light((-5.0, 25.0, -5.0), (10.0, 0.0, 0.0), (0.0, 0.0, 10.0), (30.0, 30.0, 30.0))
+    return _assemble(parts, mats, "icosphere_cluster", cam, [light])
+
+
+def _room(extent=100.0):
+    """Axis-aligned box [-extent/2, extent/2]^3 tessellated into exactly
+    100,000 triangles: x and y faces 91x92 quads, z faces 86x96 quads."""
+    h = extent / 2.0
+    E = extent
+    faces = [((-h, -h, -h), (0, E, 0), (0, 0, E), 91, 92),   # x = -h
+             ((h, -h, -h), (0, 0, E), (0, E, 0), 92, 91),    # x = +h
+             ((-h, -h, -h), (0, 0, E), (E, 0, 0), 91, 92),   # y = -h
+             ((-h, h, -h), (E, 0, 0), (0, 0, E), 92, 91),    # y = +h
+             ((-h, -h, -h), (E, 0, 0), (0, E, 0), 86, 96),   # z = -h
+             ((-h, -h, h), (0, E, 0), (E, 0, 0), 96, 86)]    # z = +h
+    vs, fs, off = [], [], 0
+    for o, du, dv, nu, nv_ in faces:
+        v, f = quad_grid(o, du, dv, nu, nv_)
+        vs.append(v)
+        fs.append(f + off)
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(fs)
+
+
+def soup_in_room(seed=2, n_soup=900_000, res=1024):
+    """900k small triangles (vertices = centre U(unit cube) + N(0, 0.005)^3)
+    inside a 100k-triangle room box of extent 100: 1,000,000 triangles."""
+    rng = np.random.default_rng(seed)
+    rv, rf = _room(100.0)
+    c = rng.uniform(0.0, 1.0, size=(n_soup, 1, 3))
+    sv = (c + rng.normal(0.0, 0.005, size=(n_soup, 3, 3))).reshape(-1, 3)
+    sf = np.arange(3 * n_soup, dtype=np.uint32).reshape(-1, 3)
+    mats = [material((0.15, 0.15, 0.15)), material((0.22, 0.18, 0.12))]
+    cam = dict(pos=(0.5, 1.4, 3.2), dir=(0.0, -0.3, -1.0), up=(0.0, 1.0, 0.0), fov=50.0,
+               res=(res, res))
+    light = area_light((-2.0, 20.0, -2.0), (4.0, 0.0, 0.0), (0.0, 0.0, 4.0), (50.0, 50.0, 50.0))
+    return _assemble([(rv, rf, 0), (sv, sf, 1)], mats, "soup_in_room", cam, [light])
+
+
+def heightfield_keyframes(seed=3, quads=708, frames=30, extent=100.0, res=(1024, 768)):
+    """(quads x quads)-quad heightfield, (quads+1)^2 vertices and 2*quads^2
+    triangles (708 -> 502,681 vertices, 1,002,528 triangles), with `frames`
+    key frames of y = sum_{i<4} A_i sin(k_i x + l_i z + phi_i + w_i f).
+    Returns (scene at frame 0, key-frame vertex array (frames, 3, nv))."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.5, 2.0, 4)
+    k = rng.uniform(-0.3, 0.3, 4)
+    l_ = rng.uniform(-0.3, 0.3, 4)
+    phi = rng.uniform(0.0, 2.0 * np.pi, 4)
+    w = rng.uniform(0.1, 0.5, 4)
+    h = extent / 2.0
+    v, f = quad_grid((-h, 0.0, -h), (extent, 0.0, 0.0), (0.0, 0.0, extent), quads, quads)
+    x = v[:, 0]
+    z = v[:, 2]
+    keys = np.empty((frames, 3, len(v)), np.float32)
+    for fr in range(frames):
+        y = np.zeros_like(x)
+        for i in range(4):
+            y += A[i] * np.sin(k[i] * x + l_[i] * z + phi[i] + w[i] * fr)
+        keys[fr, 0] = x
+        keys[fr, 1] = y
+        keys[fr, 2] = z
+    mats = [material((0.5 / np.pi, 0.5 / np.pi, 0.5 / np.pi))]
+    cam = dict(pos=(0.0, 35.0, 60.0), dir=(0.0, -0.6, -1.0), up=(0.0, 1.0, 0.0), fov=55.0,
+               res=res)
+    light = area_light((-10.0, 60.0, -10.0), (20.0, 0.0, 0.0), (0.0, 0.0, 20.0),
+                       (40.0, 40.0, 40.0))
+    sc = _assemble([(keys[0].T, f, 0)], mats, "heightfield", cam, [light])
+    return sc, keys
+
+
+def small_box(seed=0):
+    """Unit-cube room (12 triangles) plus one floating triangle: a tiny
+    fixture in the spirit of SceneLoader::loadDefaultScene
+    (src/Application/SceneLoader.cpp:335-443)."""
+    rng = np.random.default_rng(seed)
+    verts, faces = [], []
+    cube_v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                       [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float64)
+    cube_f = np.array([[0, 1, 2], [0, 2, 3], [4, 6, 5], [4, 7, 6], [0, 4, 5], [0, 5, 1],
+                       [3, 2, 6], [3, 6, 7], [0, 3, 7], [0, 7, 4], [1, 5, 6], [1, 6, 2]],
+                      np.uint32)
+    tri = rng.uniform(0.3, 0.7, size=(3, 3))
+    mats = [material((0.2, 0.2, 0.2)), material((0.1, 0.15, 0.25))]
+    cam = dict(pos=(0.5, 0.5, 0.95), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov=60.0,
+               res=(32, 32))
+    light = area_light((0.4, 0.99, 0.4), (0.2, 0.0, 0.0), (0.0, 0.0, 0.2), (5.0, 5.0, 5.0))
+    return _assemble([(cube_v, cube_f, 0), (tri, np.array([[0, 1, 2]], np.uint32), 1)], mats,
+                     "small_box", cam, [light])
+
+
+def random_soup(seed, n_tris, extent=10.0, size=0.5):
+    """n random triangles of edge scale `size` in a cube of side `extent`."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, extent, size=(n_tris, 1, 3))
+    v = (c + rng.uniform(-size, size, size=(n_tris, 3, 3))).reshape(-1, 3)
+    f = np.arange(3 * n_tris, dtype=np.uint32).reshape(-1, 3)
+    mats = [material((0.2, 0.2, 0.2))]
+    cam = dict(pos=(extent / 2, extent / 2, extent * 2.2), dir=(0.0, 0.0, -1.0),
+               up=(0.0, 1.0, 0.0), fov=40.0, res=(64, 64))
+    return _assemble([(v, f, 0)], mats, "random_soup", cam)

@@ -1,0 +1,161 @@
+/*
+ * happyray.h -- C ABI of the MI355X (gfx950) ray-tracing hot path.
+ *
+ * One opaque context replaces the static, single-instance RTEngine facade
+ * (src/RT/RTEngine.h:36-68).  Every entry point returns an hr_status; the
+ * message of the last failure on the calling thread is hr_last_error().
+ * The reference prints and exit(1)s on CUDA errors
+ * (src/Utils/CUDAUtil.h:199-206); here nothing exits the process.
+ *
+ * Threading: calls on one context must not overlap; different contexts are
+ * independent (each owns its HIP stream and device memory).
+ * Ownership: the caller owns every host buffer it passes; the context owns
+ * all device memory, which grows and is reused across frames like the
+ * reference's memory managers (src/RT/Structure/MemoryManager.h:73-139).
+ *
+ * Host buffers are plain C arrays: vertices are SoA (x[nv], y[nv], z[nv]),
+ * indices are uint32 triplets, rays are SoA.  Entry points whose name ends
+ * in _dev take device pointers instead (for callers that keep data in HBM).
+ */
+#ifndef HAPPYRAY_H
+#define HAPPYRAY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    HR_OK = 0,
+    HR_ERR_ARG = 1,        /* invalid argument                              */
+    HR_ERR_DEVICE = 2,     /* HIP runtime error / no device                 */
+    HR_ERR_OOM = 3,        /* device allocation failed                      */
+    HR_ERR_IO = 4,         /* file could not be read                        */
+    HR_ERR_STATE = 5,      /* call out of order (e.g. render before build)  */
+    HR_ERR_CAPACITY = 6,   /* a fixed capacity was exceeded                 */
+    HR_ERR_PARSE = 7,      /* malformed scene / camera / light file         */
+    HR_ERR_BUDGET = 8      /* request refused by the memory budget          */
+} hr_status;
+
+typedef enum {
+    HR_GRID_UNIFORM = 0,       /* uniform grid, bounding-box insertion          */
+    HR_GRID_UNIFORM_EXACT = 1, /* uniform grid, exact triangle/box insertion    */
+    HR_GRID_TWO_LEVEL = 2      /* two-level grid                                */
+} hr_grid_kind;
+
+typedef enum {
+    HR_MODE_AO = 3,            /* RTEngine::renderFrame mode 3 (src/RT/RTEngine.cu:433) */
+    HR_MODE_PATH = 2           /* mode 2: path-tracing megakernel                       */
+} hr_mode;
+
+typedef struct hr_ctx hr_ctx;
+
+typedef struct {
+    int device;                /* HIP device ordinal                            */
+    uint32_t seed_lo, seed_hi; /* key of the counter-based RNG                  */
+} hr_options;
+
+/* Grid description returned after a build (UniformGrid / TwoLevelGrid
+ * parameters, src/RT/Structure/UniformGrid.h:37-138,
+ * src/RT/Structure/TwoLevelGrid.h:163-230). */
+typedef struct {
+    int kind;
+    float bmin[3], bmax[3];
+    int res[3];
+    float cell_size[3], cell_size_rcp[3];
+    uint32_t num_cells;      /* (top-level) cells                         */
+    uint32_t num_leaves;     /* two-level: leaf cells                     */
+    uint32_t num_refs;       /* primitive references (length of prims)   */
+    uint32_t num_top_pairs;  /* two-level: top-level (cell, prim) pairs   */
+    uint32_t num_triangles;
+} hr_grid_info;
+
+/* AreaLightSource (src/RT/Primitive/LightSource.hpp:38-99) */
+typedef struct {
+    float position[3], normal[3], intensity[3], edge1[3], edge2[3];
+} hr_area_light;
+
+typedef struct {
+    int jitter;          /* 0: pixel centres; 1: Gaussian pixel filter (Camera.h:116-132) */
+    int ao_samples;      /* AO rays per primary hit (AOIntegrator.h:45 uses 4)            */
+    float ao_dist;       /* occlusion distance (the reference uses 0.05*|bbox diagonal|)  */
+    int pt_depth;        /* path vertices                                                 */
+} hr_render_params;
+
+typedef struct {
+    double build_ms;         /* last hr_build, device time (hipEvents)           */
+    double render_ms;        /* last hr_render, device time                      */
+    uint64_t rays_traced;    /* rays traversed by the last hr_render / hr_trace  */
+} hr_timing;
+
+/* ---- lifetime: RTEngine::init / cleanup (src/RT/RTEngine.cu:183-192, :455-472) ---- */
+const char* hr_last_error(void);
+hr_status hr_create(hr_ctx** out, const hr_options* opts);
+hr_status hr_destroy(hr_ctx* ctx);
+/* the context's HIP stream (hipStream_t), for callers that time or order work */
+hr_status hr_get_stream(hr_ctx* ctx, void** stream);
+
+/* ---- geometry: RTEngine::upload(frame1, frame2, coeff) (src/RT/RTEngine.cu:194-215,
+ *      src/Application/WFObjectUploader.h:48-119).  v = v1*(1-c) + v2*c per vertex,
+ *      bounds padded by 0.001*|diagonal|; interpolation and bounds run on the GPU. */
+hr_status hr_upload_mesh(hr_ctx* ctx, const float* xyz1, const float* xyz2, float coeff,
+                         uint32_t nv, const uint32_t* idx, uint32_t nt);
+/* PhongMaterial table (12 floats each, src/RT/Primitive/Material.hpp:37-84) and one
+ * material index per triangle (WFObjectUploader.h:254-330). */
+hr_status hr_upload_materials(hr_ctx* ctx, const float* mats, uint32_t nmat,
+                              const uint32_t* mat_idx, uint32_t nt);
+/* Key frames resident in device memory (AnimationManager, src/Application/
+ * AnimationManager.hpp:46-138): xyz holds nframes blocks of SoA vertices. */
+hr_status hr_set_keyframes(hr_ctx* ctx, const float* xyz, uint32_t nframes, uint32_t nv,
+                           const uint32_t* idx, uint32_t nt);
+/* Interpolate resident key frames k1, k2 with coefficient c on the GPU
+ * (AnimationManager::getFrameId / getNextFrameId / getInterpolationCoefficient). */
+hr_status hr_set_frame(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coeff);
+
+/* ---- build: setGridDensities + buildAccStruct (src/RT/RTEngine.cu:354-368, :449-453) ---- */
+hr_status hr_set_grid(hr_ctx* ctx, hr_grid_kind kind, float lambda_top, float lambda_leaf);
+hr_status hr_build(hr_ctx* ctx);
+hr_status hr_get_grid_info(hr_ctx* ctx, hr_grid_info* out);
+/* Copy the grid to host memory.  cells: 2*num_cells uint32 (uniform: begin,end;
+ * two-level: leafBegin, packed resolution|flags), leaves: 2*num_leaves (may be NULL
+ * for uniform grids), prims: num_refs. */
+hr_status hr_get_grid(hr_ctx* ctx, uint32_t* cells, uint32_t* leaves, uint32_t* prims);
+
+/* ---- camera and lights: RTEngine::setCamera / setLights (src/RT/RTEngine.cu:370-396) ---- */
+hr_status hr_set_camera(hr_ctx* ctx, const float pos[3], const float dir[3], const float up[3],
+                        float fov_deg, int res_x, int res_y);
+hr_status hr_set_lights(hr_ctx* ctx, const hr_area_light* lights, const float* weights, int n);
+
+/* ---- render: RTEngine::renderFrame (src/RT/RTEngine.cu:398-442) ----
+ * Accumulates into the context's float3 framebuffer with weight 1/(image_id+1)
+ * (AOIntegrator.h:181-187); image_id 0 restarts the running mean.  rgb_out
+ * (3*res_x*res_y floats) receives a copy when not NULL. */
+hr_status hr_render(hr_ctx* ctx, hr_mode mode, int image_id, const hr_render_params* params,
+                    float* rgb_out);
+/* Primary-ray hit buffer of the last render (SimpleRayBuffer, RayBuffers.h:34-195):
+ * 7 SoA float planes (hit point xyz, direction xyz, t) and primitive ids. */
+hr_status hr_get_primary_hits(hr_ctx* ctx, float* planes7, uint32_t* prims);
+/* AO occlusion flags of the last AO render (1 = occluded), ao_samples per pixel. */
+hr_status hr_get_ao_occlusion(hr_ctx* ctx, uint8_t* occ);
+
+/* ---- ray batches: the trace<> kernel with a RayLoader generator
+ * (src/RT/Algorithm/RayTracingKernels.h:47-152, RayGenerators.h:347-364).
+ * Closest hit: t_out = distance (FLT_MAX on a miss), prim_out = primitive id or
+ * num_triangles on a miss. */
+hr_status hr_trace(hr_ctx* ctx, const float* org, const float* dir, uint32_t n,
+                   float* t_out, uint32_t* prim_out);
+/* Any hit within (1e-4, tmax): occ_out[i] = 1 when occluded. */
+hr_status hr_occluded(hr_ctx* ctx, const float* org, const float* dir, uint32_t n, float tmax,
+                      uint8_t* occ_out);
+hr_status hr_trace_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n,
+                       float* d_t_out, uint32_t* d_prim_out);
+
+hr_status hr_get_timing(hr_ctx* ctx, hr_timing* out);
+hr_status hr_synchronize(hr_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,153 @@
+/*
+ * hr_oracle.h -- scalar CPU restatement of happyray's per-frame hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
+ * loaded by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg, and by nothing else.  The product (happyray_amd/,
+ * include/happyray.h) never calls it and never falls back to it.
+ *
+ * It follows the *host* branches of the reference (IEEE division and
+ * square root, no fused multiply-add; build with -ffp-contract=off), with
+ * the documented departures:
+ *   - canonical (cell, primitive id) order inside every cell and leaf
+ *     (the reference's order comes from shared-memory atomics);
+ *   - a Philox4x32-10 counter RNG keyed by (pixel, image, stream) instead
+ *     of KISS / XorShift seeded by the CUDA thread id;
+ *   - transcendental functions (pow, log, exp, sin/cos) evaluated by the
+ *     double-precision routines in or_math.c (a written spec that the GPU
+ *     kernels implement too) and rounded to float;
+ *   - AO occlusion decided against a configurable distance instead of the
+ *     bounding-box-derived alpha, any-hit termination at that distance;
+ *   - a fixed maximum path depth for the path tracer;
+ *   - exact-insertion pairs that fail the triangle/box test are dropped
+ *     instead of being written with key = #cells and sorted last.
+ * Parity status of every function is listed in the header of the file that
+ * implements it.
+ */
+#ifndef HR_ORACLE_H
+#define HR_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* grid kinds (src/RT/RTEngine.cu:64-89) */
+#define OR_GRID_UNIFORM        0  /* uniform grid, bounding-box insertion   */
+#define OR_GRID_UNIFORM_EXACT  1  /* uniform grid, exact triangle insertion */
+#define OR_GRID_TWO_LEVEL      2  /* two-level grid                          */
+
+typedef struct {
+    const float* vx;   /* SoA vertex coordinates, nv each */
+    const float* vy;
+    const float* vz;
+    uint32_t nv;
+    const uint32_t* idx; /* 3 * nt vertex indices */
+    uint32_t nt;
+} or_mesh;
+
+typedef struct {
+    int kind;
+    float bmin[3], bmax[3];
+    int res[3];
+    float cell_size[3], cell_size_rcp[3];
+    uint32_t num_cells;      /* top-level cells                                  */
+    uint32_t num_leaves;     /* two-level only                                   */
+    uint32_t num_refs;       /* length of prims[]                                */
+    uint32_t num_top_pairs;  /* two-level only: (top cell, prim) pairs           */
+    uint32_t num_bbox_pairs; /* exact uniform only: pairs before the exact test  */
+    uint32_t* cells;         /* 2*num_cells: UG (begin,end); TLG (leafBegin, packed) */
+    uint32_t* leaves;        /* 2*num_leaves (begin,end)                          */
+    uint32_t* prims;         /* num_refs primitive ids                            */
+} or_grid;
+
+typedef struct {
+    uint64_t rays;       /* rays traversed */
+    uint64_t steps;      /* DDA cell steps (top + leaf) */
+    uint64_t tests;      /* ray/triangle tests */
+} or_stats;
+
+typedef struct {
+    float center[3], top_left[3], step_x[3], step_y[3];
+    int res_x, res_y;
+} or_camera;
+
+/* PhongMaterial layout (src/RT/Primitive/Material.hpp:37-84): 12 floats */
+typedef struct {
+    float kd[3];  float ior_w;      /* diffuse, IOR (+10 when refractive) */
+    float ks[3];  float spec_exp;   /* specular, exponent                 */
+    float ke[3];  float pad;        /* emission                           */
+} or_material;
+
+/* AreaLightSource (src/RT/Primitive/LightSource.hpp:38-99) */
+typedef struct {
+    float position[3], normal[3], intensity[3], edge1[3], edge2[3];
+} or_light;
+
+typedef struct {
+    uint32_t seed_lo, seed_hi;  /* Philox key */
+    int jitter;                 /* 0: pixel centre, 1: Gaussian pixel filter */
+    int ao_samples;             /* AO rays per primary hit */
+    float ao_dist;              /* AO occlusion distance   */
+    int pt_depth;               /* max path vertices        */
+    int nthreads;               /* host threads (tiles of 64 rays) */
+} or_render_params;
+
+/* ---- spec math (or_math.c) ---- */
+float or_powf(float x, float y);
+float or_logf(float x);
+float or_expf(float x);
+void  or_sincosf(float a, float* s, float* c);
+void  or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float or_u01(uint32_t x);
+
+/* ---- key frames and bounds (or_build.c) ---- */
+void or_lerp_bounds(const float* x1, const float* y1, const float* z1,
+                    const float* x2, const float* y2, const float* z2,
+                    float coeff, uint32_t nv,
+                    float* ox, float* oy, float* oz,
+                    float bmin[3], float bmax[3]);
+void or_frame_ids(float current, float step, uint32_t num_frames,
+                  uint32_t* k1, uint32_t* k2, float* coeff);
+
+/* ---- grid builds (or_build.c) ---- */
+int  or_grid_resolution(const float bmin[3], const float bmax[3], uint32_t nprims,
+                        float lambda, int res[3]);
+int  or_build(const or_mesh* mesh, const float bmin[3], const float bmax[3],
+              int kind, float lambda_top, float lambda_leaf, or_grid* out);
+void or_grid_free(or_grid* g);
+/* per-primitive cell range and exact-test helpers, exposed for tests */
+void or_prim_cell_range(const or_mesh* m, uint32_t prim, const or_grid* g,
+                        int lo[3], int hi[3]);
+int  or_tri_box_overlap(const float v0[3], const float v1[3], const float v2[3],
+                        const float center[3], const float half[3]);
+
+/* ---- traversal (or_trace.c) ---- */
+int  or_trace(const or_mesh* mesh, const or_grid* g,
+              const float* ox, const float* oy, const float* oz,
+              const float* dx, const float* dy, const float* dz,
+              uint32_t n, int anyhit, float tmax, int nthreads,
+              float* t_out, uint32_t* prim_out, or_stats* stats);
+void or_camera_init(or_camera* cam, const float pos[3], const float dir[3],
+                    const float up[3], float fov_deg, int res_x, int res_y);
+/* primary hit buffer: hit point, stored direction, t, primitive id */
+int  or_primary(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
+                const or_render_params* p, int image_id,
+                float* hx, float* hy, float* hz, float* hdx, float* hdy, float* hdz,
+                float* ht, uint32_t* hprim, or_stats* stats);
+int  or_render_ao(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
+                  const or_material* mats, const uint32_t* mat_idx,
+                  const or_render_params* p, int image_id,
+                  float* fb_rgb, uint8_t* occ_out, or_stats* stats);
+int  or_render_pt(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
+                  const or_material* mats, const uint32_t* mat_idx,
+                  const or_light* lights, const float* light_weights, int nlights,
+                  const or_render_params* p, int image_id,
+                  float* fb_rgb, or_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

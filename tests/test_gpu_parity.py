@@ -1,0 +1,159 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer outputs (grids, primitive ids) must be
+bit-exact; floating-point outputs use the tolerances of the hot-path
+contract: hit t to 1e-5 relative, images to 1e-4 per pixel on >= 99.9 % of
+pixels.  The kernels and the oracle evaluate the same IEEE operations, so
+in practice these comparisons are exact."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import (SEED, assert_grid_equal, gpu_engine, oracle_grid, random_rays)
+from happyray_amd import (Engine, GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO,
+                          MODE_PATH, scenes)
+
+pytestmark = pytest.mark.gpu
+
+T_RTOL = 1e-5          # hit distance, relative
+IMG_TOL = 1e-4         # per pixel, absolute
+IMG_FRAC = 0.999       # fraction of pixels within IMG_TOL
+
+
+@pytest.fixture(scope="module")
+def cluster():
+    return scenes.icosphere_cluster(res=128)
+
+
+@pytest.fixture(scope="module")
+def box():
+    return scenes.small_box()
+
+
+def assert_hits_match(t_g, p_g, t_o, p_o, nt):
+    hit_g = p_g < nt
+    hit_o = p_o < nt
+    np.testing.assert_array_equal(hit_g, hit_o)
+    np.testing.assert_allclose(t_g[hit_o], t_o[hit_o], rtol=T_RTOL, atol=0)
+    # ids equal except at equal-t ties (none expected with identical arithmetic)
+    diff = p_g != p_o
+    assert np.all(t_g[diff] == t_o[diff])
+    assert diff.mean() < 1e-4
+
+
+def assert_image_match(a, b):
+    err = np.abs(a - b).max(axis=1)
+    frac = (err <= IMG_TOL).mean()
+    assert frac >= IMG_FRAC, f"only {frac:.6f} of pixels within {IMG_TOL} (max err {err.max()})"
+
+
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_UNIFORM, 5.0, 1.0),
+                                        (GRID_TWO_LEVEL, 1.0 / 16, 3.0),
+                                        (GRID_TWO_LEVEL, 1.2, 2.2)])
+def test_build_parity_cluster(cluster, kind, lt, ll):
+    print(f"build parity icosphere_cluster kind={kind} lt={lt} ll={ll}", flush=True)
+    e = gpu_engine(cluster, kind, lt, ll)
+    og = oracle_grid(cluster, kind, lt, ll)
+    assert_grid_equal(e, og)
+    e.close()
+
+
+@pytest.mark.parametrize("kind", [GRID_UNIFORM_EXACT, GRID_TWO_LEVEL])
+def test_build_parity_box_and_soup(box, kind):
+    soup = scenes.random_soup(7, 3000, extent=10.0, size=1.5)
+    for sc in (box, soup):
+        print(f"build parity {sc.name} kind={kind}", flush=True)
+        e = gpu_engine(sc, kind, 5.0 if kind != GRID_TWO_LEVEL else 0.5, 3.0)
+        assert_grid_equal(e, oracle_grid(sc, kind, 5.0 if kind != GRID_TWO_LEVEL else 0.5, 3.0))
+        e.close()
+
+
+def test_build_capacity_regrowth():
+    """Few large triangles make far more pairs than the first capacity guess,
+    so the build overflows, grows its buffers and runs again."""
+    print("build capacity regrowth", flush=True)
+    sc = scenes.random_soup(11, 40, extent=10.0, size=8.0)
+    for kind, lt, ll in ((GRID_UNIFORM_EXACT, 200.0, 1.0), (GRID_TWO_LEVEL, 50.0, 40.0)):
+        e = gpu_engine(sc, kind, lt, ll)
+        assert_grid_equal(e, oracle_grid(sc, kind, lt, ll))
+        e.close()
+
+
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 1.0 / 16, 3.0)])
+def test_trace_parity(cluster, kind, lt, ll):
+    print(f"trace parity kind={kind}", flush=True)
+    e = gpu_engine(cluster, kind, lt, ll)
+    og = oracle_grid(cluster, kind, lt, ll)
+    org, d = random_rays(cluster, 20000, seed=5)
+    t_g, p_g = e.trace(org, d)
+    t_o, p_o, _ = O.trace(og, org, d, nthreads=8)
+    assert_hits_match(t_g, p_g, t_o, p_o, cluster.nt)
+    assert (p_g < cluster.nt).mean() > 0.3
+    for tmax in (0.5, 3.0, 1e30):
+        occ_g = e.occluded(org, d, tmax)
+        _, occ_o, _ = O.trace(og, org, d, anyhit=True, tmax=tmax, nthreads=8)
+        np.testing.assert_array_equal(occ_g, occ_o.astype(np.uint8))
+    e.close()
+
+
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 1.0 / 16, 3.0)])
+@pytest.mark.parametrize("jitter", [0, 1])
+def test_primary_and_ao_parity(cluster, kind, lt, ll, jitter):
+    print(f"AO parity kind={kind} jitter={jitter}", flush=True)
+    e = gpu_engine(cluster, kind, lt, ll)
+    e.set_camera_dict(cluster.camera)
+    og = oracle_grid(cluster, kind, lt, ll)
+    cam = O.camera(**{k: cluster.camera[k] for k in ("pos", "dir", "up", "fov")},
+                   res_x=cluster.camera["res"][0], res_y=cluster.camera["res"][1])
+    p = O.params(seed=SEED, jitter=jitter, ao_samples=4, ao_dist=2.0, nthreads=8)
+    fb_g = e.render_frame(0, MODE_AO, jitter=jitter, ao_samples=4, ao_dist=2.0)
+    planes_g, prims_g = e.primary_hits()
+    planes_o, prims_o, _ = O.primary(og, cam, p)
+    np.testing.assert_array_equal(prims_g, prims_o)
+    hit = prims_o < cluster.nt
+    np.testing.assert_array_equal(planes_g[:, hit], planes_o[:, hit])
+    fb_o, occ_o, _ = O.render_ao(og, cam, p, cluster.materials, cluster.mat_idx)
+    occ_g = e.ao_occlusion(4)
+    np.testing.assert_array_equal(occ_g, occ_o)
+    assert_image_match(fb_g, fb_o)
+    # progressive accumulation over a second image
+    fb_g2 = e.render_frame(1, MODE_AO, jitter=jitter, ao_samples=4, ao_dist=2.0)
+    fb_o2, _, _ = O.render_ao(og, cam, O.params(seed=SEED, jitter=jitter, ao_samples=4,
+                                                 ao_dist=2.0, nthreads=8),
+                              cluster.materials, cluster.mat_idx, image_id=1, fb=fb_o)
+    assert_image_match(fb_g2, fb_o2)
+    e.close()
+
+
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)])
+def test_path_tracing_parity(cluster, kind, lt, ll):
+    print(f"PT parity kind={kind}", flush=True)
+    e = gpu_engine(cluster, kind, lt, ll)
+    e.set_camera_dict(cluster.camera)
+    e.set_lights(cluster.lights)
+    og = oracle_grid(cluster, kind, lt, ll)
+    cam = O.camera(**{k: cluster.camera[k] for k in ("pos", "dir", "up", "fov")},
+                   res_x=cluster.camera["res"][0], res_y=cluster.camera["res"][1])
+    p = O.params(seed=SEED, jitter=1, pt_depth=4, nthreads=8)
+    fb_g = e.render_frame(0, MODE_PATH, jitter=1, pt_depth=4)
+    fb_o, _ = O.render_pt(og, cam, p, cluster.materials, cluster.mat_idx, cluster.lights)
+    assert fb_o.max() > 0
+    assert_image_match(fb_g, fb_o)
+    e.close()
+
+
+def test_keyframe_interpolation_parity():
+    print("keyframe parity", flush=True)
+    sc, keys = scenes.heightfield_keyframes(quads=64, frames=5, res=(64, 48))
+    e = Engine(seed=SEED)
+    e.set_keyframes(keys, sc.idx)
+    e.upload_materials(sc.materials, sc.mat_idx)
+    e.set_grid(GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    for cur in (0.0, 0.4, 1.7, 3.9):
+        k1, k2, c = O.frame_ids(cur, 0.4, 5)
+        e.set_frame(k1, k2, c)
+        e.build_acc_struct()
+        verts, bmin, bmax = O.lerp_bounds(keys[k1], keys[k2], c)
+        m = O.OMesh(verts[0], verts[1], verts[2], sc.idx)
+        og = O.OGrid(m, bmin, bmax, O.GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+        assert_grid_equal(e, og)
+    e.close()

@@ -1,0 +1,289 @@
+"""Pins of the oracle's grid builds.
+
+* resolution: hand-computed closed forms and the 32^3 fallback;
+* bounding-box cell ranges: brute force over every cell in double precision;
+* exact insertion: an independent double-precision separating-axis test
+  (Akenine-Moller's 13 axes) on boxes shrunk / grown by a margin, so only
+  cases within rounding of the boundary are left undecided;
+* cells / leaves / primitive lists: rebuilt by a dictionary from the
+  per-primitive cell sets, compared entry by entry;
+* the reference's own debug invariants (UGridSortBuilder.h:253-401,
+  TLGridSortBuilder.h:411-428) and the two-level cell bit layout.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from happyray_amd import scenes
+
+
+def mesh_of(sc):
+    return O.OMesh(sc.verts[0], sc.verts[1], sc.verts[2], sc.idx)
+
+
+def build(sc, kind, lt, ll=1.0):
+    m = mesh_of(sc)
+    bmin, bmax = O.bounds(sc.verts)
+    return O.OGrid(m, bmin, bmax, kind, lt, ll)
+
+
+# ---------------------------------------------------------------------------
+# resolution and bounds
+# ---------------------------------------------------------------------------
+
+def test_resolution_closed_forms(oracle):
+    # diagonal 10^3, N = 1000, lambda = 1: magic = 1 -> 10 cells per axis
+    assert oracle.resolution([0, 0, 0], [10, 10, 10], 1000, 1.0) == (10, 10, 10)
+    # lambda = 8: magic = 8^0.3333333f, and 0.3333333f < 1/3, so magic rounds
+    # to 1.99999988f and 10 * magic truncates to 19 (as powf would in the
+    # reference)
+    third = float(np.float32(0.3333333))
+    magic = np.float32(8.0 ** third)
+    assert magic < 2.0
+    assert oracle.resolution([0, 0, 0], [10, 10, 10], 1000, 8.0) == (int(np.float32(10) * magic),) * 3
+    assert oracle.resolution([0, 0, 0], [10, 10, 10], 1000, 8.0) == (19, 19, 19)
+    # lambda = 27 * 1.0001: magic just above 3 -> 30 per axis
+    assert oracle.resolution([0, 0, 0], [10, 10, 10], 1000, 27.01) == (30, 30, 30)
+    # anisotropic box 4 x 2 x 1, N = 8, lambda = 1: magic = 1 -> (4, 2, 1)
+    assert oracle.resolution([0, 0, 0], [4, 2, 1], 8, 1.0) == (4, 2, 1)
+    # any resolution < 1 falls back to 32^3 (UGridMemoryManager.cu:338-346)
+    assert oracle.resolution([0, 0, 0], [100, 1, 1], 10, 1.0) == (32, 32, 32)
+    assert oracle.resolution([0, 0, 0], [1, 1, 1], 0, 5.0) == (32, 32, 32)
+
+
+def test_bounds_padding(oracle):
+    v = np.array([[0, 3], [0, 4], [0, 0]], np.float32)
+    _, bmin, bmax = oracle.lerp_bounds(v, v, 1.0)
+    # diagonal 5, pad 0.005
+    np.testing.assert_array_equal(bmin, np.float32(0) - np.float32(0.001) * np.float32(5))
+    np.testing.assert_array_equal(bmax, np.array([3, 4, 0], np.float32) + np.float32(0.001) * np.float32(5))
+
+
+def test_lerp_closed_form(oracle):
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=(3, 100)).astype(np.float32)
+    b = rng.normal(size=(3, 100)).astype(np.float32)
+    c = np.float32(0.3)
+    out, _, _ = oracle.lerp_bounds(a, b, float(c))
+    np.testing.assert_array_equal(out, a * (np.float32(1) - c) + b * c)
+
+
+def test_frame_ids(oracle):
+    # AnimationManager: k1 = floor(t), k2 = (k1 + 1 + int(step)) % K, c = frac(t)
+    assert oracle.frame_ids(2.5, 0.4, 10)[:2] == (2, 3)
+    assert oracle.frame_ids(9.5, 0.4, 10)[:2] == (9, 0)
+    assert oracle.frame_ids(1.0, 1.5, 10)[:2] == (1, 3)
+    assert abs(oracle.frame_ids(2.25, 0.4, 10)[2] - 0.25) < 1e-7
+
+
+# ---------------------------------------------------------------------------
+# cell ranges and the exact test
+# ---------------------------------------------------------------------------
+
+def brute_cells(lo, hi, bmin, cs, res):
+    """Cells whose closed box intersects [lo, hi] (double precision)."""
+    out = []
+    for z, y, x in itertools.product(range(res[2]), range(res[1]), range(res[0])):
+        c0 = bmin + np.array([x, y, z]) * cs
+        c1 = c0 + cs
+        if np.all(hi >= c0) and np.all(lo <= c1):
+            out.append((x, y, z))
+    return out
+
+
+def test_bbox_cell_ranges_brute_force(oracle):
+    sc = scenes.random_soup(3, 60, extent=4.0, size=0.7)
+    g = build(sc, O.GRID_UNIFORM, 3.0)
+    p = g.params()
+    bmin = p["bmin"].astype(np.float64)
+    cs = p["cell_size"].astype(np.float64)
+    undecided = 0
+    for t in range(sc.nt):
+        v = sc.verts[:, sc.idx[t]].astype(np.float64)
+        lo, hi = v.min(1), v.max(1)
+        (a0, a1, a2), (b0, b1, b2) = g.prim_cell_range(t)
+        got = {(x, y, z) for z in range(a2, b2) for y in range(a1, b1) for x in range(a0, b0)}
+        inner = set(brute_cells(lo + 1e-4, hi - 1e-4, bmin, cs, p["res"]))
+        outer = set(brute_cells(lo - 1e-4, hi + 1e-4, bmin, cs, p["res"]))
+        assert inner <= got <= outer
+        undecided += len(outer - inner)
+    assert undecided < 0.05 * 60 * 8
+
+
+def sat_double(v, center, half):
+    """Akenine-Moller triangle/box overlap in double precision."""
+    v = [np.asarray(x, np.float64) - center for x in v]
+    e = [v[1] - v[0], v[2] - v[1], v[0] - v[2]]
+    axes = [np.eye(3)[i] for i in range(3)]
+    n = np.cross(v[1] - v[0], v[2] - v[0])
+    cand = axes + [n] + [np.cross(ei, a) for ei in e for a in axes]
+    for ax in cand:
+        if np.dot(ax, ax) < 1e-30:
+            continue
+        p = [np.dot(ax, x) for x in v]
+        r = np.sum(np.abs(ax) * half)
+        if min(p) > r or max(p) < -r:
+            return False
+    return True
+
+
+def test_exact_test_against_double_sat(oracle):
+    rng = np.random.default_rng(5)
+    agree = 0
+    total = 0
+    for _ in range(3000):
+        v = rng.uniform(-1.5, 1.5, (3, 3)).astype(np.float32)
+        if rng.uniform() < 0.2:
+            v[:, rng.integers(0, 3)] = 0.4   # axis-aligned triangle
+        c = np.zeros(3, np.float32)
+        half = np.full(3, 0.5, np.float32)
+        # the reference tests only cells inside the triangle's bounding-box
+        # range, so the three box-face axes never separate: keep that case
+        if np.any(v.min(0) > 0.5 * (1 - 1e-4)) or np.any(v.max(0) < -0.5 * (1 - 1e-4)):
+            continue
+        got = oracle.tri_box_overlap(v[0], v[1], v[2], c, half)
+        in_ = sat_double(v, c.astype(np.float64), half.astype(np.float64) * (1 - 1e-4))
+        out_ = sat_double(v, c.astype(np.float64), half.astype(np.float64) * (1 + 1e-4))
+        if in_ == out_:
+            total += 1
+            agree += (got == in_)
+    assert total > 1000
+    assert agree == total
+
+
+def test_exact_test_known_cases(oracle):
+    h = [0.5, 0.5, 0.5]
+    c = [0, 0, 0]
+    # triangle inside the box
+    assert oracle.tri_box_overlap([-.1, -.1, 0], [.1, -.1, 0], [0, .1, 0], c, h)
+    # triangle far away
+    assert not oracle.tri_box_overlap([3, 3, 3], [4, 3, 3], [3, 4, 3], c, h)
+    # large triangle cutting through the box
+    assert oracle.tri_box_overlap([-10, -10, 0.1], [10, -10, 0.1], [0, 10, 0.1], c, h)
+    # triangle in a plane just outside a face
+    assert not oracle.tri_box_overlap([-10, -10, 0.6], [10, -10, 0.6], [0, 10, 0.6], c, h)
+    # corner-only bounding-box overlap rejected by an edge axis
+    assert not oracle.tri_box_overlap([0.45, 1.5, 0], [1.5, 0.45, 0], [1.5, 1.5, 0], c, h)
+    # degenerate (zero-area) triangle is never inserted
+    assert not oracle.tri_box_overlap([0, 0, 0], [0.1, 0, 0], [0.2, 0, 0], c, h)
+
+
+# ---------------------------------------------------------------------------
+# cells, leaves and primitive lists
+# ---------------------------------------------------------------------------
+
+def check_uniform(sc, g, exact):
+    p = g.params()
+    res = p["res"]
+    rx, ry = res[0], res[1]
+    cells = g.cells()
+    prims = g.prims()
+    expect = {}
+    cs = p["cell_size"]
+    half = cs * np.float32(0.505)
+    for t in range(sc.nt):
+        (a0, a1, a2), (b0, b1, b2) = g.prim_cell_range(t)
+        v = sc.verts[:, sc.idx[t]].T
+        for z in range(a2, b2):
+            for y in range(a1, b1):
+                for x in range(a0, b0):
+                    if exact:
+                        # centre by the accumulation of the reference loops
+                        cen = []
+                        for ax, k, lo_k in ((0, x, a0), (1, y, a1), (2, z, a2)):
+                            mcc = (np.float32(lo_k) * cs[ax] + p["bmin"][ax]) + half[ax]
+                            cc = np.float32(mcc - cs[ax])
+                            for _ in range(k - lo_k + 1):
+                                cc = np.float32(cc + cs[ax])
+                            cen.append(cc)
+                        if not O.tri_box_overlap(v[0], v[1], v[2], cen, half):
+                            continue
+                    expect.setdefault(x + y * rx + z * rx * ry, []).append(t)
+    n = 0
+    for k in range(len(cells)):
+        b, e = cells[k]
+        lst = expect.get(k, [])
+        assert e - b == len(lst)
+        if lst:
+            np.testing.assert_array_equal(prims[b:e], lst)
+            assert b == n
+            n = e
+    assert n == len(prims) == g.c.num_refs
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_uniform_cells_rebuilt_from_pairs(oracle, exact):
+    sc = scenes.random_soup(9, 150, extent=5.0, size=0.6)
+    g = build(sc, O.GRID_UNIFORM_EXACT if exact else O.GRID_UNIFORM, 5.0)
+    check_uniform(sc, g, exact)
+
+
+def test_uniform_invariants_cluster(oracle):
+    sc = scenes.icosphere_cluster()
+    g = build(sc, O.GRID_UNIFORM_EXACT, 5.0)
+    cells = g.cells().astype(np.int64)
+    prims = g.prims()
+    assert np.all(cells[:, 0] <= cells[:, 1])
+    assert np.all(cells[:, 1] <= g.c.num_refs)
+    assert np.all(prims < sc.nt)
+    assert (cells[:, 1] - cells[:, 0]).sum() == g.c.num_refs
+    # within a cell, primitive ids ascend
+    for b, e in cells[cells[:, 1] - cells[:, 0] > 1][:5000]:
+        assert np.all(np.diff(prims[b:e].astype(np.int64)) > 0)
+    # the exact test removes pairs but never adds any
+    gb = build(sc, O.GRID_UNIFORM, 5.0)
+    assert g.c.num_refs < gb.c.num_refs
+    assert g.c.num_bbox_pairs == gb.c.num_refs
+
+
+def unpack(y):
+    return y & 0xFF, (y >> 8) & 0xFF, (y >> 16) & 0xFF, bool(y & 0x10000000), bool(y & 0x40000000)
+
+
+def test_two_level_structure(oracle):
+    sc = scenes.random_soup(13, 4000, extent=6.0, size=0.3)
+    lt, ll = 0.25, 3.0
+    g = build(sc, O.GRID_TWO_LEVEL, lt, ll)
+    top = build(sc, O.GRID_UNIFORM, lt)          # the same top level, ranges kept
+    tcells = top.cells()
+    tprims = top.prims()
+    cells = g.cells()
+    leaves = g.leaves()
+    prims = g.prims()
+    p = g.params()
+    cs = p["cell_size"].astype(np.float32)
+    vol = np.float32(cs[0] * cs[1] * cs[2])
+    begin = 0
+    missing = 0
+    total = 0
+    for c in range(len(cells)):
+        lb, y = int(cells[c, 0]), int(cells[c, 1])
+        rx, ry, rz, leaf, notempty = unpack(y)
+        n = int(tcells[c, 1] - tcells[c, 0])
+        # leaf resolution closed form (TLGridBuildKernels.h:67-86)
+        if n <= 16:
+            assert (rx, ry, rz) == (1, 1, 1) and leaf
+        else:
+            m = oracle.powf(float(np.float32(ll) * np.float32(n) / vol), 0.3333333)
+            r = np.minimum(np.float32(255), np.maximum(np.float32(1), cs * np.float32(m)))
+            assert (rx, ry, rz) == tuple(int(v) for v in r) and not leaf
+        assert notempty == (n > 0)
+        assert lb == begin
+        begin += rx * ry * rz
+        # every leaf range lists primitives of this top cell, ascending
+        tset = set(tprims[tcells[c, 0]:tcells[c, 1]].tolist())
+        seen = set()
+        for li in range(lb, lb + rx * ry * rz):
+            b, e = leaves[li]
+            assert b <= e <= len(prims)
+            lst = prims[b:e]
+            assert np.all(np.diff(lst.astype(np.int64)) > 0)
+            assert set(lst.tolist()) <= tset
+            seen |= set(lst.tolist())
+        missing += len(tset - seen)
+        total += len(tset)
+    assert begin == g.c.num_leaves
+    assert g.c.num_top_pairs == top.c.num_refs
+    assert missing <= 0.001 * total
