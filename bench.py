@@ -1,0 +1,171 @@
+"""Benchmark of the per-frame hot path on one MI355X.
+
+Workload (default): 1,000,000 triangles -- 900k small triangles in a unit
+cube inside a 100k-triangle room box of extent 100 -- two-level grid with
+top density 1/16 and leaf density 3, 1024x1024 primary rays plus 8 ambient-
+occlusion rays per hit (occlusion distance 0.05 * |bounds diagonal|, the
+reference's alpha).  One step = one frame of the hot path on resident data:
+full two-level rebuild (interpolation, bounds, build) + AO render.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+Prints one JSON line.  `value` is Mrays/s over the timed steps (all rays:
+primary + AO), with the rebuild inside the step.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s (AO, path); grid rebuild ms/frame; p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# algorithmic work per DDA step / triangle test (flops), and bytes gathered
+FLOPS_PER_STEP = 20
+FLOPS_PER_TEST = 50
+BYTES_PER_STEP = 8          # one (begin, end) cell or leaf
+BYTES_PER_TEST = 4 + 48     # primitive id + triangle record
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    return ap.parse_args()
+
+
+def roofline_ao(ws, npix_hit, ao_rays, kernel_ms):
+    """Algorithmic bytes and flops of one AO-ray kernel launch."""
+    steps, tests = ws.secondary.steps, ws.secondary.tests
+    bytes_alg = BYTES_PER_STEP * steps + BYTES_PER_TEST * tests + 32 * npix_hit + ao_rays
+    flops = FLOPS_PER_STEP * steps + FLOPS_PER_TEST * tests
+    sec = kernel_ms * 1e-3
+    gbs = bytes_alg / sec / 1e9
+    tflops = flops / sec / 1e12
+    return bytes_alg, flops, gbs, tflops
+
+
+def cpu_baseline(scene, cam, ao_dist, threads):
+    """The CPU oracle (a scalar port of the reference algorithm) on the same
+    frame -- scene, view, 1024x1024 primary rays, 8 AO rays per hit -- on
+    `threads` host threads, and its two-level build of the full scene on one
+    thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    v = scene.verts
+    m = O.OMesh(v[0], v[1], v[2], scene.idx)
+    bmin, bmax = O.bounds(v)
+    t0 = time.perf_counter()
+    g = O.OGrid(m, bmin, bmax, O.GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    build_s = time.perf_counter() - t0
+    c = O.camera(cam["pos"], cam["dir"], cam["up"], cam["fov"], *cam["res"])
+    p = O.params(seed=(0x1234, 0x5678), jitter=1, ao_samples=8, ao_dist=ao_dist, nthreads=threads)
+    t0 = time.perf_counter()
+    _, _, st = O.render_ao(g, c, p, scene.materials, scene.mat_idx)
+    render_s = time.perf_counter() - t0
+    return {"value": round(st.rays / render_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one full AO frame of the same workload ({st.rays} rays, "
+                      f"{render_s:.2f} s on {threads} threads); two-level build of all "
+                      f"{scene.nt} triangles on 1 thread: {build_s * 1e3:.1f} ms",
+            "build_ms_1thread": round(build_s * 1e3, 2)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
+                                   "note (DESIGN.md), not a benchmark", "n_gpus": args.gpus}))
+        sys.exit(2)
+    from happyray_amd import Engine, GRID_TWO_LEVEL, MODE_AO, scenes
+
+    scene = scenes.soup_in_room()
+    cam = scene.camera
+    e = Engine(seed=(0x1234, 0x5678))
+    e.upload_scene(scene)
+    e.set_grid(GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    e.set_camera_dict(cam)
+    e.build_acc_struct()
+    ao_dist = 0.05 * e.get_bounding_box_diagonal_length()
+    S = 8
+
+    def step(i, stats=False):
+        e.build_acc_struct()
+        e.render_frame(i, MODE_AO, jitter=1, ao_samples=S, ao_dist=ao_dist, download=False,
+                       collect_stats=stats)
+
+    for i in range(args.warmup):
+        step(i)
+    e.synchronize()
+    r0 = e.timing().rays_traced
+    build_ms, prim_ms, ao_ms, shade_ms = [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i)
+        tm = e.timing()
+        build_ms.append(tm.build_ms)
+        prim_ms.append(tm.primary_ms)
+        ao_ms.append(tm.secondary_ms)
+        shade_ms.append(tm.shade_ms)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    rays = e.timing().rays_traced - r0
+    ms_step = wall * 1e3 / args.steps
+    value = rays / wall / 1e6
+
+    # work counters of one more frame (outside the timed region)
+    e.work_stats()
+    step(0, stats=True)
+    ws = e.work_stats()
+    planes, prims = e.primary_hits()
+    npix_hit = int((prims < scene.nt).sum())
+    ao_rays = ws.secondary.rays
+    ao_avg = float(np.mean(ao_ms))
+    bytes_alg, flops, gbs, tflops = roofline_ao(ws, npix_hit, ao_rays, ao_avg)
+    render_ms = np.array(prim_ms) + np.array(ao_ms) + np.array(shade_ms)
+    info = e.grid_info()
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "soup_in_room: 900k small triangles in a unit cube inside a "
+                               "100k-triangle room of extent 100 (1,000,000 triangles); two-level "
+                               "grid top 1/16 leaf 3; 1024x1024 primary + 8 AO rays per hit; "
+                               "step = rebuild + AO frame",
+                   "triangles": scene.nt, "rays_per_step": int(rays // args.steps),
+                   "top_res": list(info.res), "leaves": info.num_leaves,
+                   "leaf_refs": info.num_refs, "ao_dist": round(ao_dist, 4)},
+        "rebuild_ms": round(float(np.mean(build_ms)), 4),
+        "render_ms": round(float(np.mean(render_ms)), 4),
+        "render_mrays_s": round(rays / args.steps / (float(np.mean(render_ms)) * 1e-3) / 1e6, 3),
+        "kernel_ms": {"primary": round(float(np.mean(prim_ms)), 4), "ao_rays": round(ao_avg, 4),
+                      "ao_shade": round(float(np.mean(shade_ms)), 4)},
+        "roofline": {"kernel": "k_ao_rays<two_level>", "bound": "hbm", "achieved": round(gbs, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+                     "traffic": None,
+                     "bytes_per_launch": int(bytes_alg),
+                     "valu": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 5),
+                              "flops_per_launch": int(flops)},
+                     "work": {"ao_rays": int(ao_rays), "steps": int(ws.secondary.steps),
+                              "tests": int(ws.secondary.tests)}},
+    }
+    e.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, cam, ao_dist, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,11 +38,20 @@ class GridInfo(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("jitter", C.c_int), ("ao_samples", C.c_int), ("ao_dist", C.c_float),
-                ("pt_depth", C.c_int)]
+                ("pt_depth", C.c_int), ("collect_stats", C.c_int)]
 
 
 class Timing(C.Structure):
-    _fields_ = [("build_ms", C.c_double), ("render_ms", C.c_double), ("rays_traced", C.c_uint64)]
+    _fields_ = [("build_ms", C.c_double), ("render_ms", C.c_double), ("primary_ms", C.c_double),
+                ("secondary_ms", C.c_double), ("shade_ms", C.c_double), ("rays_traced", C.c_uint64)]
+
+
+class WorkCounts(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("steps", C.c_uint64), ("tests", C.c_uint64)]
+
+
+class WorkStats(C.Structure):
+    _fields_ = [("primary", WorkCounts), ("secondary", WorkCounts)]
 
 
 # name -> (restype, argtypes)
@@ -68,6 +77,7 @@ SIGNATURES = {
     "hr_occluded": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_float, _u8p]),
     "hr_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp]),
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
+    "hr_get_work_stats": (C.c_int, [_vp, C.POINTER(WorkStats)]),
     "hr_synchronize": (C.c_int, [_vp]),
 }
 

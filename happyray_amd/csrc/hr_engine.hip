@@ -130,8 +130,10 @@ struct hr_ctx {
 
     // timing / counters
     hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr, ev_r0 = nullptr, ev_r1 = nullptr;
+    hipEvent_t ev_p1 = nullptr, ev_p2 = nullptr;
     bool build_timed = false, render_timed = false;
     DevBuf<unsigned long long> ray_count;
+    DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
 };
 
 namespace {
@@ -386,15 +388,18 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         if ((s = c->params.reserve(1)) != HR_OK) break;
         if ((s = c->status.reserve(1)) != HR_OK) break;
         if ((s = c->ray_count.reserve(1)) != HR_OK) break;
+        if ((s = c->work.reserve(6)) != HR_OK) break;
         e = hipHostMalloc((void**)&c->h_status, sizeof(hr::BuildStatus), hipHostMallocDefault);
         if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
-        hipEvent_t* evs[4] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1 };
+        hipEvent_t* evs[6] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1, &c->ev_p1, &c->ev_p2 };
         for (hipEvent_t* ev : evs) {
             e = hipEventCreate(ev);
             if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
         }
         if (s != HR_OK) break;
         e = hipMemsetAsync(c->ray_count.p, 0, sizeof(unsigned long long), c->stream);
+        if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+        e = hipMemsetAsync(c->work.p, 0, 6 * sizeof(unsigned long long), c->stream);
         if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
     } while (0);
     if (s != HR_OK) {
@@ -426,8 +431,9 @@ hr_status hr_destroy(hr_ctx* c)
     c->params.release();
     c->status.release();
     c->ray_count.release();
+    c->work.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
-    hipEvent_t evs[4] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1 };
+    hipEvent_t evs[6] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1, c->ev_p1, c->ev_p2 };
     for (hipEvent_t ev : evs)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -640,6 +646,7 @@ hr_status hr_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_param
     ra.ao_dist = p->ao_dist;
     ra.pt_depth = p->pt_depth;
     ra.image = image_id;
+    ra.stats = p->collect_stats ? c->work.p : nullptr;
     const hr::TraceScene S = trace_scene(c);
     hr::Shading sh;
     sh.mats = c->mats.p;
@@ -649,12 +656,16 @@ hr_status hr_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_param
     sh.nlights = c->nlights;
     HR_HIP(hipEventRecord(c->ev_r0, c->stream));
     hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
+    HR_HIP(hipEventRecord(c->ev_p1, c->stream));
+    if (ra.stats) ra.stats = c->work.p + 3;
     if (mode == HR_MODE_AO) {
-        hr::launch_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->ray_count.p,
-                      c->stream);
+        hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
+        HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+        hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->stream);
         c->last_ao_samples = p->ao_samples;
     } else {
         hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
+        HR_HIP(hipEventRecord(c->ev_p2, c->stream));
     }
     HR_HIP(hipEventRecord(c->ev_r1, c->stream));
     HR_HIP(hipGetLastError());
@@ -700,7 +711,7 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
     const hr::TraceScene S = trace_scene(c);
     if (device) {
         hr::launch_trace_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->ray_count.p,
-                               c->stream);
+                               c->work.p, c->stream);
         HR_HIP(hipGetLastError());
         return HR_OK;
     }
@@ -712,7 +723,7 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
     HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
     HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
     hr::launch_trace_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
-                           c->ray_occ.p, c->ray_count.p, c->stream);
+                           c->ray_occ.p, c->ray_count.p, c->work.p, c->stream);
     HR_HIP(hipGetLastError());
     if (t_out) HR_HIP(hipMemcpyAsync(t_out, c->ray_t.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (prim_out)
@@ -748,14 +759,36 @@ hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
     HR_TRY(check_ctx(c));
     if (!out) return fail(HR_ERR_ARG, "null output pointer");
     HR_HIP(hipStreamSynchronize(c->stream));
-    out->build_ms = 0.0;
-    out->render_ms = 0.0;
+    out->build_ms = out->render_ms = out->primary_ms = out->secondary_ms = out->shade_ms = 0.0;
     float ms = 0.f;
     if (c->build_timed && hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1) == hipSuccess) out->build_ms = ms;
-    if (c->render_timed && hipEventElapsedTime(&ms, c->ev_r0, c->ev_r1) == hipSuccess) out->render_ms = ms;
+    if (c->render_timed) {
+        if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_r1) == hipSuccess) out->render_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess) out->primary_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev_p1, c->ev_p2) == hipSuccess) out->secondary_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev_p2, c->ev_r1) == hipSuccess) out->shade_ms = ms;
+    }
     unsigned long long rays = 0;
     HR_HIP(hipMemcpy(&rays, c->ray_count.p, sizeof rays, hipMemcpyDeviceToHost));
     out->rays_traced = rays;
+    return HR_OK;
+}
+
+hr_status hr_get_work_stats(hr_ctx* c, hr_work_stats* out)
+{
+    HR_TRY(check_ctx(c));
+    if (!out) return fail(HR_ERR_ARG, "null output pointer");
+    unsigned long long w[6] = { 0, 0, 0, 0, 0, 0 };
+    HR_HIP(hipStreamSynchronize(c->stream));
+    HR_HIP(hipMemcpy(w, c->work.p, sizeof w, hipMemcpyDeviceToHost));
+    hr::fill_u32((uint32_t*)c->work.p, 0u, 12, nullptr, 1, c->stream);
+    HR_HIP(hipStreamSynchronize(c->stream));
+    out->primary.rays = w[0];
+    out->primary.steps = w[1];
+    out->primary.tests = w[2];
+    out->secondary.rays = w[3];
+    out->secondary.steps = w[4];
+    out->secondary.tests = w[5];
     return HR_OK;
 }
 

@@ -29,6 +29,7 @@ struct RenderArgs {
     float ao_dist;
     int pt_depth;
     int image;
+    unsigned long long* stats;   // nullptr, or {rays, steps, tests} counters
 };
 
 struct Shading {
@@ -42,14 +43,16 @@ struct Shading {
 // hit buffer: 7 SoA planes of npix floats (hit point xyz, dir xyz, t) + prims
 void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
                     uint32_t* hit_prims, unsigned long long* ray_count, hipStream_t s);
-void launch_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
-               const uint32_t* hit_prims, uint8_t* occ, const Shading& sh, float* fb,
-               unsigned long long* ray_count, hipStream_t s);
+void launch_ao_rays(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                    const uint32_t* hit_prims, uint8_t* occ, unsigned long long* ray_count, hipStream_t s);
+void launch_ao_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                     const uint32_t* hit_prims, const uint8_t* occ, const Shading& sh, float* fb,
+                     hipStream_t s);
 void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                const uint32_t* hit_prims, const Shading& sh, float* fb,
                unsigned long long* ray_count, hipStream_t s);
 void launch_trace_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n,
                         int anyhit, float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out,
-                        unsigned long long* ray_count, hipStream_t s);
+                        unsigned long long* ray_count, unsigned long long* stats, hipStream_t s);
 
 }  // namespace hr

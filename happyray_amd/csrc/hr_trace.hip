@@ -56,11 +56,16 @@ __device__ __forceinline__ int min_dim(float x, float y, float z)
     return (x < y) ? ((x < z) ? 0 : 2) : ((y < z) ? 1 : 2);
 }
 
+struct Work {
+    uint32_t steps = 0, tests = 0;
+};
+
 template <bool kAny>
 __device__ __forceinline__ bool isect_range(const TraceScene& S, uint32_t b, uint32_t e, f3 org, f3 dir,
-                                            float& t, uint32_t& best)
+                                            float& t, uint32_t& best, Work& wk)
 {
     for (uint32_t it = b; it < e; ++it) {
+        ++wk.tests;
         const uint32_t p = S.prims[it];
         f3 v0, v1, v2;
         load_tri(S.rec, p, v0, v1, v2);
@@ -118,7 +123,7 @@ __device__ __forceinline__ bool dda_setup(const GridRegs& G, f3 org, f3 rcp, flo
 
 template <bool kAny>
 __device__ void trav_uniform(const GridRegs& G, const TraceScene& S, f3 org, f3 rcp, f3 dir, float& t,
-                             uint32_t& best, float tterm)
+                             uint32_t& best, float tterm, Work& wk)
 {
     float tentry, tx, ty, tz;
     int cx, cy, cz;
@@ -128,8 +133,9 @@ __device__ void trav_uniform(const GridRegs& G, const TraceScene& S, f3 org, f3 
     const float dx = G.cs.x * fabsf(rcp.x), dy = G.cs.y * fabsf(rcp.y), dz = G.cs.z * fabsf(rcp.z);
     const uint32_t sxy = (uint32_t)G.rx * (uint32_t)G.ry;
     while (flag) {
+        ++wk.steps;
         const uint2 cell = S.cells[(uint32_t)cx + (uint32_t)cy * (uint32_t)G.rx + (uint32_t)cz * sxy];
-        isect_range<kAny>(S, cell.x, cell.y, org, dir, t, best);
+        isect_range<kAny>(S, cell.x, cell.y, org, dir, t, best, wk);
         const int d = min_dim(tx, ty, tz);
         if (kAny && t < tterm) flag = false;
         if (d == 0) {
@@ -150,7 +156,7 @@ __device__ void trav_uniform(const GridRegs& G, const TraceScene& S, f3 org, f3 
 
 template <bool kAny>
 __device__ void trav_two_level(const GridRegs& G, const TraceScene& S, f3 org, f3 rcp, f3 dir, float& t,
-                               uint32_t& best, float tterm)
+                               uint32_t& best, float tterm, Work& wk)
 {
     float tentry, tx, ty, tz;
     int cx, cy, cz;
@@ -160,6 +166,7 @@ __device__ void trav_two_level(const GridRegs& G, const TraceScene& S, f3 org, f
     const float dx = G.cs.x * fabsf(rcp.x), dy = G.cs.y * fabsf(rcp.y), dz = G.cs.z * fabsf(rcp.z);
     const uint32_t sxy = (uint32_t)G.rx * (uint32_t)G.ry;
     while (flag) {
+        ++wk.steps;
         const uint2 cell = S.cells[(uint32_t)cx + (uint32_t)cy * (uint32_t)G.rx + (uint32_t)cz * sxy];
         bool second = (cell.y & 0x40000000u) != 0u;
         if (second) {
@@ -185,9 +192,10 @@ __device__ void trav_two_level(const GridRegs& G, const TraceScene& S, f3 org, f
             const float ddx = scx * fabsf(rcp.x), ddy = scy * fabsf(rcp.y), ddz = scz * fabsf(rcp.z);
             const uint32_t r01 = (uint32_t)r0 * (uint32_t)r1;
             while (second) {
+                ++wk.steps;
                 const uint2 leaf = S.leaves[cell.x + (uint32_t)lx + (uint32_t)r0 * (uint32_t)ly +
                                             r01 * (uint32_t)lz];
-                isect_range<kAny>(S, leaf.x, leaf.y, org, dir, t, best);
+                isect_range<kAny>(S, leaf.x, leaf.y, org, dir, t, best, wk);
                 if (kAny && t < tterm) { second = false; flag = false; }
                 const int d = min_dim(ux, uy, uz);
                 if (d == 0) {
@@ -227,13 +235,13 @@ __device__ void trav_two_level(const GridRegs& G, const TraceScene& S, f3 org, f
 // trace<> body: rcp = 1 / d, traverse, d' = 1 / rcp (RayTracingKernels.h:118-140)
 template <bool kTwo, bool kAny>
 __device__ __forceinline__ uint32_t trace_ray(const GridRegs& G, const TraceScene& S, f3 org, f3 d,
-                                              float& t, float tterm, f3& dir_out)
+                                              float& t, float tterm, f3& dir_out, Work& wk)
 {
     const f3 rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
     const f3 dir = mk(1.f / rcp.x, 1.f / rcp.y, 1.f / rcp.z);
     uint32_t best = S.nt;
-    if (kTwo) trav_two_level<kAny>(G, S, org, rcp, dir, t, best, tterm);
-    else trav_uniform<kAny>(G, S, org, rcp, dir, t, best, tterm);
+    if (kTwo) trav_two_level<kAny>(G, S, org, rcp, dir, t, best, tterm, wk);
+    else trav_uniform<kAny>(G, S, org, rcp, dir, t, best, tterm, wk);
     dir_out = dir;
     return best;
 }
@@ -242,6 +250,25 @@ __device__ __forceinline__ void count_rays(unsigned long long* counter, bool tra
 {
     const uint64_t b = __ballot(traced);
     if (counter && __lane_id() == 0 && b) atomicAdd(counter, (unsigned long long)__popcll(b));
+}
+
+// {rays, steps, tests} work counters, one atomic per wave and counter;
+// every lane of the wave must call it
+__device__ __forceinline__ void add_work(unsigned long long* stats, uint32_t rays, const Work& wk)
+{
+    if (!stats) return;
+    unsigned long long r = rays, s = wk.steps, t = wk.tests;
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) {
+        r += __shfl_xor(r, d, kWave);
+        s += __shfl_xor(s, d, kWave);
+        t += __shfl_xor(t, d, kWave);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(stats + 0, r);
+        atomicAdd(stats + 1, s);
+        atomicAdd(stats + 2, t);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -288,23 +315,26 @@ __global__ __launch_bounds__(kBlock) void k_primary(TraceScene S, CameraDev cam,
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = pix < npix;
     count_rays(ray_count, active);
-    if (!active) return;
-    const GridRegs G = load_grid(S.gp);
-    const f3 d0 = primary_direction(cam, ra, pix, npix);
-    const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
-    float t = kFltMax;
-    f3 dir;
-    const uint32_t best = trace_ray<kTwo, false>(G, S, org, d0, t, 0.f, dir);
-    const uint32_t prim = t < kFltMax ? best : S.nt;
-    const f3 hp = org + (t - kEps) * dir;
-    hits[pix] = hp.x;
-    hits[(size_t)npix + pix] = hp.y;
-    hits[2 * (size_t)npix + pix] = hp.z;
-    hits[3 * (size_t)npix + pix] = dir.x;
-    hits[4 * (size_t)npix + pix] = dir.y;
-    hits[5 * (size_t)npix + pix] = dir.z;
-    hits[6 * (size_t)npix + pix] = t;
-    hit_prims[pix] = prim;
+    Work wk;
+    if (active) {
+        const GridRegs G = load_grid(S.gp);
+        const f3 d0 = primary_direction(cam, ra, pix, npix);
+        const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
+        float t = kFltMax;
+        f3 dir;
+        const uint32_t best = trace_ray<kTwo, false>(G, S, org, d0, t, 0.f, dir, wk);
+        const uint32_t prim = t < kFltMax ? best : S.nt;
+        const f3 hp = org + (t - kEps) * dir;
+        hits[pix] = hp.x;
+        hits[(size_t)npix + pix] = hp.y;
+        hits[2 * (size_t)npix + pix] = hp.z;
+        hits[3 * (size_t)npix + pix] = dir.x;
+        hits[4 * (size_t)npix + pix] = dir.y;
+        hits[5 * (size_t)npix + pix] = dir.z;
+        hits[6 * (size_t)npix + pix] = t;
+        hit_prims[pix] = prim;
+    }
+    add_work(ra.stats, active ? 1u : 0u, wk);
 }
 
 // ---------------------------------------------------------------------------
@@ -339,8 +369,9 @@ __global__ __launch_bounds__(kBlock) void k_ao_rays(TraceScene S, RenderArgs ra,
     const float th = in ? hits[6 * (size_t)npix + pix] : kFltMax;
     const bool active = in && th < kFltMax && prim < S.nt;
     count_rays(ray_count, active);
-    if (!in) return;
-    if (!active) { occ[j] = 1u; return; }
+    Work wk;
+    if (in && !active) occ[j] = 1u;
+    if (active) {
     const GridRegs G = load_grid(S.gp);
     // AmbientOcclusionRayGenerator::operator() (RayGenerators.h:281-343)
     f3 org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
@@ -359,8 +390,10 @@ __global__ __launch_bounds__(kBlock) void k_ao_rays(TraceScene S, RenderArgs ra,
     org = org + (0.01f * ra.ao_dist) * dir;
     float t = ra.ao_dist;
     f3 d2;
-    trace_ray<kTwo, true>(G, S, org, dir, t, ra.ao_dist, d2);
+    trace_ray<kTwo, true>(G, S, org, dir, t, ra.ao_dist, d2, wk);
     occ[j] = t < ra.ao_dist ? 1u : 0u;
+    }
+    add_work(ra.stats, active ? 1u : 0u, wk);
 }
 
 // computeAOIllumination (AOIntegrator.h:82-188), one lane per pixel
@@ -429,12 +462,13 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
                                                float* __restrict__ fb, unsigned long long* ray_count)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= npix) return;
+    const bool in = pix < npix;
     const GridRegs G = load_grid(S.gp);
     f3 radiance = rep(0.f);
-    float ray_t = hits[6 * (size_t)npix + pix];
-    uint32_t prim = hit_prims[pix];
-    unsigned long long traced = 0;
+    float ray_t = in ? hits[6 * (size_t)npix + pix] : kFltMax;
+    uint32_t prim = in ? hit_prims[pix] : S.nt;
+    uint32_t traced = 0;
+    Work wk;
     if (ray_t < kFltMax && prim < S.nt) {
         f3 org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
         f3 dir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix],
@@ -511,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
                     if (cos_nl > 0.f && cos_ln > 0.f) {
                         float st = 0.9999f;
                         f3 dd;
-                        trace_ray<kTwo, true>(G, S, org, sdir, st, 0.9999f, dd);
+                        trace_ray<kTwo, true>(G, S, org, sdir, st, 0.9999f, dd, wk);
                         ++traced;
                         if (!(st < 0.9999f)) {
                             const float area = len(cross(le1, le2));
@@ -557,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
             // extension ray (PathTracer.h:318-332)
             float t = kFltMax;
             f3 d2;
-            const uint32_t hit = trace_ray<kTwo, false>(G, S, org, dir, t, 0.f, d2);
+            const uint32_t hit = trace_ray<kTwo, false>(G, S, org, dir, t, 0.f, d2, wk);
             ++traced;
             dir = d2;
             if (!(t < kFltMax)) break;
@@ -566,7 +600,9 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
             ray_t = t;
         }
     }
-    if (ray_count && traced) atomicAdd(ray_count, traced);
+    if (ray_count && traced) atomicAdd(ray_count, (unsigned long long)traced);
+    add_work(ra.stats, traced, wk);
+    if (!in) return;
     const float nw = 1.f / (float)(ra.image + 1);
     const float ow = 1.f - nw;
     float* o = fb + 3 * (size_t)pix;
@@ -585,24 +621,28 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(TraceScene S, const floa
                                                         float tmax, float* __restrict__ t_out,
                                                         uint32_t* __restrict__ prim_out,
                                                         uint8_t* __restrict__ occ_out,
-                                                        unsigned long long* ray_count)
+                                                        unsigned long long* ray_count,
+                                                        unsigned long long* stats)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     count_rays(ray_count, i < n);
-    if (i >= n) return;
-    const GridRegs G = load_grid(S.gp);
-    const f3 o = mk(org[i], org[(size_t)n + i], org[2 * (size_t)n + i]);
-    const f3 d = mk(dir[i], dir[(size_t)n + i], dir[2 * (size_t)n + i]);
-    float t = kAny ? tmax : kFltMax;
-    f3 d2;
-    const uint32_t best = trace_ray<kTwo, kAny>(G, S, o, d, t, tmax, d2);
-    if (kAny) {
-        occ_out[i] = t < tmax ? 1u : 0u;
-        if (t_out) t_out[i] = t;
-    } else {
-        t_out[i] = t;
-        prim_out[i] = t < kFltMax ? best : S.nt;
+    Work wk;
+    if (i < n) {
+        const GridRegs G = load_grid(S.gp);
+        const f3 o = mk(org[i], org[(size_t)n + i], org[2 * (size_t)n + i]);
+        const f3 d = mk(dir[i], dir[(size_t)n + i], dir[2 * (size_t)n + i]);
+        float t = kAny ? tmax : kFltMax;
+        f3 d2;
+        const uint32_t best = trace_ray<kTwo, kAny>(G, S, o, d, t, tmax, d2, wk);
+        if (kAny) {
+            occ_out[i] = t < tmax ? 1u : 0u;
+            if (t_out) t_out[i] = t;
+        } else {
+            t_out[i] = t;
+            prim_out[i] = t < kFltMax ? best : S.nt;
+        }
     }
+    add_work(stats, i < n ? 1u : 0u, wk);
 }
 
 inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
@@ -621,9 +661,8 @@ void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs&
                            hits, hit_prims, ray_count);
 }
 
-void launch_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
-               const uint32_t* hit_prims, uint8_t* occ, const Shading& sh, float* fb,
-               unsigned long long* ray_count, hipStream_t s)
+void launch_ao_rays(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                    const uint32_t* hit_prims, uint8_t* occ, unsigned long long* ray_count, hipStream_t s)
 {
     const size_t nrays = (size_t)npix * (size_t)ra.ao_samples;
     if (S.two_level)
@@ -632,6 +671,12 @@ void launch_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
     else
         hipLaunchKernelGGL(k_ao_rays<false>, dim3(blocks_for(nrays)), dim3(kBlock), 0, s, S, ra, npix, hits,
                            hit_prims, occ, ray_count);
+}
+
+void launch_ao_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                     const uint32_t* hit_prims, const uint8_t* occ, const Shading& sh, float* fb,
+                     hipStream_t s)
+{
     hipLaunchKernelGGL(k_ao_shade, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits, hit_prims,
                        occ, sh, fb);
 }
@@ -650,23 +695,23 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
 
 void launch_trace_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n,
                         int anyhit, float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out,
-                        unsigned long long* ray_count, hipStream_t s)
+                        unsigned long long* ray_count, unsigned long long* stats, hipStream_t s)
 {
     const dim3 g(blocks_for(n > 0 ? n : 1)), b(kBlock);
     if (S.two_level) {
         if (anyhit)
             hipLaunchKernelGGL((k_trace_batch<true, true>), g, b, 0, s, S, org, dir, n, tmax, t_out, prim_out,
-                               occ_out, ray_count);
+                               occ_out, ray_count, stats);
         else
             hipLaunchKernelGGL((k_trace_batch<true, false>), g, b, 0, s, S, org, dir, n, tmax, t_out, prim_out,
-                               occ_out, ray_count);
+                               occ_out, ray_count, stats);
     } else {
         if (anyhit)
             hipLaunchKernelGGL((k_trace_batch<false, true>), g, b, 0, s, S, org, dir, n, tmax, t_out, prim_out,
-                               occ_out, ray_count);
+                               occ_out, ray_count, stats);
         else
             hipLaunchKernelGGL((k_trace_batch<false, false>), g, b, 0, s, S, org, dir, n, tmax, t_out, prim_out,
-                               occ_out, ray_count);
+                               occ_out, ray_count, stats);
     }
 }
 

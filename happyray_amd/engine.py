@@ -157,8 +157,9 @@ class Engine:
 
     # -- render -----------------------------------------------------------
     def render_frame(self, image_id=0, mode=MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0,
-                     pt_depth=4, download=True):
-        p = _lib.RenderParams(int(jitter), int(ao_samples), float(ao_dist), int(pt_depth))
+                     pt_depth=4, download=True, collect_stats=False):
+        p = _lib.RenderParams(int(jitter), int(ao_samples), float(ao_dist), int(pt_depth),
+                              int(bool(collect_stats)))
         out = None
         if download:
             out = np.zeros((self._res[0] * self._res[1], 3), np.float32)
@@ -203,6 +204,11 @@ class Engine:
         t = _lib.Timing()
         check(self._L.hr_get_timing(self._h, C.byref(t)), "hr_get_timing")
         return t
+
+    def work_stats(self):
+        w = _lib.WorkStats()
+        check(self._L.hr_get_work_stats(self._h, C.byref(w)), "hr_get_work_stats")
+        return w
 
     def synchronize(self):
         check(self._L.hr_synchronize(self._h), "hr_synchronize")

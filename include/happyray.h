@@ -82,13 +82,33 @@ typedef struct {
     int ao_samples;      /* AO rays per primary hit (AOIntegrator.h:45 uses 4)            */
     float ao_dist;       /* occlusion distance (the reference uses 0.05*|bbox diagonal|)  */
     int pt_depth;        /* path vertices                                                 */
+    int collect_stats;   /* 1: count DDA steps and triangle tests (hr_get_work_stats)    */
 } hr_render_params;
 
+/* Device times from hipEvents recorded on the context's stream around each
+ * phase of the last build / render. */
 typedef struct {
-    double build_ms;         /* last hr_build, device time (hipEvents)           */
-    double render_ms;        /* last hr_render, device time                      */
-    uint64_t rays_traced;    /* rays traversed by the last hr_render / hr_trace  */
+    double build_ms;         /* last hr_build                                    */
+    double render_ms;        /* last hr_render, all kernels                      */
+    double primary_ms;       /* primary-ray kernel of the last render            */
+    double secondary_ms;     /* AO-ray kernel or path-tracing kernel             */
+    double shade_ms;         /* AO shading kernel                                */
+    uint64_t rays_traced;    /* rays traversed since the context was created     */
 } hr_timing;
+
+/* Work counters of the renders run with collect_stats = 1 and of all ray
+ * batches, split by kernel: primary rays (and ray batches), then secondary
+ * rays (AO rays, or the path tracer's shadow and extension rays). */
+typedef struct {
+    uint64_t rays;           /* rays traversed              */
+    uint64_t steps;          /* DDA cell steps (top + leaf) */
+    uint64_t tests;          /* ray/triangle tests          */
+} hr_work_counts;
+
+typedef struct {
+    hr_work_counts primary;
+    hr_work_counts secondary;
+} hr_work_stats;
 
 /* ---- lifetime: RTEngine::init / cleanup (src/RT/RTEngine.cu:183-192, :455-472) ---- */
 const char* hr_last_error(void);
@@ -153,6 +173,8 @@ hr_status hr_trace_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint
                        float* d_t_out, uint32_t* d_prim_out);
 
 hr_status hr_get_timing(hr_ctx* ctx, hr_timing* out);
+/* Read and clear the work counters. */
+hr_status hr_get_work_stats(hr_ctx* ctx, hr_work_stats* out);
 hr_status hr_synchronize(hr_ctx* ctx);
 
 #ifdef __cplusplus
