@@ -1,0 +1,156 @@
+"""Pins of the oracle's integrators by closed forms.
+
+* AO over an open plane: no AO ray is occluded and every hit pixel equals
+  the shading formula of computeAOIllumination (AOIntegrator.h:144-157)
+  evaluated on the primary hit; misses are 1;
+* AO inside a closed box: every AO ray is occluded, hit pixels are 0;
+* direct lighting: a Lambertian plane under a parallel square area light
+  (no occluders, one path vertex).  The per-frame estimator is
+  kd * I * A * cos_n cos_l / r^2 at a uniform light point, so its mean and
+  variance are integrals over the light; both are evaluated by dense
+  quadrature in double precision, and the running mean over K frames must lie
+  within 5 standard errors at every pixel;
+* the running-mean accumulation weight 1/(image+1) (AOIntegrator.h:181-187).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from happyray_amd import scenes
+
+F = np.float32
+PI = F(np.pi)
+
+
+def plane_scene(kd):
+    """A 100x100 plane at y = 0 plus one small triangle far off to the side,
+    so that y = 0 does not fall on a cell boundary (exact insertion drops a
+    triangle lying exactly on the lower face of its cell range, as the
+    reference does: its enlarged test box starts at that face)."""
+    v, f = scenes.quad_grid((-50, 0, -50), (100, 0, 0), (0, 0, 100), 4, 4)
+    tri = np.array([[49.0, 0.37, 49.0], [49.5, 0.37, 49.0], [49.0, 0.9, 49.5]])
+    parts = [(v, f, 0), (tri, np.array([[0, 1, 2]], np.uint32), 1)]
+    mats = [scenes.material(kd), scenes.material((0.1, 0.1, 0.1))]
+    cam = dict(pos=(0.0, 10.0, 0.0), dir=(0.1, -1.0, 0.2), up=(0.0, 0.0, 1.0), fov=60.0, res=(48, 40))
+    return scenes._assemble(parts, mats, "plane", cam)
+
+
+def setup(sc, kind=O.GRID_UNIFORM_EXACT, lt=5.0):
+    m = O.OMesh(sc.verts[0], sc.verts[1], sc.verts[2], sc.idx)
+    bmin, bmax = O.bounds(sc.verts)
+    g = O.OGrid(m, bmin, bmax, kind, lt, 3.0)
+    c = sc.camera
+    cam = O.camera(c["pos"], c["dir"], c["up"], c["fov"], *c["res"])
+    return g, cam
+
+
+def ao_shade_closed_form(kd, pdir, n, S):
+    v = np.ones(3, F) * (kd * PI)
+    if np.any(kd * PI < F(0.9)):
+        s = max(F(0), np.sqrt(F(1) - n[2]))
+        c = abs(F(pdir[0] * n[0] + pdir[1] * n[1]) + F(pdir[2] * n[2]))
+        w = np.array([F(0.7) * c + F(0.3) * s, F(0.7) * c + F(0.2) * s, F(0.9) * c], F)
+        v = (v * w).astype(F)
+    else:
+        v = np.ones(3, F)
+    rad = np.zeros(3, F)
+    for _ in range(S):
+        rad = (rad + (v * F(1)) / F(S)).astype(F)
+    return rad
+
+
+@pytest.mark.parametrize("kind", [O.GRID_UNIFORM_EXACT, O.GRID_TWO_LEVEL])
+def test_ao_open_plane(kind):
+    kd = np.array([0.1, 0.2, 0.25], F)
+    sc = plane_scene(kd)
+    g, cam = setup(sc, kind, 5.0 if kind != O.GRID_TWO_LEVEL else 0.5)
+    p = O.params(jitter=1, ao_samples=4, ao_dist=30.0)
+    h, hp, _ = O.primary(g, cam, p)
+    fb, occ, _ = O.render_ao(g, cam, p, sc.materials, sc.mat_idx)
+    hit = hp < sc.nt
+    assert hit.mean() > 0.5
+    assert occ.reshape(-1, 4)[hit].sum() == 0
+    np.testing.assert_array_equal(fb[~hit], np.ones((np.sum(~hit), 3), F))
+    for i in np.nonzero(hit)[0][::7]:
+        tri = sc.idx[hp[i]]
+        v0, v1, v2 = (sc.verts[:, tri[k]] for k in range(3))
+        e1 = (v1 - v0).astype(F)
+        e2 = (v2 - v0).astype(F)
+        rn = np.array([e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                       e1[0] * e2[1] - e1[1] * e2[0]], F)
+        n = (rn * (F(1) / np.sqrt(F(rn[0] * rn[0] + rn[1] * rn[1]) + F(rn[2] * rn[2])))).astype(F)
+        np.testing.assert_array_equal(fb[i], ao_shade_closed_form(kd, h[3:6, i], n, 4))
+
+
+def test_ao_closed_box():
+    """Inside the unit cube every AO ray hits a wall within sqrt(3) < 2, except
+    those whose start, moved 0.01 * ao_dist along the ray
+    (RayGenerators.h:340), leaves the box: only hit points within that
+    distance of an edge can do so."""
+    sc = scenes.small_box()
+    g, cam = setup(sc)
+    S, dist = 8, 2.0
+    p = O.params(jitter=1, ao_samples=S, ao_dist=dist)
+    fb, occ, _ = O.render_ao(g, cam, p, sc.materials, sc.mat_idx)
+    h, hp, _ = O.primary(g, cam, p)
+    assert np.all(hp < sc.nt)
+    free = np.nonzero(occ.reshape(-1, S).min(1) == 0)[0]
+    pts = h[0:3, free].T
+    near = np.minimum(pts, 1 - pts) < 0.01 * dist + 1e-4
+    assert np.all(near.sum(1) >= 2)          # near two faces: an edge
+    assert occ.mean() > 0.95
+    closed = np.setdiff1d(np.arange(len(hp)), free)
+    assert np.all(fb[closed] == 0)
+
+
+def test_running_mean_weights():
+    sc = scenes.small_box()
+    g, cam = setup(sc)
+    p = O.params(jitter=1, ao_samples=2, ao_dist=0.3)
+    f0, _, _ = O.render_ao(g, cam, p, sc.materials, sc.mat_idx, image_id=0)
+    f1, _, _ = O.render_ao(g, cam, p, sc.materials, sc.mat_idx, image_id=1, fb=f0)
+    n1, _, _ = O.render_ao(g, cam, p, sc.materials, sc.mat_idx, image_id=1)   # fresh buffer
+    # fb1 = fb0 * (1 - 1/2) + new * 1/2, and a zero buffer gives new * 1/2
+    np.testing.assert_array_equal(f1, (f0 * F(0.5) + (n1 / F(0.5)) * F(0.5)).astype(F))
+
+
+def irradiance_integrals(p, n, light, m=400):
+    """E[f] and E[f^2] of f = I*A*cos_n*cos_l/r^2 at a uniform light point."""
+    pos, ln, e1, e2 = (light[0:3].astype(np.float64), light[3:6].astype(np.float64),
+                       light[9:12].astype(np.float64), light[12:15].astype(np.float64))
+    u = (np.arange(m) + 0.5) / m
+    uu, vv = np.meshgrid(u, u, indexing="ij")
+    q = pos + uu[..., None] * e1 + vv[..., None] * e2
+    d = q - p
+    r2 = (d * d).sum(-1)
+    dn = d / np.sqrt(r2)[..., None]
+    cn = np.maximum(0, dn @ n)
+    cl = np.maximum(0, -(dn @ ln))
+    area = np.linalg.norm(np.cross(e1, e2))
+    f = area * cn * cl / r2
+    return f.mean(), (f * f).mean()
+
+
+def test_direct_lighting_integral():
+    kd = np.array([0.3, 0.3, 0.3], F)
+    sc = plane_scene(kd)
+    light = scenes.area_light((-1.0, 4.0, -1.0), (2.0, 0.0, 0.0), (0.0, 0.0, 2.0), (5.0, 5.0, 5.0))
+    assert light[4] < 0                                  # faces down
+    g, cam = setup(sc)
+    K = 48
+    fb = None
+    for k in range(K):
+        p = O.params(seed=(7, 9), jitter=0, pt_depth=1)
+        fb, _ = O.render_pt(g, cam, p, sc.materials, sc.mat_idx, light[None], image_id=k, fb=fb)
+    h, hp, _ = O.primary(g, cam, O.params(jitter=0))
+    hit = np.nonzero(hp < sc.nt)[0]
+    z = []
+    for i in hit[::5]:
+        pnt = h[0:3, i].astype(np.float64) + 5 * 1e-5 * np.array([0, 1, 0])
+        ef, ef2 = irradiance_integrals(pnt, np.array([0, 1.0, 0]), light)
+        mean = 0.3 * 5.0 * ef
+        sd = 0.3 * 5.0 * np.sqrt(max(ef2 - ef * ef, 1e-30) / K)
+        z.append((fb[i, 0] - mean) / max(sd, 1e-6 * mean + 1e-12))
+    z = np.array(z)
+    assert np.abs(z).max() < 5.0
+    assert abs(z.mean()) < 5.0 / np.sqrt(len(z)) + 0.05
