@@ -40,7 +40,24 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes=256 << 20):
+    """One plain copy kernel of known size through the HIP runtime, so that a
+    profiler's FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0      # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def roofline_ao(ws, npix_hit, ao_rays, kernel_ms):
@@ -162,6 +179,8 @@ def main():
                               "tests": int(ws.secondary.tests)}},
     }
     e.close()
+    if args.copy_calibration:
+        copy_calibration()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cam, ao_dist, args.cpu_threads)
     print(json.dumps(out), flush=True)

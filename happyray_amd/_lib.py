@@ -46,6 +46,11 @@ class Timing(C.Structure):
                 ("secondary_ms", C.c_double), ("shade_ms", C.c_double), ("rays_traced", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("persistent_primary", C.c_int), ("persistent_secondary", C.c_int),
+                ("refill", C.c_int)]
+
+
 class WorkCounts(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("steps", C.c_uint64), ("tests", C.c_uint64)]
 
@@ -76,6 +81,8 @@ SIGNATURES = {
     "hr_trace": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, _f32p, _u32p]),
     "hr_occluded": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_float, _u8p]),
     "hr_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp]),
+    "hr_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+    "hr_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
     "hr_get_work_stats": (C.c_int, [_vp, C.POINTER(WorkStats)]),
     "hr_synchronize": (C.c_int, [_vp]),

@@ -18,6 +18,7 @@
 
 #include "../../include/happyray.h"
 #include "hr_build.h"
+#include "hr_persist.h"
 #include "hr_trace.h"
 
 namespace {
@@ -134,6 +135,8 @@ struct hr_ctx {
     bool build_timed = false, render_timed = false;
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
+    DevBuf<uint32_t> pool;               // persistent kernels' ray counter
+    hr_tuning tuning{ 0, 1, 48 };
 };
 
 namespace {
@@ -389,6 +392,7 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         if ((s = c->status.reserve(1)) != HR_OK) break;
         if ((s = c->ray_count.reserve(1)) != HR_OK) break;
         if ((s = c->work.reserve(6)) != HR_OK) break;
+        if ((s = c->pool.reserve(1)) != HR_OK) break;
         e = hipHostMalloc((void**)&c->h_status, sizeof(hr::BuildStatus), hipHostMallocDefault);
         if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
         hipEvent_t* evs[6] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1, &c->ev_p1, &c->ev_p2 };
@@ -432,6 +436,7 @@ hr_status hr_destroy(hr_ctx* c)
     c->status.release();
     c->ray_count.release();
     c->work.release();
+    c->pool.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
     hipEvent_t evs[6] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1, c->ev_p1, c->ev_p2 };
     for (hipEvent_t ev : evs)
@@ -655,11 +660,21 @@ hr_status hr_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_param
     sh.light_cdf = c->light_cdf.p;
     sh.nlights = c->nlights;
     HR_HIP(hipEventRecord(c->ev_r0, c->stream));
-    hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
+    const bool pers = c->tuning.persistent_secondary != 0;
+    const uint32_t refill = (uint32_t)c->tuning.refill;
+    if (c->tuning.persistent_primary)
+        hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
+                            c->stream);
+    else
+        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
     HR_HIP(hipEventRecord(c->ev_p1, c->stream));
     if (ra.stats) ra.stats = c->work.p + 3;
     if (mode == HR_MODE_AO) {
-        hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
+        if (pers)
+            hr::persist_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->pool.p, refill, c->ray_count.p,
+                           c->stream);
+        else
+            hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
         HR_HIP(hipEventRecord(c->ev_p2, c->stream));
         hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->stream);
         c->last_ao_samples = p->ao_samples;
@@ -709,9 +724,15 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
     if (n == 0) return HR_OK;
     if (!org || !dir) return fail(HR_ERR_ARG, "null ray buffer");
     const hr::TraceScene S = trace_scene(c);
+    const bool pers = c->tuning.persistent_secondary != 0;
+    const uint32_t refill = (uint32_t)c->tuning.refill;
     if (device) {
-        hr::launch_trace_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->ray_count.p,
-                               c->work.p, c->stream);
+        if (pers)
+            hr::persist_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->pool.p, refill,
+                              c->ray_count.p, c->work.p, c->stream);
+        else
+            hr::launch_trace_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->ray_count.p,
+                                   c->work.p, c->stream);
         HR_HIP(hipGetLastError());
         return HR_OK;
     }
@@ -722,8 +743,12 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
     HR_TRY(c->ray_occ.reserve(n));
     HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
     HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
-    hr::launch_trace_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
-                           c->ray_occ.p, c->ray_count.p, c->work.p, c->stream);
+    if (pers)
+        hr::persist_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
+                          c->ray_occ.p, c->pool.p, refill, c->ray_count.p, c->work.p, c->stream);
+    else
+        hr::launch_trace_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
+                               c->ray_occ.p, c->ray_count.p, c->work.p, c->stream);
     HR_HIP(hipGetLastError());
     if (t_out) HR_HIP(hipMemcpyAsync(t_out, c->ray_t.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (prim_out)
@@ -789,6 +814,26 @@ hr_status hr_get_work_stats(hr_ctx* c, hr_work_stats* out)
     out->secondary.rays = w[3];
     out->secondary.steps = w[4];
     out->secondary.tests = w[5];
+    return HR_OK;
+}
+
+hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
+{
+    HR_TRY(check_ctx(c));
+    if (!t) return fail(HR_ERR_ARG, "null tuning");
+    if ((t->persistent_primary != 0 && t->persistent_primary != 1) ||
+        (t->persistent_secondary != 0 && t->persistent_secondary != 1))
+        return fail(HR_ERR_ARG, "persistent flags must be 0 or 1");
+    if (t->refill < 1 || t->refill > 64) return fail(HR_ERR_ARG, "refill must be in [1, 64]");
+    c->tuning = *t;
+    return HR_OK;
+}
+
+hr_status hr_get_tuning(hr_ctx* c, hr_tuning* t)
+{
+    HR_TRY(check_ctx(c));
+    if (!t) return fail(HR_ERR_ARG, "null output pointer");
+    *t = c->tuning;
     return HR_OK;
 }
 

@@ -1,0 +1,465 @@
+// hr_persist.hip -- persistent traversal kernels with dynamic ray fetch.
+//
+// The reference's trace<> kernel uses a persistent warp pool only on old
+// architectures and otherwise a grid-stride loop with one ray per thread
+// (src/RT/Algorithm/RayTracingKernels.h:65-106); its traversers nest a cell
+// loop, a leaf loop and a candidate loop (src/RT/Algorithm/TLGridTraverser.h:
+// 132-350, RayTriangleIntersector.h:52-90), so a wave waits for its slowest
+// lane at every level.  Here each lane runs a flat state machine in which one
+// loop iteration does one unit of work -- one ray/triangle test, or one DDA
+// advance that loads the next cell or leaf range -- and a wave refills idle
+// lanes with new rays from a global counter whenever at least `refill` of its
+// 64 lanes are idle.  The operations applied to each ray, and their order,
+// are those of the reference traversal (same cells, same candidates, same
+// termination tests), so results are bit-identical to the one-ray-per-lane
+// kernels and to the CPU oracle.
+#include "hr_persist.h"
+#include "hr_traverse.h"
+
+namespace hr {
+
+namespace {
+
+using namespace trav;
+constexpr int kBlock = 256;
+
+struct RayState {
+    f3 org, dir, rcp;
+    float t, tterm;
+    uint32_t best;
+    int cx, cy, cz;
+    float tx, ty, tz, tentry;
+    int lx, ly, lz;
+    float ux, uy, uz;
+    float ddx, ddy, ddz;
+    uint32_t cell;      // packed word of the current top cell (two-level)
+    uint32_t lbeg;      // its first leaf
+    uint32_t it, end;   // candidate range still to test
+    bool in_leaf;
+};
+
+__device__ __forceinline__ uint32_t cell_index(const GridRegs& G, int cx, int cy, int cz)
+{
+    return (uint32_t)cx + (uint32_t)cy * (uint32_t)G.rx + (uint32_t)cz * ((uint32_t)G.rx * (uint32_t)G.ry);
+}
+
+__device__ __forceinline__ void leaf_load(const TraceScene& S, RayState& r, Work& wk)
+{
+    ++wk.steps;
+    const uint32_t r0 = r.cell & 0xFFu, r1 = (r.cell >> 8) & 0xFFu;
+    const uint2 leaf = S.leaves[r.lbeg + (uint32_t)r.lx + r0 * (uint32_t)r.ly + r0 * r1 * (uint32_t)r.lz];
+    r.it = leaf.x;
+    r.end = leaf.y;
+}
+
+// visit the current top cell: non-empty cells start their leaf DDA at
+// tentry (TLGridTraverser.h:147-197) and load the first leaf range
+__device__ __forceinline__ void top_visit(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+{
+    ++wk.steps;
+    const uint2 cell = S.cells[cell_index(G, r.cx, r.cy, r.cz)];
+    r.it = r.end = 0u;
+    r.in_leaf = false;
+    if (!(cell.y & 0x40000000u)) return;
+    const int r0 = (int)(cell.y & 0xFFu), r1 = (int)((cell.y >> 8) & 0xFFu), r2 = (int)((cell.y >> 16) & 0xFFu);
+    const f3 ep = r.org + rep(r.tentry) / r.rcp;
+    const f3 cres = mk((float)r0, (float)r1, (float)r2);
+    const f3 srcp = G.csr * cres;
+    f3 mb;
+    mb.x = (float)r.cx * G.cs.x + G.bmin.x;
+    mb.y = (float)r.cy * G.cs.y + G.bmin.y;
+    mb.z = (float)r.cz * G.cs.z + G.bmin.z;
+    f3 cf;
+    cf.x = floorf((ep - mb).x * srcp.x);
+    cf.y = floorf((ep - mb).y * srcp.y);
+    cf.z = floorf((ep - mb).z * srcp.z);
+    const float scx = 1.f / srcp.x, scy = 1.f / srcp.y, scz = 1.f / srcp.z;
+    cf = vmin(cres - rep(1.f), vmax(cf, rep(0.f)));
+    r.ux = ((cf.x + (r.rcp.x > 0.f ? 1.f : 0.f)) * scx + mb.x - r.org.x) * r.rcp.x;
+    r.uy = ((cf.y + (r.rcp.y > 0.f ? 1.f : 0.f)) * scy + mb.y - r.org.y) * r.rcp.y;
+    r.uz = ((cf.z + (r.rcp.z > 0.f ? 1.f : 0.f)) * scz + mb.z - r.org.z) * r.rcp.z;
+    r.lx = (int)cf.x;
+    r.ly = (int)cf.y;
+    r.lz = (int)cf.z;
+    r.ddx = scx * fabsf(r.rcp.x);
+    r.ddy = scy * fabsf(r.rcp.y);
+    r.ddz = scz * fabsf(r.rcp.z);
+    r.cell = cell.y;
+    r.lbeg = cell.x;
+    r.in_leaf = true;
+    leaf_load(S, r, wk);
+}
+
+__device__ __forceinline__ void uniform_load(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+{
+    ++wk.steps;
+    const uint2 cell = S.cells[cell_index(G, r.cx, r.cy, r.cz)];
+    r.it = cell.x;
+    r.end = cell.y;
+}
+
+// one ray/triangle test; true when an any-hit ray is finished
+template <bool kAny>
+__device__ __forceinline__ bool test_one(const TraceScene& S, RayState& r, Work& wk)
+{
+    ++wk.tests;
+    const uint32_t p = S.prims[r.it];
+    ++r.it;
+    f3 v0, v1, v2;
+    load_tri(S.rec, p, v0, v1, v2);
+    const f3 e1 = v1 - v0;
+    const f3 e2 = v2 - v0;
+    const f3 pv = cross(r.dir, e2);
+    const float det_rcp = 1.f / dot(e1, pv);
+    f3 tv = r.org - v0;
+    const float alpha = det_rcp * dot(tv, pv);
+    tv = cross(tv, e1);
+    const float beta = det_rcp * dot(tv, r.dir);
+    const float dist = det_rcp * dot(e2, tv);
+    if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps && dist < r.t) {
+        r.t = dist;
+        r.best = p;
+        return kAny;
+    }
+    return false;
+}
+
+// the step after a cell's candidates are exhausted; false when the ray ends
+template <bool kAny>
+__device__ __forceinline__ bool advance_uniform(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+{
+    const int d = min_dim(r.tx, r.ty, r.tz);
+    if (kAny && r.t < r.tterm) return false;
+    bool flag;
+    if (d == 0) {
+        flag = r.t > r.tx;
+        r.cx += r.rcp.x > 0.f ? 1 : -1;
+        r.tx += G.cs.x * fabsf(r.rcp.x);
+        flag = flag && r.cx < G.rx && r.cx > -1;
+    } else if (d == 1) {
+        flag = r.t > r.ty;
+        r.cy += r.rcp.y > 0.f ? 1 : -1;
+        r.ty += G.cs.y * fabsf(r.rcp.y);
+        flag = flag && r.cy < G.ry && r.cy > -1;
+    } else {
+        flag = r.t > r.tz;
+        r.cz += r.rcp.z > 0.f ? 1 : -1;
+        r.tz += G.cs.z * fabsf(r.rcp.z);
+        flag = flag && r.cz < G.rz && r.cz > -1;
+    }
+    if (!flag) return false;
+    uniform_load(G, S, r, wk);
+    return true;
+}
+
+template <bool kAny>
+__device__ __forceinline__ bool advance_two_level(const GridRegs& G, const TraceScene& S, RayState& r,
+                                                  Work& wk)
+{
+    if (r.in_leaf) {
+        // end of a leaf's candidates (TLGridTraverser.h:209-286)
+        if (kAny && r.t < r.tterm) return false;
+        const int d = min_dim(r.ux, r.uy, r.uz);
+        const int r0 = (int)(r.cell & 0xFFu), r1 = (int)((r.cell >> 8) & 0xFFu),
+                  r2 = (int)((r.cell >> 16) & 0xFFu);
+        bool second;
+        if (d == 0) {
+            if (r.t < r.ux) return false;
+            r.lx += r.rcp.x > 0.f ? 1 : -1;
+            r.ux += r.ddx;
+            second = r.lx != r0 && r.lx != -1;
+        } else if (d == 1) {
+            if (r.t < r.uy) return false;
+            r.ly += r.rcp.y > 0.f ? 1 : -1;
+            r.uy += r.ddy;
+            second = r.ly != r1 && r.ly != -1;
+        } else {
+            if (r.t < r.uz) return false;
+            r.lz += r.rcp.z > 0.f ? 1 : -1;
+            r.uz += r.ddz;
+            second = r.lz != r2 && r.lz != -1;
+        }
+        if (second) {
+            leaf_load(S, r, wk);
+            return true;
+        }
+        r.in_leaf = false;
+    }
+    // next top cell (TLGridTraverser.h:292-347)
+    const int d = min_dim(r.tx, r.ty, r.tz);
+    bool flag;
+    if (d == 0) {
+        flag = r.t > r.tx;
+        r.tentry = r.tx;
+        r.cx += r.rcp.x > 0.f ? 1 : -1;
+        r.tx += G.cs.x * fabsf(r.rcp.x);
+        flag = flag && r.cx != G.rx && r.cx != -1;
+    } else if (d == 1) {
+        flag = r.t > r.ty;
+        r.tentry = r.ty;
+        r.cy += r.rcp.y > 0.f ? 1 : -1;
+        r.ty += G.cs.y * fabsf(r.rcp.y);
+        flag = flag && r.cy != G.ry && r.cy != -1;
+    } else {
+        flag = r.t > r.tz;
+        r.tentry = r.tz;
+        r.cz += r.rcp.z > 0.f ? 1 : -1;
+        r.tz += G.cs.z * fabsf(r.rcp.z);
+        flag = flag && r.cz != G.rz && r.cz != -1;
+    }
+    if (!flag) return false;
+    top_visit(G, S, r, wk);
+    return true;
+}
+
+// set up the DDA of a generated ray; false when it does not enter the grid
+template <bool kTwo, bool kAny>
+__device__ __forceinline__ bool ray_begin(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+{
+    r.best = S.nt;
+    r.it = r.end = 0u;
+    r.in_leaf = false;
+    bool flag = !(kAny && r.t < r.tterm);
+    flag = dda_setup<!kTwo>(G, r.org, r.rcp, r.tentry, r.cx, r.cy, r.cz, r.tx, r.ty, r.tz) && flag;
+    if (!flag) return false;
+    if (kTwo) top_visit(G, S, r, wk);
+    else uniform_load(G, S, r, wk);
+    return true;
+}
+
+__device__ __forceinline__ void set_ray(RayState& r, f3 org, f3 d)
+{
+    r.org = org;
+    r.rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    r.dir = mk(1.f / r.rcp.x, 1.f / r.rcp.y, 1.f / r.rcp.z);
+}
+
+// ---------------------------------------------------------------------------
+// ray generators / sinks
+// ---------------------------------------------------------------------------
+
+struct GenPrimary {
+    CameraDev cam;
+    RenderArgs ra;
+    uint32_t npix;
+    float* hits;
+    uint32_t* hit_prims;
+
+    __device__ bool start(uint32_t j, RayState& r, const TraceScene&) const
+    {
+        const f3 d0 = primary_direction(cam, ra, j, npix);
+        set_ray(r, mk(cam.center[0], cam.center[1], cam.center[2]), d0);
+        r.t = kFltMax;
+        r.tterm = 0.f;
+        return true;
+    }
+    __device__ void finish(uint32_t j, const RayState& r, bool started, const TraceScene& S) const
+    {
+        (void)started;
+        const uint32_t prim = r.t < kFltMax ? r.best : S.nt;
+        const f3 hp = r.org + (r.t - kEps) * r.dir;
+        hits[j] = hp.x;
+        hits[(size_t)npix + j] = hp.y;
+        hits[2 * (size_t)npix + j] = hp.z;
+        hits[3 * (size_t)npix + j] = r.dir.x;
+        hits[4 * (size_t)npix + j] = r.dir.y;
+        hits[5 * (size_t)npix + j] = r.dir.z;
+        hits[6 * (size_t)npix + j] = r.t;
+        hit_prims[j] = prim;
+    }
+};
+
+struct GenAO {
+    RenderArgs ra;
+    uint32_t npix;
+    const float* hits;
+    const uint32_t* hit_prims;
+    uint8_t* occ;
+
+    // AmbientOcclusionRayGenerator::operator() (RayGenerators.h:281-343)
+    __device__ bool start(uint32_t j, RayState& r, const TraceScene& S) const
+    {
+        const uint32_t ns = (uint32_t)ra.ao_samples;
+        const uint32_t pix = j / ns, s = j % ns;
+        const uint32_t prim = hit_prims[pix];
+        const float th = hits[6 * (size_t)npix + pix];
+        if (!(th < kFltMax) || prim >= S.nt) return false;
+        f3 org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+        const f3 pdir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix],
+                           hits[5 * (size_t)npix + pix]);
+        f3 v0, v1, v2;
+        load_tri(S.rec, prim, v0, v1, v2);
+        f3 n = normalize(cross(v1 - v0, v2 - v0));
+        if (dot(n, pdir) > 0.f) n = -n;
+        float u[4];
+        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)ra.image, kStreamAO + s, u);
+        const f3 h = cosine_dir(u[0], u[1]);
+        f3 tg, bn;
+        local_frame(n, tg, bn);
+        const f3 dir = normalize(n * h.z + tg * h.x + bn * h.y);
+        org = org + (0.01f * ra.ao_dist) * dir;
+        set_ray(r, org, dir);
+        r.t = ra.ao_dist;
+        r.tterm = ra.ao_dist;
+        return true;
+    }
+    __device__ void finish(uint32_t j, const RayState& r, bool started, const TraceScene&) const
+    {
+        occ[j] = (!started || r.t < ra.ao_dist) ? 1u : 0u;
+    }
+};
+
+struct GenBatch {
+    const float* org;
+    const float* dir;
+    uint32_t n;
+    float tmax;
+    int any;
+    float* t_out;
+    uint32_t* prim_out;
+    uint8_t* occ_out;
+
+    __device__ bool start(uint32_t j, RayState& r, const TraceScene&) const
+    {
+        set_ray(r, mk(org[j], org[(size_t)n + j], org[2 * (size_t)n + j]),
+                mk(dir[j], dir[(size_t)n + j], dir[2 * (size_t)n + j]));
+        r.t = any ? tmax : kFltMax;
+        r.tterm = tmax;
+        return true;
+    }
+    __device__ void finish(uint32_t j, const RayState& r, bool, const TraceScene& S) const
+    {
+        if (any) {
+            occ_out[j] = r.t < tmax ? 1u : 0u;
+            if (t_out) t_out[j] = r.t;
+        } else {
+            t_out[j] = r.t;
+            prim_out[j] = r.t < kFltMax ? r.best : S.nt;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// the persistent kernel
+// ---------------------------------------------------------------------------
+
+template <bool kTwo, bool kAny, class Gen>
+__global__ __launch_bounds__(kBlock) void k_persist(TraceScene S, Gen gen, uint32_t nrays,
+                                                    uint32_t* __restrict__ pool, uint32_t refill,
+                                                    unsigned long long* ray_count,
+                                                    unsigned long long* stats)
+{
+    const GridRegs G = load_grid(S.gp);
+    const uint64_t lt = __lanemask_lt();
+    RayState r;
+    Work wk;
+    uint32_t j = 0, started = 0;
+    bool live = false;
+    bool exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill || idle == ~0ull)) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(pool, nidle);
+            base = __shfl(base, 0, kWave);
+            if (base + nidle >= nrays) exhausted = true;
+            if (!live) {
+                j = base + (uint32_t)__popcll(idle & lt);
+                if (j < nrays) {
+                    const bool ok = gen.start(j, r, S);
+                    started += ok ? 1u : 0u;
+                    live = ok && ray_begin<kTwo, kAny>(G, S, r, wk);
+                    if (!live) gen.finish(j, r, ok, S);
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (live) {
+            bool done;
+            if (r.it < r.end) done = test_one<kAny>(S, r, wk);
+            else done = kTwo ? !advance_two_level<kAny>(G, S, r, wk) : !advance_uniform<kAny>(G, S, r, wk);
+            if (done) {
+                live = false;
+                gen.finish(j, r, true, S);
+            }
+        }
+    }
+    if (ray_count) {
+        unsigned long long c = started;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if (ray_count && __lane_id() == 0 && c) atomicAdd(ray_count, c);
+    }
+    add_work(stats, started, wk);
+}
+
+template <class K>
+uint32_t persistent_blocks(K kernel)
+{
+    static uint32_t cached = 0;
+    if (cached) return cached;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0);
+    if (cus <= 0) cus = 256;
+    if (per <= 0) per = 1;
+    cached = (uint32_t)(cus * per);
+    return cached;
+}
+
+template <bool kTwo, bool kAny, class Gen>
+void launch(const TraceScene& S, const Gen& gen, uint32_t nrays, uint32_t* pool, uint32_t refill,
+            unsigned long long* ray_count, unsigned long long* stats, hipStream_t s)
+{
+    if (nrays == 0) return;
+    fill_u32(pool, 0u, 1, nullptr, 1, s);
+    auto kern = k_persist<kTwo, kAny, Gen>;
+    uint32_t blocks = persistent_blocks(kern);
+    const uint32_t need = (nrays + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, S, gen, nrays, pool, refill, ray_count,
+                       stats);
+}
+
+}  // namespace
+
+void persist_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
+                     uint32_t* hit_prims, uint32_t* pool, uint32_t refill, unsigned long long* ray_count,
+                     hipStream_t s)
+{
+    const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
+    GenPrimary g{ cam, ra, npix, hits, hit_prims };
+    if (S.two_level) launch<true, false>(S, g, npix, pool, refill, ray_count, ra.stats, s);
+    else launch<false, false>(S, g, npix, pool, refill, ray_count, ra.stats, s);
+}
+
+void persist_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                const uint32_t* hit_prims, uint8_t* occ, uint32_t* pool, uint32_t refill,
+                unsigned long long* ray_count, hipStream_t s)
+{
+    GenAO g{ ra, npix, hits, hit_prims, occ };
+    const uint32_t n = npix * (uint32_t)ra.ao_samples;
+    if (S.two_level) launch<true, true>(S, g, n, pool, refill, ray_count, ra.stats, s);
+    else launch<false, true>(S, g, n, pool, refill, ray_count, ra.stats, s);
+}
+
+void persist_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
+                   float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s)
+{
+    GenBatch g{ org, dir, n, tmax, anyhit, t_out, prim_out, occ_out };
+    if (S.two_level) {
+        if (anyhit) launch<true, true>(S, g, n, pool, refill, ray_count, stats, s);
+        else launch<true, false>(S, g, n, pool, refill, ray_count, stats, s);
+    } else {
+        if (anyhit) launch<false, true>(S, g, n, pool, refill, ray_count, stats, s);
+        else launch<false, false>(S, g, n, pool, refill, ray_count, stats, s);
+    }
+}
+
+}  // namespace hr

@@ -205,6 +205,15 @@ class Engine:
         check(self._L.hr_get_timing(self._h, C.byref(t)), "hr_get_timing")
         return t
 
+    def set_tuning(self, persistent_primary=0, persistent_secondary=1, refill=48):
+        t = _lib.Tuning(int(persistent_primary), int(persistent_secondary), int(refill))
+        check(self._L.hr_set_tuning(self._h, C.byref(t)), "hr_set_tuning")
+
+    def tuning(self):
+        t = _lib.Tuning()
+        check(self._L.hr_get_tuning(self._h, C.byref(t)), "hr_get_tuning")
+        return t
+
     def work_stats(self):
         w = _lib.WorkStats()
         check(self._L.hr_get_work_stats(self._h, C.byref(w)), "hr_get_work_stats")

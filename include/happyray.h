@@ -172,6 +172,18 @@ hr_status hr_occluded(hr_ctx* ctx, const float* org, const float* dir, uint32_t 
 hr_status hr_trace_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n,
                        float* d_t_out, uint32_t* d_prim_out);
 
+/* Kernel choices of a context (no reference counterpart: the reference fixes
+ * its launch shapes at compile time, src/RT/Algorithm/RayTracingKernels.h:34-43). */
+typedef struct {
+    int persistent_primary;    /* camera rays: 0 (default) one ray per lane, 1 persistent
+                                  traversal with dynamic ray fetch                       */
+    int persistent_secondary;  /* AO rays and ray batches: 1 (default) persistent, 0 not */
+    int refill;                /* idle lanes of a 64-lane wave that trigger a fetch,
+                                  1..64 (default 48)                                     */
+} hr_tuning;
+hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
+hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);
+
 hr_status hr_get_timing(hr_ctx* ctx, hr_timing* out);
 /* Read and clear the work counters. */
 hr_status hr_get_work_stats(hr_ctx* ctx, hr_work_stats* out);

@@ -157,3 +157,54 @@ def test_keyframe_interpolation_parity():
         og = O.OGrid(m, bmin, bmax, O.GRID_TWO_LEVEL, 1.0 / 16, 3.0)
         assert_grid_equal(e, og)
     e.close()
+
+
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 1.0 / 16, 3.0)])
+def test_persistent_matches_one_ray_per_lane(cluster, kind, lt, ll):
+    """The persistent kernels (dynamic ray fetch) and the one-ray-per-lane
+    kernels apply the same operations to every ray: bit-identical outputs and
+    identical work counters, for several refill thresholds."""
+    print(f"persistent vs one-ray-per-lane kind={kind}", flush=True)
+    e = gpu_engine(cluster, kind, lt, ll)
+    e.set_camera_dict(cluster.camera)
+    org, d = random_rays(cluster, 30000, seed=17)
+    out = {}
+    for pers, refill in ((0, 16), (1, 1), (1, 16), (1, 64)):
+        e.set_tuning(pers, pers, refill)
+        e.work_stats()
+        img = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0, collect_stats=True)
+        ws = e.work_stats()
+        occ = e.ao_occlusion(4)
+        planes, prims = e.primary_hits()
+        t, p = e.trace(org, d)
+        o = e.occluded(org, d, 1.5)
+        out[(pers, refill)] = (img, occ, planes, prims, t, p, o,
+                               (ws.primary.rays, ws.primary.steps, ws.primary.tests,
+                                ws.secondary.rays, ws.secondary.steps, ws.secondary.tests))
+    ref = out[(0, 16)]
+    for key, val in out.items():
+        for a, b in zip(val[:7], ref[:7]):
+            np.testing.assert_array_equal(a, b, err_msg=str(key))
+        assert val[7] == ref[7], (key, val[7], ref[7])
+    e.close()
+
+
+def test_work_counters_match_oracle(cluster):
+    """DDA steps and triangle tests counted by the kernels equal the oracle's."""
+    print("work counters vs oracle", flush=True)
+    kind, lt, ll = GRID_TWO_LEVEL, 1.0 / 16, 3.0
+    e = gpu_engine(cluster, kind, lt, ll)
+    e.set_camera_dict(cluster.camera)
+    e.work_stats()
+    e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0, collect_stats=True)
+    ws = e.work_stats()
+    og = oracle_grid(cluster, kind, lt, ll)
+    cam = O.camera(**{k: cluster.camera[k] for k in ("pos", "dir", "up", "fov")},
+                   res_x=cluster.camera["res"][0], res_y=cluster.camera["res"][1])
+    p = O.params(seed=SEED, jitter=1, ao_samples=4, ao_dist=2.0, nthreads=8)
+    _, _, sp = O.primary(og, cam, p)
+    _, _, sa = O.render_ao(og, cam, p, cluster.materials, cluster.mat_idx)
+    assert (ws.primary.rays, ws.primary.steps, ws.primary.tests) == (sp.rays, sp.steps, sp.tests)
+    assert (ws.secondary.rays, ws.secondary.steps, ws.secondary.tests) == \
+        (sa.rays - sp.rays, sa.steps - sp.steps, sa.tests - sp.tests)
+    e.close()
