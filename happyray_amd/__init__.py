@@ -6,6 +6,7 @@ tracing, behind the C ABI in include/happyray.h.
 from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH,
                    HappyRayError, LIB_PATH, load)
 from .engine import Engine
+from .scenefile import SceneFile
 
-__all__ = ["Engine", "HappyRayError", "load", "LIB_PATH", "GRID_UNIFORM", "GRID_UNIFORM_EXACT",
+__all__ = ["Engine", "SceneFile", "HappyRayError", "load", "LIB_PATH", "GRID_UNIFORM", "GRID_UNIFORM_EXACT",
            "GRID_TWO_LEVEL", "MODE_AO", "MODE_PATH"]

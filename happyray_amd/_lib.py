@@ -59,6 +59,14 @@ class WorkStats(C.Structure):
     _fields_ = [("primary", WorkCounts), ("secondary", WorkCounts)]
 
 
+class SceneInfo(C.Structure):
+    _fields_ = [("num_frames", C.c_uint32), ("num_vertices", C.c_uint32), ("num_normals", C.c_uint32),
+                ("num_triangles", C.c_uint32), ("num_materials", C.c_uint32),
+                ("num_lights", C.c_uint32), ("frame_step", C.c_float), ("has_camera", C.c_int),
+                ("cam_pos", C.c_float * 3), ("cam_dir", C.c_float * 3), ("cam_up", C.c_float * 3),
+                ("cam_fov", C.c_float), ("cam_res_x", C.c_int), ("cam_res_y", C.c_int)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "hr_last_error": (C.c_char_p, []),
@@ -86,6 +94,19 @@ SIGNATURES = {
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
     "hr_get_work_stats": (C.c_int, [_vp, C.POINTER(WorkStats)]),
     "hr_synchronize": (C.c_int, [_vp]),
+    "hr_set_normals": (C.c_int, [_vp, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint32]),
+    "hr_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "hr_scene_load_obj": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "hr_scene_free": (C.c_int, [_vp]),
+    "hr_scene_get_info": (C.c_int, [_vp, C.POINTER(SceneInfo)]),
+    "hr_scene_get_vertices": (C.c_int, [_vp, C.c_uint32, _f32p]),
+    "hr_scene_get_normals": (C.c_int, [_vp, C.c_uint32, _f32p]),
+    "hr_scene_get_triangles": (C.c_int, [_vp, _u32p, _u32p, _u32p]),
+    "hr_scene_get_materials": (C.c_int, [_vp, _f32p]),
+    "hr_scene_get_lights": (C.c_int, [_vp, _f32p, _f32p]),
+    "hr_scene_upload": (C.c_int, [_vp, _vp]),
+    "hr_scene_frame": (C.c_int, [_vp, C.c_float, _u32p, _u32p, _f32p]),
+    "hr_scene_next_frame": (C.c_int, [_vp, _f32p]),
 }
 
 _lib = None

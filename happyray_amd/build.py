@@ -18,6 +18,7 @@ LIB = os.path.join(HERE, "libhappyray.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_persist.hip", "hr_engine.hip"]
+CXX_SOURCES = ["hr_scene.cpp"]          # host-only C++ (scene files)
 ARCH = os.environ.get("HR_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -36,19 +37,25 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-I", INCLUDE, "-I", CSRC]
+
+
 def _compile(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc(), "-c", "-x", "hip", os.path.join(CSRC, src), "-o", out] + FLAGS
+    if src.endswith(".cpp"):
+        cmd = [os.environ.get("CXX", "g++"), "-c", os.path.join(CSRC, src), "-o", out] + CXX_FLAGS
+    else:
+        cmd = [_hipcc(), "-c", "-x", "hip", os.path.join(CSRC, src), "-o", out] + FLAGS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"compile failed for {src}:\n{r.stdout}\n{r.stderr}")
     return out, r.stderr
 
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
-        results = list(ex.map(_compile, SOURCES))
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        results = list(ex.map(_compile, SOURCES + CXX_SOURCES))
     objs = [o for o, _ in results]
     if verbose:
         for _, err in results:

@@ -18,6 +18,7 @@
 
 #include "../../include/happyray.h"
 #include "hr_build.h"
+#include "hr_error.h"
 #include "hr_persist.h"
 #include "hr_trace.h"
 
@@ -74,6 +75,8 @@ struct DevBuf {
 
 }  // namespace
 
+hr_status hr::set_error(hr_status s, const std::string& msg) { return fail(s, msg); }
+
 struct hr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -84,9 +87,18 @@ struct hr_ctx {
     DevBuf<float> key_a, key_b;       // hr_upload_mesh sources
     DevBuf<float> keyframes;          // hr_set_keyframes
     uint32_t nframes = 0;
+    uint32_t geo_frames = 0;          // key frames of the geometry (2 after hr_upload_mesh with two)
+    uint32_t k1 = 0, k2 = 0;          // key frames of the current frame
     const float* src1 = nullptr;
     const float* src2 = nullptr;
     float coeff = 1.f;
+
+    // per-corner normals
+    DevBuf<float> normal_keys;        // nframes_n blocks of SoA xyz
+    DevBuf<uint32_t> nidx;
+    DevBuf<float4> normals;           // current frame
+    uint32_t nn = 0, nframes_n = 0;
+    bool has_normals = false, normals_dirty = false;
     DevBuf<float> verts;
     DevBuf<uint32_t> idx;
 
@@ -419,6 +431,9 @@ hr_status hr_destroy(hr_ctx* c)
     if (!c) return HR_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->normals.release();
+    c->nidx.release();
+    c->normal_keys.release();
     DevBuf<float>* fbufs[] = { &c->key_a, &c->key_b, &c->keyframes, &c->verts, &c->mats, &c->lights,
                                &c->light_cdf, &c->hits, &c->fb, &c->ray_org, &c->ray_dir, &c->ray_t };
     for (auto* b : fbufs) b->release();
@@ -474,6 +489,10 @@ hr_status hr_upload_mesh(hr_ctx* c, const float* xyz1, const float* xyz2, float 
     }
     c->src1 = c->key_a.p;
     c->coeff = coeff;
+    c->geo_frames = c->src2 == c->src1 ? 1u : 2u;
+    c->k1 = 0;
+    c->k2 = c->geo_frames - 1u;
+    c->has_normals = false;
     HR_TRY(c->verts.reserve(3 * (size_t)nv));
     HR_TRY(c->idx.reserve(3 * (size_t)std::max(nt, 1u)));
     HR_TRY(upload(c->idx.p, idx, 3 * (size_t)nt * sizeof(uint32_t), c->stream));
@@ -518,6 +537,9 @@ hr_status hr_set_keyframes(hr_ctx* c, const float* xyz, uint32_t nframes, uint32
     HR_TRY(c->idx.reserve(3 * (size_t)std::max(nt, 1u)));
     HR_TRY(upload(c->idx.p, idx, 3 * (size_t)nt * sizeof(uint32_t), c->stream));
     c->nframes = nframes;
+    c->geo_frames = nframes;
+    c->k1 = c->k2 = 0;
+    c->has_normals = false;
     c->nv = nv;
     c->nt = nt;
     c->src1 = c->src2 = c->keyframes.p;
@@ -535,7 +557,35 @@ hr_status hr_set_frame(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff)
     c->src1 = c->keyframes.p + (size_t)k1 * 3 * c->nv;
     c->src2 = c->keyframes.p + (size_t)k2 * 3 * c->nv;
     c->coeff = coeff;
+    c->k1 = k1;
+    c->k2 = k2;
+    c->normals_dirty = true;
     c->built = false;
+    return HR_OK;
+}
+
+hr_status hr_set_normals(hr_ctx* c, const float* normals, uint32_t nframes, uint32_t nn, const uint32_t* nidx,
+                         uint32_t nt)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->src1) return fail(HR_ERR_STATE, "normals before geometry");
+    if (!normals || !nidx || nn == 0) return fail(HR_ERR_ARG, "empty normals");
+    if (nt != c->nt) return fail(HR_ERR_ARG, "normal index count differs from the triangle count");
+    if (nframes != 1 && nframes != c->geo_frames)
+        return fail(HR_ERR_ARG, "normals need 1 block or one per geometry key frame");
+    for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+        if (nidx[i] >= nn) return fail(HR_ERR_ARG, "normal index out of range");
+    const size_t n = (size_t)nframes * 3 * nn;
+    HR_TRY(c->normal_keys.reserve(n));
+    HR_TRY(c->nidx.reserve(3 * (size_t)std::max(nt, 1u)));
+    HR_TRY(c->normals.reserve(nn));
+    HR_TRY(upload(c->normal_keys.p, normals, n * sizeof(float), c->stream));
+    HR_TRY(upload(c->nidx.p, nidx, 3 * (size_t)nt * sizeof(uint32_t), c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    c->nn = nn;
+    c->nframes_n = nframes;
+    c->has_normals = true;
+    c->normals_dirty = true;
     return HR_OK;
 }
 
@@ -659,6 +709,19 @@ hr_status hr_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_param
     sh.lights = c->lights.p;
     sh.light_cdf = c->light_cdf.p;
     sh.nlights = c->nlights;
+    sh.normals = nullptr;
+    sh.nidx = nullptr;
+    if (mode == HR_MODE_PATH && c->has_normals) {
+        if (c->normals_dirty) {
+            const size_t blk = 3 * (size_t)c->nn;
+            const float* a = c->normal_keys.p + (c->nframes_n == 1 ? 0 : (size_t)c->k1 * blk);
+            const float* b = c->normal_keys.p + (c->nframes_n == 1 ? 0 : (size_t)c->k2 * blk);
+            hr::launch_frame_normals(a, b, c->nn, c->normals.p, c->stream);
+            c->normals_dirty = false;
+        }
+        sh.normals = c->normals.p;
+        sh.nidx = c->nidx.p;
+    }
     HR_HIP(hipEventRecord(c->ev_r0, c->stream));
     const bool pers = c->tuning.persistent_secondary != 0;
     const uint32_t refill = (uint32_t)c->tuning.refill;

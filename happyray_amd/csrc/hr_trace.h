@@ -38,6 +38,8 @@ struct Shading {
     const float* lights;        // 15 floats per AreaLightSource
     const float* light_cdf;     // inclusive sum of light weights
     int nlights;
+    const float4* normals;      // current-frame normals (xyz, 0), nullptr -> geometric
+    const uint32_t* nidx;       // 3 normal indices per triangle
 };
 
 // hit buffer: 7 SoA planes of npix floats (hit point xyz, dir xyz, t) + prims
@@ -51,6 +53,10 @@ void launch_ao_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, c
 void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                const uint32_t* hit_prims, const Shading& sh, float* fb,
                unsigned long long* ray_count, hipStream_t s);
+// current-frame normals: a * (1 - 0) + b * 0 per component, as the reference
+// uploads normals with coefficient 0 (src/RT/RTEngine.cu:209-210); a, b are
+// SoA key-frame blocks of nn normals
+void launch_frame_normals(const float* a, const float* b, uint32_t nn, float4* out, hipStream_t s);
 void launch_trace_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n,
                         int anyhit, float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out,
                         unsigned long long* ray_count, unsigned long long* stats, hipStream_t s);

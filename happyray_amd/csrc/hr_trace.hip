@@ -13,6 +13,7 @@
 //   AmbientOcclusionRayGenerator     src/RT/Algorithm/RayGenerators.h:273-344
 //   computeAOIllumination            src/RT/Integrator/AOIntegrator.h:64-191
 //   pathTrace                        src/RT/Integrator/PathTracer.h:41-372
+//   uploadObjFrameNormalData         src/Application/WFObjectUploader.h:121-165
 //
 // Departures, shared with the CPU oracle: the counter-based RNG keyed by
 // (pixel, image, stream); any-hit rays start at t = tmax and stop at the
@@ -32,6 +33,24 @@ namespace {
 
 constexpr int kBlock = 256;
 using namespace trav;
+
+// Shading normal of the path tracer (PathTracer.h:109-135): barycentric
+// weights from the sub-triangle areas seen from the hit point, then the
+// normalized blend of the corner normals; the geometric normal when the
+// scene has no normals.
+__device__ __forceinline__ f3 shading_normal(const Shading& sh, uint32_t prim, f3 org, f3 v0, f3 v1, f3 v2)
+{
+    const f3 real_normal = cross(v1 - v0, v2 - v0);
+    if (!sh.nidx) return normalize(real_normal);
+    const f3 a = v0 - org, b = v1 - org, c = v2 - org;
+    const f3 n0 = cross(b, c), n1 = cross(c, a);
+    const float rcp = 1.f / sqrtf(dot(real_normal, real_normal));
+    const float u = len(n0) * rcp, v = len(n1) * rcp;
+    const float4 c0 = sh.normals[sh.nidx[3 * (size_t)prim]];
+    const float4 c1 = sh.normals[sh.nidx[3 * (size_t)prim + 1]];
+    const float4 c2 = sh.normals[sh.nidx[3 * (size_t)prim + 2]];
+    return normalize(u * mk(c0.x, c0.y, c0.z) + v * mk(c1.x, c1.y, c1.z) + (1.f - u - v) * mk(c2.x, c2.y, c2.z));
+}
 
 // ---------------------------------------------------------------------------
 // primary rays
@@ -201,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
                   w);
             f3 v0, v1, v2;
             load_tri(S.rec, prim, v0, v1, v2);
-            f3 n = normalize(cross(v1 - v0, v2 - v0));
+            f3 n = shading_normal(sh, prim, org, v0, v1, v2);
             float cos_ne = -dot(n, dir);
             const float* m = sh.mats + 12 * (size_t)(sh.mat_idx ? sh.mat_idx[prim] : 0u);
             const f3 kd = mk(m[0], m[1], m[2]);
@@ -331,6 +350,23 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
 }
 
 // ---------------------------------------------------------------------------
+// per-corner normals
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_frame_normals(const float* __restrict__ a,
+                                                          const float* __restrict__ b, uint32_t nn,
+                                                          float4* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const float c = 0.f;
+    const float x = a[i] * (1.f - c) + b[i] * c;
+    const float y = a[(size_t)nn + i] * (1.f - c) + b[(size_t)nn + i] * c;
+    const float z = a[2 * (size_t)nn + i] * (1.f - c) + b[2 * (size_t)nn + i] * c;
+    out[i] = make_float4(x, y, z, 0.f);
+}
+
+// ---------------------------------------------------------------------------
 // ray batches (trace<> with RayLoader, RayGenerators.h:347-364)
 // ---------------------------------------------------------------------------
 
@@ -410,6 +446,12 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
     else
         hipLaunchKernelGGL(k_pt<false>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits,
                            hit_prims, sh, fb, ray_count);
+}
+
+void launch_frame_normals(const float* a, const float* b, uint32_t nn, float4* out, hipStream_t s)
+{
+    if (nn == 0) return;
+    hipLaunchKernelGGL(k_frame_normals, dim3(blocks_for(nn)), dim3(kBlock), 0, s, a, b, nn, out);
 }
 
 void launch_trace_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n,

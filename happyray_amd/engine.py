@@ -106,6 +106,23 @@ class Engine:
     def set_frame(self, k1, k2, coeff):
         check(self._L.hr_set_frame(self._h, k1, k2, float(coeff)), "hr_set_frame")
 
+    def set_normals(self, normals, nidx):
+        """normals: (3, nn) or (frames, 3, nn) float32 SoA; nidx: (nt, 3) normal indices."""
+        n = np.ascontiguousarray(normals, np.float32)
+        if n.ndim == 2:
+            n = n[None]
+        ni = np.ascontiguousarray(nidx, np.uint32).reshape(-1)
+        check(self._L.hr_set_normals(self._h, _f(n), n.shape[0], n.shape[2], _u(ni), len(ni) // 3),
+              "hr_set_normals")
+
+    def load_scene(self, scene_file):
+        """Make a parsed SceneFile this engine's geometry (hr_scene_upload)."""
+        check(self._L.hr_scene_upload(self._h, scene_file.handle), "hr_scene_upload")
+        self._nt = scene_file.info.num_triangles
+        if scene_file.info.has_camera:
+            i = scene_file.info
+            self._res = (i.cam_res_x, i.cam_res_y)
+
     # -- build ------------------------------------------------------------
     def set_grid(self, kind, lambda_top, lambda_leaf=2.2):
         check(self._L.hr_set_grid(self._h, kind, float(lambda_top), float(lambda_leaf)),

@@ -133,6 +133,18 @@ hr_status hr_set_keyframes(hr_ctx* ctx, const float* xyz, uint32_t nframes, uint
 /* Interpolate resident key frames k1, k2 with coefficient c on the GPU
  * (AnimationManager::getFrameId / getNextFrameId / getInterpolationCoefficient). */
 hr_status hr_set_frame(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coeff);
+/* Per-corner vertex normals (ObjUploader::uploadObjFrameNormalData /
+ * uploadObjFrameNormalIndexData, src/Application/WFObjectUploader.h:121-252, called
+ * from src/RT/RTEngine.cu:209-213): normals holds nframes blocks of SoA xyz (nn per
+ * block), nidx three normal indices per triangle.  nframes is 1 (one block for every
+ * frame) or the number of geometry key frames (2 after hr_upload_mesh with two
+ * frames).  The path tracer interpolates them at the hit point
+ * (src/RT/Integrator/PathTracer.h:109-135); the normals used are those of the first
+ * key frame of the current frame, as the reference uploads them with coefficient 0.
+ * Without normals the path tracer shades with the geometric normal.  Uploading new
+ * geometry drops the normals. */
+hr_status hr_set_normals(hr_ctx* ctx, const float* normals, uint32_t nframes, uint32_t nn,
+                         const uint32_t* nidx, uint32_t nt);
 
 /* ---- build: setGridDensities + buildAccStruct (src/RT/RTEngine.cu:354-368, :449-453) ---- */
 hr_status hr_set_grid(hr_ctx* ctx, hr_grid_kind kind, float lambda_top, float lambda_leaf);
@@ -188,6 +200,53 @@ hr_status hr_get_timing(hr_ctx* ctx, hr_timing* out);
 /* Read and clear the work counters. */
 hr_status hr_get_work_stats(hr_ctx* ctx, hr_work_stats* out);
 hr_status hr_synchronize(hr_ctx* ctx);
+
+/* ---- scene files: SceneLoader::loadScene (src/Application/SceneLoader.cpp:212-332) ----
+ * hr_scene_load parses a scene.cfg (src/Application/SceneConfigurationReader.cpp:29-190)
+ * and the files it names: OBJ/MTL key frames (src/Application/WFObjectReader.cpp), a
+ * camera.cfg (src/Application/CameraManager.cpp:96-168) and a light.cfg
+ * (src/Application/LightSourceLoader.cpp:29-100).  Without a light file, area lights
+ * are derived from emissive quads of the first key frame, or one default light is
+ * placed above the scene (SceneLoader.cpp:94-210).  Relative paths are tried against
+ * the working directory first, then against the directory of the scene.cfg.
+ * Everything stays in host memory until hr_scene_upload; no GPU is needed to parse.
+ * All key frames must have the same vertex, normal and triangle counts. */
+typedef struct hr_scene hr_scene;
+typedef struct {
+    uint32_t num_frames;          /* key frames                                         */
+    uint32_t num_vertices, num_normals, num_triangles, num_materials;
+    uint32_t num_lights;
+    float frame_step;             /* AnimationManager step size (scene.cfg "frame step") */
+    int has_camera;               /* 1 when the scene.cfg names a camera file            */
+    float cam_pos[3], cam_dir[3], cam_up[3];   /* after the camera file's rotation      */
+    float cam_fov;
+    int cam_res_x, cam_res_y;
+} hr_scene_info;
+hr_status hr_scene_load(const char* cfg_path, hr_scene** out);
+/* one OBJ file as a static scene (AnimationManager::read(const char*) + light setup) */
+hr_status hr_scene_load_obj(const char* obj_path, hr_scene** out);
+hr_status hr_scene_free(hr_scene* scene);
+hr_status hr_scene_get_info(const hr_scene* scene, hr_scene_info* out);
+/* SoA xyz of key frame `frame`: 3*num_vertices floats (vertices) or 3*num_normals
+ * floats (normals) */
+hr_status hr_scene_get_vertices(const hr_scene* scene, uint32_t frame, float* xyz);
+hr_status hr_scene_get_normals(const hr_scene* scene, uint32_t frame, float* xyz);
+/* 3 vertex indices, 3 normal indices and 1 material index per triangle; any may be NULL */
+hr_status hr_scene_get_triangles(const hr_scene* scene, uint32_t* idx, uint32_t* nidx,
+                                 uint32_t* mat_idx);
+/* PhongMaterial rows after Material::setupPhongCoefficients
+ * (src/Application/WFObjectReader.cpp:49-57): 12 floats per material */
+hr_status hr_scene_get_materials(const hr_scene* scene, float* mats);
+/* area lights and their sampling weights (area * |intensity|) */
+hr_status hr_scene_get_lights(const hr_scene* scene, hr_area_light* lights, float* weights);
+/* Make the scene a context's geometry: key frames resident (frame 0 current),
+ * normals, materials, lights and, when the scene has one, the camera. */
+hr_status hr_scene_upload(hr_ctx* ctx, const hr_scene* scene);
+/* AnimationManager frame arithmetic (src/Application/AnimationManager.hpp:68-112):
+ * the key frames and coefficient of animation time `current`, and the next time. */
+hr_status hr_scene_frame(const hr_scene* scene, float current, uint32_t* k1, uint32_t* k2,
+                         float* coeff);
+hr_status hr_scene_next_frame(const hr_scene* scene, float* current);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,12 @@ typedef struct {
     uint32_t nv;
     const uint32_t* idx; /* 3 * nt vertex indices */
     uint32_t nt;
+    /* optional per-corner normals (NULL nidx: geometric normals) */
+    const float* nx;     /* SoA normal coordinates, nn each */
+    const float* ny;
+    const float* nz;
+    uint32_t nn;
+    const uint32_t* nidx; /* 3 * nt normal indices */
 } or_mesh;
 
 typedef struct {
@@ -123,6 +129,10 @@ void or_prim_cell_range(const or_mesh* m, uint32_t prim, const or_grid* g,
                         int lo[3], int hi[3]);
 int  or_tri_box_overlap(const float v0[3], const float v1[3], const float v2[3],
                         const float center[3], const float half[3]);
+
+/* path-tracer shading normal of primitive prim seen from hit point org
+ * (PathTracer.h:109-135), exposed for tests */
+void or_shading_normal(const or_mesh* mesh, uint32_t prim, const float org[3], float out[3]);
 
 /* ---- traversal (or_trace.c) ---- */
 int  or_trace(const or_mesh* mesh, const or_grid* g,

@@ -74,6 +74,25 @@ static inline void tri_verts(const or_mesh* m, uint32_t p, f3* v0, f3* v1, f3* v
     *v2 = mk3(m->vx[i2], m->vy[i2], m->vz[i2]);
 }
 
+/* Shading normal of the path tracer (src/RT/Integrator/PathTracer.h:109-135):
+ * barycentric weights u, v from the sub-triangle areas seen from the hit
+ * point org, then the normalized blend u*n0 + v*n1 + (1-u-v)*n2 of the corner
+ * normals.  Meshes without normals use the geometric normal. */
+static inline f3 pt_shading_normal(const or_mesh* m, uint32_t p, f3 org, f3 v0, f3 v1, f3 v2)
+{
+    const f3 real_normal = cross3(sub3(v1, v0), sub3(v2, v0));
+    if (!m->nidx) return norm3(real_normal);
+    const f3 a = sub3(v0, org), b = sub3(v1, org), c = sub3(v2, org);
+    const f3 n0 = cross3(b, c), n1 = cross3(c, a);
+    const float rcp = lenrcp3(real_normal);
+    const float u = len3(n0) * rcp, v = len3(n1) * rcp;
+    const uint32_t i0 = m->nidx[3 * p], i1 = m->nidx[3 * p + 1], i2 = m->nidx[3 * p + 2];
+    const f3 c0 = mk3(m->nx[i0], m->ny[i0], m->nz[i0]);
+    const f3 c1 = mk3(m->nx[i1], m->ny[i1], m->nz[i1]);
+    const f3 c2 = mk3(m->nx[i2], m->ny[i2], m->nz[i2]);
+    return norm3(add3(add3(smul3(u, c0), smul3(v, c1)), smul3(1.f - u - v, c2)));
+}
+
 /* BBoxExtractor<Triangle> (src/RT/Primitive/BBox.hpp:96-110) */
 static inline void tri_bbox(f3 v0, f3 v1, f3 v2, f3* lo, f3* hi)
 {

@@ -578,8 +578,7 @@ static void pt_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
                 or_rand4(q->p, pix, (uint32_t)q->image, OR_STREAM_PT + 2u * (uint32_t)depth + 1u, w);
                 f3 v0, v1, v2;
                 tri_verts(cx->m, prim, &v0, &v1, &v2);
-                const f3 real_normal = cross3(sub3(v1, v0), sub3(v2, v0));
-                f3 n = norm3(real_normal);
+                f3 n = pt_shading_normal(cx->m, prim, org, v0, v1, v2);
                 float cos_ne = -dot3(n, dir);
                 const or_material* mt = &q->mats[q->mat_idx ? q->mat_idx[prim] : 0u];
                 const f3 kd = ld3(mt->kd), ks = ld3(mt->ks), ke = ld3(mt->ke);
@@ -715,4 +714,12 @@ int or_render_pt(const or_mesh* m, const or_grid* g, const or_camera* cam,
     run_tiles(m, g, npix, p->nthreads, pt_tile, &a, stats);
     free(h); free(hp);
     return 0;
+}
+
+void or_shading_normal(const or_mesh* m, uint32_t prim, const float org[3], float out[3])
+{
+    f3 v0, v1, v2;
+    tri_verts(m, prim, &v0, &v1, &v2);
+    const f3 n = pt_shading_normal(m, prim, ld3(org), v0, v1, v2);
+    out[0] = n.x; out[1] = n.y; out[2] = n.z;
 }

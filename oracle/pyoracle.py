@@ -23,7 +23,8 @@ _u8p = C.POINTER(C.c_uint8)
 
 class Mesh(C.Structure):
     _fields_ = [("vx", _f32p), ("vy", _f32p), ("vz", _f32p), ("nv", C.c_uint32),
-                ("idx", _u32p), ("nt", C.c_uint32)]
+                ("idx", _u32p), ("nt", C.c_uint32), ("nx", _f32p), ("ny", _f32p),
+                ("nz", _f32p), ("nn", C.c_uint32), ("nidx", _u32p)]
 
 
 class Grid(C.Structure):
@@ -83,6 +84,7 @@ def lib():
         L.or_prim_cell_range.argtypes = [C.POINTER(Mesh), C.c_uint32, C.POINTER(Grid),
                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_tri_box_overlap.argtypes = [_f32p] * 5
+        L.or_shading_normal.argtypes = [C.POINTER(Mesh), C.c_uint32, _f32p, _f32p]
         L.or_trace.argtypes = [C.POINTER(Mesh), C.POINTER(Grid)] + [_f32p] * 6 + [
             C.c_uint32, C.c_int, C.c_float, C.c_int, _f32p, _u32p, C.POINTER(Stats)]
         L.or_camera_init.argtypes = [C.POINTER(Camera), _f32p, _f32p, _f32p, C.c_float,
@@ -115,13 +117,23 @@ def _f3(v):
 class OMesh:
     """Keeps the numpy arrays alive behind an or_mesh."""
 
-    def __init__(self, vx, vy, vz, idx):
+    def __init__(self, vx, vy, vz, idx, normals=None, nidx=None):
+        """normals: (3, nn) SoA per-corner normals, nidx: (nt, 3) normal indices."""
         self.vx = np.ascontiguousarray(vx, dtype=np.float32)
         self.vy = np.ascontiguousarray(vy, dtype=np.float32)
         self.vz = np.ascontiguousarray(vz, dtype=np.float32)
         self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
         self.c = Mesh(_fp(self.vx), _fp(self.vy), _fp(self.vz), len(self.vx),
                       _up(self.idx), len(self.idx) // 3)
+        if normals is not None:
+            n = np.ascontiguousarray(normals, dtype=np.float32).reshape(3, -1)
+            self.nx, self.ny, self.nz = (np.ascontiguousarray(n[k]) for k in range(3))
+            self.nidx = np.ascontiguousarray(nidx, dtype=np.uint32).reshape(-1)
+            assert len(self.nidx) == len(self.idx)
+            assert len(self.nidx) == 0 or self.nidx.max() < n.shape[1]
+            self.c.nx, self.c.ny, self.c.nz = _fp(self.nx), _fp(self.ny), _fp(self.nz)
+            self.c.nn = n.shape[1]
+            self.c.nidx = _up(self.nidx)
 
     @property
     def nt(self):
@@ -310,3 +322,10 @@ def render_pt(grid, cam, p, materials, mat_idx, lights, weights=None, image_id=0
                        None if mi is None else _up(mi), _fp(L), None if w is None else _fp(w),
                        len(L), C.byref(p), image_id, _fp(fb), C.byref(st))
     return fb, st
+
+
+def shading_normal(mesh, prim, org):
+    """Path-tracer shading normal of primitive prim seen from hit point org."""
+    out = np.zeros(3, np.float32)
+    lib().or_shading_normal(C.byref(mesh.c), prim, _fp(_f3(org)), _fp(out))
+    return out

