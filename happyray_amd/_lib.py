@@ -94,6 +94,8 @@ SIGNATURES = {
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
     "hr_get_work_stats": (C.c_int, [_vp, C.POINTER(WorkStats)]),
     "hr_synchronize": (C.c_int, [_vp]),
+    "hr_render_frame_graph": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int,
+                                         C.POINTER(RenderParams), _f32p]),
     "hr_set_normals": (C.c_int, [_vp, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint32]),
     "hr_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "hr_scene_load_obj": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),

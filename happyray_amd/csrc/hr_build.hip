@@ -58,13 +58,25 @@ __global__ void k_reset(BuildStatus* st)
 
 // v = v1 * (1 - c) + v2 * c (WFObjectUploader.h:82-88) and min / max of all
 // vertices (WFObjectUploader.h:90-95)
+// With a frame selection fs (captured frames) the key frames and the
+// coefficient come from device memory: blocks fs->k1 and fs->k2 of keys.
 __global__ __launch_bounds__(kBlock) void k_lerp_bounds(
-    const float* __restrict__ x1, const float* __restrict__ y1, const float* __restrict__ z1,
-    const float* __restrict__ x2, const float* __restrict__ y2, const float* __restrict__ z2,
-    float c, uint32_t nv, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
-    BuildStatus* __restrict__ st)
+    const float* __restrict__ k1, const float* __restrict__ k2, float c, const FrameSel* __restrict__ fs,
+    const float* __restrict__ keys, uint32_t nv, float* __restrict__ ox, float* __restrict__ oy,
+    float* __restrict__ oz, BuildStatus* __restrict__ st)
 {
     __shared__ float red[6][kWavesPerBlock];
+    if (fs) {
+        k1 = keys + (size_t)fs->k1 * 3 * nv;
+        k2 = keys + (size_t)fs->k2 * 3 * nv;
+        c = fs->coeff;
+    }
+    const float* __restrict__ x1 = k1;
+    const float* __restrict__ y1 = k1 + nv;
+    const float* __restrict__ z1 = k1 + 2 * (size_t)nv;
+    const float* __restrict__ x2 = k2;
+    const float* __restrict__ y2 = k2 + nv;
+    const float* __restrict__ z2 = k2 + 2 * (size_t)nv;
     f3 lo = rep(kFltMax), hi = rep(-kFltMax);
     const float ic = 1.f - c;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
@@ -138,6 +150,16 @@ __global__ void k_grid_setup(GridParams* __restrict__ gp, BuildStatus* __restric
 }
 
 // total > cap sets the overflow bit and zeroes the effective size
+// After an overflow the grid is unusable: a zero resolution makes every ray
+// miss at its entry test, so kernels that run after the build without a host
+// check (captured frames) read nothing out of bounds.
+__global__ void k_seal(GridParams* gp, const BuildStatus* st)
+{
+    if (threadIdx.x != 0 || st->overflow == 0u) return;
+    gp->res[0] = gp->res[1] = gp->res[2] = 0;
+    gp->num_cells = 0u;
+}
+
 __global__ void k_check(BuildStatus* st, uint32_t* total, uint32_t* eff, uint32_t cap, uint32_t bit)
 {
     if (threadIdx.x != 0) return;
@@ -491,12 +513,12 @@ inline int bits_for(uint64_t n)
 // ---------------------------------------------------------------------------
 
 void launch_lerp_bounds(const float* k1, const float* k2, float c, uint32_t nv, float* out,
-                        BuildStatus* st, hipStream_t s)
+                        BuildStatus* st, hipStream_t s, const FrameSel* fs, const float* keys)
 {
     uint32_t blocks = cdiv(nv > 0 ? nv : 1, kBlock);
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_lerp_bounds, dim3(blocks), dim3(kBlock), 0, s, k1, k1 + nv, k1 + 2 * (size_t)nv,
-                       k2, k2 + nv, k2 + 2 * (size_t)nv, c, nv, out, out + nv, out + 2 * (size_t)nv, st);
+    hipLaunchKernelGGL(k_lerp_bounds, dim3(blocks), dim3(kBlock), 0, s, k1, k2, c, fs, keys, nv, out, out + nv,
+                       out + 2 * (size_t)nv, st);
 }
 
 void launch_reset(BuildStatus* st, hipStream_t s)
@@ -547,7 +569,10 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     const uint32_t rblocks = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256) > 4096 ? 4096
                                  : cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256);
     hipLaunchKernelGGL(k_ranges, dim3(rblocks), dim3(256), 0, s, b.keys_sorted, &st->n_pairs_eff, b.cells, st);
-    if (!two) return;
+    if (!two) {
+        hipLaunchKernelGGL(k_seal, dim3(1), dim3(64), 0, s, b.params, st);
+        return;
+    }
 
     // ---- leaf resolution, leaf begin offsets ----
     const uint32_t cblocks = cdiv(b.cells_cap, 256) > 4096 ? 4096 : cdiv(b.cells_cap, 256);
@@ -580,6 +605,7 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
                                   : cdiv(b.leaf_pairs_cap > 0 ? b.leaf_pairs_cap : 1, 256);
     hipLaunchKernelGGL(k_ranges, dim3(lrblocks), dim3(256), 0, s, b.keys_sorted, &st->n_leaf_pairs_eff,
                        b.leaves, st);
+    hipLaunchKernelGGL(k_seal, dim3(1), dim3(64), 0, s, b.params, st);
 }
 
 }  // namespace hr

@@ -38,6 +38,15 @@ struct BuildStatus {
     uint32_t pad[2];
 };
 
+// The per-frame selection of a captured frame (HIP graph): key frames, the
+// interpolation coefficient and the image id, read by the kernels from device
+// memory so that one recorded graph serves every frame.
+struct FrameSel {
+    uint32_t k1, k2;
+    float coeff;
+    int image;
+};
+
 constexpr uint32_t kOverflowCells = 1u;
 constexpr uint32_t kOverflowPairs = 2u;
 constexpr uint32_t kOverflowLeaves = 4u;

@@ -149,6 +149,14 @@ struct hr_ctx {
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
     hr_tuning tuning{ 0, 1, 48 };
+
+    // captured frames: a rebuild graph and a render graph, replayed per frame
+    hipGraphExec_t g_build = nullptr, g_render = nullptr;
+    std::vector<unsigned char> graph_sig;    // state the graphs were recorded with
+    DevBuf<hr::FrameSel> frame_sel;
+    hr::FrameSel* h_frame = nullptr;         // pinned: source of the frame selection
+    hr::GridParams* h_params_pin = nullptr;  // pinned: grid parameters after a replay
+    bool graph_timed = false;
 };
 
 namespace {
@@ -245,10 +253,16 @@ hr::BuildBuffers build_buffers(hr_ctx* c)
     return b;
 }
 
-void enqueue_build(hr_ctx* c)
+// interpolation, bounds and the whole grid build; with fs the key frames and
+// coefficient are read on the device (captured frames)
+void enqueue_build(hr_ctx* c, const hr::FrameSel* fs = nullptr)
 {
     hr::launch_reset(c->status.p, c->stream);
-    hr::launch_lerp_bounds(c->src1, c->src2, c->coeff, c->nv, c->verts.p, c->status.p, c->stream);
+    if (fs)
+        hr::launch_lerp_bounds(nullptr, nullptr, 0.f, c->nv, c->verts.p, c->status.p, c->stream, fs,
+                               c->keyframes.p);
+    else
+        hr::launch_lerp_bounds(c->src1, c->src2, c->coeff, c->nv, c->verts.p, c->status.p, c->stream);
     hr::BuildArgs a;
     a.kind = c->kind;
     a.lambda_top = c->lambda_top;
@@ -266,6 +280,35 @@ uint32_t grow(uint64_t cur, uint64_t need)
 {
     uint64_t v = std::max<uint64_t>(need + need / 4 + 1024, cur + cur / 2);
     return (uint32_t)std::min<uint64_t>(v, 0xFFFFFFF0ull);
+}
+
+// grid description after a successful build (c->h_params holds the parameters)
+void set_info(hr_ctx* c, const hr::BuildStatus& st)
+{
+    const bool two = c->kind == HR_GRID_TWO_LEVEL;
+    hr_grid_info& in = c->info;
+    in.kind = c->kind;
+    for (int i = 0; i < 3; ++i) {
+        in.bmin[i] = c->h_params.bmin[i];
+        in.bmax[i] = c->h_params.bmax[i];
+        in.res[i] = c->h_params.res[i];
+        in.cell_size[i] = c->h_params.cs[i];
+        in.cell_size_rcp[i] = c->h_params.csr[i];
+    }
+    in.num_cells = st.num_cells;
+    in.num_leaves = two ? st.num_leaves : 0u;
+    in.num_refs = two ? st.num_leaf_pairs : st.num_pairs;
+    in.num_top_pairs = two ? st.num_pairs : 0u;
+    in.num_triangles = c->nt;
+    c->built = true;
+}
+
+void drop_graphs(hr_ctx* c)
+{
+    if (c->g_build) (void)hipGraphExecDestroy(c->g_build);
+    if (c->g_render) (void)hipGraphExecDestroy(c->g_render);
+    c->g_build = c->g_render = nullptr;
+    c->graph_sig.clear();
 }
 
 hr_status do_build(hr_ctx* c)
@@ -296,21 +339,7 @@ hr_status do_build(hr_ctx* c)
         if (st.overflow == 0u) {
             c->build_timed = true;
             HR_HIP(hipMemcpy(&c->h_params, c->params.p, sizeof(hr::GridParams), hipMemcpyDeviceToHost));
-            hr_grid_info& in = c->info;
-            in.kind = c->kind;
-            for (int i = 0; i < 3; ++i) {
-                in.bmin[i] = c->h_params.bmin[i];
-                in.bmax[i] = c->h_params.bmax[i];
-                in.res[i] = c->h_params.res[i];
-                in.cell_size[i] = c->h_params.cs[i];
-                in.cell_size_rcp[i] = c->h_params.csr[i];
-            }
-            in.num_cells = st.num_cells;
-            in.num_leaves = two ? st.num_leaves : 0u;
-            in.num_refs = two ? st.num_leaf_pairs : st.num_pairs;
-            in.num_top_pairs = two ? st.num_pairs : 0u;
-            in.num_triangles = c->nt;
-            c->built = true;
+            set_info(c, st);
             return HR_OK;
         }
         if (st.overflow & hr::kOverflowCells) {
@@ -370,6 +399,200 @@ hr_status ensure_default_material(hr_ctx* c)
     return HR_OK;
 }
 
+hr_status check_render_args(hr_ctx* c, hr_mode mode, int image_id, const hr_render_params* p)
+{
+    if (!c->has_camera) return fail(HR_ERR_STATE, "render before setCamera");
+    if (!p) return fail(HR_ERR_ARG, "null render parameters");
+    if (mode != HR_MODE_AO && mode != HR_MODE_PATH) return fail(HR_ERR_ARG, "unknown render mode");
+    if (image_id < 0) return fail(HR_ERR_ARG, "negative image id");
+    if (mode == HR_MODE_AO && (p->ao_samples < 1 || p->ao_samples > 256))
+        return fail(HR_ERR_ARG, "ao_samples must be in [1, 256]");
+    if (mode == HR_MODE_PATH && (p->pt_depth < 1 || p->pt_depth > 64))
+        return fail(HR_ERR_ARG, "pt_depth must be in [1, 64]");
+    return HR_OK;
+}
+
+// buffers a render writes
+hr_status prepare_render(hr_ctx* c, hr_mode mode, const hr_render_params* p)
+{
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    HR_TRY(c->hits.reserve(7 * (size_t)npix));
+    HR_TRY(c->hit_prims.reserve(npix));
+    HR_TRY(c->fb.reserve(3 * (size_t)npix));
+    if (mode == HR_MODE_AO) HR_TRY(c->occ.reserve((size_t)npix * (size_t)p->ao_samples));
+    HR_TRY(ensure_default_material(c));
+    return HR_OK;
+}
+
+// current-frame normals from the normal key frames; with fs the key frames
+// are chosen on the device
+void enqueue_normals(hr_ctx* c, const hr::FrameSel* fs)
+{
+    const size_t blk = 3 * (size_t)c->nn;
+    if (c->nframes_n == 1) {
+        hr::launch_frame_normals(c->normal_keys.p, c->normal_keys.p, c->nn, c->normals.p, c->stream);
+    } else if (fs) {
+        hr::launch_frame_normals(nullptr, nullptr, c->nn, c->normals.p, c->stream, fs, c->normal_keys.p);
+    } else {
+        hr::launch_frame_normals(c->normal_keys.p + (size_t)c->k1 * blk, c->normal_keys.p + (size_t)c->k2 * blk,
+                                 c->nn, c->normals.p, c->stream);
+    }
+}
+
+// Primary rays and the integrator.  fs != nullptr: the image id is read on the
+// device (captured frames); events = false while a graph is being recorded.
+hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_params* p,
+                         const hr::FrameSel* fs, bool events)
+{
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    hr::RenderArgs ra;
+    ra.seed_lo = c->seed_lo;
+    ra.seed_hi = c->seed_hi;
+    ra.jitter = p->jitter;
+    ra.ao_samples = p->ao_samples;
+    ra.ao_dist = p->ao_dist;
+    ra.pt_depth = p->pt_depth;
+    ra.image = image_id;
+    ra.stats = p->collect_stats ? c->work.p : nullptr;
+    ra.frame = fs;
+    const hr::TraceScene S = trace_scene(c);
+    hr::Shading sh;
+    sh.mats = c->mats.p;
+    sh.mat_idx = c->has_mat_idx ? c->mat_idx.p : nullptr;
+    sh.lights = c->lights.p;
+    sh.light_cdf = c->light_cdf.p;
+    sh.nlights = c->nlights;
+    sh.normals = nullptr;
+    sh.nidx = nullptr;
+    if (mode == HR_MODE_PATH && c->has_normals) {
+        sh.normals = c->normals.p;
+        sh.nidx = c->nidx.p;
+    }
+    if (events) HR_HIP(hipEventRecord(c->ev_r0, c->stream));
+    const bool pers = c->tuning.persistent_secondary != 0;
+    const uint32_t refill = (uint32_t)c->tuning.refill;
+    if (c->tuning.persistent_primary)
+        hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
+                            c->stream);
+    else
+        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
+    if (events) HR_HIP(hipEventRecord(c->ev_p1, c->stream));
+    if (ra.stats) ra.stats = c->work.p + 3;
+    if (mode == HR_MODE_AO) {
+        if (pers)
+            hr::persist_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->pool.p, refill, c->ray_count.p,
+                           c->stream);
+        else
+            hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
+        if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+        hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->stream);
+        c->last_ao_samples = p->ao_samples;
+    } else {
+        hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
+        if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+    }
+    if (events) HR_HIP(hipEventRecord(c->ev_r1, c->stream));
+    HR_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+template <class T>
+void put(std::vector<unsigned char>& v, const T& x)
+{
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&x);
+    v.insert(v.end(), b, b + sizeof(T));
+}
+
+// everything a recorded frame bakes into its kernel arguments
+std::vector<unsigned char> graph_signature(hr_ctx* c, hr_mode mode, const hr_render_params* p)
+{
+    std::vector<unsigned char> v;
+    put(v, (int)mode);
+    put(v, p->jitter);
+    put(v, p->ao_samples);
+    put(v, p->ao_dist);
+    put(v, p->pt_depth);
+    put(v, p->collect_stats);
+    put(v, c->seed_lo);
+    put(v, c->seed_hi);
+    put(v, c->kind);
+    put(v, c->lambda_top);
+    put(v, c->lambda_leaf);
+    put(v, c->nt);
+    put(v, c->nv);
+    put(v, c->nframes);
+    const void* ptrs[] = { c->keyframes.p, c->verts.p, c->idx.p, c->rec.p, c->wave_counts.p, c->keys.p,
+                           c->vals.p, c->keys_sorted.p, c->top_vals_sorted.p, c->prims.p, c->cells.p,
+                           c->leaves.p, c->leaf_counts.p, c->scan_tmp.p, c->sort_keys_tmp.p,
+                           c->sort_vals_tmp.p, c->sort_hist.p, c->mats.p, c->mat_idx.p, c->lights.p,
+                           c->light_cdf.p, c->normal_keys.p, c->normals.p, c->nidx.p, c->hits.p,
+                           c->hit_prims.p, c->fb.p, c->occ.p };
+    for (const void* q : ptrs) put(v, q);
+    put(v, c->cells_cap);
+    put(v, c->pairs_cap);
+    put(v, c->leaves_cap);
+    put(v, c->leaf_pairs_cap);
+    put(v, c->wave_counts.cap);
+    put(v, c->scan_tmp.cap);
+    put(v, c->cam);
+    put(v, c->nlights);
+    put(v, c->has_mat_idx);
+    put(v, c->has_normals);
+    put(v, c->nn);
+    put(v, c->nframes_n);
+    put(v, c->tuning);
+    return v;
+}
+
+hr_status end_capture(hr_ctx* c, hipGraphExec_t* out)
+{
+    hipGraph_t g = nullptr;
+    HR_HIP(hipStreamEndCapture(c->stream, &g));
+    const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(HR_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return HR_OK;
+}
+
+// Record the rebuild graph (frame selection upload, interpolation, build,
+// normals, status read-back into pinned memory) and the render graph
+// (first-image clear, primary rays, integrator).  No memset node is recorded.
+hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
+{
+    drop_graphs(c);
+    hr::FrameSel* fs = c->frame_sel.p;
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    HR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipMemcpyAsync(fs, c->h_frame, sizeof(hr::FrameSel), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        enqueue_build(c, fs);
+        if (c->has_normals) enqueue_normals(c, fs);
+        e = hipMemcpyAsync(c->h_status, c->status.p, sizeof(hr::BuildStatus), hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->h_params_pin, c->params.p, sizeof(hr::GridParams), hipMemcpyDeviceToHost,
+                           c->stream);
+    hr_status st = end_capture(c, &c->g_build);
+    if (e != hipSuccess) st = fail(HR_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+    if (st != HR_OK) {
+        drop_graphs(c);
+        return st;
+    }
+    HR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    hr::launch_clear_first(c->fb.p, 3 * (size_t)npix, fs, c->stream);
+    const hr_status rs = enqueue_render(c, mode, 0, p, fs, false);
+    st = end_capture(c, &c->g_render);
+    if (rs != HR_OK || st != HR_OK) {
+        drop_graphs(c);
+        return rs != HR_OK ? rs : st;
+    }
+    c->graph_sig = graph_signature(c, mode, p);
+    return HR_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -407,6 +630,11 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         if ((s = c->pool.reserve(1)) != HR_OK) break;
         e = hipHostMalloc((void**)&c->h_status, sizeof(hr::BuildStatus), hipHostMallocDefault);
         if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
+        e = hipHostMalloc((void**)&c->h_frame, sizeof(hr::FrameSel), hipHostMallocDefault);
+        if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
+        e = hipHostMalloc((void**)&c->h_params_pin, sizeof(hr::GridParams), hipHostMallocDefault);
+        if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
+        if ((s = c->frame_sel.reserve(1)) != HR_OK) break;
         hipEvent_t* evs[6] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1, &c->ev_p1, &c->ev_p2 };
         for (hipEvent_t* ev : evs) {
             e = hipEventCreate(ev);
@@ -452,7 +680,11 @@ hr_status hr_destroy(hr_ctx* c)
     c->ray_count.release();
     c->work.release();
     c->pool.release();
+    drop_graphs(c);
+    c->frame_sel.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->h_frame) (void)hipHostFree(c->h_frame);
+    if (c->h_params_pin) (void)hipHostFree(c->h_params_pin);
     hipEvent_t evs[6] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1, c->ev_p1, c->ev_p2 };
     for (hipEvent_t ev : evs)
         if (ev) (void)hipEventDestroy(ev);
@@ -675,85 +907,81 @@ hr_status hr_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_param
 {
     HR_TRY(check_ctx(c));
     if (!c->built) return fail(HR_ERR_STATE, "render before build");
-    if (!c->has_camera) return fail(HR_ERR_STATE, "render before setCamera");
-    if (!p) return fail(HR_ERR_ARG, "null render parameters");
-    if (mode != HR_MODE_AO && mode != HR_MODE_PATH) return fail(HR_ERR_ARG, "unknown render mode");
-    if (image_id < 0) return fail(HR_ERR_ARG, "negative image id");
-    if (mode == HR_MODE_AO && (p->ao_samples < 1 || p->ao_samples > 256))
-        return fail(HR_ERR_ARG, "ao_samples must be in [1, 256]");
-    if (mode == HR_MODE_PATH && (p->pt_depth < 1 || p->pt_depth > 64))
-        return fail(HR_ERR_ARG, "pt_depth must be in [1, 64]");
+    HR_TRY(check_render_args(c, mode, image_id, p));
+    HR_TRY(prepare_render(c, mode, p));
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
-    HR_TRY(c->hits.reserve(7 * (size_t)npix));
-    HR_TRY(c->hit_prims.reserve(npix));
-    HR_TRY(c->fb.reserve(3 * (size_t)npix));
-    if (mode == HR_MODE_AO) HR_TRY(c->occ.reserve((size_t)npix * (size_t)p->ao_samples));
-    HR_TRY(ensure_default_material(c));
     if (image_id == 0 || npix != c->last_npix) {
         hr::fill_u32((uint32_t*)c->fb.p, 0u, 3 * (size_t)npix, nullptr, 1, c->stream);
         c->last_npix = npix;
     }
-    hr::RenderArgs ra;
-    ra.seed_lo = c->seed_lo;
-    ra.seed_hi = c->seed_hi;
-    ra.jitter = p->jitter;
-    ra.ao_samples = p->ao_samples;
-    ra.ao_dist = p->ao_dist;
-    ra.pt_depth = p->pt_depth;
-    ra.image = image_id;
-    ra.stats = p->collect_stats ? c->work.p : nullptr;
-    const hr::TraceScene S = trace_scene(c);
-    hr::Shading sh;
-    sh.mats = c->mats.p;
-    sh.mat_idx = c->has_mat_idx ? c->mat_idx.p : nullptr;
-    sh.lights = c->lights.p;
-    sh.light_cdf = c->light_cdf.p;
-    sh.nlights = c->nlights;
-    sh.normals = nullptr;
-    sh.nidx = nullptr;
-    if (mode == HR_MODE_PATH && c->has_normals) {
-        if (c->normals_dirty) {
-            const size_t blk = 3 * (size_t)c->nn;
-            const float* a = c->normal_keys.p + (c->nframes_n == 1 ? 0 : (size_t)c->k1 * blk);
-            const float* b = c->normal_keys.p + (c->nframes_n == 1 ? 0 : (size_t)c->k2 * blk);
-            hr::launch_frame_normals(a, b, c->nn, c->normals.p, c->stream);
-            c->normals_dirty = false;
-        }
-        sh.normals = c->normals.p;
-        sh.nidx = c->nidx.p;
+    if (mode == HR_MODE_PATH && c->has_normals && c->normals_dirty) {
+        enqueue_normals(c, nullptr);
+        c->normals_dirty = false;
     }
-    HR_HIP(hipEventRecord(c->ev_r0, c->stream));
-    const bool pers = c->tuning.persistent_secondary != 0;
-    const uint32_t refill = (uint32_t)c->tuning.refill;
-    if (c->tuning.persistent_primary)
-        hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
-                            c->stream);
-    else
-        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
-    HR_HIP(hipEventRecord(c->ev_p1, c->stream));
-    if (ra.stats) ra.stats = c->work.p + 3;
-    if (mode == HR_MODE_AO) {
-        if (pers)
-            hr::persist_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->pool.p, refill, c->ray_count.p,
-                           c->stream);
-        else
-            hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
-        HR_HIP(hipEventRecord(c->ev_p2, c->stream));
-        hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->stream);
-        c->last_ao_samples = p->ao_samples;
-    } else {
-        hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
-        HR_HIP(hipEventRecord(c->ev_p2, c->stream));
-    }
-    HR_HIP(hipEventRecord(c->ev_r1, c->stream));
-    HR_HIP(hipGetLastError());
+    HR_TRY(enqueue_render(c, mode, image_id, p, nullptr, true));
     c->render_timed = true;
+    c->graph_timed = false;
     if (rgb_out) {
         HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, 3 * (size_t)npix * sizeof(float), hipMemcpyDeviceToHost,
                               c->stream));
         HR_HIP(hipStreamSynchronize(c->stream));
     }
     return HR_OK;
+}
+
+hr_status hr_render_frame_graph(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff, hr_mode mode, int image_id,
+                                const hr_render_params* p, float* rgb_out)
+{
+    HR_TRY(check_ctx(c));
+    if (c->nframes == 0) return fail(HR_ERR_STATE, "captured frames need resident key frames");
+    if (k1 >= c->nframes || k2 >= c->nframes) return fail(HR_ERR_ARG, "key frame index out of range");
+    HR_TRY(check_render_args(c, mode, image_id, p));
+    c->src1 = c->keyframes.p + (size_t)k1 * 3 * c->nv;
+    c->src2 = c->keyframes.p + (size_t)k2 * 3 * c->nv;
+    c->coeff = coeff;
+    c->k1 = k1;
+    c->k2 = k2;
+    c->normals_dirty = true;
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    const size_t fb_bytes = 3 * (size_t)npix * sizeof(float);
+    if (c->g_build && c->g_render && npix == c->last_npix && c->graph_sig == graph_signature(c, mode, p)) {
+        *c->h_frame = hr::FrameSel{ k1, k2, coeff, image_id };
+        HR_HIP(hipEventRecord(c->ev_b0, c->stream));
+        HR_HIP(hipGraphLaunch(c->g_build, c->stream));
+        HR_HIP(hipEventRecord(c->ev_b1, c->stream));
+        HR_HIP(hipGraphLaunch(c->g_render, c->stream));
+        HR_HIP(hipEventRecord(c->ev_r1, c->stream));
+        if (rgb_out) HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, fb_bytes, hipMemcpyDeviceToHost, c->stream));
+        HR_HIP(hipStreamSynchronize(c->stream));
+        if (c->h_status->overflow == 0u) {
+            c->h_params = *c->h_params_pin;
+            set_info(c, *c->h_status);
+            c->build_timed = c->render_timed = c->graph_timed = true;
+            c->normals_dirty = false;
+            if (mode == HR_MODE_AO) c->last_ao_samples = p->ao_samples;
+            return HR_OK;
+        }
+        // the rebuild outgrew its buffers: render this frame without the
+        // graphs (the build grows them) and record again
+        drop_graphs(c);
+    }
+    c->built = false;
+    HR_TRY(do_build(c));
+    HR_TRY(prepare_render(c, mode, p));
+    if (image_id == 0 || npix != c->last_npix) {
+        hr::fill_u32((uint32_t*)c->fb.p, 0u, 3 * (size_t)npix, nullptr, 1, c->stream);
+        c->last_npix = npix;
+    }
+    if (mode == HR_MODE_PATH && c->has_normals) {
+        enqueue_normals(c, nullptr);
+        c->normals_dirty = false;
+    }
+    HR_TRY(enqueue_render(c, mode, image_id, p, nullptr, true));
+    c->render_timed = true;
+    c->graph_timed = false;
+    if (rgb_out) HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, fb_bytes, hipMemcpyDeviceToHost, c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return record_graphs(c, mode, p);
 }
 
 hr_status hr_get_primary_hits(hr_ctx* c, float* planes7, uint32_t* prims)
@@ -850,7 +1078,9 @@ hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
     out->build_ms = out->render_ms = out->primary_ms = out->secondary_ms = out->shade_ms = 0.0;
     float ms = 0.f;
     if (c->build_timed && hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1) == hipSuccess) out->build_ms = ms;
-    if (c->render_timed) {
+    if (c->graph_timed) {
+        if (hipEventElapsedTime(&ms, c->ev_b1, c->ev_r1) == hipSuccess) out->render_ms = ms;
+    } else if (c->render_timed) {
         if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_r1) == hipSuccess) out->render_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess) out->primary_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev_p1, c->ev_p2) == hipSuccess) out->secondary_ms = ms;

@@ -292,7 +292,7 @@ struct GenAO {
         f3 n = normalize(cross(v1 - v0, v2 - v0));
         if (dot(n, pdir) > 0.f) n = -n;
         float u[4];
-        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)ra.image, kStreamAO + s, u);
+        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamAO + s, u);
         const f3 h = cosine_dir(u[0], u[1]);
         f3 tg, bn;
         local_frame(n, tg, bn);

@@ -30,7 +30,11 @@ struct RenderArgs {
     int pt_depth;
     int image;
     unsigned long long* stats;   // nullptr, or {rays, steps, tests} counters
+    const FrameSel* frame;       // nullptr, or the frame's selection in device memory
+                                 // (its image id then replaces `image`)
 };
+
+__device__ __forceinline__ int image_of(const RenderArgs& ra) { return ra.frame ? ra.frame->image : ra.image; }
 
 struct Shading {
     const float* mats;          // 12 floats per PhongMaterial
@@ -56,7 +60,11 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
 // current-frame normals: a * (1 - 0) + b * 0 per component, as the reference
 // uploads normals with coefficient 0 (src/RT/RTEngine.cu:209-210); a, b are
 // SoA key-frame blocks of nn normals
-void launch_frame_normals(const float* a, const float* b, uint32_t nn, float4* out, hipStream_t s);
+// zero n floats of fb when fs->image == 0 (read on the device)
+void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s);
+// fs != nullptr: a and b are blocks fs->k1 and fs->k2 of keys, chosen on the device
+void launch_frame_normals(const float* a, const float* b, uint32_t nn, float4* out, hipStream_t s,
+                          const FrameSel* fs = nullptr, const float* keys = nullptr);
 void launch_trace_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n,
                         int anyhit, float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out,
                         unsigned long long* ray_count, unsigned long long* stats, hipStream_t s);

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_rays(TraceScene S, RenderArgs ra,
     f3 n = normalize(cross(v1 - v0, v2 - v0));
     if (dot(n, pdir) > 0.f) n = -n;
     float u[4];
-    rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)ra.image, kStreamAO + s, u);
+    rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamAO + s, u);
     const f3 h = cosine_dir(u[0], u[1]);
     f3 tg, bn;
     local_frame(n, tg, bn);
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_shade(TraceScene S, RenderArgs ra
         }
         rad = rad + (v * 1.f) / (float)S_;
     }
-    const float nw = 1.f / (float)(ra.image + 1);
+    const float nw = 1.f / (float)(image_of(ra) + 1);
     const float ow = 1.f - nw;
     float* o = fb + 3 * (size_t)pix;
     o[0] = o[0] * ow + rad.x * nw;
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
         bool last_spec = true;
         for (int depth = 0; depth < ra.pt_depth; ++depth) {
             float u[4], w[4];
-            rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)ra.image, kStreamPT + 2u * (uint32_t)depth, u);
-            rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)ra.image, kStreamPT + 2u * (uint32_t)depth + 1u,
+            rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPT + 2u * (uint32_t)depth, u);
+            rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPT + 2u * (uint32_t)depth + 1u,
                   w);
             f3 v0, v1, v2;
             load_tri(S.rec, prim, v0, v1, v2);
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
     if (ray_count && traced) atomicAdd(ray_count, (unsigned long long)traced);
     add_work(ra.stats, traced, wk);
     if (!in) return;
-    const float nw = 1.f / (float)(ra.image + 1);
+    const float nw = 1.f / (float)(image_of(ra) + 1);
     const float ow = 1.f - nw;
     float* o = fb + 3 * (size_t)pix;
     o[0] = o[0] * ow + radiance.x * nw;
@@ -355,15 +355,31 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
 
 __global__ __launch_bounds__(kBlock) void k_frame_normals(const float* __restrict__ a,
                                                           const float* __restrict__ b, uint32_t nn,
+                                                          const FrameSel* __restrict__ fs,
+                                                          const float* __restrict__ keys,
                                                           float4* __restrict__ out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn) return;
+    if (fs) {
+        a = keys + (size_t)fs->k1 * 3 * nn;
+        b = keys + (size_t)fs->k2 * 3 * nn;
+    }
     const float c = 0.f;
     const float x = a[i] * (1.f - c) + b[i] * c;
     const float y = a[(size_t)nn + i] * (1.f - c) + b[(size_t)nn + i] * c;
     const float z = a[2 * (size_t)nn + i] * (1.f - c) + b[2 * (size_t)nn + i] * c;
     out[i] = make_float4(x, y, z, 0.f);
+}
+
+// clears the framebuffer when the captured frame is the first image of a
+// running mean (the uncaptured path clears it from the host)
+__global__ __launch_bounds__(kBlock) void k_clear_first(float* __restrict__ fb, size_t n,
+                                                        const FrameSel* __restrict__ fs)
+{
+    if (fs->image != 0) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        fb[i] = 0.f;
 }
 
 // ---------------------------------------------------------------------------
@@ -448,10 +464,18 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
                            hit_prims, sh, fb, ray_count);
 }
 
-void launch_frame_normals(const float* a, const float* b, uint32_t nn, float4* out, hipStream_t s)
+void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s)
+{
+    uint32_t blocks = blocks_for(n > 0 ? n : 1);
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_clear_first, dim3(blocks), dim3(kBlock), 0, s, fb, n, fs);
+}
+
+void launch_frame_normals(const float* a, const float* b, uint32_t nn, float4* out, hipStream_t s,
+                          const FrameSel* fs, const float* keys)
 {
     if (nn == 0) return;
-    hipLaunchKernelGGL(k_frame_normals, dim3(blocks_for(nn)), dim3(kBlock), 0, s, a, b, nn, out);
+    hipLaunchKernelGGL(k_frame_normals, dim3(blocks_for(nn)), dim3(kBlock), 0, s, a, b, nn, fs, keys, out);
 }
 
 void launch_trace_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n,

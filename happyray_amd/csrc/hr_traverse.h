@@ -271,7 +271,7 @@ __device__ __forceinline__ f3 primary_direction(const CameraDev& cam, const Rend
     } else {
         // GaussianPixelSampler (Camera.h:121-131)
         float u[4];
-        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)ra.image, kStreamPrimary, u);
+        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPrimary, u);
         const float tmp = pid / resx;
         const float rad = 0.5f * sqrtf(-spec_logf(1 - u[0]));
         const float angle = 2.f * kPi * u[1];

@@ -184,6 +184,20 @@ class Engine:
                                 None if out is None else _f(out)), "hr_render")
         return out
 
+    def render_frame_graph(self, k1, k2, coeff, image_id=0, mode=MODE_PATH, jitter=1, ao_samples=4,
+                           ao_dist=2.0, pt_depth=4, download=True, collect_stats=False):
+        """One captured frame of the resident key frames: interpolation, rebuild,
+        primary rays and the integrator replayed from recorded HIP graphs."""
+        p = _lib.RenderParams(int(jitter), int(ao_samples), float(ao_dist), int(pt_depth),
+                              int(bool(collect_stats)))
+        out = None
+        if download:
+            out = np.zeros((self._res[0] * self._res[1], 3), np.float32)
+        check(self._L.hr_render_frame_graph(self._h, int(k1), int(k2), float(coeff), mode, int(image_id),
+                                            C.byref(p), None if out is None else _f(out)),
+              "hr_render_frame_graph")
+        return out
+
     def primary_hits(self):
         n = self._res[0] * self._res[1]
         planes = np.zeros((7, n), np.float32)

@@ -166,6 +166,20 @@ hr_status hr_set_lights(hr_ctx* ctx, const hr_area_light* lights, const float* w
  * (3*res_x*res_y floats) receives a copy when not NULL. */
 hr_status hr_render(hr_ctx* ctx, hr_mode mode, int image_id, const hr_render_params* params,
                     float* rgb_out);
+/* Captured frames (no reference counterpart: the reference runs upload, build and
+ * render as synchronous phases, src/RT/RTEngine.cu:194-453).  One call renders a
+ * frame of the resident key frames: interpolation of k1, k2 with coeff, a full
+ * rebuild, primary rays and the integrator, with results equal to hr_set_frame +
+ * hr_build + hr_render.  The work is recorded once as two HIP graphs (rebuild;
+ * render) and replayed by later calls, the frame's (k1, k2, coeff, image_id)
+ * reaching the kernels through device memory.  The graphs are recorded again when
+ * the mode, render parameters, grid settings, camera, light count, geometry,
+ * tuning or any buffer changes.  A replayed rebuild that outgrows its buffers is
+ * detected after the replay; the frame is then rendered again without graphs
+ * while the buffers grow.  hr_get_timing reports the rebuild graph as build_ms
+ * and the render graph as render_ms. */
+hr_status hr_render_frame_graph(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coeff, hr_mode mode,
+                                int image_id, const hr_render_params* params, float* rgb_out);
 /* Primary-ray hit buffer of the last render (SimpleRayBuffer, RayBuffers.h:34-195):
  * 7 SoA float planes (hit point xyz, direction xyz, t) and primitive ids. */
 hr_status hr_get_primary_hits(hr_ctx* ctx, float* planes7, uint32_t* prims);

@@ -1,0 +1,93 @@
+"""Captured frames (hr_render_frame_graph): interpolation, rebuild, primary
+rays and the integrator replayed from recorded HIP graphs give the same grids
+and images, bit for bit, as the same frames run call by call; eager work and
+synchronizations run between the replays, and a replay whose rebuild outgrows
+its buffers falls back and re-records."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import SEED, random_rays
+from happyray_amd import GRID_TWO_LEVEL, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, Engine, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def make_engine(sc, keys, kind, lt, ll):
+    e = Engine(seed=SEED)
+    e.set_keyframes(keys, sc.idx)
+    e.upload_materials(sc.materials, sc.mat_idx)
+    e.set_grid(kind, lt, ll)
+    e.set_camera_dict(sc.camera)
+    e.set_lights(sc.lights)
+    return e
+
+
+def grids_equal(a, b):
+    ia, ca, la, pa = a.grid()
+    ib, cb, lb, pb = b.grid()
+    assert tuple(ia.res) == tuple(ib.res)
+    assert ia.num_refs == ib.num_refs and ia.num_cells == ib.num_cells
+    np.testing.assert_array_equal(ca, cb)
+    if la is not None or lb is not None:
+        np.testing.assert_array_equal(la, lb)
+    np.testing.assert_array_equal(pa, pb)
+
+
+@pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO])
+def test_frame_graph_matches_uncaptured_frames(mode):
+    sc, keys = scenes.heightfield_keyframes(quads=64, frames=5, res=(64, 48))
+    g = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    e = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    org, d = random_rays(sc, 4000, seed=9)
+    kw = dict(jitter=1, pt_depth=3) if mode == MODE_PATH else dict(jitter=1, ao_samples=4, ao_dist=5.0)
+    for i, cur in enumerate([0.0, 0.4, 1.7, 3.9, 2.2, 0.1, 4.5]):
+        k1, k2, c = O.frame_ids(cur, 0.4, 5)
+        image = i % 3
+        print(f"graph frame mode={mode} t={cur} keys=({k1},{k2}) image={image}", flush=True)
+        fb_g = g.render_frame_graph(k1, k2, c, image_id=image, mode=mode, **kw)
+        e.set_frame(k1, k2, c)
+        e.build_acc_struct()
+        fb_e = e.render_frame(image, mode, **kw)
+        np.testing.assert_array_equal(fb_g, fb_e)
+        grids_equal(g, e)
+        # eager work on the captured context between replays
+        t_g, p_g = g.trace(org, d)
+        t_e, p_e = e.trace(org, d)
+        np.testing.assert_array_equal(t_g, t_e)
+        np.testing.assert_array_equal(p_g, p_e)
+        tm = g.timing()
+        assert tm.build_ms > 0 and tm.render_ms > 0
+    g.close()
+    e.close()
+
+
+def test_frame_graph_regrows_after_overflow():
+    """Key frame 1 turns small triangles into large ones: the replayed rebuild
+    overflows the pair buffers sized for key frame 0; the frame is rendered
+    again without graphs and the graphs are recorded anew."""
+    rng = np.random.default_rng(21)
+    n = 3000
+    c = rng.uniform(0, 10, (n, 1, 3))
+    small = c + rng.normal(0, 0.02, (n, 3, 3))
+    big = c + rng.normal(0, 2.5, (n, 3, 3))
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    keys = np.stack([small.reshape(-1, 3).T, big.reshape(-1, 3).T]).astype(np.float32)
+    sc = scenes._assemble([(small.reshape(-1, 3), idx, 0)], [scenes.material((0.3, 0.3, 0.3))], "soup",
+                          dict(pos=(5.0, 5.0, 30.0), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov=50.0,
+                               res=(48, 40)),
+                          [scenes.area_light((2.0, 14.0, 2.0), (6.0, 0.0, 0.0), (0.0, 0.0, 6.0),
+                                             (20.0, 20.0, 20.0))])
+    for kind, lt, ll in ((GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)):
+        g = make_engine(sc, keys, kind, lt, ll)
+        e = make_engine(sc, keys, kind, lt, ll)
+        for i, (k1, k2, cf) in enumerate([(0, 0, 0.0), (0, 0, 0.0), (1, 1, 0.0), (1, 1, 0.0), (0, 1, 0.5)]):
+            print(f"graph regrowth kind={kind} frame={i}", flush=True)
+            fb_g = g.render_frame_graph(k1, k2, cf, image_id=0, mode=MODE_PATH, pt_depth=2)
+            e.set_frame(k1, k2, cf)
+            e.build_acc_struct()
+            fb_e = e.render_frame(0, MODE_PATH, pt_depth=2)
+            np.testing.assert_array_equal(fb_g, fb_e)
+            grids_equal(g, e)
+        g.close()
+        e.close()
