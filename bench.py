@@ -42,6 +42,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
+    ap.add_argument("--workload", default="soup_ao", choices=["soup_ao", "keyframes_pt"],
+                    help="soup_ao: 1M-triangle two-level rebuild + AO frame (default); "
+                         "keyframes_pt: 708x708-quad heightfield over 30 key frames, "
+                         "interpolation + rebuild + path tracing per frame, captured as HIP graphs")
     return ap.parse_args()
 
 
@@ -71,6 +75,21 @@ def roofline_ao(ws, npix_hit, ao_rays, kernel_ms):
     return bytes_alg, flops, gbs, tflops
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_soup_in_room_ao_1024.md")
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed two-counter
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    import re
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch " + re.escape(kernel) + r" = (\d+)$", text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(scene, cam, ao_dist, threads):
     """The CPU oracle (a scalar port of the reference algorithm) on the same
     frame -- scene, view, 1024x1024 primary rays, 8 AO rays per hit -- on
@@ -97,12 +116,107 @@ def cpu_baseline(scene, cam, ao_dist, threads):
             "build_ms_1thread": round(build_s * 1e3, 2)}
 
 
+def cpu_baseline_pt(scene, keys, cam, threads, t_frame):
+    """The CPU oracle on one frame of the key-frame workload: interpolation,
+    two-level build (1 thread) and the path-traced frame on `threads` threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    k1, k2, c = O.frame_ids(t_frame, 0.4, keys.shape[0])
+    t0 = time.perf_counter()
+    v, bmin, bmax = O.lerp_bounds(keys[k1], keys[k2], c)
+    m = O.OMesh(v[0], v[1], v[2], scene.idx)
+    g = O.OGrid(m, bmin, bmax, O.GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    build_s = time.perf_counter() - t0
+    oc = O.camera(cam["pos"], cam["dir"], cam["up"], cam["fov"], *cam["res"])
+    p = O.params(seed=(0x1234, 0x5678), jitter=1, pt_depth=4, nthreads=threads)
+    t0 = time.perf_counter()
+    _, st = O.render_pt(g, oc, p, scene.materials, scene.mat_idx, scene.lights)   # primary rays included
+    render_s = time.perf_counter() - t0
+    rays = st.rays
+    return {"value": round(rays / render_s / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"one path-traced frame of the same workload ({rays} rays, {render_s:.2f} s on "
+                      f"{threads} threads); interpolation + two-level build of all {scene.nt} "
+                      f"triangles on 1 thread: {build_s * 1e3:.1f} ms",
+            "build_ms_1thread": round(build_s * 1e3, 2)}
+
+
+def main_keyframes(args):
+    """Config: 708x708-quad heightfield (1,002,528 triangles, 502,681 vertices)
+    over 30 key frames; per frame: interpolation (step 0.4), full two-level
+    rebuild (top 1/16, leaf 3), 1024x768 primary rays and 1-spp path tracing
+    with 4 path vertices and Russian roulette, all replayed from HIP graphs."""
+    from happyray_amd import Engine, GRID_TWO_LEVEL, MODE_PATH, scenes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    scene, keys = scenes.heightfield_keyframes()
+    cam = scene.camera
+    e = Engine(seed=(0x1234, 0x5678))
+    e.set_keyframes(keys, scene.idx)
+    e.upload_materials(scene.materials, scene.mat_idx)
+    e.set_lights(scene.lights)
+    e.set_camera_dict(cam)
+    e.set_grid(GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    nf = keys.shape[0]
+    step_size = 0.4
+
+    def frame_ids(t):
+        k1 = int(t)
+        return k1, (k1 + 1 + int(step_size)) % nf, np.float32(t) - np.float32(k1)
+
+    state = {"t": np.float32(0.0)}
+
+    def step(i):
+        k1, k2, c = frame_ids(float(state["t"]))
+        e.render_frame_graph(k1, k2, float(c), image_id=i, mode=MODE_PATH, jitter=1, pt_depth=4,
+                             download=False)
+        t = np.float32(state["t"]) + np.float32(step_size)
+        if int(t) >= nf:
+            t -= np.float32(nf)
+        state["t"] = t
+
+    for i in range(args.warmup):
+        step(i)
+    e.synchronize()
+    r0 = e.timing().rays_traced
+    build_ms, render_ms = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+        tm = e.timing()
+        build_ms.append(tm.build_ms)
+        render_ms.append(tm.render_ms)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    rays = e.timing().rays_traced - r0
+    info = e.grid_info()
+    out = {
+        "metric": METRIC, "value": round(rays / wall / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "keyframes_pt: 708x708-quad heightfield (1,002,528 triangles, 502,681 "
+                               "vertices), 30 key frames of a seeded sum of 4 sinusoids, step 0.4; per "
+                               "frame interpolation + two-level rebuild (top 1/16, leaf 3) + 1024x768 "
+                               "1-spp path tracing (4 vertices, Russian roulette), replayed HIP graphs",
+                   "triangles": scene.nt, "rays_per_step": int(rays // args.steps),
+                   "top_res": list(info.res), "leaves": info.num_leaves, "leaf_refs": info.num_refs},
+        "rebuild_ms": round(float(np.mean(build_ms)), 4),
+        "render_ms": round(float(np.mean(render_ms)), 4),
+        "render_mrays_s": round(rays / args.steps / (float(np.mean(render_ms)) * 1e-3) / 1e6, 3),
+    }
+    e.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_pt(scene, keys, cam, args.cpu_threads, 0.0)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
                                    "note (DESIGN.md), not a benchmark", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "keyframes_pt":
+        return main_keyframes(args)
     from happyray_amd import Engine, GRID_TWO_LEVEL, MODE_AO, scenes
 
     scene = scenes.soup_in_room()
@@ -150,6 +264,8 @@ def main():
     bytes_alg, flops, gbs, tflops = roofline_ao(ws, npix_hit, ao_rays, ao_avg)
     render_ms = np.array(prim_ms) + np.array(ao_ms) + np.array(shade_ms)
     info = e.grid_info()
+    tun = e.tuning()
+    ao_kernel = "k_persist<true, true, GenAO>" if tun.persistent_secondary else "k_ao_rays<true>"
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1,
@@ -168,9 +284,10 @@ def main():
         "render_mrays_s": round(rays / args.steps / (float(np.mean(render_ms)) * 1e-3) / 1e6, 3),
         "kernel_ms": {"primary": round(float(np.mean(prim_ms)), 4), "ao_rays": round(ao_avg, 4),
                       "ao_shade": round(float(np.mean(shade_ms)), 4)},
-        "roofline": {"kernel": "k_ao_rays<two_level>", "bound": "hbm", "achieved": round(gbs, 2),
+        "roofline": {"kernel": ao_kernel, "bound": "hbm", "achieved": round(gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
-                     "traffic": None,
+                     "traffic": committed_traffic(ao_kernel),
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                      "bytes_per_launch": int(bytes_alg),
                      "valu": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 5),

@@ -7,6 +7,7 @@ from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MO
                    HappyRayError, LIB_PATH, load)
 from .engine import Engine
 from .scenefile import SceneFile
+from .batch import Batch
 
-__all__ = ["Engine", "SceneFile", "HappyRayError", "load", "LIB_PATH", "GRID_UNIFORM", "GRID_UNIFORM_EXACT",
+__all__ = ["Engine", "Batch", "SceneFile", "HappyRayError", "load", "LIB_PATH", "GRID_UNIFORM", "GRID_UNIFORM_EXACT",
            "GRID_TWO_LEVEL", "MODE_AO", "MODE_PATH"]

@@ -67,6 +67,25 @@ class SceneInfo(C.Structure):
                 ("cam_fov", C.c_float), ("cam_res_x", C.c_int), ("cam_res_y", C.c_int)]
 
 
+class MeshDesc(C.Structure):
+    _fields_ = [("xyz", _f32p), ("nv", C.c_uint32), ("idx", _u32p), ("nt", C.c_uint32)]
+
+
+class RayBatch(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("n", C.c_uint32), ("anyhit", C.c_int), ("tmax", C.c_float),
+                ("org", _vp), ("dir", _vp), ("t_out", _vp), ("prim_out", _vp), ("occ_out", _vp)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("refill", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+class AutotuneResult(C.Structure):
+    _fields_ = [("lambda_top", C.c_float * 4), ("lambda_leaf", C.c_float * 4), ("tuning", BatchTuning),
+                ("evaluations", C.c_int), ("best_build_trace_ms", C.c_float),
+                ("best_trace_ms", C.c_float)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "hr_last_error": (C.c_char_p, []),
@@ -96,6 +115,22 @@ SIGNATURES = {
     "hr_synchronize": (C.c_int, [_vp]),
     "hr_render_frame_graph": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int,
                                          C.POINTER(RenderParams), _f32p]),
+    "hr_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(Options)]),
+    "hr_batch_destroy": (C.c_int, [_vp]),
+    "hr_batch_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "hr_batch_set_scenes": (C.c_int, [_vp, C.POINTER(MeshDesc), C.c_uint32]),
+    "hr_batch_set_densities": (C.c_int, [_vp, _f32p, _f32p]),
+    "hr_batch_build": (C.c_int, [_vp]),
+    "hr_batch_get_grid_info": (C.c_int, [_vp, C.c_uint32, C.POINTER(GridInfo)]),
+    "hr_batch_get_grid": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, _u32p]),
+    "hr_batch_trace_dev": (C.c_int, [_vp, C.POINTER(RayBatch), C.c_uint32]),
+    "hr_batch_set_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning)]),
+    "hr_batch_get_tuning": (C.c_int, [_vp, C.POINTER(BatchTuning), _f32p, _f32p]),
+    "hr_batch_autotune": (C.c_int, [_vp, C.POINTER(RayBatch), C.c_uint32, C.POINTER(AutotuneResult)]),
+    "hr_batch_get_timing": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_uint64)]),
+    "hr_batch_get_work_stats": (C.c_int, [_vp, C.POINTER(WorkStats)]),
+    "hr_batch_synchronize": (C.c_int, [_vp]),
     "hr_set_normals": (C.c_int, [_vp, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint32]),
     "hr_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "hr_scene_load_obj": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),

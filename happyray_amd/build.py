@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhappyray.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_persist.hip", "hr_engine.hip"]
+SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_persist.hip", "hr_engine.hip", "hr_batch.hip"]
 CXX_SOURCES = ["hr_scene.cpp"]          # host-only C++ (scene files)
 ARCH = os.environ.get("HR_OFFLOAD_ARCH", "gfx950")
 

@@ -39,6 +39,25 @@ struct BuildBuffers {
     SortTemp sort;
 };
 
+// The scenes of a batch, built together: the triangles, cells, leaves and
+// references of all scenes live in one set of arrays, and the cell keys are
+// global cell indices (scene s owns cells [cell_base[s], cell_base[s+1])).
+struct SceneTable {
+    uint32_t ns;
+    const uint32_t* tri_scene;   // per triangle: its scene
+    const uint32_t* tri_base;    // ns + 1 triangle offsets
+    const uint32_t* vert_base;   // ns + 1 vertex offsets
+    const float* lambda_top;     // per scene
+    const float* lambda_leaf;    // per scene
+    GridParams* gps;             // per scene
+    uint32_t* bounds_enc;        // 6 per scene
+    uint32_t* cell_counts;       // per scene
+    uint32_t* cell_base;         // ns + 1
+    uint32_t* top_base;          // ns + 1: first top-level pair of each scene
+    uint32_t* leaf_base;         // ns + 1: first leaf of each scene
+    uint32_t* pair_base;         // ns + 1: first leaf-level reference of each scene
+};
+
 void launch_reset(BuildStatus* st, hipStream_t s);
 // fs != nullptr: the key frames are blocks fs->k1, fs->k2 of keys and the
 // coefficient is fs->coeff, all read on the device
@@ -49,5 +68,11 @@ void launch_grid_setup(GridParams* gp, BuildStatus* st, uint32_t nprims, float l
                        uint64_t cells_cap, hipStream_t s);
 // everything after the bounds: setup, pairs, sort, ranges (and the leaf level)
 void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s);
+// two-level build of every scene of a batch in one pass: reset, per-scene
+// bounds, per-scene resolution, then the same count / scan / write / sort /
+// range kernels over all scenes at once.  a.idx holds global vertex indices,
+// a.nt is the total triangle count; b.params receives the total cell count.
+void run_build_batch(const BuildArgs& a, BuildBuffers& b, const SceneTable& T, uint32_t nv_total,
+                     hipStream_t s);
 
 }  // namespace hr

@@ -107,13 +107,12 @@ __global__ __launch_bounds__(kBlock) void k_lerp_bounds(
 // Bounds padding (WFObjectUploader.h:112-114) and resolution from density
 // (UGridSortBuilder.h:78-92 / TLGridSortBuilder.h:81-98, checkResolution
 // UGridMemoryManager.cu:338-346); cell size and its reciprocal as the memory
-// managers compute them (UGridMemoryManager.h:107-115).
-__global__ void k_grid_setup(GridParams* __restrict__ gp, BuildStatus* __restrict__ st,
-                             uint32_t nprims, float lambda, uint64_t cells_cap)
+// managers compute them (UGridMemoryManager.h:107-115).  Fills every field
+// of gp but num_cells and returns the cell count.
+__device__ uint64_t grid_from_bounds(const uint32_t* bounds_enc, uint32_t nprims, float lambda, GridParams* gp)
 {
-    if (threadIdx.x != 0) return;
-    f3 lo = mk(dec_f(st->bounds_enc[0]), dec_f(st->bounds_enc[1]), dec_f(st->bounds_enc[2]));
-    f3 hi = mk(dec_f(st->bounds_enc[3]), dec_f(st->bounds_enc[4]), dec_f(st->bounds_enc[5]));
+    f3 lo = mk(dec_f(bounds_enc[0]), dec_f(bounds_enc[1]), dec_f(bounds_enc[2]));
+    f3 hi = mk(dec_f(bounds_enc[3]), dec_f(bounds_enc[4]), dec_f(bounds_enc[5]));
     const float diag = len(hi - lo);
     lo = lo - rep(0.001f * diag);
     hi = hi + rep(0.001f * diag);
@@ -138,7 +137,14 @@ __global__ void k_grid_setup(GridParams* __restrict__ gp, BuildStatus* __restric
     gp->res[0] = res[0]; gp->res[1] = res[1]; gp->res[2] = res[2];
     gp->cs[0] = cs.x; gp->cs[1] = cs.y; gp->cs[2] = cs.z;
     gp->csr[0] = csr.x; gp->csr[1] = csr.y; gp->csr[2] = csr.z;
-    const uint64_t nc = (uint64_t)res[0] * (uint64_t)res[1] * (uint64_t)res[2];
+    return (uint64_t)res[0] * (uint64_t)res[1] * (uint64_t)res[2];
+}
+
+__global__ void k_grid_setup(GridParams* __restrict__ gp, BuildStatus* __restrict__ st,
+                             uint32_t nprims, float lambda, uint64_t cells_cap)
+{
+    if (threadIdx.x != 0) return;
+    const uint64_t nc = grid_from_bounds(st->bounds_enc, nprims, lambda, gp);
     if (nc > cells_cap || nc > 0xFFFFFFF0ull) {
         st->overflow |= kOverflowCells;
         st->num_cells = nc > 0xFFFFFFF0ull ? 0xFFFFFFFFu : (uint32_t)nc;
@@ -149,7 +155,6 @@ __global__ void k_grid_setup(GridParams* __restrict__ gp, BuildStatus* __restric
     }
 }
 
-// total > cap sets the overflow bit and zeroes the effective size
 // After an overflow the grid is unusable: a zero resolution makes every ray
 // miss at its entry test, so kernels that run after the build without a host
 // check (captured frames) read nothing out of bounds.
@@ -160,6 +165,7 @@ __global__ void k_seal(GridParams* gp, const BuildStatus* st)
     gp->num_cells = 0u;
 }
 
+// total > cap sets the overflow bit and zeroes the effective size
 __global__ void k_check(BuildStatus* st, uint32_t* total, uint32_t* eff, uint32_t cap, uint32_t bit)
 {
     if (threadIdx.x != 0) return;
@@ -235,16 +241,34 @@ struct WaveItems {
     // leaf level
     uint32_t lbeg[kWave];
     uint32_t r0[kWave], r1[kWave];
+    // scenes of a batch: per-owner grid width, z stride and first cell
+    uint32_t gx[kWave], gz[kWave], cbase[kWave];
 };
 
-template <bool kExact, bool kWrite>
+// scene of a global cell index: the last s with cell_base[s] <= c (cell
+// counts are positive, so the bases increase strictly)
+__device__ __forceinline__ uint32_t scene_of(const uint32_t* base, uint32_t ns, uint32_t c)
+{
+    uint32_t lo = 0, hi = ns;          // base[lo] <= c < base[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// kMulti: the triangles of a batch of scenes; each lane takes the grid of its
+// triangle's scene and writes global cell keys (cell_base of the scene + key)
+template <bool kExact, bool kWrite, bool kMulti = false>
 __global__ __launch_bounds__(kBlock) void k_top_pairs(
     const uint32_t* __restrict__ idx, const float* __restrict__ vx, const float* __restrict__ vy,
     const float* __restrict__ vz, uint32_t nt, const GridParams* __restrict__ gp,
     const BuildStatus* __restrict__ st, float4* __restrict__ rec, uint32_t* __restrict__ wave_counts,
     const uint32_t* __restrict__ wave_offsets, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-    uint32_t waves_cap)
+    uint32_t waves_cap, SceneTable T = SceneTable{})
 {
+    static_assert(!(kExact && kMulti), "batches use bounding-box insertion");
     __shared__ WaveItems sh[kWavesPerBlock];
     if (stop(st)) return;
     const int wl = (int)threadIdx.x / kWave;
@@ -257,6 +281,12 @@ __global__ __launch_bounds__(kBlock) void k_top_pairs(
         return;
     }
 
+    uint32_t cbase = 0;
+    if (kMulti) {
+        const uint32_t sc = T.tri_scene[valid ? p : nt - 1u];
+        gp = T.gps + sc;
+        cbase = T.cell_base[sc];
+    }
     const f3 bmin = mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]);
     const f3 cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
     const f3 csr = mk(gp->csr[0], gp->csr[1], gp->csr[2]);
@@ -298,6 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_top_pairs(
     W.dx[lane] = mx[0] - mn[0];
     W.dy[lane] = mx[1] - mn[1];
     W.id[lane] = p;
+    if (kMulti) {
+        W.gx[lane] = (uint32_t)rx;
+        W.gz[lane] = kcell_z;
+        W.cbase[lane] = cbase;
+    }
     if (kExact) {
         W.v[0][lane] = v0.x; W.v[1][lane] = v0.y; W.v[2][lane] = v0.z;
         W.v[3][lane] = v1.x; W.v[4][lane] = v1.y; W.v[5][lane] = v1.z;
@@ -327,7 +362,8 @@ __global__ __launch_bounds__(kBlock) void k_top_pairs(
             const int iy = t % (int)dyo, iz = t / (int)dyo;
             const int x = W.lo[0][o] + ix, y = W.lo[1][o] + iy, z = W.lo[2][o] + iz;
             id = W.id[o];
-            key = (uint32_t)x + (uint32_t)y * (uint32_t)rx + (uint32_t)z * kcell_z;
+            if (kMulti) key = W.cbase[o] + (uint32_t)x + (uint32_t)y * W.gx[o] + (uint32_t)z * W.gz[o];
+            else key = (uint32_t)x + (uint32_t)y * (uint32_t)rx + (uint32_t)z * kcell_z;
             if (kExact) {
                 const f3 c = mk(centre_along(W.mcc[0][o], cs.x, ix), centre_along(W.mcc[1][o], cs.y, iy),
                                 centre_along(W.mcc[2][o], cs.z, iz));
@@ -371,15 +407,23 @@ __global__ void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* __re
 
 // countLeafLevelCells<false> (TLGridBuildKernels.h:57-114): leaf resolution
 // per top cell and the packed TwoLevelGridCell (TwoLevelGrid.h:38-161)
+template <bool kMulti = false>
 __global__ void k_leaf_res(const GridParams* __restrict__ gp, uint2* __restrict__ cells,
                            uint32_t* __restrict__ counts, float lambda_leaf,
-                           const BuildStatus* __restrict__ st)
+                           const BuildStatus* __restrict__ st, SceneTable T = SceneTable{})
 {
     if (stop(st)) return;
     const uint32_t nc = gp->num_cells;
-    const f3 cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
-    const float vol = cs.x * cs.y * cs.z;
+    f3 cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
+    float vol = cs.x * cs.y * cs.z;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        if (kMulti) {
+            const uint32_t sc = scene_of(T.cell_base, T.ns, c);
+            const GridParams* g = T.gps + sc;
+            cs = mk(g->cs[0], g->cs[1], g->cs[2]);
+            vol = cs.x * cs.y * cs.z;
+            lambda_leaf = T.lambda_leaf[sc];
+        }
         const uint2 r = cells[c];
         const uint32_t n = r.y - r.x;
         const float magic = spec_powf(lambda_leaf * (float)n / vol, 0.3333333f);
@@ -399,14 +443,14 @@ __global__ void k_leaf_res(const GridParams* __restrict__ gp, uint2* __restrict_
 
 // countLeafLevelPairs / writeLeafLevelKeysAndValues
 // (TLGridBuildKernels.h:304-409, :536-663) over the sorted top pairs
-template <bool kWrite>
+template <bool kWrite, bool kMulti = false>
 __global__ __launch_bounds__(kBlock) void k_leaf_pairs(
     const uint32_t* __restrict__ top_keys, const uint32_t* __restrict__ top_vals,
     const uint32_t* __restrict__ n_top, const uint2* __restrict__ cells,
     const GridParams* __restrict__ gp, const BuildStatus* __restrict__ st,
     const float4* __restrict__ rec, uint32_t* __restrict__ wave_counts,
     const uint32_t* __restrict__ wave_offsets, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-    uint32_t waves_cap)
+    uint32_t waves_cap, SceneTable T = SceneTable{})
 {
     __shared__ WaveItems sh[kWavesPerBlock];
     if (stop(st)) return;
@@ -421,6 +465,12 @@ __global__ __launch_bounds__(kBlock) void k_leaf_pairs(
     const uint32_t i = wg * kWave + (uint32_t)lane;
     const bool valid = i < n;
 
+    uint32_t cbase = 0;
+    if (kMulti) {
+        const uint32_t sc = T.tri_scene[top_vals[valid ? i : n - 1u]];
+        gp = T.gps + sc;
+        cbase = T.cell_base[sc];
+    }
     const f3 bmin = mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]);
     const f3 cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
     const uint32_t gx = (uint32_t)gp->res[0], gy = (uint32_t)gp->res[1];
@@ -428,7 +478,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_pairs(
     uint32_t cnt = 0, prim = 0, lb = 0, r0 = 1, r1 = 1;
     int mn[3] = { 0, 0, 0 }, mx[3] = { 0, 0, 0 };
     if (valid) {
-        const uint32_t key = top_keys[i];
+        const uint32_t gkey = top_keys[i];
+        const uint32_t key = gkey - cbase;
         prim = top_vals[i];
         f3 v0, v1, v2, lo, hi;
         load_tri(rec, prim, v0, v1, v2);
@@ -436,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_pairs(
         const uint32_t tx = key % gx;
         const uint32_t ty = (key % (gx * gy)) / gx;
         const uint32_t tz = key / (gx * gy);
-        const uint2 cell = cells[key];
+        const uint2 cell = cells[gkey];
         lb = cell.x;
         const int c0 = (int)(cell.y & 0xFFu), c1 = (int)((cell.y >> 8) & 0xFFu),
                   c2 = (int)((cell.y >> 16) & 0xFFu);
@@ -494,6 +545,123 @@ __global__ __launch_bounds__(kBlock) void k_leaf_pairs(
             vals[pos] = id;
         }
         run += (uint32_t)__popcll(bal);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batches of scenes
+// ---------------------------------------------------------------------------
+
+__global__ void k_batch_reset(SceneTable T)
+{
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T.ns; s += gridDim.x * blockDim.x)
+        for (int i = 0; i < 3; ++i) {
+            T.bounds_enc[6 * s + i] = 0xFFFFFFFFu;
+            T.bounds_enc[6 * s + 3 + i] = 0u;
+        }
+}
+
+// per-scene bounds (the reduction of k_lerp_bounds, over static vertices):
+// a wave whose 64 vertices share a scene reduces them and issues one atomic
+// per bound; a wave across a scene boundary updates per lane
+__global__ __launch_bounds__(kBlock) void k_batch_bounds(const float* __restrict__ vx,
+                                                         const float* __restrict__ vy,
+                                                         const float* __restrict__ vz, uint32_t nv,
+                                                         SceneTable T)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~(uint32_t)(kWave - 1); base < nv;
+         base += stride) {
+        const uint32_t i = base + __lane_id();
+        const bool valid = i < nv;
+        f3 v = rep(0.f);
+        if (valid) v = mk(vx[i], vy[i], vz[i]);
+        const uint32_t last = (base + kWave - 1 < nv ? base + kWave - 1 : nv - 1);
+        const uint32_t s0 = scene_of(T.vert_base, T.ns, base);
+        const uint32_t s1 = scene_of(T.vert_base, T.ns, last);
+        if (s0 == s1) {
+            f3 lo = valid ? v : rep(kFltMax), hi = valid ? v : rep(-kFltMax);
+            lo.x = wave_min(lo.x); lo.y = wave_min(lo.y); lo.z = wave_min(lo.z);
+            hi.x = wave_max(hi.x); hi.y = wave_max(hi.y); hi.z = wave_max(hi.z);
+            if (__lane_id() == 0) {
+                uint32_t* b = T.bounds_enc + 6 * (size_t)s0;
+                atomicMin(b + 0, enc_f(lo.x)); atomicMin(b + 1, enc_f(lo.y)); atomicMin(b + 2, enc_f(lo.z));
+                atomicMax(b + 3, enc_f(hi.x)); atomicMax(b + 4, enc_f(hi.y)); atomicMax(b + 5, enc_f(hi.z));
+            }
+        } else if (valid) {
+            uint32_t* b = T.bounds_enc + 6 * (size_t)scene_of(T.vert_base, T.ns, i);
+            atomicMin(b + 0, enc_f(v.x)); atomicMin(b + 1, enc_f(v.y)); atomicMin(b + 2, enc_f(v.z));
+            atomicMax(b + 3, enc_f(v.x)); atomicMax(b + 4, enc_f(v.y)); atomicMax(b + 5, enc_f(v.z));
+        }
+    }
+}
+
+// per-scene resolution and cell count
+__global__ void k_batch_setup(SceneTable T, BuildStatus* __restrict__ st)
+{
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T.ns; s += gridDim.x * blockDim.x) {
+        const uint32_t nprims = T.tri_base[s + 1] - T.tri_base[s];
+        GridParams* g = T.gps + s;
+        const uint64_t nc = grid_from_bounds(T.bounds_enc + 6 * (size_t)s, nprims, T.lambda_top[s], g);
+        if (nc > 0xFFFFFFF0ull) {
+            atomicOr(&st->overflow, kOverflowCells);
+            T.cell_counts[s] = 0u;
+            g->num_cells = 0u;
+        } else {
+            T.cell_counts[s] = (uint32_t)nc;
+            g->num_cells = (uint32_t)nc;
+        }
+    }
+}
+
+// total cells after the scan of the per-scene counts
+__global__ void k_batch_total(SceneTable T, GridParams* agg, BuildStatus* st, uint64_t cells_cap)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t total = st->num_cells;
+    T.cell_base[T.ns] = total;
+    if ((uint64_t)total > cells_cap || (st->overflow & kOverflowCells)) {
+        st->overflow |= kOverflowCells;
+        agg->num_cells = 0u;
+    } else {
+        agg->num_cells = total;
+    }
+}
+
+// first position of each scene in sorted keys: lower bound of the scene's
+// first key (top level: its first cell; leaf level: its first leaf)
+template <bool kLeaf>
+__global__ void k_batch_bases(SceneTable T, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_dev,
+                              const uint2* __restrict__ cells, const uint32_t* __restrict__ num_leaves,
+                              const BuildStatus* __restrict__ st)
+{
+    if (stop(st)) return;
+    const uint32_t n = *n_dev;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= T.ns; s += gridDim.x * blockDim.x) {
+        uint32_t first;
+        if (kLeaf) {
+            first = s < T.ns ? cells[T.cell_base[s]].x : *num_leaves;
+            T.leaf_base[s] = first;
+        } else {
+            first = T.cell_base[s];
+        }
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (keys[mid] < first) lo = mid + 1;
+            else hi = mid;
+        }
+        if (kLeaf) T.pair_base[s] = s < T.ns ? lo : n;
+        else T.top_base[s] = s < T.ns ? lo : n;
+    }
+}
+
+__global__ void k_batch_seal(SceneTable T, const BuildStatus* __restrict__ st)
+{
+    if (st->overflow == 0u) return;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T.ns; s += gridDim.x * blockDim.x) {
+        T.gps[s].res[0] = T.gps[s].res[1] = T.gps[s].res[2] = 0;
+        T.gps[s].num_cells = 0u;
     }
 }
 
@@ -606,6 +774,81 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     hipLaunchKernelGGL(k_ranges, dim3(lrblocks), dim3(256), 0, s, b.keys_sorted, &st->n_leaf_pairs_eff,
                        b.leaves, st);
     hipLaunchKernelGGL(k_seal, dim3(1), dim3(64), 0, s, b.params, st);
+}
+
+}  // namespace hr
+
+namespace hr {
+
+void run_build_batch(const BuildArgs& a, BuildBuffers& b, const SceneTable& T, uint32_t nv_total, hipStream_t s)
+{
+    const uint32_t nt = a.nt;
+    const uint32_t waves = cdiv(nt > 0 ? nt : 1, kWave);
+    const uint32_t blocks = cdiv(waves, kWavesPerBlock);
+    BuildStatus* st = b.status;
+    const uint32_t sblocks = cdiv(T.ns + 1, 256);
+
+    launch_reset(st, s);
+    hipLaunchKernelGGL(k_batch_reset, dim3(sblocks), dim3(256), 0, s, T);
+    uint32_t vblocks = cdiv(nv_total > 0 ? nv_total : 1, kBlock);
+    if (vblocks > 2048) vblocks = 2048;
+    hipLaunchKernelGGL(k_batch_bounds, dim3(vblocks), dim3(kBlock), 0, s, a.vx, a.vy, a.vz, nv_total, T);
+    hipLaunchKernelGGL(k_batch_setup, dim3(sblocks), dim3(256), 0, s, T, st);
+    scan_exclusive(T.cell_counts, T.cell_base, T.ns, nullptr, &st->num_cells, b.scan, s);
+    hipLaunchKernelGGL(k_batch_total, dim3(1), dim3(64), 0, s, T, b.params, st, b.cells_cap);
+
+    // ---- top level: count, scan, write ----
+    hipLaunchKernelGGL((k_top_pairs<false, false, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx, a.vy,
+                       a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap, T);
+    scan_exclusive(b.wave_counts, b.wave_counts, waves, nullptr, &st->num_pairs, b.scan, s);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_pairs, &st->n_pairs_eff, b.pairs_cap,
+                       kOverflowPairs);
+    hipLaunchKernelGGL((k_top_pairs<false, true, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx, a.vy,
+                       a.vz, nt, b.params, st, b.rec, nullptr, b.wave_counts, b.keys, b.vals, b.waves_cap, T);
+
+    // ---- sort by global cell key, cell ranges ----
+    radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.top_vals_sorted, b.pairs_cap, &st->n_pairs_eff,
+                     bits_for(b.cells_cap), b.sort, s);
+    hipLaunchKernelGGL((k_batch_bases<false>), dim3(sblocks), dim3(256), 0, s, T, b.keys_sorted, &st->n_pairs_eff,
+                       b.cells, nullptr, st);
+    fill_u32((uint32_t*)b.cells, 0u, 2 * (size_t)b.cells_cap, &b.params->num_cells, 2, s);
+    const uint32_t rblocks = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256) > 4096 ? 4096
+                                 : cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256);
+    hipLaunchKernelGGL(k_ranges, dim3(rblocks), dim3(256), 0, s, b.keys_sorted, &st->n_pairs_eff, b.cells, st);
+
+    // ---- leaf resolution, leaf begin offsets ----
+    const uint32_t cblocks = cdiv(b.cells_cap, 256) > 4096 ? 4096 : cdiv(b.cells_cap, 256);
+    hipLaunchKernelGGL((k_leaf_res<true>), dim3(cblocks), dim3(256), 0, s, b.params, b.cells, b.leaf_counts, 0.f,
+                       st, T);
+    scan_exclusive_strided(b.leaf_counts, (uint32_t*)b.cells, 2, (uint32_t)b.cells_cap, &b.params->num_cells,
+                           &st->num_leaves, b.scan, s);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaves, &st->n_leaves_eff, b.leaves_cap,
+                       kOverflowLeaves);
+
+    // ---- leaf pairs ----
+    const uint32_t lwaves = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, kWave);
+    const uint32_t lblocks = cdiv(lwaves, kWavesPerBlock);
+    hipLaunchKernelGGL((k_leaf_pairs<false, true>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
+                       b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, b.wave_counts, nullptr,
+                       nullptr, nullptr, b.waves_cap, T);
+    scan_exclusive(b.wave_counts, b.wave_counts, lwaves, nullptr, &st->num_leaf_pairs, b.scan, s);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaf_pairs, &st->n_leaf_pairs_eff,
+                       b.leaf_pairs_cap, kOverflowLeafPairs);
+    hipLaunchKernelGGL((k_leaf_pairs<true, true>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
+                       b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, nullptr, b.wave_counts,
+                       b.keys, b.vals, b.waves_cap, T);
+
+    // ---- leaf sort and ranges ----
+    radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.prims, b.leaf_pairs_cap, &st->n_leaf_pairs_eff,
+                     bits_for(b.leaves_cap), b.sort, s);
+    hipLaunchKernelGGL((k_batch_bases<true>), dim3(sblocks), dim3(256), 0, s, T, b.keys_sorted,
+                       &st->n_leaf_pairs_eff, b.cells, &st->n_leaves_eff, st);
+    fill_u32((uint32_t*)b.leaves, 0u, 2 * (size_t)b.leaves_cap, &st->n_leaves_eff, 2, s);
+    const uint32_t lrblocks = cdiv(b.leaf_pairs_cap > 0 ? b.leaf_pairs_cap : 1, 256) > 4096 ? 4096
+                                  : cdiv(b.leaf_pairs_cap > 0 ? b.leaf_pairs_cap : 1, 256);
+    hipLaunchKernelGGL(k_ranges, dim3(lrblocks), dim3(256), 0, s, b.keys_sorted, &st->n_leaf_pairs_eff, b.leaves,
+                       st);
+    hipLaunchKernelGGL(k_batch_seal, dim3(sblocks), dim3(256), 0, s, T, st);
 }
 
 }  // namespace hr

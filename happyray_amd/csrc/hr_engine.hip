@@ -19,63 +19,23 @@
 #include "../../include/happyray.h"
 #include "hr_build.h"
 #include "hr_error.h"
+#include "hr_host.h"
 #include "hr_persist.h"
 #include "hr_trace.h"
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-hr_status fail(hr_status s, const std::string& msg)
+hr_status hr::set_error(hr_status s, const std::string& msg)
 {
     g_last_error = msg;
     return s;
 }
 
-#define HR_HIP(call)                                                                            \
-    do {                                                                                        \
-        hipError_t e_ = (call);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            return fail(e_ == hipErrorOutOfMemory ? HR_ERR_OOM : HR_ERR_DEVICE,                 \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                     \
-    } while (0)
+using hr::DevBuf;
+using hr::fail;
 
-#define HR_TRY(call)                                                                            \
-    do {                                                                                        \
-        hr_status s_ = (call);                                                                  \
-        if (s_ != HR_OK) return s_;                                                             \
-    } while (0)
-
-// grow-only device buffer
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t cap = 0;
-    hr_status reserve(size_t n)
-    {
-        if (n <= cap && p) return HR_OK;
-        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
-        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-        hipError_t e = hipMalloc((void**)&p, bytes);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(HR_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
-                                        hipGetErrorString(e));
-        }
-        cap = std::max<size_t>(n, 1);
-        return HR_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-}  // namespace
-
-hr_status hr::set_error(hr_status s, const std::string& msg) { return fail(s, msg); }
 
 struct hr_ctx {
     int device = 0;

@@ -17,4 +17,33 @@ void persist_batch(const TraceScene& S, const float* org, const float* dir, uint
                    float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
                    unsigned long long* ray_count, unsigned long long* stats, hipStream_t s);
 
+// A scene of a batch as the traversal sees it: its grid (cells from the
+// scene's first cell; leaves, references and triangle records shared by the
+// batch, with global indices) and the offset of its triangle ids.
+struct SceneDesc {
+    TraceScene S;
+    uint32_t tri_base;   // global id of the scene's triangle 0
+    uint32_t nt;         // the scene's triangle count (miss id of its rays)
+};
+
+// One ray batch of a multi-scene launch (device pointers, SoA rays).
+struct RayBatchDev {
+    uint32_t scene;
+    uint32_t n;
+    float tmax;
+    uint32_t pad;
+    const float* org;
+    const float* dir;
+    float* t_out;        // closest hit: distance (FLT_MAX on a miss); any hit: optional
+    uint32_t* prim_out;  // closest hit: scene-local triangle id, the scene's nt on a miss
+    uint8_t* occ_out;    // any hit: 1 when occluded
+};
+
+// One persistent launch over the rays of nb batches of many scenes (all
+// closest-hit or all any-hit): rays [prefix[b], prefix[b+1]) belong to batch
+// b; lanes refill from a global counter and may hold rays of different scenes.
+void persist_multi(const SceneDesc* scenes, const RayBatchDev* batches, const uint32_t* prefix, uint32_t nb,
+                   uint32_t nrays, int anyhit, uint32_t* pool, uint32_t refill, uint32_t blocks_per_cu,
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s);
+
 }  // namespace hr

@@ -397,6 +397,97 @@ __global__ __launch_bounds__(kBlock) void k_persist(TraceScene S, Gen gen, uint3
     add_work(stats, started, wk);
 }
 
+// The multi-scene kernel: the loop of k_persist with a per-lane scene.  A
+// lane that starts a ray finds its batch by binary search over the ray
+// prefix, then loads that scene's descriptor and grid parameters.
+template <bool kAny>
+__global__ __launch_bounds__(kBlock) void k_persist_multi(const SceneDesc* __restrict__ scenes,
+                                                          const RayBatchDev* __restrict__ batches,
+                                                          const uint32_t* __restrict__ prefix, uint32_t nb,
+                                                          uint32_t nrays, uint32_t* __restrict__ pool,
+                                                          uint32_t refill, unsigned long long* ray_count,
+                                                          unsigned long long* stats)
+{
+    const uint64_t lt = __lanemask_lt();
+    RayState r;
+    Work wk;
+    TraceScene S = scenes[0].S;
+    GridRegs G = load_grid(S.gp);
+    uint32_t j = 0, bi = 0, started = 0;
+    bool live = false;
+    bool exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill || idle == ~0ull)) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(pool, nidle);
+            base = __shfl(base, 0, kWave);
+            if (base + nidle >= nrays) exhausted = true;
+            if (!live) {
+                j = base + (uint32_t)__popcll(idle & lt);
+                if (j < nrays) {
+                    uint32_t lo = 0, hi = nb;            // prefix[lo] <= j < prefix[hi]
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (prefix[mid] <= j) lo = mid;
+                        else hi = mid;
+                    }
+                    bi = lo;
+                    const RayBatchDev& B = batches[bi];
+                    const uint32_t jl = j - prefix[bi];
+                    S = scenes[B.scene].S;
+                    G = load_grid(S.gp);
+                    set_ray(r, mk(B.org[jl], B.org[(size_t)B.n + jl], B.org[2 * (size_t)B.n + jl]),
+                            mk(B.dir[jl], B.dir[(size_t)B.n + jl], B.dir[2 * (size_t)B.n + jl]));
+                    r.t = kAny ? B.tmax : kFltMax;
+                    r.tterm = B.tmax;
+                    ++started;
+                    live = ray_begin<true, kAny>(G, S, r, wk);
+                    j = jl;
+                    if (!live) {
+                        if (kAny) {
+                            B.occ_out[jl] = r.t < B.tmax ? 1u : 0u;
+                            if (B.t_out) B.t_out[jl] = r.t;
+                        } else {
+                            B.t_out[jl] = r.t;
+                            B.prim_out[jl] = scenes[B.scene].nt;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (live) {
+            bool done;
+            if (r.it < r.end) done = test_one<kAny>(S, r, wk);
+            else done = !advance_two_level<kAny>(G, S, r, wk);
+            if (done) {
+                live = false;
+                const RayBatchDev& B = batches[bi];
+                if (kAny) {
+                    B.occ_out[j] = r.t < B.tmax ? 1u : 0u;
+                    if (B.t_out) B.t_out[j] = r.t;
+                } else {
+                    const SceneDesc& D = scenes[B.scene];
+                    B.t_out[j] = r.t;
+                    B.prim_out[j] = r.t < kFltMax ? r.best - D.tri_base : D.nt;
+                }
+            }
+        }
+    }
+    if (ray_count) {
+        unsigned long long c = started;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if (__lane_id() == 0 && c) atomicAdd(ray_count, c);
+    }
+    add_work(stats, started, wk);
+}
+
 template <class K>
 uint32_t persistent_blocks(K kernel)
 {
@@ -460,6 +551,30 @@ void persist_batch(const TraceScene& S, const float* org, const float* dir, uint
         if (anyhit) launch<false, true>(S, g, n, pool, refill, ray_count, stats, s);
         else launch<false, false>(S, g, n, pool, refill, ray_count, stats, s);
     }
+}
+
+void persist_multi(const SceneDesc* scenes, const RayBatchDev* batches, const uint32_t* prefix, uint32_t nb,
+                   uint32_t nrays, int anyhit, uint32_t* pool, uint32_t refill, uint32_t blocks_per_cu,
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s)
+{
+    if (nrays == 0 || nb == 0) return;
+    fill_u32(pool, 0u, 1, nullptr, 1, s);
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+    uint32_t blocks = (uint32_t)cus * (blocks_per_cu ? blocks_per_cu : 1u);
+    if (blocks_per_cu == 0) {
+        blocks = anyhit ? persistent_blocks(k_persist_multi<true>) : persistent_blocks(k_persist_multi<false>);
+    }
+    const uint32_t need = (nrays + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    if (anyhit)
+        hipLaunchKernelGGL(k_persist_multi<true>, dim3(blocks), dim3(kBlock), 0, s, scenes, batches, prefix, nb, nrays,
+                           pool, refill, ray_count, stats);
+    else
+        hipLaunchKernelGGL(k_persist_multi<false>, dim3(blocks), dim3(kBlock), 0, s, scenes, batches, prefix, nb,
+                           nrays, pool, refill, ray_count, stats);
 }
 
 }  // namespace hr

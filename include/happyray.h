@@ -215,6 +215,76 @@ hr_status hr_get_timing(hr_ctx* ctx, hr_timing* out);
 hr_status hr_get_work_stats(hr_ctx* ctx, hr_work_stats* out);
 hr_status hr_synchronize(hr_ctx* ctx);
 
+/* ---- batches of scenes (no reference counterpart: the reference renders one scene
+ * per process, src/RT/RTEngine.h:36-68) ----
+ * A batch keeps many independent scenes resident on one device.  hr_batch_build
+ * builds the two-level grid of every scene in one segmented pass (the kernels of
+ * hr_build over all scenes at once); hr_batch_trace_dev traces ray batches of any
+ * scenes in persistent launches that mix the rays of all scenes.  Each scene's grid
+ * (read back scene-local by hr_batch_get_grid) equals the grid hr_build makes for
+ * that scene alone with the same densities, and each ray's result equals hr_trace /
+ * hr_occluded on that grid.  Calls on one batch must not overlap. */
+typedef struct hr_batch hr_batch;
+typedef struct {
+    const float* xyz;          /* host: SoA vertices, 3 * nv floats                */
+    uint32_t nv;
+    const uint32_t* idx;       /* host: 3 * nt vertex indices (scene-local)        */
+    uint32_t nt;
+} hr_mesh_desc;
+typedef struct {
+    uint32_t scene;            /* index into the batch's scenes                    */
+    uint32_t n;                /* rays                                            */
+    int anyhit;                /* 0: closest hit; 1: occluded within (1e-4, tmax) */
+    float tmax;                /* any hit only                                    */
+    const float* org;          /* device: SoA origins, 3 * n                       */
+    const float* dir;          /* device: SoA directions, 3 * n                    */
+    float* t_out;              /* device: closest-hit distance (optional for any)  */
+    uint32_t* prim_out;        /* device: scene-local triangle id, nt on a miss    */
+    uint8_t* occ_out;          /* device: any hit: 1 when occluded                 */
+} hr_ray_batch;
+typedef struct {
+    int refill;                /* idle lanes of a wave that trigger a fetch, 1..64 */
+    int blocks_per_cu;         /* persistent workgroups per CU, 0: occupancy limit */
+} hr_batch_tuning;
+typedef struct {
+    float lambda_top[4];       /* per size class: < 4K, < 16K, < 64K, >= 64K tris */
+    float lambda_leaf[4];
+    hr_batch_tuning tuning;
+    int evaluations;           /* timed candidates                                 */
+    float best_build_trace_ms; /* build + trace of the chosen densities            */
+    float best_trace_ms;       /* trace with the chosen persistence settings        */
+} hr_autotune_result;
+hr_status hr_batch_create(hr_batch** out, const hr_options* opts);
+hr_status hr_batch_destroy(hr_batch* batch);
+hr_status hr_batch_get_stream(hr_batch* batch, void** stream);
+/* upload the scenes (replacing any previous ones); densities reset to 1.2 / 2.2 */
+hr_status hr_batch_set_scenes(hr_batch* batch, const hr_mesh_desc* scenes, uint32_t n);
+/* per-scene top-level and leaf densities (NULL: 1.2 / 2.2, src/RT/RTEngine.cu:79-80) */
+hr_status hr_batch_set_densities(hr_batch* batch, const float* lambda_top, const float* lambda_leaf);
+hr_status hr_batch_build(hr_batch* batch);
+hr_status hr_batch_get_grid_info(hr_batch* batch, uint32_t scene, hr_grid_info* out);
+/* scene-local copy of one scene's grid, in the layout of hr_get_grid */
+hr_status hr_batch_get_grid(hr_batch* batch, uint32_t scene, uint32_t* cells, uint32_t* leaves,
+                            uint32_t* prims);
+/* enqueue the traces of nb ray batches (one closest-hit and one any-hit persistent
+ * launch); returns before they finish (hr_batch_synchronize) */
+hr_status hr_batch_trace_dev(hr_batch* batch, const hr_ray_batch* batches, uint32_t nb);
+hr_status hr_batch_set_tuning(hr_batch* batch, const hr_batch_tuning* t);
+/* current tuning and per-scene densities (arrays of the scene count, may be NULL) */
+hr_status hr_batch_get_tuning(hr_batch* batch, hr_batch_tuning* t, float* lambda_top,
+                              float* lambda_leaf);
+/* Choose densities per scene-size class and the persistence settings by timing
+ * builds and the traces of the given ray batches on the device; the choice stays
+ * set on the batch, which is left built with it. */
+hr_status hr_batch_autotune(hr_batch* batch, const hr_ray_batch* batches, uint32_t nb,
+                            hr_autotune_result* out);
+/* device times of the last build and the last trace; rays traced so far */
+hr_status hr_batch_get_timing(hr_batch* batch, double* build_ms, double* trace_ms,
+                              uint64_t* rays_traced);
+/* read and clear the work counters (primary: closest hit, secondary: any hit) */
+hr_status hr_batch_get_work_stats(hr_batch* batch, hr_work_stats* out);
+hr_status hr_batch_synchronize(hr_batch* batch);
+
 /* ---- scene files: SceneLoader::loadScene (src/Application/SceneLoader.cpp:212-332) ----
  * hr_scene_load parses a scene.cfg (src/Application/SceneConfigurationReader.cpp:29-190)
  * and the files it names: OBJ/MTL key frames (src/Application/WFObjectReader.cpp), a

@@ -42,7 +42,9 @@ def test_struct_layouts_match_header(tmp_path):
     structs = {"hr_options": _lib.Options, "hr_grid_info": _lib.GridInfo,
                "hr_render_params": _lib.RenderParams, "hr_timing": _lib.Timing,
                "hr_work_stats": _lib.WorkStats, "hr_tuning": _lib.Tuning,
-               "hr_scene_info": _lib.SceneInfo}
+               "hr_scene_info": _lib.SceneInfo, "hr_mesh_desc": _lib.MeshDesc,
+               "hr_ray_batch": _lib.RayBatch, "hr_batch_tuning": _lib.BatchTuning,
+               "hr_autotune_result": _lib.AutotuneResult}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "happyray.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
