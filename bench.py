@@ -42,10 +42,13 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
-    ap.add_argument("--workload", default="soup_ao", choices=["soup_ao", "keyframes_pt"],
+    ap.add_argument("--scenes", type=int, default=256, help="batched workload: number of scenes")
+    ap.add_argument("--workload", default="soup_ao", choices=["soup_ao", "keyframes_pt", "batched"],
                     help="soup_ao: 1M-triangle two-level rebuild + AO frame (default); "
                          "keyframes_pt: 708x708-quad heightfield over 30 key frames, "
-                         "interpolation + rebuild + path tracing per frame, captured as HIP graphs")
+                         "interpolation + rebuild + path tracing per frame, captured as HIP graphs; "
+                         "batched: 256 scenes of 1k-200k triangles with 4k-1M rays each, one segmented "
+                         "build + mixed persistent traces after autotuning, against per-scene launches")
     return ap.parse_args()
 
 
@@ -209,6 +212,120 @@ def main_keyframes(args):
     print(json.dumps(out), flush=True)
 
 
+def main_batched(args):
+    """Config: 256 scenes (random soups and icosphere clusters) of 1k-200k
+    triangles, log-uniform, each with 4k-1M rays (half primary closest-hit,
+    half ambient-occlusion any-hit from random surface points).  One step
+    rebuilds every scene and traces every ray.  Batched: one segmented
+    two-level build and two persistent launches (closest, any) over all
+    scenes, densities and persistence chosen by the autotuner beforehand.
+    Plain: per scene, its own context, hr_build then hr_trace_dev and
+    hr_occluded_dev with the default densities, waiting for each request."""
+    import ctypes as C
+    from happyray_amd import Batch, Engine, GRID_TWO_LEVEL, scenes
+    from happyray_amd.devmem import hip
+    t0 = time.perf_counter()
+    scs, items = scenes.batched_workload(n_scenes=args.scenes)
+    gen_s = time.perf_counter() - t0
+    nt_total = sum(int(f.shape[0]) for _, f in scs)
+    b = Batch()
+    b.set_scenes(scs)
+    rays = b.make_rays(items)
+    b.build()
+    t0 = time.perf_counter()
+    tune = b.autotune(rays)
+    tune_s = time.perf_counter() - t0
+
+    def step():
+        b.build()
+        b.trace(rays)
+        b.synchronize()
+
+    for _ in range(args.warmup):
+        step()
+    builds, traces = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        bm, tm, _ = b.timing()
+        builds.append(bm)
+        traces.append(tm)
+    wall = time.perf_counter() - t0
+    batched = rays.rays * args.steps / wall / 1e6
+    res_b = rays.results()
+
+    # plain per-request path on the same resident rays
+    engines = []
+    for v, f in scs:
+        e = Engine(seed=(0x1234, 0x5678))
+        e.upload(v, None, 1.0, f)
+        e.set_grid(GRID_TWO_LEVEL, 1.2, 2.2)
+        engines.append(e)
+    by_scene = {}
+    for (s, org, d, anyhit, tmax), bufs in zip(items, rays.bufs):
+        by_scene.setdefault(s, []).append((org.shape[1], anyhit, tmax, bufs))
+
+    def plain_step():
+        for s, e in enumerate(engines):
+            try:
+                e.build_acc_struct()
+            except Exception as ex:
+                free, total = C.c_size_t(), C.c_size_t()
+                hip().hipMemGetInfo(C.byref(free), C.byref(total))
+                print(f"plain build of scene {s} ({scs[s][1].shape[0]} triangles) failed: {ex}; device memory "
+                      f"free {free.value / 2**30:.1f} of {total.value / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+                raise
+            for n, anyhit, tmax, (o, d, t, p, occ) in by_scene.get(s, []):
+                if anyhit:
+                    e.occluded_dev(o.value, d.value, n, tmax, occ.value)
+                else:
+                    e.trace_dev(o.value, d.value, n, t.value, p.value)
+            e.synchronize()
+
+    plain_steps = max(1, min(args.steps, 3))
+    plain_step()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        plain_step()
+    plain_wall = time.perf_counter() - t0
+    plain = rays.rays * plain_steps / plain_wall / 1e6
+    res_p = rays.results()
+    same = total = 0
+    for (s, org, d, anyhit, tmax), rb_, rp_ in zip(items, res_b, res_p):
+        same += int(np.sum(rb_[0] == rp_[0]) if anyhit else np.sum((rb_[0] == rp_[0]) & (rb_[1] == rp_[1])))
+        total += org.shape[1]
+    lt, ll, _ = b.densities()
+    out = {
+        "metric": METRIC, "value": round(batched, 3), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched: {len(scs)} scenes (even: random soups, odd: icosphere clusters), "
+                               "1k-200k triangles log-uniform, 4k-1M rays each log-uniform (half primary "
+                               "closest hit, half AO any hit from random surface points); step = rebuild of "
+                               "every scene + trace of every ray",
+                   "scenes": len(scs), "triangles": nt_total, "rays_per_step": rays.rays,
+                   "seed": 4},
+        "batched_build_ms": round(float(np.mean(builds)), 3),
+        "batched_trace_ms": round(float(np.mean(traces)), 3),
+        "plain_mrays_s": round(plain, 3),
+        "plain_ms_per_step": round(plain_wall * 1e3 / plain_steps, 3),
+        "speedup_vs_plain": round(batched / plain, 3),
+        "results_identical_to_plain": round(same / max(total, 1), 6),
+        "autotune": {"lambda_top": [round(x, 4) for x in tune.lambda_top],
+                     "lambda_leaf": [round(x, 4) for x in tune.lambda_leaf],
+                     "size_classes": ["<4K", "<16K", "<64K", ">=64K"],
+                     "refill": tune.tuning.refill, "blocks_per_cu": tune.tuning.blocks_per_cu,
+                     "evaluations": tune.evaluations, "seconds": round(tune_s, 2)},
+        "generation_s": round(gen_s, 1),
+    }
+    for e in engines:
+        e.close()
+    rays.free()
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -217,6 +334,8 @@ def main():
         sys.exit(2)
     if args.workload == "keyframes_pt":
         return main_keyframes(args)
+    if args.workload == "batched":
+        return main_batched(args)
     from happyray_amd import Engine, GRID_TWO_LEVEL, MODE_AO, scenes
 
     scene = scenes.soup_in_room()

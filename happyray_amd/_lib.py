@@ -108,6 +108,7 @@ SIGNATURES = {
     "hr_trace": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, _f32p, _u32p]),
     "hr_occluded": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_float, _u8p]),
     "hr_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp]),
+    "hr_occluded_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_float, _vp]),
     "hr_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -210,7 +212,7 @@ hr_status do_build(hr_batch* b)
     a.vz = b->verts.p + 2 * (size_t)b->nv;
     a.nt = b->nt;
     const hr::SceneTable T = table(b);
-    for (int attempt = 0; attempt < 8; ++attempt) {
+    for (int attempt = 0; attempt < 16; ++attempt) {
         HR_TRY(reserve_build(b));
         hr::BuildBuffers B = buffers(b);
         HR_HIP(hipEventRecord(b->ev_b0, b->stream));
@@ -251,14 +253,22 @@ hr_status do_build(hr_batch* b)
             b->built = true;
             return HR_OK;
         }
+        if (std::getenv("HR_DEBUG_BUILD"))
+            std::fprintf(stderr, "hr_batch_build: attempt %d overflow 0x%x cells %u/%llu pairs %u/%u leaves %u/%u leaf pairs %u/%u\n",
+                         attempt, st.overflow, st.num_cells, (unsigned long long)b->cells_cap, st.num_pairs,
+                         b->pairs_cap, st.num_leaves, b->leaves_cap, st.num_leaf_pairs, b->leaf_pairs_cap);
+        // counts are valid up to the first stage that overflowed: grow that one
         if (st.overflow & hr::kOverflowCells) {
             if (st.num_cells == 0xFFFFFFFFu || st.num_cells == 0u)
                 return fail(HR_ERR_CAPACITY, "grid resolution too large");
             b->cells_cap = grow(b->cells_cap, st.num_cells);
+        } else if (st.overflow & hr::kOverflowPairs) {
+            b->pairs_cap = grow(b->pairs_cap, st.num_pairs);
+        } else if (st.overflow & hr::kOverflowLeaves) {
+            b->leaves_cap = grow(b->leaves_cap, st.num_leaves);
+        } else if (st.overflow & hr::kOverflowLeafPairs) {
+            b->leaf_pairs_cap = grow(b->leaf_pairs_cap, st.num_leaf_pairs);
         }
-        if (st.overflow & hr::kOverflowPairs) b->pairs_cap = grow(b->pairs_cap, st.num_pairs);
-        if (st.overflow & hr::kOverflowLeaves) b->leaves_cap = grow(b->leaves_cap, st.num_leaves);
-        if (st.overflow & hr::kOverflowLeafPairs) b->leaf_pairs_cap = grow(b->leaf_pairs_cap, st.num_leaf_pairs);
     }
     return fail(HR_ERR_CAPACITY, "batch build did not fit after repeated growth");
 }

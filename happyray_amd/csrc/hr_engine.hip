@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -286,7 +287,7 @@ hr_status do_build(hr_ctx* c)
         if (c->leaf_pairs_cap == 0) c->leaf_pairs_cap = std::max<uint32_t>(65536, 8 * c->nt);
     }
     c->built = false;
-    for (int attempt = 0; attempt < 8; ++attempt) {
+    for (int attempt = 0; attempt < 16; ++attempt) {
         HR_TRY(reserve_build(c));
         HR_HIP(hipEventRecord(c->ev_b0, c->stream));
         enqueue_build(c);
@@ -302,14 +303,23 @@ hr_status do_build(hr_ctx* c)
             set_info(c, st);
             return HR_OK;
         }
+        if (std::getenv("HR_DEBUG_BUILD"))
+            std::fprintf(stderr, "hr_build: attempt %d overflow 0x%x cells %u/%llu pairs %u/%u leaves %u/%u leaf pairs %u/%u\n",
+                         attempt, st.overflow, st.num_cells, (unsigned long long)c->cells_cap, st.num_pairs,
+                         c->pairs_cap, st.num_leaves, c->leaves_cap, st.num_leaf_pairs, c->leaf_pairs_cap);
+        // the counts are valid up to the first stage that overflowed; the
+        // later stages ran on nothing (or on stale data), so only the first
+        // overflowed buffer grows per attempt
         if (st.overflow & hr::kOverflowCells) {
             if (st.num_cells == 0xFFFFFFFFu) return fail(HR_ERR_CAPACITY, "grid resolution too large");
             c->cells_cap = grow(c->cells_cap, st.num_cells);
-        }
-        if (st.overflow & hr::kOverflowPairs) c->pairs_cap = grow(c->pairs_cap, st.num_pairs);
-        if (st.overflow & hr::kOverflowLeaves) c->leaves_cap = grow(c->leaves_cap, st.num_leaves);
-        if (st.overflow & hr::kOverflowLeafPairs)
+        } else if (st.overflow & hr::kOverflowPairs) {
+            c->pairs_cap = grow(c->pairs_cap, st.num_pairs);
+        } else if (st.overflow & hr::kOverflowLeaves) {
+            c->leaves_cap = grow(c->leaves_cap, st.num_leaves);
+        } else if (st.overflow & hr::kOverflowLeafPairs) {
             c->leaf_pairs_cap = grow(c->leaf_pairs_cap, st.num_leaf_pairs);
+        }
     }
     return fail(HR_ERR_CAPACITY, "grid build did not fit after repeated growth");
 }
@@ -1028,6 +1038,14 @@ hr_status hr_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, uint32
 {
     if (!d_t_out || !d_prim_out) return fail(HR_ERR_ARG, "null output buffer");
     return trace_common(c, d_org, d_dir, n, 0, 0.f, d_t_out, d_prim_out, nullptr, true);
+}
+
+hr_status hr_occluded_dev(hr_ctx* c, const float* d_org, const float* d_dir, uint32_t n, float tmax,
+                          uint8_t* d_occ_out)
+{
+    if (!d_occ_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (!(tmax > 0.f)) return fail(HR_ERR_ARG, "tmax must be positive");
+    return trace_common(c, d_org, d_dir, n, 1, tmax, nullptr, nullptr, d_occ_out, true);
 }
 
 hr_status hr_get_timing(hr_ctx* c, hr_timing* out)

@@ -231,6 +231,13 @@ class Engine:
               "hr_occluded")
         return occ
 
+    def trace_dev(self, org, dirs, n, t_out, prim_out):
+        """Closest hit of n rays whose SoA buffers are device pointers (ints)."""
+        check(self._L.hr_trace_dev(self._h, org, dirs, n, t_out, prim_out), "hr_trace_dev")
+
+    def occluded_dev(self, org, dirs, n, tmax, occ_out):
+        check(self._L.hr_occluded_dev(self._h, org, dirs, n, float(tmax), occ_out), "hr_occluded_dev")
+
     def timing(self):
         t = _lib.Timing()
         check(self._L.hr_get_timing(self._h, C.byref(t)), "hr_get_timing")

@@ -241,3 +241,111 @@ def random_soup(seed, n_tris, extent=10.0, size=0.5):
     cam = dict(pos=(extent / 2, extent / 2, extent * 2.2), dir=(0.0, 0.0, -1.0),
                up=(0.0, 1.0, 0.0), fov=40.0, res=(64, 64))
     return _assemble([(v, f, 0)], mats, "random_soup", cam)
+
+
+def sphere_cluster_n(seed, n_target, extent=20.0):
+    """Icospheres (subdivision 3: 1,280 triangles; 2: 320 for small targets)
+    with centres U[0, extent]^3 and radii U[0.5, 2] until about n_target
+    triangles."""
+    rng = np.random.default_rng(seed)
+    sub = 3 if n_target >= 1280 else 2
+    sv, sf = icosphere(sub)
+    k = max(1, int(round(n_target / len(sf))))
+    vs, fs = [], []
+    for i in range(k):
+        c = rng.uniform(0.0, extent, 3)
+        r = rng.uniform(0.5, 2.0)
+        vs.append(sv * r + c)
+        fs.append(sf + i * len(sv))
+    return np.concatenate(vs).astype(np.float32).T.copy(), np.concatenate(fs).astype(np.uint32)
+
+
+def soup_n(seed, n, extent=20.0):
+    """n small random triangles (edge scale extent / n^(1/3)) in a cube."""
+    rng = np.random.default_rng(seed)
+    size = extent / max(n, 1) ** (1.0 / 3.0)
+    c = rng.uniform(0.0, extent, size=(n, 1, 3))
+    v = (c + rng.normal(0.0, 0.5 * size, size=(n, 3, 3))).reshape(-1, 3)
+    return v.astype(np.float32).T.copy(), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+
+
+def primary_rays(seed, verts, n, fov_deg=60.0):
+    """n pinhole rays from a point outside the bounds (1.5 diagonals from the
+    centre, random direction) through random image points aimed at the centre."""
+    rng = np.random.default_rng(seed)
+    lo, hi = verts.min(1), verts.max(1)
+    c = (lo + hi) / 2
+    diag = float(np.linalg.norm(hi - lo))
+    w = rng.normal(size=3)
+    w /= np.linalg.norm(w)
+    eye = c + 1.5 * diag * w
+    fwd = -w
+    up = np.array([0.0, 1.0, 0.0]) if abs(fwd[1]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    right = np.cross(fwd, up)
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    h = np.tan(np.radians(fov_deg) / 2)
+    sx = rng.uniform(-h, h, n)
+    sy = rng.uniform(-h, h, n)
+    d = fwd[:, None] + right[:, None] * sx + up[:, None] * sy
+    org = np.repeat(eye[:, None], n, 1)
+    return org.astype(np.float32), d.astype(np.float32)
+
+
+def ao_rays(seed, verts, idx, n, dist):
+    """n ambient-occlusion rays from random points of random triangles:
+    cosine-weighted directions about the face normal (random side), origin
+    offset 0.01 * dist along the direction (as the AO generator offsets),
+    to be traced as any-hit within dist."""
+    rng = np.random.default_rng(seed)
+    t = rng.integers(0, idx.shape[0], n)
+    a, b, c = (verts[:, idx[t, k]].astype(np.float64) for k in range(3))
+    u = rng.uniform(size=n)
+    v = rng.uniform(size=n)
+    flip = u + v > 1
+    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+    p = a + (b - a) * u + (c - a) * v
+    nrm = np.cross((b - a).T, (c - a).T).T
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=0), 1e-30)
+    nrm *= np.where(rng.uniform(size=n) < 0.5, -1.0, 1.0)
+    r1 = rng.uniform(size=n)
+    r2 = rng.uniform(size=n)
+    phi = 2 * np.pi * r1
+    lx, ly, lz = np.cos(phi) * np.sqrt(1 - r2), np.sin(phi) * np.sqrt(1 - r2), np.sqrt(r2)
+    helper = np.where(np.abs(nrm[0]) > 0.9, 0.0, 1.0)
+    tx = np.stack([helper, 1 - helper, np.zeros(n)])
+    tng = np.cross(nrm.T, tx.T).T
+    tng /= np.maximum(np.linalg.norm(tng, axis=0), 1e-30)
+    bit = np.cross(nrm.T, tng.T).T
+    d = tng * lx + bit * ly + nrm * lz
+    org = p + 0.01 * dist * d
+    return org.astype(np.float32), d.astype(np.float32)
+
+
+def batched_workload(seed=4, n_scenes=256, tri_range=(1000, 200000), ray_range=(4000, 1000000),
+                     scale=1.0):
+    """The batched configuration: n_scenes scenes, sizes log-uniform in
+    tri_range (even scenes random soups, odd scenes icosphere clusters); per
+    scene a ray count log-uniform in ray_range, half primary (closest hit) and
+    half ambient occlusion (any hit within 0.05 of the bounds diagonal) from
+    random surface points.  `scale` shrinks the ray counts (tests).
+    Returns (scenes [(verts, idx)], ray items [(scene, org, dir, anyhit, tmax)])."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]), n_scenes)).astype(np.int64)
+    rays = np.exp(rng.uniform(np.log(ray_range[0]), np.log(ray_range[1]), n_scenes)) * scale
+    rays = np.maximum(rays.astype(np.int64), 2)
+    out_scenes, items = [], []
+    for s in range(n_scenes):
+        if s % 2 == 0:
+            v, f = soup_n(seed * 1000 + s, int(sizes[s]))
+        else:
+            v, f = sphere_cluster_n(seed * 1000 + s, int(sizes[s]))
+        out_scenes.append((v, f))
+        diag = float(np.linalg.norm(v.max(1) - v.min(1)))
+        n_p = int(rays[s] // 2)
+        n_a = int(rays[s] - n_p)
+        o, d = primary_rays(seed * 7919 + s, v, n_p)
+        items.append((s, o, d, False, 0.0))
+        o, d = ao_rays(seed * 104729 + s, v, f, n_a, 0.05 * diag)
+        items.append((s, o, d, True, 0.05 * diag))
+    return out_scenes, items

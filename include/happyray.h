@@ -197,6 +197,8 @@ hr_status hr_occluded(hr_ctx* ctx, const float* org, const float* dir, uint32_t 
                       uint8_t* occ_out);
 hr_status hr_trace_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n,
                        float* d_t_out, uint32_t* d_prim_out);
+hr_status hr_occluded_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n, float tmax,
+                          uint8_t* d_occ_out);
 
 /* Kernel choices of a context (no reference counterpart: the reference fixes
  * its launch shapes at compile time, src/RT/Algorithm/RayTracingKernels.h:34-43). */

@@ -208,3 +208,24 @@ def test_work_counters_match_oracle(cluster):
     assert (ws.secondary.rays, ws.secondary.steps, ws.secondary.tests) == \
         (sa.rays - sp.rays, sa.steps - sp.steps, sa.tests - sp.tests)
     e.close()
+
+
+def test_device_pointer_entry_points(cluster):
+    """hr_trace_dev / hr_occluded_dev on resident rays equal hr_trace / hr_occluded."""
+    from happyray_amd.devmem import DeviceArray
+    print("device-pointer trace entry points", flush=True)
+    e = gpu_engine(cluster, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    org, d = random_rays(cluster, 5000, seed=23)
+    o_d, d_d = DeviceArray.from_host(org), DeviceArray.from_host(d)
+    t_d, p_d = DeviceArray((5000,), np.float32), DeviceArray((5000,), np.uint32)
+    occ_d = DeviceArray((5000,), np.uint8)
+    e.trace_dev(o_d.value, d_d.value, 5000, t_d.value, p_d.value)
+    e.occluded_dev(o_d.value, d_d.value, 5000, 2.5, occ_d.value)
+    e.synchronize()
+    t, p = e.trace(org, d)
+    np.testing.assert_array_equal(t_d.to_host(), t)
+    np.testing.assert_array_equal(p_d.to_host(), p)
+    np.testing.assert_array_equal(occ_d.to_host(), e.occluded(org, d, 2.5))
+    for a in (o_d, d_d, t_d, p_d, occ_d):
+        a.free()
+    e.close()
