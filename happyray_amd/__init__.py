@@ -8,6 +8,7 @@ from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MO
 from .engine import Engine
 from .scenefile import SceneFile
 from .batch import Batch
+from .server import Server
 
-__all__ = ["Engine", "Batch", "SceneFile", "HappyRayError", "load", "LIB_PATH", "GRID_UNIFORM", "GRID_UNIFORM_EXACT",
+__all__ = ["Engine", "Batch", "Server", "SceneFile", "HappyRayError", "load", "LIB_PATH", "GRID_UNIFORM", "GRID_UNIFORM_EXACT",
            "GRID_TWO_LEVEL", "MODE_AO", "MODE_PATH"]

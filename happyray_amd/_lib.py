@@ -86,6 +86,36 @@ class AutotuneResult(C.Structure):
                 ("best_trace_ms", C.c_float)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("device", C.c_int), ("memory_budget", C.c_uint64), ("max_wave_rays", C.c_uint32),
+                ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("scene", C.c_uint32), ("n", C.c_uint32), ("tmax", C.c_float),
+                ("depth", C.c_int), ("on_device", C.c_int), ("org", _vp), ("dir", _vp), ("t_out", _vp),
+                ("prim_out", _vp), ("occ_out", _vp), ("rgb_out", _vp), ("k1", C.c_uint32),
+                ("k2", C.c_uint32), ("coeff", C.c_float)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("latency_ms", C.c_double), ("wave", C.c_uint64)]
+
+
+class ServerScene(C.Structure):
+    _fields_ = [("keys", _f32p), ("nframes", C.c_uint32), ("nv", C.c_uint32), ("idx", _u32p),
+                ("nt", C.c_uint32), ("mats", _f32p), ("nmat", C.c_uint32), ("mat_idx", _u32p),
+                ("lights", _f32p), ("light_weights", _f32p), ("nlights", C.c_int),
+                ("lambda_top", C.c_float), ("lambda_leaf", C.c_float)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("rays", C.c_uint64), ("waves", C.c_uint64),
+                ("builds", C.c_uint64), ("rebuilds", C.c_uint64), ("evictions", C.c_uint64),
+                ("resident_bytes", C.c_uint64), ("peak_resident_bytes", C.c_uint64),
+                ("resident_scenes", C.c_uint32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "hr_last_error": (C.c_char_p, []),
@@ -109,6 +139,7 @@ SIGNATURES = {
     "hr_occluded": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_float, _u8p]),
     "hr_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp]),
     "hr_occluded_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_float, _vp]),
+    "hr_path_trace": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_int, _f32p]),
     "hr_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
@@ -132,6 +163,13 @@ SIGNATURES = {
                                       C.POINTER(C.c_uint64)]),
     "hr_batch_get_work_stats": (C.c_int, [_vp, C.POINTER(WorkStats)]),
     "hr_batch_synchronize": (C.c_int, [_vp]),
+    "hr_server_create": (C.c_int, [C.POINTER(_vp), C.POINTER(ServerOptions)]),
+    "hr_server_destroy": (C.c_int, [_vp]),
+    "hr_server_add_scene": (C.c_int, [_vp, C.POINTER(ServerScene), C.POINTER(C.c_uint32)]),
+    "hr_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "hr_server_wait": (C.c_int, [_vp, C.c_uint64, C.POINTER(RequestResult)]),
+    "hr_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int)]),
+    "hr_server_get_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "hr_set_normals": (C.c_int, [_vp, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint32]),
     "hr_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "hr_scene_load_obj": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),

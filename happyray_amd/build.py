@@ -17,7 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhappyray.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_persist.hip", "hr_engine.hip", "hr_batch.hip"]
+SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_persist.hip", "hr_engine.hip", "hr_batch.hip",
+           "hr_server.hip"]
 CXX_SOURCES = ["hr_scene.cpp"]          # host-only C++ (scene files)
 ARCH = os.environ.get("HR_OFFLOAD_ARCH", "gfx950")
 

@@ -330,6 +330,7 @@ hr_status enqueue_trace(hr_batch* b, const hr_ray_batch* rb, uint32_t nb, bool t
             d.t_out = r.t_out;
             d.prim_out = r.prim_out;
             d.occ_out = r.occ_out;
+            d.hits = nullptr;
             total[k] += r.n;
             if (total[k] > 0xFFFFFFFFull) return fail(HR_ERR_CAPACITY, "more than 2^32 rays in one launch");
             pre[k][q + 1] = (uint32_t)total[k];

@@ -19,6 +19,7 @@
 
 #include "../../include/happyray.h"
 #include "hr_build.h"
+#include "hr_ctx_internal.h"
 #include "hr_error.h"
 #include "hr_host.h"
 #include "hr_persist.h"
@@ -98,7 +99,7 @@ struct hr_ctx {
     int last_ao_samples = 0;
 
     // ray batches
-    DevBuf<float> ray_org, ray_dir, ray_t;
+    DevBuf<float> ray_org, ray_dir, ray_t, ray_hits, ray_rgb;
     DevBuf<uint32_t> ray_prim;
     DevBuf<uint8_t> ray_occ;
 
@@ -566,6 +567,64 @@ hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// internal interface (hr_ctx_internal.h)
+// ---------------------------------------------------------------------------
+
+hr::TraceScene hr::ctx_trace_scene(hr_ctx* c) { return trace_scene(c); }
+
+size_t hr::ctx_device_bytes(hr_ctx* c)
+{
+    size_t b = 0;
+    b += c->key_a.bytes() + c->key_b.bytes() + c->keyframes.bytes() + c->verts.bytes() + c->idx.bytes();
+    b += c->normal_keys.bytes() + c->nidx.bytes() + c->normals.bytes() + c->mats.bytes() + c->mat_idx.bytes();
+    b += c->rec.bytes() + c->wave_counts.bytes() + c->keys.bytes() + c->vals.bytes() + c->keys_sorted.bytes();
+    b += c->top_vals_sorted.bytes() + c->prims.bytes() + c->leaf_counts.bytes() + c->cells.bytes();
+    b += c->leaves.bytes() + c->scan_tmp.bytes() + c->sort_keys_tmp.bytes() + c->sort_vals_tmp.bytes();
+    b += c->sort_hist.bytes() + c->lights.bytes() + c->light_cdf.bytes() + c->hits.bytes() + c->fb.bytes();
+    b += c->hit_prims.bytes() + c->occ.bytes() + c->ray_org.bytes() + c->ray_dir.bytes() + c->ray_t.bytes();
+    b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes();
+    return b;
+}
+
+hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims, uint32_t n, int depth, float* rgb,
+                             hipStream_t s)
+{
+    if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
+    if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
+    if (n == 0) return HR_OK;
+    HR_TRY(ensure_default_material(c));
+    if (c->has_normals && c->normals_dirty) {
+        hipStream_t keep = c->stream;
+        c->stream = s;
+        enqueue_normals(c, nullptr);
+        c->stream = keep;
+        c->normals_dirty = false;
+    }
+    hr::RenderArgs ra;
+    ra.seed_lo = c->seed_lo;
+    ra.seed_hi = c->seed_hi;
+    ra.jitter = 0;
+    ra.ao_samples = 1;
+    ra.ao_dist = 0.f;
+    ra.pt_depth = depth;
+    ra.image = 0;
+    ra.stats = nullptr;
+    ra.frame = nullptr;
+    hr::Shading sh;
+    sh.mats = c->mats.p;
+    sh.mat_idx = c->has_mat_idx ? c->mat_idx.p : nullptr;
+    sh.lights = c->lights.p;
+    sh.light_cdf = c->light_cdf.p;
+    sh.nlights = c->nlights;
+    sh.normals = c->has_normals ? c->normals.p : nullptr;
+    sh.nidx = c->has_normals ? c->nidx.p : nullptr;
+    hr::fill_u32((uint32_t*)rgb, 0u, 3 * (size_t)n, nullptr, 1, s);
+    hr::launch_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, c->ray_count.p, s);
+    HR_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 
@@ -633,7 +692,8 @@ hr_status hr_destroy(hr_ctx* c)
     c->nidx.release();
     c->normal_keys.release();
     DevBuf<float>* fbufs[] = { &c->key_a, &c->key_b, &c->keyframes, &c->verts, &c->mats, &c->lights,
-                               &c->light_cdf, &c->hits, &c->fb, &c->ray_org, &c->ray_dir, &c->ray_t };
+                               &c->light_cdf, &c->hits, &c->fb, &c->ray_org, &c->ray_dir, &c->ray_t,
+                               &c->ray_hits, &c->ray_rgb };
     for (auto* b : fbufs) b->release();
     DevBuf<uint32_t>* ubufs[] = { &c->idx, &c->mat_idx, &c->wave_counts, &c->keys, &c->vals,
                                   &c->keys_sorted, &c->top_vals_sorted, &c->prims, &c->leaf_counts,
@@ -1046,6 +1106,30 @@ hr_status hr_occluded_dev(hr_ctx* c, const float* d_org, const float* d_dir, uin
     if (!d_occ_out) return fail(HR_ERR_ARG, "null output buffer");
     if (!(tmax > 0.f)) return fail(HR_ERR_ARG, "tmax must be positive");
     return trace_common(c, d_org, d_dir, n, 1, tmax, nullptr, nullptr, d_occ_out, true);
+}
+
+hr_status hr_path_trace(hr_ctx* c, const float* org, const float* dir, uint32_t n, int depth, float* rgb_out)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
+    if (!rgb_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
+    if (n == 0) return HR_OK;
+    if (!org || !dir) return fail(HR_ERR_ARG, "null ray buffer");
+    HR_TRY(c->ray_org.reserve(3 * (size_t)n));
+    HR_TRY(c->ray_dir.reserve(3 * (size_t)n));
+    HR_TRY(c->ray_t.reserve(n));
+    HR_TRY(c->ray_prim.reserve(n));
+    HR_TRY(c->ray_hits.reserve(7 * (size_t)n));
+    HR_TRY(c->ray_rgb.reserve(3 * (size_t)n));
+    HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
+    HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
+    hr::persist_batch(trace_scene(c), c->ray_org.p, c->ray_dir.p, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr,
+                      c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p);
+    HR_TRY(hr::ctx_path_trace(c, c->ray_hits.p, c->ray_prim.p, n, depth, c->ray_rgb.p, c->stream));
+    HR_HIP(hipMemcpyAsync(rgb_out, c->ray_rgb.p, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
 }
 
 hr_status hr_get_timing(hr_ctx* c, hr_timing* out)

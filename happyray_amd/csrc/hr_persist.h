@@ -13,9 +13,11 @@ void persist_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs
 void persist_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                 const uint32_t* hit_prims, uint8_t* occ, uint32_t* pool, uint32_t refill,
                 unsigned long long* ray_count, hipStream_t s);
+// hits != nullptr (closest hit only): also the 7-plane hit buffer of the rays
 void persist_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
                    float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
-                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s);
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s,
+                   float* hits = nullptr);
 
 // A scene of a batch as the traversal sees it: its grid (cells from the
 // scene's first cell; leaves, references and triangle records shared by the
@@ -37,6 +39,9 @@ struct RayBatchDev {
     float* t_out;        // closest hit: distance (FLT_MAX on a miss); any hit: optional
     uint32_t* prim_out;  // closest hit: scene-local triangle id, the scene's nt on a miss
     uint8_t* occ_out;    // any hit: 1 when occluded
+    float* hits;         // closest hit, optional: 7 SoA planes of n floats (hit point
+                         // org + (t - EPS) dir, direction, t), the layout of the
+                         // primary-hit buffer the path tracer starts from
 };
 
 // One persistent launch over the rays of nb batches of many scenes (all

@@ -309,6 +309,20 @@ struct GenAO {
     }
 };
 
+// the primary-hit record of a finished closest-hit ray (SimpleRayBuffer::store,
+// RayBuffers.h:62-87): hit point org + (t - EPS) dir, direction, t
+__device__ __forceinline__ void store_hit(float* hits, uint32_t n, uint32_t j, const RayState& r)
+{
+    const f3 hp = r.org + (r.t - kEps) * r.dir;
+    hits[j] = hp.x;
+    hits[(size_t)n + j] = hp.y;
+    hits[2 * (size_t)n + j] = hp.z;
+    hits[3 * (size_t)n + j] = r.dir.x;
+    hits[4 * (size_t)n + j] = r.dir.y;
+    hits[5 * (size_t)n + j] = r.dir.z;
+    hits[6 * (size_t)n + j] = r.t;
+}
+
 struct GenBatch {
     const float* org;
     const float* dir;
@@ -318,6 +332,7 @@ struct GenBatch {
     float* t_out;
     uint32_t* prim_out;
     uint8_t* occ_out;
+    float* hits;
 
     __device__ bool start(uint32_t j, RayState& r, const TraceScene&) const
     {
@@ -335,6 +350,7 @@ struct GenBatch {
         } else {
             t_out[j] = r.t;
             prim_out[j] = r.t < kFltMax ? r.best : S.nt;
+            if (hits) store_hit(hits, n, j, r);
         }
     }
 };
@@ -452,6 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_persist_multi(const SceneDesc* __res
                         } else {
                             B.t_out[jl] = r.t;
                             B.prim_out[jl] = scenes[B.scene].nt;
+                            if (B.hits) store_hit(B.hits, B.n, jl, r);
                         }
                     }
                 }
@@ -475,6 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_persist_multi(const SceneDesc* __res
                     const SceneDesc& D = scenes[B.scene];
                     B.t_out[j] = r.t;
                     B.prim_out[j] = r.t < kFltMax ? r.best - D.tri_base : D.nt;
+                    if (B.hits) store_hit(B.hits, B.n, j, r);
                 }
             }
         }
@@ -541,9 +559,9 @@ void persist_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const 
 
 void persist_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
                    float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
-                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s)
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, float* hits)
 {
-    GenBatch g{ org, dir, n, tmax, anyhit, t_out, prim_out, occ_out };
+    GenBatch g{ org, dir, n, tmax, anyhit, t_out, prim_out, occ_out, anyhit ? nullptr : hits };
     if (S.two_level) {
         if (anyhit) launch<true, true>(S, g, n, pool, refill, ray_count, stats, s);
         else launch<true, false>(S, g, n, pool, refill, ray_count, stats, s);

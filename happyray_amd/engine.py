@@ -231,6 +231,15 @@ class Engine:
               "hr_occluded")
         return occ
 
+    def path_trace(self, org, dirs, depth=4):
+        """org, dirs: (3, n) float32. Radiance (n, 3) of a path from each ray."""
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[1]
+        rgb = np.zeros((n, 3), np.float32)
+        check(self._L.hr_path_trace(self._h, _f(o), _f(d), n, int(depth), _f(rgb)), "hr_path_trace")
+        return rgb
+
     def trace_dev(self, org, dirs, n, t_out, prim_out):
         """Closest hit of n rays whose SoA buffers are device pointers (ints)."""
         check(self._L.hr_trace_dev(self._h, org, dirs, n, t_out, prim_out), "hr_trace_dev")

@@ -199,6 +199,12 @@ hr_status hr_trace_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint
                        float* d_t_out, uint32_t* d_prim_out);
 hr_status hr_occluded_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n, float tmax,
                           uint8_t* d_occ_out);
+/* Paths from given rays: each ray's closest hit starts a path of `depth` vertices
+ * shaded as the path-tracing render shades a primary hit (src/RT/Integrator/
+ * PathTracer.h:41-372), ray i taking the random streams of pixel i of image 0;
+ * rgb_out receives 3 floats of radiance per ray (0 for rays that miss). */
+hr_status hr_path_trace(hr_ctx* ctx, const float* org, const float* dir, uint32_t n, int depth,
+                        float* rgb_out);
 
 /* Kernel choices of a context (no reference counterpart: the reference fixes
  * its launch shapes at compile time, src/RT/Algorithm/RayTracingKernels.h:34-43). */
@@ -286,6 +292,83 @@ hr_status hr_batch_get_timing(hr_batch* batch, double* build_ms, double* trace_m
 /* read and clear the work counters (primary: closest hit, secondary: any hit) */
 hr_status hr_batch_get_work_stats(hr_batch* batch, hr_work_stats* out);
 hr_status hr_batch_synchronize(hr_batch* batch);
+
+/* ---- request server (no reference counterpart) ----
+ * A server keeps scenes resident on one device and serves trace, occlusion, path and
+ * rebuild requests submitted from any thread.  A worker thread batches continuously:
+ * while one wave of requests runs on the GPU, requests that arrive are admitted into
+ * the next wave as soon as a wave slot and device memory are free; a wave traces the
+ * rays of all its requests, of any scenes, in persistent launches (hr_batch_trace_dev's
+ * kernels).  A scene's grid is built on first use and evicted least-recently-used when
+ * the memory budget requires; a rebuild request (new key-frame interpolation) builds
+ * into a second grid while earlier waves still read the first.  Each request's result
+ * equals the single-scene entry point's result for its rays (hr_trace, hr_occluded,
+ * hr_path_trace) on its scene's current frame, alone or batched with others. */
+typedef struct hr_server hr_server;
+typedef struct {
+    int device;
+    uint64_t memory_budget;    /* device bytes for grids and wave scratch (0: 200 GiB) */
+    uint32_t max_wave_rays;    /* rays admitted into one wave (0: 16M); a larger request
+                                  forms a wave alone                                    */
+    uint32_t seed_lo, seed_hi; /* RNG key of path requests                               */
+} hr_server_options;
+typedef enum {
+    HR_REQ_CLOSEST = 0,        /* t_out, prim_out (either may be NULL)                  */
+    HR_REQ_OCCLUDED = 1,       /* occ_out within (1e-4, tmax)                           */
+    HR_REQ_PATH = 2,           /* rgb_out: 3 floats per ray, `depth` path vertices      */
+    HR_REQ_REBUILD = 3         /* the scene's frame becomes (k1, k2, coeff)              */
+} hr_request_kind;
+typedef struct {
+    int kind;                  /* hr_request_kind                                      */
+    uint32_t scene;
+    uint32_t n;                /* rays (at most 2^30)                                   */
+    float tmax;
+    int depth;
+    int on_device;             /* 1: rays and outputs are device pointers; NULL outputs
+                                  are computed and dropped                              */
+    const float* org;          /* SoA, 3 * n                                            */
+    const float* dir;
+    float* t_out;
+    uint32_t* prim_out;
+    uint8_t* occ_out;
+    float* rgb_out;
+    uint32_t k1, k2;           /* rebuild                                               */
+    float coeff;
+} hr_request;
+typedef struct {
+    int status;                /* hr_status of the request                              */
+    double latency_ms;         /* submit to completion                                  */
+    uint64_t wave;             /* wave that ran it (0: none)                            */
+} hr_request_result;
+typedef struct {
+    const float* keys;         /* host: nframes blocks of SoA vertices (3 * nv each)    */
+    uint32_t nframes, nv;
+    const uint32_t* idx;
+    uint32_t nt;
+    const float* mats;         /* 12 floats each, may be NULL                           */
+    uint32_t nmat;
+    const uint32_t* mat_idx;
+    const hr_area_light* lights;
+    const float* light_weights;
+    int nlights;
+    float lambda_top, lambda_leaf;   /* two-level densities (0: 1.2 / 2.2)              */
+} hr_server_scene;
+typedef struct {
+    uint64_t requests, rays, waves, builds, rebuilds, evictions;
+    uint64_t resident_bytes, peak_resident_bytes;
+    uint32_t resident_scenes;
+} hr_server_stats;
+hr_status hr_server_create(hr_server** out, const hr_server_options* opts);
+/* finishes the requests already submitted, then releases everything */
+hr_status hr_server_destroy(hr_server* server);
+/* the scene's data is copied; it becomes resident on first use (frame (0, 0, 0)) */
+hr_status hr_server_add_scene(hr_server* server, const hr_server_scene* scene, uint32_t* id);
+/* the caller keeps the request's buffers alive until hr_server_wait returns */
+hr_status hr_server_submit(hr_server* server, const hr_request* request, uint64_t* id);
+/* blocks until the request is done; returns its status (result may be NULL) */
+hr_status hr_server_wait(hr_server* server, uint64_t id, hr_request_result* result);
+hr_status hr_server_poll(hr_server* server, uint64_t id, int* done);
+hr_status hr_server_get_stats(hr_server* server, hr_server_stats* out);
 
 /* ---- scene files: SceneLoader::loadScene (src/Application/SceneLoader.cpp:212-332) ----
  * hr_scene_load parses a scene.cfg (src/Application/SceneConfigurationReader.cpp:29-190)

@@ -130,6 +130,18 @@ void or_prim_cell_range(const or_mesh* m, uint32_t prim, const or_grid* g,
 int  or_tri_box_overlap(const float v0[3], const float v1[3], const float v2[3],
                         const float center[3], const float half[3]);
 
+/* the camera ray directions of all pixels (before the traversal's reciprocal) */
+void or_primary_dirs(const or_camera* cam, const or_render_params* p, int image_id,
+                     float* dx, float* dy, float* dz);
+/* paths from given rays: closest hit of each ray stored as a primary hit, then
+ * the path tracer of or_render_pt from it (image 0, ray i = pixel i) */
+int  or_path_trace_rays(const or_mesh* mesh, const or_grid* g,
+                        const float* ox, const float* oy, const float* oz,
+                        const float* dx, const float* dy, const float* dz, uint32_t n,
+                        const or_material* mats, const uint32_t* mat_idx,
+                        const or_light* lights, const float* light_weights, int nlights,
+                        const or_render_params* p, float* rgb, or_stats* stats);
+
 /* path-tracer shading normal of primitive prim seen from hit point org
  * (PathTracer.h:109-135), exposed for tests */
 void or_shading_normal(const or_mesh* mesh, uint32_t prim, const float org[3], float out[3]);

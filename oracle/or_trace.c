@@ -412,6 +412,16 @@ static void primary_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
     }
 }
 
+void or_primary_dirs(const or_camera* cam, const or_render_params* p, int image_id,
+                     float* dx, float* dy, float* dz)
+{
+    const uint32_t npix = (uint32_t)cam->res_x * (uint32_t)cam->res_y;
+    for (uint32_t i = 0; i < npix; ++i) {
+        const f3 d = primary_dir(cam, p, i, npix, image_id);
+        dx[i] = d.x; dy[i] = d.y; dz[i] = d.z;
+    }
+}
+
 int or_primary(const or_mesh* m, const or_grid* g, const or_camera* cam,
                const or_render_params* p, int image_id,
                float* hx, float* hy, float* hz, float* hdx, float* hdy, float* hdz,
@@ -712,6 +722,51 @@ int or_render_pt(const or_mesh* m, const or_grid* g, const or_camera* cam,
     pt_arg a = { mats, mat_idx, lights, cdf, nlights < 64 ? nlights : 64, p, image_id, npix,
                  hx, hy, hz, hdx, hdy, hdz, ht, hp, fb };
     run_tiles(m, g, npix, p->nthreads, pt_tile, &a, stats);
+    free(h); free(hp);
+    return 0;
+}
+
+typedef struct {
+    const float *ox, *oy, *oz, *dx, *dy, *dz;
+    float *hx, *hy, *hz, *hdx, *hdy, *hdz, *ht; uint32_t* hprim;
+} rays_arg;
+
+/* closest hits of given rays stored as primary hits (SimpleRayBuffer::store) */
+static void rays_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
+{
+    rays_arg* q = (rays_arg*)a;
+    for (uint32_t i = b; i < e; ++i) {
+        const f3 org = mk3(q->ox[i], q->oy[i], q->oz[i]);
+        float t;
+        f3 dir;
+        const uint32_t prim = trace_one(cx, org, mk3(q->dx[i], q->dy[i], q->dz[i]), FLT_MAX, 0, 0.f, &t, &dir);
+        const f3 hp = add3(org, smul3(t - OR_EPS, dir));
+        q->hx[i] = hp.x; q->hy[i] = hp.y; q->hz[i] = hp.z;
+        q->hdx[i] = dir.x; q->hdy[i] = dir.y; q->hdz[i] = dir.z;
+        q->ht[i] = t; q->hprim[i] = prim;
+    }
+}
+
+int or_path_trace_rays(const or_mesh* m, const or_grid* g,
+                       const float* ox, const float* oy, const float* oz,
+                       const float* dx, const float* dy, const float* dz, uint32_t n,
+                       const or_material* mats, const uint32_t* mat_idx,
+                       const or_light* lights, const float* weights, int nlights,
+                       const or_render_params* p, float* rgb, or_stats* stats)
+{
+    float* h = (float*)malloc((size_t)(n ? n : 1) * 7 * sizeof(float));
+    uint32_t* hp = (uint32_t*)malloc((size_t)(n ? n : 1) * sizeof(uint32_t));
+    float *hx = h, *hy = h + n, *hz = h + 2 * (size_t)n, *hdx = h + 3 * (size_t)n,
+          *hdy = h + 4 * (size_t)n, *hdz = h + 5 * (size_t)n, *ht = h + 6 * (size_t)n;
+    rays_arg ra = { ox, oy, oz, dx, dy, dz, hx, hy, hz, hdx, hdy, hdz, ht, hp };
+    run_tiles(m, g, n, p->nthreads, rays_tile, &ra, stats);
+    float cdf[64];
+    float acc = 0.f;
+    for (int i = 0; i < nlights && i < 64; ++i) { acc += weights ? weights[i] : 1.f; cdf[i] = acc; }
+    for (size_t i = 0; i < 3 * (size_t)n; ++i) rgb[i] = 0.f;
+    pt_arg a = { mats, mat_idx, lights, cdf, nlights < 64 ? nlights : 64, p, 0, n,
+                 hx, hy, hz, hdx, hdy, hdz, ht, hp, rgb };
+    run_tiles(m, g, n, p->nthreads, pt_tile, &a, stats);
     free(h); free(hp);
     return 0;
 }

@@ -85,6 +85,9 @@ def lib():
                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_tri_box_overlap.argtypes = [_f32p] * 5
         L.or_shading_normal.argtypes = [C.POINTER(Mesh), C.c_uint32, _f32p, _f32p]
+        L.or_primary_dirs.argtypes = [C.POINTER(Camera), C.POINTER(RenderParams), C.c_int, _f32p, _f32p, _f32p]
+        L.or_path_trace_rays.argtypes = [C.POINTER(Mesh), C.POINTER(Grid)] + [_f32p] * 6 + [
+            C.c_uint32, _f32p, _u32p, _f32p, _f32p, C.c_int, C.POINTER(RenderParams), _f32p, C.POINTER(Stats)]
         L.or_trace.argtypes = [C.POINTER(Mesh), C.POINTER(Grid)] + [_f32p] * 6 + [
             C.c_uint32, C.c_int, C.c_float, C.c_int, _f32p, _u32p, C.POINTER(Stats)]
         L.or_camera_init.argtypes = [C.POINTER(Camera), _f32p, _f32p, _f32p, C.c_float,
@@ -329,3 +332,28 @@ def shading_normal(mesh, prim, org):
     out = np.zeros(3, np.float32)
     lib().or_shading_normal(C.byref(mesh.c), prim, _fp(_f3(org)), _fp(out))
     return out
+
+
+def path_trace_rays(grid, org, dirs, p, materials, mat_idx, lights, weights=None):
+    """Radiance (n, 3) of a path from each given ray (image 0, ray i = pixel i)."""
+    o = np.ascontiguousarray(org, np.float32).reshape(3, -1)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(3, -1)
+    n = o.shape[1]
+    rgb = np.zeros((n, 3), np.float32)
+    m, mi = _mats(materials, mat_idx)
+    L = np.ascontiguousarray(lights, np.float32).reshape(-1, 15)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    st = Stats()
+    lib().or_path_trace_rays(C.byref(grid.mesh.c), C.byref(grid.c), _fp(o[0]), _fp(o[1]), _fp(o[2]),
+                             _fp(d[0]), _fp(d[1]), _fp(d[2]), n, _fp(m), None if mi is None else _up(mi),
+                             _fp(L), None if w is None else _fp(w), len(L), C.byref(p), _fp(rgb),
+                             C.byref(st))
+    return rgb, st
+
+
+def primary_dirs(cam, p, image_id=0):
+    """(3, npix) camera ray directions of every pixel."""
+    n = cam.res_x * cam.res_y
+    d = np.zeros((3, n), np.float32)
+    lib().or_primary_dirs(C.byref(cam), C.byref(p), image_id, _fp(d[0]), _fp(d[1]), _fp(d[2]))
+    return d

@@ -44,7 +44,9 @@ def test_struct_layouts_match_header(tmp_path):
                "hr_work_stats": _lib.WorkStats, "hr_tuning": _lib.Tuning,
                "hr_scene_info": _lib.SceneInfo, "hr_mesh_desc": _lib.MeshDesc,
                "hr_ray_batch": _lib.RayBatch, "hr_batch_tuning": _lib.BatchTuning,
-               "hr_autotune_result": _lib.AutotuneResult}
+               "hr_autotune_result": _lib.AutotuneResult, "hr_server_options": _lib.ServerOptions,
+               "hr_request": _lib.Request, "hr_request_result": _lib.RequestResult,
+               "hr_server_scene": _lib.ServerScene, "hr_server_stats": _lib.ServerStats}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "happyray.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -180,3 +180,18 @@ def test_shading_normal_interpolation():
     flat = O.OMesh(v[:, 0], v[:, 1], v[:, 2], np.array([[0, 1, 2]], np.uint32))
     g = np.cross(v[1] - v[0], v[2] - v[0]).astype(np.float64)
     np.testing.assert_allclose(O.shading_normal(flat, 0, v.mean(0)), g / np.linalg.norm(g), atol=1e-6)
+
+
+def test_paths_from_camera_rays_equal_the_render():
+    """Paths from given rays, fed the camera's own pixel rays, reproduce the
+    path-traced image 0 exactly: the hit record of a given ray is the primary
+    hit record, and ray i takes the streams of pixel i."""
+    sc = scenes.small_box()
+    g, cam = setup(sc, O.GRID_TWO_LEVEL, 1.2)
+    p = O.params(seed=(3, 4), jitter=1, pt_depth=4, nthreads=4)
+    fb, _ = O.render_pt(g, cam, p, sc.materials, sc.mat_idx, sc.lights)
+    d = O.primary_dirs(cam, p)
+    org = np.repeat(np.asarray(cam.center[:], np.float32)[:, None], d.shape[1], 1)
+    rgb, _ = O.path_trace_rays(g, org, d, p, sc.materials, sc.mat_idx, sc.lights)
+    assert fb.max() > 0
+    np.testing.assert_array_equal(rgb, fb)
