@@ -93,7 +93,8 @@ class ServerOptions(C.Structure):
 
 class Request(C.Structure):
     _fields_ = [("kind", C.c_int), ("scene", C.c_uint32), ("n", C.c_uint32), ("tmax", C.c_float),
-                ("depth", C.c_int), ("on_device", C.c_int), ("org", _vp), ("dir", _vp), ("t_out", _vp),
+                ("depth", C.c_int), ("on_device", C.c_int), ("org", _vp), ("dir", _vp),
+                ("stride", C.c_uint32), ("t_out", _vp),
                 ("prim_out", _vp), ("occ_out", _vp), ("rgb_out", _vp), ("k1", C.c_uint32),
                 ("k2", C.c_uint32), ("coeff", C.c_float)]
 
@@ -140,6 +141,7 @@ SIGNATURES = {
     "hr_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp]),
     "hr_occluded_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_float, _vp]),
     "hr_path_trace": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_int, _f32p]),
+    "hr_path_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, _vp]),
     "hr_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),

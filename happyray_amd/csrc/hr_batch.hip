@@ -324,7 +324,7 @@ hr_status enqueue_trace(hr_batch* b, const hr_ray_batch* rb, uint32_t nb, bool t
             d.scene = r.scene;
             d.n = r.n;
             d.tmax = r.anyhit ? r.tmax : 0.f;
-            d.pad = 0;
+            d.stride = 0;
             d.org = r.org;
             d.dir = r.dir;
             d.t_out = r.t_out;

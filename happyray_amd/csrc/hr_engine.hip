@@ -1132,6 +1132,23 @@ hr_status hr_path_trace(hr_ctx* c, const float* org, const float* dir, uint32_t 
     return HR_OK;
 }
 
+hr_status hr_path_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, uint32_t n, int depth,
+                            float* d_rgb_out)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
+    if (!d_rgb_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
+    if (n == 0) return HR_OK;
+    if (!d_org || !d_dir) return fail(HR_ERR_ARG, "null ray buffer");
+    HR_TRY(c->ray_t.reserve(n));
+    HR_TRY(c->ray_prim.reserve(n));
+    HR_TRY(c->ray_hits.reserve(7 * (size_t)n));
+    hr::persist_batch(trace_scene(c), d_org, d_dir, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr, c->pool.p,
+                      (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p);
+    return hr::ctx_path_trace(c, c->ray_hits.p, c->ray_prim.p, n, depth, d_rgb_out, c->stream);
+}
+
 hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
 {
     HR_TRY(check_ctx(c));

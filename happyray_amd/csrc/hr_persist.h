@@ -33,7 +33,7 @@ struct RayBatchDev {
     uint32_t scene;
     uint32_t n;
     float tmax;
-    uint32_t pad;
+    uint32_t stride;     // elements between the x, y and z planes of org / dir (0: n)
     const float* org;
     const float* dir;
     float* t_out;        // closest hit: distance (FLT_MAX on a miss); any hit: optional

@@ -454,8 +454,9 @@ __global__ __launch_bounds__(kBlock) void k_persist_multi(const SceneDesc* __res
                     const uint32_t jl = j - prefix[bi];
                     S = scenes[B.scene].S;
                     G = load_grid(S.gp);
-                    set_ray(r, mk(B.org[jl], B.org[(size_t)B.n + jl], B.org[2 * (size_t)B.n + jl]),
-                            mk(B.dir[jl], B.dir[(size_t)B.n + jl], B.dir[2 * (size_t)B.n + jl]));
+                    const size_t st = B.stride ? B.stride : B.n;
+                    set_ray(r, mk(B.org[jl], B.org[st + jl], B.org[2 * st + jl]),
+                            mk(B.dir[jl], B.dir[st + jl], B.dir[2 * st + jl]));
                     r.t = kAny ? B.tmax : kFltMax;
                     r.tterm = B.tmax;
                     ++started;

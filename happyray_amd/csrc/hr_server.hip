@@ -424,20 +424,26 @@ void launch_wave(hr_server* S, Slot& sl, std::vector<Req*>& list)
             it = desc_of.emplace(grid[i], ndesc++).first;
         }
         const size_t n = r.n;
+        const size_t stride = r.stride ? r.stride : n;
         const float* org = r.org;
         const float* dir = r.dir;
+        uint32_t dev_stride = (uint32_t)stride;
         if (!r.on_device) {
             float* o = (float*)take(12 * n);
             float* d = (float*)take(12 * n);
-            (void)hipMemcpyAsync(o, r.org, 12 * n, hipMemcpyHostToDevice, S->stream);
-            (void)hipMemcpyAsync(d, r.dir, 12 * n, hipMemcpyHostToDevice, S->stream);
+            for (int k = 0; k < 3; ++k) {
+                (void)hipMemcpyAsync(o + k * n, r.org + k * stride, 4 * n, hipMemcpyHostToDevice, S->stream);
+                (void)hipMemcpyAsync(d + k * n, r.dir + k * stride, 4 * n, hipMemcpyHostToDevice, S->stream);
+            }
             org = o;
             dir = d;
+            dev_stride = (uint32_t)n;
         }
         hr::RayBatchDev& b = rbs[i];
         std::memset(&b, 0, sizeof b);
         b.scene = it->second;
         b.n = r.n;
+        b.stride = dev_stride;
         b.org = org;
         b.dir = dir;
         if (r.kind == HR_REQ_OCCLUDED) {
@@ -665,6 +671,7 @@ hr_status hr_server_submit(hr_server* S, const hr_request* r, uint64_t* id)
     if (r->kind == HR_REQ_OCCLUDED && !(r->tmax > 0.f)) return fail(HR_ERR_ARG, "tmax must be positive");
     if (r->kind == HR_REQ_PATH && (r->depth < 1 || r->depth > 64)) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
     if (r->n > (1u << 30)) return fail(HR_ERR_ARG, "at most 2^30 rays per request");
+    if (r->stride != 0 && r->stride < r->n) return fail(HR_ERR_ARG, "stride smaller than the ray count");
     auto q = std::make_unique<Req>();
     q->r = *r;
     q->t_submit = Clock::now();

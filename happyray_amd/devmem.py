@@ -16,7 +16,18 @@ def hip():
         _hip.hipFree.argtypes = [C.c_void_p]
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipMemcpy2DAsync.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
+                                          C.c_size_t, C.c_int, C.c_void_p]
+        _hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     return _hip
+
+
+def copy_planes(dst, src, n, src_stride, planes=3, stream=None):
+    """Copy `planes` rows of n floats from device address src (rows
+    src_stride floats apart) to dst (rows n floats apart), on `stream`."""
+    rc = hip().hipMemcpy2DAsync(dst, 4 * n, src, 4 * src_stride, 4 * n, planes, D2D, stream)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy2DAsync failed with {rc}")
 
 
 H2D, D2H, D2D = 1, 2, 3

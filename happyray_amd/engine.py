@@ -240,6 +240,9 @@ class Engine:
         check(self._L.hr_path_trace(self._h, _f(o), _f(d), n, int(depth), _f(rgb)), "hr_path_trace")
         return rgb
 
+    def path_trace_dev(self, org, dirs, n, depth, rgb_out):
+        check(self._L.hr_path_trace_dev(self._h, org, dirs, n, int(depth), rgb_out), "hr_path_trace_dev")
+
     def trace_dev(self, org, dirs, n, t_out, prim_out):
         """Closest hit of n rays whose SoA buffers are device pointers (ints)."""
         check(self._L.hr_trace_dev(self._h, org, dirs, n, t_out, prim_out), "hr_trace_dev")

@@ -349,3 +349,49 @@ def batched_workload(seed=4, n_scenes=256, tri_range=(1000, 200000), ray_range=(
         o, d = ao_rays(seed * 104729 + s, v, f, n_a, 0.05 * diag)
         items.append((s, o, d, True, 0.05 * diag))
     return out_scenes, items
+
+
+def served_workload(seed=5, n_scenes=64, tri_range=(10000, 2000000), n_requests=2000, rate=2000.0,
+                    ray_range=(1000, 4000000), pool=262144, rebuild_frac=0.05):
+    """The served configuration: n_scenes resident scenes (even: random soups,
+    odd: icosphere clusters) of tri_range triangles, log-uniform, each with two
+    key frames (the second displaced by a smooth field) and an area light
+    above it; per scene a pool of primary rays and one of ambient-occlusion
+    rays (any hit within 0.05 of the diagonal); n_requests Poisson arrivals at
+    `rate` per second, each a rebuild (probability rebuild_frac) or a closest
+    hit, occlusion or depth-4 path request of a log-uniform ray count.
+    Returns (scenes [(keys (2,3,nv), idx, light)], pools [(prim_org, prim_dir,
+    ao_org, ao_dir, ao_tmax)], requests [(t, kind, scene, n, offset, k1, k2, c)])."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]), n_scenes)).astype(np.int64)
+    out_scenes, pools = [], []
+    for s in range(n_scenes):
+        if s % 2 == 0:
+            v, f = soup_n(seed * 1000 + s, int(sizes[s]))
+        else:
+            v, f = sphere_cluster_n(seed * 1000 + s, int(sizes[s]))
+        lo, hi = v.min(1), v.max(1)
+        ext = hi - lo
+        disp = 0.01 * float(ext.max()) * np.sin(v[0] * (3.0 / max(ext[0], 1e-3)) + s)
+        v2 = v.copy()
+        v2[1] += disp.astype(np.float32)
+        keys = np.stack([v, v2]).astype(np.float32)
+        c = (lo + hi) / 2
+        light = area_light((c[0] - 0.2 * ext[0], hi[1] + 0.3 * ext[1], c[2] - 0.2 * ext[2]),
+                           (0.4 * ext[0], 0.0, 0.0), (0.0, 0.0, 0.4 * ext[2]), (30.0, 30.0, 30.0))
+        out_scenes.append((keys, f, light))
+        diag = float(np.linalg.norm(ext))
+        po, pd = primary_rays(seed * 7919 + s, v, pool)
+        ao, ad = ao_rays(seed * 104729 + s, v, f, pool, 0.05 * diag)
+        pools.append((po, pd, ao, ad, 0.05 * diag))
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    reqs = []
+    for i in range(n_requests):
+        s = int(rng.integers(0, n_scenes))
+        if rng.uniform() < rebuild_frac:
+            reqs.append((float(t[i]), "rebuild", s, 0, 0, 0, 1, float(rng.uniform())))
+            continue
+        kind = ("closest", "occluded", "path")[int(rng.integers(0, 3))]
+        n = int(np.exp(rng.uniform(np.log(ray_range[0]), np.log(ray_range[1]))))
+        reqs.append((float(t[i]), kind, s, n, 0, 0, 0, 0.0))
+    return out_scenes, pools, reqs

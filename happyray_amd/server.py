@@ -98,33 +98,34 @@ class Server:
         o, d, n = self._rays(org, dirs)
         t = np.zeros(n, np.float32)
         p = np.zeros(n, np.uint32)
-        r = _lib.Request(REQ_CLOSEST, scene, n, 0.0, 0, 0, o.ctypes.data, d.ctypes.data, t.ctypes.data,
+        r = _lib.Request(REQ_CLOSEST, scene, n, 0.0, 0, 0, o.ctypes.data, d.ctypes.data, 0, t.ctypes.data,
                          p.ctypes.data, None, None, 0, 0, 0.0)
         return self._submit(r, (t, p), [o, d])
 
     def submit_occluded(self, scene, org, dirs, tmax):
         o, d, n = self._rays(org, dirs)
         occ = np.zeros(n, np.uint8)
-        r = _lib.Request(REQ_OCCLUDED, scene, n, float(tmax), 0, 0, o.ctypes.data, d.ctypes.data, None, None,
-                         occ.ctypes.data, None, 0, 0, 0.0)
+        r = _lib.Request(REQ_OCCLUDED, scene, n, float(tmax), 0, 0, o.ctypes.data, d.ctypes.data, 0, None,
+                         None, occ.ctypes.data, None, 0, 0, 0.0)
         return self._submit(r, occ, [o, d])
 
     def submit_path(self, scene, org, dirs, depth=4):
         o, d, n = self._rays(org, dirs)
         rgb = np.zeros((n, 3), np.float32)
-        r = _lib.Request(REQ_PATH, scene, n, 0.0, int(depth), 0, o.ctypes.data, d.ctypes.data, None, None, None,
-                         rgb.ctypes.data, 0, 0, 0.0)
+        r = _lib.Request(REQ_PATH, scene, n, 0.0, int(depth), 0, o.ctypes.data, d.ctypes.data, 0, None, None,
+                         None, rgb.ctypes.data, 0, 0, 0.0)
         return self._submit(r, rgb, [o, d])
 
     def submit_rebuild(self, scene, k1, k2, coeff):
-        r = _lib.Request(REQ_REBUILD, scene, 0, 0.0, 0, 0, None, None, None, None, None, None, int(k1), int(k2),
-                         float(coeff))
+        r = _lib.Request(REQ_REBUILD, scene, 0, 0.0, 0, 0, None, None, 0, None, None, None, None, int(k1),
+                         int(k2), float(coeff))
         return self._submit(r, None, [])
 
-    def submit_device(self, kind, scene, n, org_ptr, dir_ptr, tmax=0.0, depth=4):
-        """Rays already in device memory; outputs are computed and dropped."""
-        r = _lib.Request(kind, scene, int(n), float(tmax), int(depth), 1, org_ptr, dir_ptr, None, None, None,
-                         None, 0, 0, 0.0)
+    def submit_device(self, kind, scene, n, org_ptr, dir_ptr, stride=0, tmax=0.0, depth=4):
+        """Rays already in device memory (SoA planes `stride` floats apart);
+        outputs are computed and dropped."""
+        r = _lib.Request(kind, scene, int(n), float(tmax), int(depth), 1, org_ptr, dir_ptr, int(stride), None,
+                         None, None, None, 0, 0, 0.0)
         return self._submit(r, None, [])
 
     def stats(self):

@@ -205,6 +205,9 @@ hr_status hr_occluded_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, u
  * rgb_out receives 3 floats of radiance per ray (0 for rays that miss). */
 hr_status hr_path_trace(hr_ctx* ctx, const float* org, const float* dir, uint32_t n, int depth,
                         float* rgb_out);
+/* the same with device buffers; returns before the work finishes (hr_synchronize) */
+hr_status hr_path_trace_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n, int depth,
+                            float* d_rgb_out);
 
 /* Kernel choices of a context (no reference counterpart: the reference fixes
  * its launch shapes at compile time, src/RT/Algorithm/RayTracingKernels.h:34-43). */
@@ -326,8 +329,9 @@ typedef struct {
     int depth;
     int on_device;             /* 1: rays and outputs are device pointers; NULL outputs
                                   are computed and dropped                              */
-    const float* org;          /* SoA, 3 * n                                            */
+    const float* org;          /* SoA: x, y and z planes of n floats, `stride` apart   */
     const float* dir;
+    uint32_t stride;           /* floats between the planes (0: n)                     */
     float* t_out;
     uint32_t* prim_out;
     uint8_t* occ_out;
