@@ -16,8 +16,10 @@ STATUS_NAMES = {0: "HR_OK", 1: "HR_ERR_ARG", 2: "HR_ERR_DEVICE", 3: "HR_ERR_OOM"
 GRID_UNIFORM = 0
 GRID_UNIFORM_EXACT = 1
 GRID_TWO_LEVEL = 2
-MODE_AO = 3
+MODE_SIMPLE = 0
+MODE_PATH_MULTI = 1
 MODE_PATH = 2
+MODE_AO = 3
 
 _f32p = C.POINTER(C.c_float)
 _u32p = C.POINTER(C.c_uint32)
@@ -185,6 +187,9 @@ SIGNATURES = {
     "hr_scene_upload": (C.c_int, [_vp, _vp]),
     "hr_scene_frame": (C.c_int, [_vp, C.c_float, _u32p, _u32p, _f32p]),
     "hr_scene_next_frame": (C.c_int, [_vp, _f32p]),
+    "hr_image_to_rgb8": (C.c_int, [_f32p, C.c_int, C.c_int, C.c_float, _u8p]),
+    "hr_write_png": (C.c_int, [C.c_char_p, _u8p, C.c_int, C.c_int]),
+    "hr_write_screenshot": (C.c_int, [C.c_char_p, _f32p, C.c_int, C.c_int]),
 }
 
 _lib = None

@@ -15,11 +15,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhappyray.so")
+CLI = os.path.join(HERE, "happyray")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_persist.hip", "hr_engine.hip", "hr_batch.hip",
-           "hr_server.hip"]
-CXX_SOURCES = ["hr_scene.cpp"]          # host-only C++ (scene files)
+SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_pathmk.hip", "hr_persist.hip", "hr_engine.hip",
+           "hr_batch.hip", "hr_server.hip"]
+CXX_SOURCES = ["hr_scene.cpp", "hr_image.cpp"]   # host-only C++ (scene files, images)
 ARCH = os.environ.get("HR_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -63,11 +64,19 @@ def build(verbose=False):
             if err.strip():
                 print(err, file=sys.stderr)
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-fPIC"]
+    cmd = [_hipcc(), "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-fPIC", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the headless renderer over the C ABI (Main.cpp's -nw path)
+    cli_tmp = CLI + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), os.path.join(CSRC, "hr_cli.cpp"), "-o", cli_tmp] + CXX_FLAGS + \
+          ["-L", HERE, "-lhappyray", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building the happyray executable failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(cli_tmp, CLI)
     return LIB
 
 

@@ -52,9 +52,9 @@ struct hr_batch {
     DevBuf<hr::BuildStatus> status;
     hr::BuildStatus* h_status = nullptr;          // pinned
     DevBuf<float4> rec;
-    DevBuf<uint32_t> wave_counts, keys, vals, keys_sorted, top_vals_sorted, prims, leaf_counts;
+    DevBuf<uint32_t> wave_counts, wave_offsets, keys, vals, keys_sorted, top_vals_sorted, prims, leaf_counts;
     DevBuf<uint2> cells, leaves;
-    DevBuf<uint32_t> scan_tmp, sort_keys_tmp, sort_vals_tmp, sort_hist;
+    DevBuf<uint32_t> scan_chain, sort_chain, sort_keys_tmp, sort_vals_tmp, sort_hist;
     uint64_t cells_cap = 0;
     uint32_t pairs_cap = 0, leaves_cap = 0, leaf_pairs_cap = 0;
     bool built = false;
@@ -100,6 +100,7 @@ hr_status reserve_build(hr_batch* b)
     const uint32_t waves = cdiv(std::max<uint64_t>(std::max(b->nt, b->pairs_cap), 1), 64);
     const uint32_t waves_cap = (waves + 3) / 4 * 4 + 4;
     HR_TRY(b->wave_counts.reserve(waves_cap));
+    HR_TRY(b->wave_offsets.reserve(waves_cap));
     HR_TRY(b->keys.reserve(pair_max));
     HR_TRY(b->vals.reserve(pair_max));
     HR_TRY(b->keys_sorted.reserve(pair_max));
@@ -108,12 +109,22 @@ hr_status reserve_build(hr_batch* b)
     HR_TRY(b->top_vals_sorted.reserve(b->pairs_cap));
     HR_TRY(b->leaf_counts.reserve(b->cells_cap));
     HR_TRY(b->leaves.reserve(b->leaves_cap));
-    const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)b->cells_cap, (size_t)b->ns + 1,
-                                             hr::sort_hist_words(pair_max) });
-    HR_TRY(b->scan_tmp.reserve(hr::scan_temp_words((uint32_t)std::min<size_t>(scan_n, 0xFFFFFFFFu))));
+    const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)b->cells_cap, (size_t)b->ns + 1});
+    {
+        // the single-pass scan's tile flags must start cleared
+        const size_t before = b->scan_chain.cap;
+        HR_TRY(b->scan_chain.reserve(hr::scan_chain_words((uint32_t)std::min<size_t>(scan_n, 0xFFFFFFFFu))));
+        if (b->scan_chain.cap != before) hr::fill_u32(b->scan_chain.p, 0u, b->scan_chain.cap, nullptr, 1, b->stream);
+    }
     HR_TRY(b->sort_keys_tmp.reserve(pair_max));
     HR_TRY(b->sort_vals_tmp.reserve(pair_max));
     HR_TRY(b->sort_hist.reserve(hr::sort_hist_words(pair_max)));
+    {
+        // the sort passes' tile flags must start cleared
+        const size_t before = b->sort_chain.cap;
+        HR_TRY(b->sort_chain.reserve(hr::sort_chain_words(pair_max)));
+        if (b->sort_chain.cap != before) hr::fill_u32(b->sort_chain.p, 0u, b->sort_chain.cap, nullptr, 1, b->stream);
+    }
     return HR_OK;
 }
 
@@ -124,6 +135,7 @@ hr::BuildBuffers buffers(hr_batch* b)
     B.status = b->status.p;
     B.rec = b->rec.p;
     B.wave_counts = b->wave_counts.p;
+    B.wave_offsets = b->wave_offsets.p;
     B.keys = b->keys.p;
     B.vals = b->vals.p;
     B.keys_sorted = b->keys_sorted.p;
@@ -137,12 +149,12 @@ hr::BuildBuffers buffers(hr_batch* b)
     B.leaves_cap = b->leaves_cap;
     B.leaf_pairs_cap = b->leaf_pairs_cap;
     B.waves_cap = (uint32_t)b->wave_counts.cap;
-    B.scan.block_sums = b->scan_tmp.p;
-    B.scan.cap = b->scan_tmp.cap;
+    B.scan.chain = b->scan_chain.p;
+    B.scan.chain_cap = b->scan_chain.cap;
     B.sort.keys_tmp = b->sort_keys_tmp.p;
     B.sort.vals_tmp = b->sort_vals_tmp.p;
     B.sort.hist = b->sort_hist.p;
-    B.sort.scan = B.scan;
+    B.sort.chain = b->sort_chain.p;
     return B;
 }
 
@@ -417,8 +429,9 @@ hr_status hr_batch_destroy(hr_batch* b)
     for (auto* x : f) x->release();
     DevBuf<uint32_t>* u[] = { &b->idx, &b->tri_scene, &b->vbase, &b->tbase, &b->bounds, &b->cell_counts,
                               &b->cell_base, &b->top_base, &b->leaf_base, &b->pair_base, &b->wave_counts,
+                              &b->wave_offsets, &b->scan_chain,
                               &b->keys, &b->vals, &b->keys_sorted, &b->top_vals_sorted, &b->prims,
-                              &b->leaf_counts, &b->scan_tmp, &b->sort_keys_tmp, &b->sort_vals_tmp,
+                              &b->leaf_counts, &b->sort_chain, &b->sort_keys_tmp, &b->sort_vals_tmp,
                               &b->sort_hist, &b->prefix_dev[0], &b->prefix_dev[1], &b->pool };
     for (auto* x : u) x->release();
     b->gps.release();

@@ -21,7 +21,8 @@ struct BuildBuffers {
     GridParams* params;
     BuildStatus* status;
     float4* rec;                // 3 * nt triangle records
-    uint32_t* wave_counts;      // per-wave pair counts, then offsets
+    uint32_t* wave_counts;      // per-wave pair counts
+    uint32_t* wave_offsets;     // their exclusive scan
     uint32_t* keys;             // unsorted keys / values (pairs, then leaf pairs)
     uint32_t* vals;
     uint32_t* keys_sorted;

@@ -165,14 +165,6 @@ __global__ void k_seal(GridParams* gp, const BuildStatus* st)
     gp->num_cells = 0u;
 }
 
-// total > cap sets the overflow bit and zeroes the effective size
-__global__ void k_check(BuildStatus* st, uint32_t* total, uint32_t* eff, uint32_t cap, uint32_t bit)
-{
-    if (threadIdx.x != 0) return;
-    const uint32_t t = *total;
-    if (t > cap) st->overflow |= bit;
-    *eff = st->overflow ? 0u : t;
-}
 
 // ---------------------------------------------------------------------------
 // exact triangle / cell test (UniformGridBuildKernels.cu:8-48, :305-361)
@@ -718,15 +710,14 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     else
         hipLaunchKernelGGL((k_top_pairs<false, false>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
                            a.vy, a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap);
-    scan_exclusive(b.wave_counts, b.wave_counts, waves, nullptr, &st->num_pairs, b.scan, s);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_pairs, &st->n_pairs_eff,
-                       b.pairs_cap, kOverflowPairs);
+    const ScanCheck pairs_check{ st, b.pairs_cap, kOverflowPairs, &st->n_pairs_eff };
+    scan_exclusive_1p(b.wave_counts, b.wave_offsets, 1, waves, nullptr, &st->num_pairs, &pairs_check, b.scan, s);
     if (exact)
         hipLaunchKernelGGL((k_top_pairs<true, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
-                           a.vy, a.vz, nt, b.params, st, b.rec, nullptr, b.wave_counts, b.keys, b.vals, b.waves_cap);
+                           a.vy, a.vz, nt, b.params, st, b.rec, nullptr, b.wave_offsets, b.keys, b.vals, b.waves_cap);
     else
         hipLaunchKernelGGL((k_top_pairs<false, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
-                           a.vy, a.vz, nt, b.params, st, b.rec, nullptr, b.wave_counts, b.keys, b.vals, b.waves_cap);
+                           a.vy, a.vz, nt, b.params, st, b.rec, nullptr, b.wave_offsets, b.keys, b.vals, b.waves_cap);
 
     // ---- sort by cell key, then cell ranges ----
     const int cell_bits = bits_for(b.cells_cap);
@@ -746,10 +737,9 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     const uint32_t cblocks = cdiv(b.cells_cap, 256) > 4096 ? 4096 : cdiv(b.cells_cap, 256);
     hipLaunchKernelGGL(k_leaf_res, dim3(cblocks), dim3(256), 0, s, b.params, b.cells, b.leaf_counts,
                        a.lambda_leaf, st);
-    scan_exclusive_strided(b.leaf_counts, (uint32_t*)b.cells, 2, (uint32_t)b.cells_cap,
-                           &b.params->num_cells, &st->num_leaves, b.scan, s);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaves, &st->n_leaves_eff,
-                       b.leaves_cap, kOverflowLeaves);
+    const ScanCheck leaves_check{ st, b.leaves_cap, kOverflowLeaves, &st->n_leaves_eff };
+    scan_exclusive_1p(b.leaf_counts, (uint32_t*)b.cells, 2, (uint32_t)b.cells_cap, &b.params->num_cells,
+                      &st->num_leaves, &leaves_check, b.scan, s);
 
     // ---- leaf pairs: count, scan, write ----
     const uint32_t lwaves = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, kWave);
@@ -757,12 +747,12 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     hipLaunchKernelGGL((k_leaf_pairs<false>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
                        b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, b.wave_counts,
                        nullptr, nullptr, nullptr, b.waves_cap);
-    scan_exclusive(b.wave_counts, b.wave_counts, lwaves, nullptr, &st->num_leaf_pairs, b.scan, s);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaf_pairs, &st->n_leaf_pairs_eff,
-                       b.leaf_pairs_cap, kOverflowLeafPairs);
+    const ScanCheck lp_check{ st, b.leaf_pairs_cap, kOverflowLeafPairs, &st->n_leaf_pairs_eff };
+    scan_exclusive_1p(b.wave_counts, b.wave_offsets, 1, lwaves, nullptr, &st->num_leaf_pairs, &lp_check, b.scan,
+                      s);
     hipLaunchKernelGGL((k_leaf_pairs<true>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
                        b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, nullptr,
-                       b.wave_counts, b.keys, b.vals, b.waves_cap);
+                       b.wave_offsets, b.keys, b.vals, b.waves_cap);
 
     // ---- leaf sort and ranges ----
     const int leaf_bits = bits_for(b.leaves_cap);
@@ -794,17 +784,16 @@ void run_build_batch(const BuildArgs& a, BuildBuffers& b, const SceneTable& T, u
     if (vblocks > 2048) vblocks = 2048;
     hipLaunchKernelGGL(k_batch_bounds, dim3(vblocks), dim3(kBlock), 0, s, a.vx, a.vy, a.vz, nv_total, T);
     hipLaunchKernelGGL(k_batch_setup, dim3(sblocks), dim3(256), 0, s, T, st);
-    scan_exclusive(T.cell_counts, T.cell_base, T.ns, nullptr, &st->num_cells, b.scan, s);
+    scan_exclusive_1p(T.cell_counts, T.cell_base, 1, T.ns, nullptr, &st->num_cells, nullptr, b.scan, s);
     hipLaunchKernelGGL(k_batch_total, dim3(1), dim3(64), 0, s, T, b.params, st, b.cells_cap);
 
     // ---- top level: count, scan, write ----
     hipLaunchKernelGGL((k_top_pairs<false, false, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx, a.vy,
                        a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap, T);
-    scan_exclusive(b.wave_counts, b.wave_counts, waves, nullptr, &st->num_pairs, b.scan, s);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_pairs, &st->n_pairs_eff, b.pairs_cap,
-                       kOverflowPairs);
+    const ScanCheck pairs_check{ st, b.pairs_cap, kOverflowPairs, &st->n_pairs_eff };
+    scan_exclusive_1p(b.wave_counts, b.wave_offsets, 1, waves, nullptr, &st->num_pairs, &pairs_check, b.scan, s);
     hipLaunchKernelGGL((k_top_pairs<false, true, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx, a.vy,
-                       a.vz, nt, b.params, st, b.rec, nullptr, b.wave_counts, b.keys, b.vals, b.waves_cap, T);
+                       a.vz, nt, b.params, st, b.rec, nullptr, b.wave_offsets, b.keys, b.vals, b.waves_cap, T);
 
     // ---- sort by global cell key, cell ranges ----
     radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.top_vals_sorted, b.pairs_cap, &st->n_pairs_eff,
@@ -820,10 +809,9 @@ void run_build_batch(const BuildArgs& a, BuildBuffers& b, const SceneTable& T, u
     const uint32_t cblocks = cdiv(b.cells_cap, 256) > 4096 ? 4096 : cdiv(b.cells_cap, 256);
     hipLaunchKernelGGL((k_leaf_res<true>), dim3(cblocks), dim3(256), 0, s, b.params, b.cells, b.leaf_counts, 0.f,
                        st, T);
-    scan_exclusive_strided(b.leaf_counts, (uint32_t*)b.cells, 2, (uint32_t)b.cells_cap, &b.params->num_cells,
-                           &st->num_leaves, b.scan, s);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaves, &st->n_leaves_eff, b.leaves_cap,
-                       kOverflowLeaves);
+    const ScanCheck leaves_check{ st, b.leaves_cap, kOverflowLeaves, &st->n_leaves_eff };
+    scan_exclusive_1p(b.leaf_counts, (uint32_t*)b.cells, 2, (uint32_t)b.cells_cap, &b.params->num_cells,
+                      &st->num_leaves, &leaves_check, b.scan, s);
 
     // ---- leaf pairs ----
     const uint32_t lwaves = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, kWave);
@@ -831,11 +819,11 @@ void run_build_batch(const BuildArgs& a, BuildBuffers& b, const SceneTable& T, u
     hipLaunchKernelGGL((k_leaf_pairs<false, true>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
                        b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, b.wave_counts, nullptr,
                        nullptr, nullptr, b.waves_cap, T);
-    scan_exclusive(b.wave_counts, b.wave_counts, lwaves, nullptr, &st->num_leaf_pairs, b.scan, s);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, st, &st->num_leaf_pairs, &st->n_leaf_pairs_eff,
-                       b.leaf_pairs_cap, kOverflowLeafPairs);
+    const ScanCheck lp_check{ st, b.leaf_pairs_cap, kOverflowLeafPairs, &st->n_leaf_pairs_eff };
+    scan_exclusive_1p(b.wave_counts, b.wave_offsets, 1, lwaves, nullptr, &st->num_leaf_pairs, &lp_check, b.scan,
+                      s);
     hipLaunchKernelGGL((k_leaf_pairs<true, true>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
-                       b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, nullptr, b.wave_counts,
+                       b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, nullptr, b.wave_offsets,
                        b.keys, b.vals, b.waves_cap, T);
 
     // ---- leaf sort and ranges ----

@@ -165,33 +165,47 @@ __device__ __forceinline__ void box_cells(f3 lo, f3 hi, f3 origin, f3 rcp, int r
 // ---------------------------------------------------------------------------
 
 struct ScanTemp {
-    uint32_t* block_sums;   // capacity: ceil(n_cap / kScanTile) + 1
-    size_t cap;
+    uint32_t* chain;        // scan_chain_words(n_cap), zeroed when allocated
+    size_t chain_cap;
+};
+
+// A capacity check fused into a single-pass scan: the workgroup that knows the
+// total sets st->overflow |= bit when total > cap, and *eff = overflow ? 0 : total.
+struct ScanCheck {
+    BuildStatus* st;
+    uint32_t cap;
+    uint32_t bit;
+    uint32_t* eff;
 };
 
 constexpr uint32_t kScanTile = 2048;     // 256 threads x 8
 constexpr uint32_t kSortTile = 4096;     // 256 threads x 16
 
-// exclusive scan of n (host value, or *n_dev when n_dev != nullptr) uint32;
-// *total_dev (may be nullptr) receives the sum.
-void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_cap, const uint32_t* n_dev,
-                    uint32_t* total_dev, ScanTemp& tmp, hipStream_t s);
-// same, writing out[i * out_stride]
-void scan_exclusive_strided(const uint32_t* in, uint32_t* out, uint32_t out_stride, uint32_t n_cap,
-                            const uint32_t* n_dev, uint32_t* total_dev, ScanTemp& tmp,
-                            hipStream_t s);
-size_t scan_temp_words(uint32_t n_cap);
+// Exclusive scan of n uint32 (n_cap, or *n_dev when n_dev != nullptr) into
+// out[i * out_stride], with decoupled look-back: one launch; tiles take
+// ids in launch order, publish their sum, then their inclusive prefix, and a
+// tile reads its predecessors' published values.  A predecessor that has not
+// published after a bounded wait has its sum recomputed from `in`, so every
+// workgroup finishes.  `out` must not alias `in`.  *total_dev and the check
+// (both may be null) are written by the tile holding the last element.
+void scan_exclusive_1p(const uint32_t* in, uint32_t* out, uint32_t out_stride, uint32_t n_cap,
+                       const uint32_t* n_dev, uint32_t* total_dev, const ScanCheck* check, ScanTemp& tmp,
+                       hipStream_t s);
+size_t scan_chain_words(uint32_t n_cap);
 
-// stable LSD radix sort of (key, value) pairs on the low `bits` key bits.
-// n = *n_dev (<= n_cap).  keys_in / vals_in are overwritten when the number
-// of passes exceeds 2.  The result lands in keys_out / vals_out.
+// stable LSD radix sort of (key, value) pairs on the low `bits` key bits
+// (at most 32).  n = *n_dev (<= n_cap).  keys_in / vals_in are overwritten
+// when the number of passes exceeds 2.  The result lands in keys_out /
+// vals_out.  One upsweep counts the digits of every pass; each pass is one
+// scatter launch with decoupled look-back over the tiles.
 struct SortTemp {
     uint32_t* keys_tmp;
     uint32_t* vals_tmp;
-    uint32_t* hist;         // 256 * tiles(n_cap)
-    ScanTemp scan;
+    uint32_t* hist;         // sort_hist_words: digit counts of every pass
+    uint32_t* chain;        // sort_chain_words, zeroed when allocated: per-tile digit flags
 };
 size_t sort_hist_words(uint32_t n_cap);
+size_t sort_chain_words(uint32_t n_cap);
 void radix_sort_pairs(uint32_t* keys_in, uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
                       uint32_t n_cap, const uint32_t* n_dev, int bits, SortTemp& tmp, hipStream_t s);
 

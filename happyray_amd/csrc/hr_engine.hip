@@ -76,9 +76,9 @@ struct hr_ctx {
     DevBuf<hr::BuildStatus> status;
     hr::BuildStatus* h_status = nullptr;  // pinned
     DevBuf<float4> rec;
-    DevBuf<uint32_t> wave_counts, keys, vals, keys_sorted, top_vals_sorted, prims, leaf_counts;
+    DevBuf<uint32_t> wave_counts, wave_offsets, keys, vals, keys_sorted, top_vals_sorted, prims, leaf_counts;
     DevBuf<uint2> cells, leaves;
-    DevBuf<uint32_t> scan_tmp, sort_keys_tmp, sort_vals_tmp, sort_hist;
+    DevBuf<uint32_t> scan_chain, sort_chain, sort_keys_tmp, sort_vals_tmp, sort_hist;
     uint64_t cells_cap = 0;
     uint32_t pairs_cap = 0, leaves_cap = 0, leaf_pairs_cap = 0;
     bool built = false;
@@ -97,6 +97,12 @@ struct hr_ctx {
     DevBuf<uint8_t> occ;
     uint32_t last_npix = 0;
     int last_ao_samples = 0;
+    // multi-kernel path tracer: end vertices (7 planes), then 7 rgb buffers
+    // (importance, light, light direction, new radiance, residue, blended,
+    // contribution); cleared with the camera
+    DevBuf<float> pm;
+    DevBuf<uint32_t> pm_prims;
+    uint32_t pm_npix = 0;
 
     // ray batches
     DevBuf<float> ray_org, ray_dir, ray_t, ray_hits, ray_rgb;
@@ -167,6 +173,7 @@ hr_status reserve_build(hr_ctx* c)
     const uint32_t waves = cdiv(std::max<uint64_t>(std::max(nt, c->pairs_cap), 1), 64);
     const uint32_t waves_cap = (waves + 3) / 4 * 4 + 4;
     HR_TRY(c->wave_counts.reserve(waves_cap));
+    HR_TRY(c->wave_offsets.reserve(waves_cap));
     HR_TRY(c->keys.reserve(pair_max));
     HR_TRY(c->vals.reserve(pair_max));
     HR_TRY(c->keys_sorted.reserve(pair_max));
@@ -177,12 +184,22 @@ hr_status reserve_build(hr_ctx* c)
         HR_TRY(c->leaf_counts.reserve(c->cells_cap));
         HR_TRY(c->leaves.reserve(c->leaves_cap));
     }
-    const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)c->cells_cap,
-                                             hr::sort_hist_words(pair_max) });
-    HR_TRY(c->scan_tmp.reserve(hr::scan_temp_words((uint32_t)std::min<size_t>(scan_n, 0xFFFFFFFFu))));
+    const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)c->cells_cap});
+    {
+        // the single-pass scan's tile flags must start cleared
+        const size_t before = c->scan_chain.cap;
+        HR_TRY(c->scan_chain.reserve(hr::scan_chain_words((uint32_t)std::min<size_t>(scan_n, 0xFFFFFFFFu))));
+        if (c->scan_chain.cap != before) hr::fill_u32(c->scan_chain.p, 0u, c->scan_chain.cap, nullptr, 1, c->stream);
+    }
     HR_TRY(c->sort_keys_tmp.reserve(pair_max));
     HR_TRY(c->sort_vals_tmp.reserve(pair_max));
     HR_TRY(c->sort_hist.reserve(hr::sort_hist_words(pair_max)));
+    {
+        // the sort passes' tile flags must start cleared
+        const size_t before = c->sort_chain.cap;
+        HR_TRY(c->sort_chain.reserve(hr::sort_chain_words(pair_max)));
+        if (c->sort_chain.cap != before) hr::fill_u32(c->sort_chain.p, 0u, c->sort_chain.cap, nullptr, 1, c->stream);
+    }
     return HR_OK;
 }
 
@@ -193,6 +210,7 @@ hr::BuildBuffers build_buffers(hr_ctx* c)
     b.status = c->status.p;
     b.rec = c->rec.p;
     b.wave_counts = c->wave_counts.p;
+    b.wave_offsets = c->wave_offsets.p;
     b.keys = c->keys.p;
     b.vals = c->vals.p;
     b.keys_sorted = c->keys_sorted.p;
@@ -206,12 +224,12 @@ hr::BuildBuffers build_buffers(hr_ctx* c)
     b.leaves_cap = c->leaves_cap;
     b.leaf_pairs_cap = c->leaf_pairs_cap;
     b.waves_cap = (uint32_t)c->wave_counts.cap;
-    b.scan.block_sums = c->scan_tmp.p;
-    b.scan.cap = c->scan_tmp.cap;
+    b.scan.chain = c->scan_chain.p;
+    b.scan.chain_cap = c->scan_chain.cap;
     b.sort.keys_tmp = c->sort_keys_tmp.p;
     b.sort.vals_tmp = c->sort_vals_tmp.p;
     b.sort.hist = c->sort_hist.p;
-    b.sort.scan = b.scan;
+    b.sort.chain = c->sort_chain.p;
     return b;
 }
 
@@ -374,7 +392,8 @@ hr_status check_render_args(hr_ctx* c, hr_mode mode, int image_id, const hr_rend
 {
     if (!c->has_camera) return fail(HR_ERR_STATE, "render before setCamera");
     if (!p) return fail(HR_ERR_ARG, "null render parameters");
-    if (mode != HR_MODE_AO && mode != HR_MODE_PATH) return fail(HR_ERR_ARG, "unknown render mode");
+    if (mode != HR_MODE_AO && mode != HR_MODE_PATH && mode != HR_MODE_SIMPLE && mode != HR_MODE_PATH_MULTI)
+        return fail(HR_ERR_ARG, "unknown render mode");
     if (image_id < 0) return fail(HR_ERR_ARG, "negative image id");
     if (mode == HR_MODE_AO && (p->ao_samples < 1 || p->ao_samples > 256))
         return fail(HR_ERR_ARG, "ao_samples must be in [1, 256]");
@@ -391,9 +410,38 @@ hr_status prepare_render(hr_ctx* c, hr_mode mode, const hr_render_params* p)
     HR_TRY(c->hit_prims.reserve(npix));
     HR_TRY(c->fb.reserve(3 * (size_t)npix));
     if (mode == HR_MODE_AO) HR_TRY(c->occ.reserve((size_t)npix * (size_t)p->ao_samples));
+    if (mode == HR_MODE_PATH_MULTI) {
+        const size_t before = c->pm.cap;
+        HR_TRY(c->pm.reserve(28 * (size_t)npix));
+        HR_TRY(c->pm_prims.reserve(npix));
+        if (c->pm.cap != before || c->pm_npix != npix) {
+            // PathTracer::setResolution clears the carried buffers (PathTracer.h:1028-1035)
+            hr::fill_u32((uint32_t*)c->pm.p, 0u, c->pm.cap, nullptr, 1, c->stream);
+            c->pm_npix = npix;
+        }
+    }
     HR_TRY(ensure_default_material(c));
     return HR_OK;
 }
+
+hr::PathBuffers path_buffers(hr_ctx* c)
+{
+    const size_t n = c->pm_npix;
+    float* q = c->pm.p;
+    hr::PathBuffers pb;
+    pb.end = q;
+    pb.end_prims = c->pm_prims.p;
+    pb.importance = q + 7 * n;
+    pb.ls = q + 10 * n;
+    pb.dir_ls = q + 13 * n;
+    pb.new_radiance = q + 16 * n;
+    pb.residue = q + 19 * n;
+    pb.blended = q + 22 * n;
+    pb.contrib = q + 25 * n;
+    return pb;
+}
+
+bool uses_normals(hr_mode mode) { return mode == HR_MODE_PATH || mode == HR_MODE_PATH_MULTI; }
 
 // current-frame normals from the normal key frames; with fs the key frames
 // are chosen on the device
@@ -419,7 +467,8 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     hr::RenderArgs ra;
     ra.seed_lo = c->seed_lo;
     ra.seed_hi = c->seed_hi;
-    ra.jitter = p->jitter;
+    ra.jitter = mode == HR_MODE_PATH_MULTI ? 1 : p->jitter;   // PathTracer's RandomPrimaryRayGenerator
+    ra.simple_sampler = mode == HR_MODE_SIMPLE ? 1 : 0;
     ra.ao_samples = p->ao_samples;
     ra.ao_dist = p->ao_dist;
     ra.pt_depth = p->pt_depth;
@@ -435,11 +484,20 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     sh.nlights = c->nlights;
     sh.normals = nullptr;
     sh.nidx = nullptr;
-    if (mode == HR_MODE_PATH && c->has_normals) {
+    if (uses_normals(mode) && c->has_normals) {
         sh.normals = c->normals.p;
         sh.nidx = c->nidx.p;
     }
     if (events) HR_HIP(hipEventRecord(c->ev_r0, c->stream));
+    if (mode == HR_MODE_PATH_MULTI && c->nlights == 0) {
+        // PathTracer::integrate returns at once without lights (PathTracer.h:1049-1053)
+        if (events) {
+            HR_HIP(hipEventRecord(c->ev_p1, c->stream));
+            HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+            HR_HIP(hipEventRecord(c->ev_r1, c->stream));
+        }
+        return HR_OK;
+    }
     const bool pers = c->tuning.persistent_secondary != 0;
     const uint32_t refill = (uint32_t)c->tuning.refill;
     if (c->tuning.persistent_primary)
@@ -458,9 +516,15 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
         hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->stream);
         c->last_ao_samples = p->ao_samples;
-    } else {
+    } else if (mode == HR_MODE_PATH) {
         hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+    } else if (mode == HR_MODE_SIMPLE) {
+        if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+        hr::launch_simple_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->fb.p, c->stream);
+    } else {
+        hr::launch_path_multi(S, c->cam, ra, c->hits.p, c->hit_prims.p, sh, path_buffers(c), c->fb.p,
+                              c->ray_count.p, c->stream, events ? c->ev_p2 : nullptr);
     }
     if (events) HR_HIP(hipEventRecord(c->ev_r1, c->stream));
     HR_HIP(hipGetLastError());
@@ -494,17 +558,19 @@ std::vector<unsigned char> graph_signature(hr_ctx* c, hr_mode mode, const hr_ren
     put(v, c->nframes);
     const void* ptrs[] = { c->keyframes.p, c->verts.p, c->idx.p, c->rec.p, c->wave_counts.p, c->keys.p,
                            c->vals.p, c->keys_sorted.p, c->top_vals_sorted.p, c->prims.p, c->cells.p,
-                           c->leaves.p, c->leaf_counts.p, c->scan_tmp.p, c->sort_keys_tmp.p,
+                           c->leaves.p, c->leaf_counts.p, c->wave_offsets.p, c->scan_chain.p,
+                           c->sort_chain.p, c->sort_keys_tmp.p,
                            c->sort_vals_tmp.p, c->sort_hist.p, c->mats.p, c->mat_idx.p, c->lights.p,
                            c->light_cdf.p, c->normal_keys.p, c->normals.p, c->nidx.p, c->hits.p,
-                           c->hit_prims.p, c->fb.p, c->occ.p };
+                           c->hit_prims.p, c->fb.p, c->occ.p, c->pm.p, c->pm_prims.p };
     for (const void* q : ptrs) put(v, q);
     put(v, c->cells_cap);
     put(v, c->pairs_cap);
     put(v, c->leaves_cap);
     put(v, c->leaf_pairs_cap);
     put(v, c->wave_counts.cap);
-    put(v, c->scan_tmp.cap);
+    put(v, c->wave_offsets.cap);
+    put(v, c->pm_npix);
     put(v, c->cam);
     put(v, c->nlights);
     put(v, c->has_mat_idx);
@@ -577,12 +643,13 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     size_t b = 0;
     b += c->key_a.bytes() + c->key_b.bytes() + c->keyframes.bytes() + c->verts.bytes() + c->idx.bytes();
     b += c->normal_keys.bytes() + c->nidx.bytes() + c->normals.bytes() + c->mats.bytes() + c->mat_idx.bytes();
-    b += c->rec.bytes() + c->wave_counts.bytes() + c->keys.bytes() + c->vals.bytes() + c->keys_sorted.bytes();
+    b += c->rec.bytes() + c->wave_counts.bytes() + c->wave_offsets.bytes() + c->scan_chain.bytes();
+    b += c->keys.bytes() + c->vals.bytes() + c->keys_sorted.bytes();
     b += c->top_vals_sorted.bytes() + c->prims.bytes() + c->leaf_counts.bytes() + c->cells.bytes();
-    b += c->leaves.bytes() + c->scan_tmp.bytes() + c->sort_keys_tmp.bytes() + c->sort_vals_tmp.bytes();
+    b += c->leaves.bytes() + c->sort_chain.bytes() + c->sort_keys_tmp.bytes() + c->sort_vals_tmp.bytes();
     b += c->sort_hist.bytes() + c->lights.bytes() + c->light_cdf.bytes() + c->hits.bytes() + c->fb.bytes();
     b += c->hit_prims.bytes() + c->occ.bytes() + c->ray_org.bytes() + c->ray_dir.bytes() + c->ray_t.bytes();
-    b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes();
+    b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes() + c->pm.bytes() + c->pm_prims.bytes();
     return b;
 }
 
@@ -695,9 +762,12 @@ hr_status hr_destroy(hr_ctx* c)
                                &c->light_cdf, &c->hits, &c->fb, &c->ray_org, &c->ray_dir, &c->ray_t,
                                &c->ray_hits, &c->ray_rgb };
     for (auto* b : fbufs) b->release();
-    DevBuf<uint32_t>* ubufs[] = { &c->idx, &c->mat_idx, &c->wave_counts, &c->keys, &c->vals,
+    c->pm.release();
+    c->pm_prims.release();
+    DevBuf<uint32_t>* ubufs[] = { &c->idx, &c->mat_idx, &c->wave_counts, &c->wave_offsets, &c->scan_chain,
+                                  &c->keys, &c->vals,
                                   &c->keys_sorted, &c->top_vals_sorted, &c->prims, &c->leaf_counts,
-                                  &c->scan_tmp, &c->sort_keys_tmp, &c->sort_vals_tmp, &c->sort_hist,
+                                  &c->sort_chain, &c->sort_keys_tmp, &c->sort_vals_tmp, &c->sort_hist,
                                   &c->hit_prims, &c->ray_prim };
     for (auto* b : ubufs) b->release();
     c->occ.release();
@@ -910,6 +980,8 @@ hr_status hr_set_camera(hr_ctx* c, const float pos[3], const float dir[3], const
         return fail(HR_ERR_ARG, "invalid resolution");
     camera_init(c->cam, pos, dir, up, fov_deg, res_x, res_y);
     c->has_camera = true;
+    // RTEngine::setCamera -> PathTracer::setResolution (RTEngine.cu:389)
+    if (c->pm.p) hr::fill_u32((uint32_t*)c->pm.p, 0u, c->pm.cap, nullptr, 1, c->stream);
     return HR_OK;
 }
 
@@ -944,7 +1016,7 @@ hr_status hr_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_param
         hr::fill_u32((uint32_t*)c->fb.p, 0u, 3 * (size_t)npix, nullptr, 1, c->stream);
         c->last_npix = npix;
     }
-    if (mode == HR_MODE_PATH && c->has_normals && c->normals_dirty) {
+    if (uses_normals(mode) && c->has_normals && c->normals_dirty) {
         enqueue_normals(c, nullptr);
         c->normals_dirty = false;
     }
@@ -1002,7 +1074,7 @@ hr_status hr_render_frame_graph(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff
         hr::fill_u32((uint32_t*)c->fb.p, 0u, 3 * (size_t)npix, nullptr, 1, c->stream);
         c->last_npix = npix;
     }
-    if (mode == HR_MODE_PATH && c->has_normals) {
+    if (uses_normals(mode) && c->has_normals) {
         enqueue_normals(c, nullptr);
         c->normals_dirty = false;
     }

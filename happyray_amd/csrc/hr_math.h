@@ -219,6 +219,8 @@ HR_HD float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 constexpr uint32_t kStreamPrimary = 0u;
 constexpr uint32_t kStreamAO = 16u;      // + sample index
 constexpr uint32_t kStreamPT = 1024u;    // + 2 * vertex + {0, 1}
+constexpr uint32_t kStreamPathMK = 2048u;     // multi-kernel path tracer: + 2 * vertex + {0, 1}
+constexpr uint32_t kStreamPathLight = 4096u;  // its shadow-ray light sample
 constexpr uint32_t kCounterTag = 0x68617070u;
 
 HR_HD void rand4(uint32_t seed_lo, uint32_t seed_hi, uint32_t pixel, uint32_t image,

@@ -25,6 +25,9 @@ struct CameraDev {
 struct RenderArgs {
     uint32_t seed_lo, seed_hi;
     int jitter;
+    int simple_sampler;          // 1: RTEngine::renderFrame mode 0's choice (RTEngine.cu:414-423):
+                                 // RegularPixelSampler<2,2> with sample id = image for images
+                                 // 0-3, the Gaussian pixel filter after; 0: `jitter` decides
     int ao_samples;
     float ao_dist;
     int pt_depth;
@@ -62,6 +65,31 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
 // SoA key-frame blocks of nn normals
 // zero n floats of fb when fs->image == 0 (read on the device)
 void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s);
+// simpleShade (src/RT/Integrator/SimpleIntegrator.h:42-125): barycentric colour
+// times |cos| at each primary hit, background (0.15, 0.2, 0.3), running mean
+void launch_simple_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                         const uint32_t* hit_prims, float* fb, hipStream_t s);
+
+// Buffers of the multi-kernel path tracer (PathTracer.h:1038-1215), npix pixels.
+// rgb buffers hold 3 floats per pixel; new_radiance and residue carry over from
+// one image to the next and are cleared with the camera.
+struct PathBuffers {
+    float* end;                 // 7 SoA planes: end-of-path hit point, direction, t
+    uint32_t* end_prims;
+    float* importance;          // path throughput (ImportanceBuffer)
+    float* ls;                  // light radiance of the shadow sample
+    float* dir_ls;              // unit direction to the light, 0 if occluded or none
+    float* new_radiance;        // running mean of throughput x direct light
+    float* residue;             // what the blur held back (addIndirectIllumination)
+    float* blended;             // fb * old_w + contribution * new_w (blur input)
+    float* contrib;             // importance * new_radiance + residue
+};
+constexpr int kPathMaxBounces = 8;   // RFRACTION_PATH_MAX_DEPTH (PathTracer.h:23)
+// tracePath, shadow rays, computeDirectIllumination and addIndirectIllumination;
+// hits / hit_prims hold the primary hits
+void launch_path_multi(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, const float* hits,
+                       const uint32_t* hit_prims, const Shading& sh, const PathBuffers& pb, float* fb,
+                       unsigned long long* ray_count, hipStream_t s, hipEvent_t after_trace = nullptr);
 // fs != nullptr: a and b are blocks fs->k1 and fs->k2 of keys, chosen on the device
 void launch_frame_normals(const float* a, const float* b, uint32_t nn, float4* out, hipStream_t s,
                           const FrameSel* fs = nullptr, const float* keys = nullptr);

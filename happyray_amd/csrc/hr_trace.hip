@@ -13,6 +13,7 @@
 //   AmbientOcclusionRayGenerator     src/RT/Algorithm/RayGenerators.h:273-344
 //   computeAOIllumination            src/RT/Integrator/AOIntegrator.h:64-191
 //   pathTrace                        src/RT/Integrator/PathTracer.h:41-372
+//   simpleShade                      src/RT/Integrator/SimpleIntegrator.h:42-125
 //   uploadObjFrameNormalData         src/Application/WFObjectUploader.h:121-165
 //
 // Departures, shared with the CPU oracle: the counter-based RNG keyed by
@@ -33,24 +34,6 @@ namespace {
 
 constexpr int kBlock = 256;
 using namespace trav;
-
-// Shading normal of the path tracer (PathTracer.h:109-135): barycentric
-// weights from the sub-triangle areas seen from the hit point, then the
-// normalized blend of the corner normals; the geometric normal when the
-// scene has no normals.
-__device__ __forceinline__ f3 shading_normal(const Shading& sh, uint32_t prim, f3 org, f3 v0, f3 v1, f3 v2)
-{
-    const f3 real_normal = cross(v1 - v0, v2 - v0);
-    if (!sh.nidx) return normalize(real_normal);
-    const f3 a = v0 - org, b = v1 - org, c = v2 - org;
-    const f3 n0 = cross(b, c), n1 = cross(c, a);
-    const float rcp = 1.f / sqrtf(dot(real_normal, real_normal));
-    const float u = len(n0) * rcp, v = len(n1) * rcp;
-    const float4 c0 = sh.normals[sh.nidx[3 * (size_t)prim]];
-    const float4 c1 = sh.normals[sh.nidx[3 * (size_t)prim + 1]];
-    const float4 c2 = sh.normals[sh.nidx[3 * (size_t)prim + 2]];
-    return normalize(u * mk(c0.x, c0.y, c0.z) + v * mk(c1.x, c1.y, c1.z) + (1.f - u - v) * mk(c2.x, c2.y, c2.z));
-}
 
 // ---------------------------------------------------------------------------
 // primary rays
@@ -350,6 +333,45 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
 }
 
 // ---------------------------------------------------------------------------
+// simple integrator (SimpleIntegrator.h:42-125), one lane per pixel
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_simple_shade(TraceScene S, RenderArgs ra, uint32_t npix,
+                                                         const float* __restrict__ hits,
+                                                         const uint32_t* __restrict__ hit_prims,
+                                                         float* __restrict__ fb)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    const float t = hits[6 * (size_t)npix + pix];
+    const uint32_t prim = hit_prims[pix];
+    f3 rad;
+    if (t < kFltMax && prim < S.nt) {
+        f3 v0, v1, v2;
+        load_tri(S.rec, prim, v0, v1, v2);
+        const f3 org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+        const f3 dir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix],
+                          hits[5 * (size_t)npix + pix]);
+        const f3 real_normal = cross(v1 - v0, v2 - v0);
+        const f3 a = v0 - org, b = v1 - org, c = v2 - org;
+        const f3 n0 = cross(b, c), n1 = cross(c, a);
+        const float rcp = 1.f / sqrtf(dot(real_normal, real_normal));
+        const float u = len(n0) * rcp, v = len(n1) * rcp;
+        const f3 n = normalize(real_normal);
+        const float w = fmaxf(0.f, fabsf(dot(-n, normalize(dir))));
+        rad = mk(u, v, 1.f - u - v) * w;
+    } else {
+        rad = mk(0.15f, 0.2f, 0.3f);
+    }
+    const float nw = 1.f / (float)(image_of(ra) + 1);
+    const float ow = 1.f - nw;
+    float* o = fb + 3 * (size_t)pix;
+    o[0] = o[0] * ow + rad.x * nw;
+    o[1] = o[1] * ow + rad.y * nw;
+    o[2] = o[2] * ow + rad.z * nw;
+}
+
+// ---------------------------------------------------------------------------
 // per-corner normals
 // ---------------------------------------------------------------------------
 
@@ -462,6 +484,13 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
     else
         hipLaunchKernelGGL(k_pt<false>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits,
                            hit_prims, sh, fb, ray_count);
+}
+
+void launch_simple_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                         const uint32_t* hit_prims, float* fb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_simple_shade, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits, hit_prims,
+                       fb);
 }
 
 void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s)

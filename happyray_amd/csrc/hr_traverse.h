@@ -261,17 +261,23 @@ __device__ __forceinline__ f3 primary_direction(const CameraDev& cam, const Rend
     const float pid = (float)(pix < npix - 1u ? pix : npix - 1u);
     const float resx = (float)cam.res_x;
     float sx, sy;
-    if (!ra.jitter) {
-        // RegularPixelSampler<1,1>, sample 0 (Camera.h:82-97)
-        sy = 0.5f;
-        sx = 0.5f;
-        const float tmp = pid / resx;
+    const int image = image_of(ra);
+    const bool regular = ra.simple_sampler ? image < 4 : !ra.jitter;
+    if (regular) {
+        // RegularPixelSampler<k,k> (Camera.h:82-97): k = 2 with sample id =
+        // image for the simple integrator, else k = 1 (the pixel centre)
+        const int k = ra.simple_sampler ? 2 : 1;
+        float tmp = (float)((int)(float)image % (k * k)) / (float)k;
+        if (!ra.simple_sampler) tmp = 0.f;
+        sy = (truncf(tmp) + 0.5f) / (float)k;
+        sx = tmp - truncf(tmp) + 0.5f / (float)k;
+        tmp = pid / resx;
         sy += truncf(tmp);
         sx += (tmp - truncf(tmp)) * resx;
     } else {
         // GaussianPixelSampler (Camera.h:121-131)
         float u[4];
-        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPrimary, u);
+        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image, kStreamPrimary, u);
         const float tmp = pid / resx;
         const float rad = 0.5f * sqrtf(-spec_logf(1 - u[0]));
         const float angle = 2.f * kPi * u[1];
@@ -284,6 +290,24 @@ __device__ __forceinline__ f3 primary_direction(const CameraDev& cam, const Rend
     const f3 stx = mk(cam.step_x[0], cam.step_x[1], cam.step_x[2]);
     const f3 sty = mk(cam.step_y[0], cam.step_y[1], cam.step_y[2]);
     return tl + sx * stx + sy * sty;
+}
+
+// Shading normal of the path tracers (PathTracer.h:109-135, :526-553): barycentric
+// weights from the sub-triangle areas seen from the hit point, then the
+// normalized blend of the corner normals; the geometric normal when the scene
+// has no normals.
+__device__ __forceinline__ f3 shading_normal(const Shading& sh, uint32_t prim, f3 org, f3 v0, f3 v1, f3 v2)
+{
+    const f3 real_normal = cross(v1 - v0, v2 - v0);
+    if (!sh.nidx) return normalize(real_normal);
+    const f3 a = v0 - org, b = v1 - org, c = v2 - org;
+    const f3 n0 = cross(b, c), n1 = cross(c, a);
+    const float rcp = 1.f / sqrtf(dot(real_normal, real_normal));
+    const float u = len(n0) * rcp, v = len(n1) * rcp;
+    const float4 c0 = sh.normals[sh.nidx[3 * (size_t)prim]];
+    const float4 c1 = sh.normals[sh.nidx[3 * (size_t)prim + 1]];
+    const float4 c2 = sh.normals[sh.nidx[3 * (size_t)prim + 2]];
+    return normalize(u * mk(c0.x, c0.y, c0.z) + v * mk(c1.x, c1.y, c1.z) + (1.f - u - v) * mk(c2.x, c2.y, c2.z));
 }
 
 // CosineHemisphereSampler (HemisphereSamplers.hpp:58-71)

@@ -17,8 +17,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH,
-                   check)
+from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, MODE_PATH_MULTI,
+                   MODE_SIMPLE, check)
 
 _f32p = C.POINTER(C.c_float)
 _u32p = C.POINTER(C.c_uint32)
@@ -274,4 +274,4 @@ class Engine:
 
 
 __all__ = ["Engine", "GRID_UNIFORM", "GRID_UNIFORM_EXACT", "GRID_TWO_LEVEL", "MODE_AO",
-           "MODE_PATH"]
+           "MODE_PATH", "MODE_PATH_MULTI", "MODE_SIMPLE"]

@@ -32,7 +32,7 @@ typedef enum {
     HR_ERR_ARG = 1,        /* invalid argument                              */
     HR_ERR_DEVICE = 2,     /* HIP runtime error / no device                 */
     HR_ERR_OOM = 3,        /* device allocation failed                      */
-    HR_ERR_IO = 4,         /* file could not be read                        */
+    HR_ERR_IO = 4,         /* file could not be read or written             */
     HR_ERR_STATE = 5,      /* call out of order (e.g. render before build)  */
     HR_ERR_CAPACITY = 6,   /* a fixed capacity was exceeded                 */
     HR_ERR_PARSE = 7,      /* malformed scene / camera / light file         */
@@ -45,9 +45,15 @@ typedef enum {
     HR_GRID_TWO_LEVEL = 2      /* two-level grid                                */
 } hr_grid_kind;
 
+/* RTEngine::renderFrame modes (src/RT/RTEngine.cu:398-444) */
 typedef enum {
-    HR_MODE_AO = 3,            /* RTEngine::renderFrame mode 3 (src/RT/RTEngine.cu:433) */
-    HR_MODE_PATH = 2           /* mode 2: path-tracing megakernel                       */
+    HR_MODE_SIMPLE = 0,        /* mode 0: primary rays + simpleShade; the 2x2 regular
+                                  pixel sampler for images 0-3, the Gaussian filter after */
+    HR_MODE_PATH_MULTI = 1,    /* mode 1: multi-kernel path tracer with the blur / residue
+                                  merge (PathTracer.h:374-1215); up to 8 bounces; jitter
+                                  and pt_depth are not used                             */
+    HR_MODE_PATH = 2,          /* mode 2: path-tracing megakernel                       */
+    HR_MODE_AO = 3             /* mode 3: ambient occlusion                             */
 } hr_mode;
 
 typedef struct hr_ctx hr_ctx;
@@ -420,6 +426,18 @@ hr_status hr_scene_upload(hr_ctx* ctx, const hr_scene* scene);
 hr_status hr_scene_frame(const hr_scene* scene, float current, uint32_t* k1, uint32_t* k2,
                          float* coeff);
 hr_status hr_scene_next_frame(const hr_scene* scene, float* current);
+
+/* ---- image output (host only): Image::gammaCorrect + Image::writePNG
+ * (src/Utils/ImagePNG.cpp:64-72, :268-330) ----
+ * rgb: 3 floats per pixel, row-major from the top-left pixel, as the framebuffer.
+ * hr_image_to_rgb8: powf(v, 1/gamma) per channel (gamma <= 0: none), clamped to
+ * [0, 1], times 255, truncated to a byte (NaN -> 0).
+ * hr_write_png: an 8-bit RGB PNG (no interlace, filter 0, zlib deflate).
+ * hr_write_screenshot: both, with gamma 2.2, as SDLGLApplication::writeScreenShot
+ * (src/Application/SDLGLApplication.cpp:120-143). */
+hr_status hr_image_to_rgb8(const float* rgb, int width, int height, float gamma, uint8_t* out);
+hr_status hr_write_png(const char* path, const uint8_t* rgb8, int width, int height);
+hr_status hr_write_screenshot(const char* path, const float* rgb, int width, int height);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,8 @@ typedef struct {
     float ao_dist;              /* AO occlusion distance   */
     int pt_depth;               /* max path vertices        */
     int nthreads;               /* host threads (tiles of 64 rays) */
+    int simple_sampler;         /* 1: renderFrame mode 0's pixel sampler (RTEngine.cu:414-423):
+                                   RegularPixelSampler<2,2> for images 0-3, Gaussian after */
 } or_render_params;
 
 /* ---- spec math (or_math.c) ---- */
@@ -129,6 +131,21 @@ void or_prim_cell_range(const or_mesh* m, uint32_t prim, const or_grid* g,
                         int lo[3], int hi[3]);
 int  or_tri_box_overlap(const float v0[3], const float v1[3], const float v2[3],
                         const float center[3], const float half[3]);
+
+/* renderFrame mode 0: primary rays + simpleShade (SimpleIntegrator.h:42-125);
+ * fb (3 floats per pixel) is the running mean, weight 1/(image_id+1) */
+int  or_render_simple(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
+                      const or_render_params* p, int image_id, float* fb, or_stats* stats);
+/* renderFrame mode 1: the multi-kernel path tracer (PathTracer.h:374-1215).
+ * state holds 6 floats per pixel carried from image to image (new radiance,
+ * residue; zero them with the camera); end / end_prims (may be NULL) receive
+ * the end vertices (7 SoA planes) and importance (may be NULL) the path
+ * throughput (3 floats per pixel). */
+int  or_render_pt_multi(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
+                        const or_material* mats, const uint32_t* mat_idx,
+                        const or_light* lights, const float* light_weights, int nlights,
+                        const or_render_params* p, int image_id, float* fb, float* state,
+                        float* end, uint32_t* end_prims, float* importance, or_stats* stats);
 
 /* the camera ray directions of all pixels (before the traversal's reciprocal) */
 void or_primary_dirs(const or_camera* cam, const or_render_params* p, int image_id,

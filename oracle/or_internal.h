@@ -107,6 +107,8 @@ static inline void tri_bbox(f3 v0, f3 v1, f3 v2, f3* lo, f3* hi)
 #define OR_STREAM_PRIMARY 0u
 #define OR_STREAM_AO      16u   /* + sample index */
 #define OR_STREAM_PT      1024u /* + 2 * vertex index + {0,1} */
+#define OR_STREAM_PATH_MK 2048u /* multi-kernel path tracer: + 2 * vertex index + {0,1} */
+#define OR_STREAM_PATH_LIGHT 4096u /* its shadow-ray light sample */
 
 static inline void or_rand4(const or_render_params* p, uint32_t pixel, uint32_t image,
                             uint32_t stream, float u[4])

@@ -367,7 +367,16 @@ static f3 primary_dir(const or_camera* cam, const or_render_params* p, uint32_t 
     const float pid = (float)(pix < npix - 1u ? pix : npix - 1u);
     const float resx = (float)cam->res_x;
     float sx, sy;
-    if (!p->jitter) {
+    if (p->simple_sampler && image < 4) {
+        /* RegularPixelSampler<2,2> with sample id = image (Camera.h:82-97,
+         * RTEngine.cu:414-418) */
+        float tmp = (float)((int)(float)image % (2 * 2)) / (float)2;
+        sy = (truncf(tmp) + 0.5f) / (float)2;
+        sx = tmp - truncf(tmp) + 0.5f / (float)2;
+        tmp = pid / resx;
+        sy += truncf(tmp);
+        sx += (tmp - truncf(tmp)) * resx;
+    } else if (!p->jitter && !p->simple_sampler) {
         /* RegularPixelSampler<1,1> with sample 0 (Camera.h:82-97) */
         float tmp = (float)(0 % 1) / 1.f;
         sy = (truncf(tmp) + 0.5f) / 1.f;
@@ -777,4 +786,403 @@ void or_shading_normal(const or_mesh* m, uint32_t prim, const float org[3], floa
     tri_verts(m, prim, &v0, &v1, &v2);
     const f3 n = pt_shading_normal(m, prim, ld3(org), v0, v1, v2);
     out[0] = n.x; out[1] = n.y; out[2] = n.z;
+}
+
+/* ------------------------------------------------------------------ */
+/* renderFrame mode 0: simple integrator                               */
+/* ------------------------------------------------------------------ */
+
+int or_render_simple(const or_mesh* m, const or_grid* g, const or_camera* cam,
+                     const or_render_params* p, int image_id, float* fb, or_stats* stats)
+{
+    const uint32_t npix = (uint32_t)cam->res_x * (uint32_t)cam->res_y;
+    float* h = (float*)malloc((size_t)npix * 7 * sizeof(float));
+    uint32_t* hp = (uint32_t*)malloc((size_t)npix * sizeof(uint32_t));
+    float *hx = h, *hy = h + npix, *hz = h + 2 * (size_t)npix, *hdx = h + 3 * (size_t)npix,
+          *hdy = h + 4 * (size_t)npix, *hdz = h + 5 * (size_t)npix, *ht = h + 6 * (size_t)npix;
+    or_primary(m, g, cam, p, image_id, hx, hy, hz, hdx, hdy, hdz, ht, hp, stats);
+    /* simpleShade (SimpleIntegrator.h:42-125) */
+    const float new_w = 1.f / (float)(image_id + 1);
+    const float old_w = 1.f - new_w;
+    for (uint32_t pix = 0; pix < npix; ++pix) {
+        f3 rad;
+        const uint32_t prim = hp[pix];
+        if (ht[pix] < FLT_MAX && prim < m->nt) {
+            f3 v0, v1, v2;
+            tri_verts(m, prim, &v0, &v1, &v2);
+            const f3 org = mk3(hx[pix], hy[pix], hz[pix]);
+            const f3 dir = mk3(hdx[pix], hdy[pix], hdz[pix]);
+            const f3 real_normal = cross3(sub3(v1, v0), sub3(v2, v0));
+            const f3 a = sub3(v0, org), b = sub3(v1, org), c = sub3(v2, org);
+            const f3 n0 = cross3(b, c), n1 = cross3(c, a);
+            const float rcp = lenrcp3(real_normal);
+            const float u = len3(n0) * rcp, v = len3(n1) * rcp;
+            const f3 n = norm3(real_normal);
+            const float w = fmaxf(0.f, fabsf(dot3(neg3(n), norm3(dir))));
+            rad = muls3(mk3(u, v, 1.f - u - v), w);
+        } else {
+            rad = mk3(0.15f, 0.2f, 0.3f);
+        }
+        float* o = fb + 3 * (size_t)pix;
+        o[0] = o[0] * old_w + rad.x * new_w;
+        o[1] = o[1] * old_w + rad.y * new_w;
+        o[2] = o[2] * old_w + rad.z * new_w;
+    }
+    free(h); free(hp);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* renderFrame mode 1: multi-kernel path tracer                        */
+/* ------------------------------------------------------------------ */
+
+#define OR_PATH_MAX_BOUNCES 8   /* RFRACTION_PATH_MAX_DEPTH (PathTracer.h:23) */
+
+typedef struct {
+    const or_material* mats; const uint32_t* mat_idx;
+    const or_light* lights; const float* cdf; int nlights;
+    const or_render_params* p; int image; uint32_t npix;
+    const float *hx, *hy, *hz, *hdx, *hdy, *hdz, *ht; const uint32_t* hprim;
+    float* end; uint32_t* end_prims; float* imp; float* ls; float* dls;
+} pm_arg;
+
+/* Russian roulette at an opaque vertex (PathTracer.h:489-520, :704-735):
+ * 0 diffuse, 1 specular, 2 absorbed (throughput already divided) */
+static int pm_roulette(const or_material* mt, float rnum, f3* att, int at_primary)
+{
+    const f3 kd = ld3(mt->kd), ks = ld3(mt->ks);
+    const float se = mt->spec_exp;
+    const float albedo_d = maxc3(kd) * OR_PI;
+    const float albedo_s = maxc3(ks) * 2.f * OR_PI / (se + 2.f);
+    if (rnum < albedo_d) {
+        *att = divs3(muls3(mul3(*att, kd), OR_PI), albedo_d);
+        return 0;
+    }
+    if (rnum < albedo_d + albedo_s) {
+        *att = divs3(muls3(mul3(*att, ks), OR_PI), ((se + 1.f) * 0.5f * albedo_s));
+        return 1;
+    }
+    if (at_primary)
+        *att = mul3(*att, rep3(1.f / (1.f - albedo_s - albedo_d)));
+    else
+        *att = divs3(*att, (1.f - albedo_s - albedo_d));
+    return 2;
+}
+
+/* tracePath (PathTracer.h:374-748) */
+static void pm_path_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
+{
+    pm_arg* q = (pm_arg*)a;
+    const uint32_t n = q->npix;
+    for (uint32_t pix = b; pix < e; ++pix) {
+        f3 org = mk3(q->hx[pix], q->hy[pix], q->hz[pix]);
+        f3 dir = mk3(q->hdx[pix], q->hdy[pix], q->hdz[pix]);
+        float t = q->ht[pix];
+        uint32_t prim = q->hprim[pix];
+        /* the ray buffer holds the primary hit until a store */
+        f3 e_hp = org, e_dir = dir;
+        float e_t = t;
+        uint32_t e_prim = prim;
+        f3 att = rep3(1.f);
+        int alive = t < FLT_MAX && prim < cx->m->nt;
+        int diffuse = 0;
+        const or_material* mt = NULL;
+        if (alive) {
+            mt = &q->mats[q->mat_idx ? q->mat_idx[prim] : 0u];
+            if (!(mt->ior_w - 10.f > 0.f)) {
+                float u[4];
+                or_rand4(q->p, pix, (uint32_t)q->image, OR_STREAM_PATH_MK, u);
+                const int r = pm_roulette(mt, u[0], &att, 1);
+                diffuse = r == 0;
+                if (r == 2) alive = 0;
+            }
+        }
+        int depth = 0;
+        while (alive) {
+            float ra[4], rb[4];
+            or_rand4(q->p, pix, (uint32_t)q->image, OR_STREAM_PATH_MK + 2u * (uint32_t)depth, ra);
+            or_rand4(q->p, pix, (uint32_t)q->image, OR_STREAM_PATH_MK + 2u * (uint32_t)depth + 1u, rb);
+            f3 v0, v1, v2;
+            tri_verts(cx->m, prim, &v0, &v1, &v2);
+            f3 nrm = pt_shading_normal(cx->m, prim, org, v0, v1, v2);
+            float dnd = -dot3(nrm, dir);
+            const float ior = mt->ior_w - 10.f;
+            if (ior > 0.f) {
+                /* transparent (PathTracer.h:556-622) */
+                const f3 kd = ld3(mt->kd);
+                float n1 = 1.000293f, n2 = ior;
+                if (dnd < 0.f) {
+                    n1 = ior; n2 = 1.000293f;
+                    nrm = neg3(nrm);
+                    dnd = -dnd;
+                    att.x *= or_expf(-t * kd.x * OR_PI);
+                    att.y *= or_expf(-t * kd.y * OR_PI);
+                    att.z *= or_expf(-t * kd.z * OR_PI);
+                }
+                const float f0 = (n1 - n2) * (n1 - n2) / ((n1 + n2) * (n1 + n2));
+                const float fres = f0 + (1.f - f0) * or_powf(1.f - fabsf(dnd), 5.f);
+                const float ni = n1 / n2;
+                const float sin2 = ni * ni * (1 - dnd * dnd);
+                if (sin2 > 1.f || ra[1] < fres) {
+                    org = add3(org, smul3(5.f * OR_EPS, nrm));
+                    dir = add3(dir, smul3(2 * dnd, nrm));
+                    if (sin2 >= 1.f) {
+                        if (ra[2] < 0.5f) {
+                            att = muls3(att, 2.f);
+                        } else {
+                            att = rep3(0.f);
+                            e_hp = add3(org, smul3(t - OR_EPS, dir));
+                            e_dir = dir; e_t = t; e_prim = prim;
+                            break;
+                        }
+                    }
+                } else {
+                    const float nf = ni * dnd - sqrtf(1.f - sin2);
+                    org = sub3(org, smul3(10.f * OR_EPS, nrm));
+                    dir = add3(smul3(ni, dir), smul3(nf, nrm));
+                }
+            } else if (diffuse) {
+                /* PathTracer.h:623-641 */
+                if (dnd < 0.f) nrm = neg3(nrm);
+                const f3 rd = cosine_sample(rb[0], rb[1]);
+                f3 tg, bn;
+                local_coords(nrm, &tg, &bn);
+                dir = add3(add3(muls3(tg, rd.x), muls3(bn, rd.y)), muls3(nrm, rd.z));
+                org = add3(org, smul3(5.f * OR_EPS, nrm));
+            } else {
+                /* specular (PathTracer.h:642-672) with PowerCosineHemisphereSampler
+                 * (HemisphereSamplers.hpp:78-94) */
+                const float se = mt->spec_exp;
+                if (dnd < 0.f) { nrm = neg3(nrm); dnd = -dnd; }
+                f3 rd;
+                rd.z = or_powf(rb[1], 1.f / (se + 1.f));
+                float s_, c_;
+                or_sincosf(2.f * OR_PI * rb[0], &s_, &c_);
+                rd.y = s_; rd.x = c_;
+                rd.x *= sqrtf(1 - rd.z * rd.z);
+                rd.y *= sqrtf(1 - rd.z * rd.z);
+                const f3 up = add3(dir, smul3(2.f * dnd, nrm));
+                f3 rt, fw;
+                local_coords(up, &rt, &fw);
+                dir = add3(add3(muls3(rt, rd.x), muls3(fw, rd.y)), muls3(up, rd.z));
+                org = add3(org, smul3(5.f * OR_EPS, nrm));
+                if (dot3(dir, nrm) < OR_EPS) att = muls3(att, 0.f);
+                else att = muls3(att, dot3(dir, nrm));
+            }
+            /* extension ray (PathTracer.h:676-700) */
+            f3 d2;
+            const uint32_t hit = trace_one(cx, org, dir, FLT_MAX, 0, 0.f, &t, &d2);
+            dir = d2;
+            if (!(t < FLT_MAX) || depth >= OR_PATH_MAX_BOUNCES) {
+                e_hp = add3(org, smul3(t - OR_EPS, dir));
+                e_dir = dir; e_t = t; e_prim = t < FLT_MAX ? hit : prim;
+                break;
+            }
+            prim = hit;
+            mt = &q->mats[q->mat_idx ? q->mat_idx[prim] : 0u];
+            if (!(mt->ior_w - 10.f > 0.f)) {
+                float u[4];
+                or_rand4(q->p, pix, (uint32_t)q->image, OR_STREAM_PATH_MK + 2u * (uint32_t)(depth + 1), u);
+                const int r = pm_roulette(mt, u[0], &att, 0);
+                diffuse = r == 0;
+                if (r == 2) {
+                    e_hp = add3(org, smul3(t - OR_EPS, dir));
+                    e_dir = dir; e_t = t; e_prim = prim;
+                    break;
+                }
+            }
+            ++depth;
+            org = add3(org, smul3(t - OR_EPS, dir));
+        }
+        q->end[pix] = e_hp.x; q->end[n + pix] = e_hp.y; q->end[2 * (size_t)n + pix] = e_hp.z;
+        q->end[3 * (size_t)n + pix] = e_dir.x; q->end[4 * (size_t)n + pix] = e_dir.y;
+        q->end[5 * (size_t)n + pix] = e_dir.z; q->end[6 * (size_t)n + pix] = e_t;
+        q->end_prims[pix] = e_prim;
+        q->imp[3 * (size_t)pix] = att.x; q->imp[3 * (size_t)pix + 1] = att.y; q->imp[3 * (size_t)pix + 2] = att.z;
+    }
+}
+
+/* AreaLightSource::isOnLS (LightSource.hpp:50-76) */
+static int pm_on_light(const or_light* L, f3 pt)
+{
+    const f3 pos = ld3(L->position), nrm = ld3(L->normal), e1 = ld3(L->edge1), e2 = ld3(L->edge2);
+    const float eps = 0.001f * fmaxf(maxc3(abs3(e1)), maxc3(abs3(e2)));
+    if (!(fabsf(dot3(sub3(pt, pos), nrm)) < eps)) return 0;
+    const f3 a1 = abs3(e1), a2 = abs3(e2);
+    const int d1 = (a1.x > a1.y) ? ((a1.x > a1.z) ? 0 : 2) : ((a1.y > a1.z) ? 1 : 2);
+    const int d2 = (a2.x > a2.y) ? ((a2.x > a2.z) ? 0 : 2) : ((a2.y > a2.z) ? 1 : 2);
+    const float e1x = comp3(e1, d1), e1y = comp3(e1, d2), e2x = comp3(e2, d1), e2y = comp3(e2, d2);
+    const float px = comp3(pos, d1), py = comp3(pos, d2), xx = comp3(pt, d1), xy = comp3(pt, d2);
+    float beta = (xy - py) - (e1y / e1x) * (xx - px);
+    beta /= e2y - e2x * (e1y / e1x);
+    const float alpha = ((xx - px) - beta * e2x) / e1x;
+    return alpha > -eps && alpha < 1.f + eps && beta > -eps && beta < 1.f + eps;
+}
+
+/* AreaLightShadowRayGenerator<1,1> (RayGenerators.h:118-206), the shadow
+ * trace and DirectIlluminationBuffer::store (RayBuffers.h:211-235) */
+static void pm_shadow_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
+{
+    pm_arg* q = (pm_arg*)a;
+    const uint32_t n = q->npix;
+    for (uint32_t pix = b; pix < e; ++pix) {
+        const float t = q->end[6 * (size_t)n + pix];
+        f3 ls = rep3(0.f), dl = rep3(0.f);
+        if (t < FLT_MAX) {
+            float u[4];
+            or_rand4(q->p, pix, (uint32_t)q->image, OR_STREAM_PATH_LIGHT, u);
+            int li = q->nlights - 1;
+            for (int i = 0; i < q->nlights - 1; ++i)
+                if (u[0] < q->cdf[i] / q->cdf[q->nlights - 1]) { li = i; break; }
+            const or_light* L = &q->lights[li];
+            const f3 org = mk3(q->end[pix], q->end[n + pix], q->end[2 * (size_t)n + pix]);
+            if (pm_on_light(L, org)) {
+                const f3 dir = mk3(q->end[3 * (size_t)n + pix], q->end[4 * (size_t)n + pix],
+                                   q->end[5 * (size_t)n + pix]);
+                const float cos_ln = -dot3(dir, ld3(L->normal));
+                ls = muls3(ld3(L->intensity), cos_ln > 0.f ? .5f : 0.f);
+            } else {
+                /* StratifiedSampleGenerator<1,1> (RandomNumberGenerators.hpp:188-205) */
+                const float r1 = 1.f * u[1] + 0.f * 1.f;
+                const float r2 = 1.f * u[2] + 0.f * 1.f;
+                const f3 lp = add3(add3(ld3(L->position), smul3(r1, ld3(L->edge1))), smul3(r2, ld3(L->edge2)));
+                const f3 dir = sub3(lp, org);
+                const float att = 1.f / dot3(dir, dir);
+                const f3 dn = muls3(dir, sqrtf(att));
+                const float cos_ln = dot3(ld3(L->normal), neg3(dn));
+                const float area = len3(cross3(ld3(L->edge1), ld3(L->edge2)));
+                ls = muls3(muls3(muls3(muls3(ld3(L->intensity), area), att), fmaxf(0.f, cos_ln)), 0.5f);
+                float st;
+                f3 d2;
+                if (trace_one(cx, org, dir, 0.9999f, 1, 0.9999f, &st, &d2)) ls = rep3(0.f);
+                else dl = norm3(d2);
+            }
+        }
+        q->ls[3 * (size_t)pix] = ls.x; q->ls[3 * (size_t)pix + 1] = ls.y; q->ls[3 * (size_t)pix + 2] = ls.z;
+        q->dls[3 * (size_t)pix] = dl.x; q->dls[3 * (size_t)pix + 1] = dl.y; q->dls[3 * (size_t)pix + 2] = dl.z;
+    }
+}
+
+int or_render_pt_multi(const or_mesh* m, const or_grid* g, const or_camera* cam,
+                       const or_material* mats, const uint32_t* mat_idx,
+                       const or_light* lights, const float* weights, int nlights,
+                       const or_render_params* p, int image_id, float* fb, float* state,
+                       float* end_out, uint32_t* end_prims_out, float* imp_out, or_stats* stats)
+{
+    /* PathTracer::integrate returns at once without lights (PathTracer.h:1049-1053) */
+    if (nlights <= 0) return 0;
+    const uint32_t npix = (uint32_t)cam->res_x * (uint32_t)cam->res_y;
+    const size_t n = npix;
+    float* h = (float*)malloc(n * 7 * sizeof(float));
+    uint32_t* hp = (uint32_t*)malloc(n * sizeof(uint32_t));
+    float* end = end_out ? end_out : (float*)malloc(n * 7 * sizeof(float));
+    uint32_t* endp = end_prims_out ? end_prims_out : (uint32_t*)malloc(n * sizeof(uint32_t));
+    float* imp = imp_out ? imp_out : (float*)malloc(n * 3 * sizeof(float));
+    float* ls = (float*)malloc(n * 3 * sizeof(float));
+    float* dls = (float*)malloc(n * 3 * sizeof(float));
+    float* blended = (float*)malloc(n * 3 * sizeof(float));
+    float* contrib = (float*)malloc(n * 3 * sizeof(float));
+    float* nrad = state;
+    float* residue = state + 3 * n;
+    float *hx = h, *hy = h + n, *hz = h + 2 * n, *hdx = h + 3 * n, *hdy = h + 4 * n, *hdz = h + 5 * n,
+          *ht = h + 6 * n;
+    /* the PathTracer's RandomPrimaryRayGenerator<GaussianPixelSampler> */
+    or_render_params pp = *p;
+    pp.jitter = 1;
+    pp.simple_sampler = 0;
+    or_primary(m, g, cam, &pp, image_id, hx, hy, hz, hdx, hdy, hdz, ht, hp, stats);
+    float cdf[64];
+    float acc = 0.f;
+    for (int i = 0; i < nlights && i < 64; ++i) { acc += weights ? weights[i] : 1.f; cdf[i] = acc; }
+    pm_arg a = { mats, mat_idx, lights, cdf, nlights < 64 ? nlights : 64, &pp, image_id, npix,
+                 hx, hy, hz, hdx, hdy, hdz, ht, hp, end, endp, imp, ls, dls };
+    or_stats st2 = { 0, 0, 0 };
+    run_tiles(m, g, npix, p->nthreads, pm_path_tile, &a, &st2);
+    run_tiles(m, g, npix, p->nthreads, pm_shadow_tile, &a, &st2);
+    if (stats) { stats->rays += st2.rays; stats->steps += st2.steps; stats->tests += st2.tests; }
+    const float new_w = 1.f / (float)(image_id + 1);
+    const float old_w = 1.f - new_w;
+    /* computeDirectIllumination (PathTracer.h:751-880), one sample per pixel */
+    for (uint32_t pix = 0; pix < npix; ++pix) {
+        const float t = end[6 * n + pix];
+        const uint32_t prim = endp[pix];
+        f3 v = ld3(ls + 3 * (size_t)pix);
+        if (t < FLT_MAX) {
+            if (v.x + v.y + v.z > 0.f && prim < m->nt) {
+                const f3 org = mk3(end[pix], end[n + pix], end[2 * n + pix]);
+                const f3 dir = mk3(end[3 * n + pix], end[4 * n + pix], end[5 * n + pix]);
+                f3 v0, v1, v2;
+                tri_verts(m, prim, &v0, &v1, &v2);
+                const f3 nrm = pt_shading_normal(m, prim, org, v0, v1, v2);
+                const or_material* mt = &mats[mat_idx ? mat_idx[prim] : 0u];
+                const f3 dl = ld3(dls + 3 * (size_t)pix);
+                const float cos_nl = dot3(nrm, dl);
+                const float cos_ne = -dot3(nrm, dir);
+                const float cos_hl = fmaxf(0.f, dot3(norm3(sub3(dl, dir)), nrm));
+                if (fabsf(dl.x) + fabsf(dl.y) + fabsf(dl.z) > 0.f && cos_ne > 0.f && cos_nl > 0.f) {
+                    const f3 lsr = v;
+                    v = muls3(mul3(ld3(mt->kd), lsr), cos_nl);
+                    v = add3(v, muls3(mul3(muls3(ld3(mt->ks), fmaxf(0.f, or_powf(cos_hl, mt->spec_exp))), lsr),
+                                      cos_nl));
+                }
+            }
+        } else {
+            v = rep3(0.f);
+        }
+        const f3 rad = add3(rep3(0.f), divs3(muls3(v, 1.f), (float)1));
+        const f3 att = ld3(imp + 3 * (size_t)pix);
+        const f3 nr = add3(muls3(ld3(nrad + 3 * (size_t)pix), old_w), muls3(mul3(rad, att), new_w));
+        nrad[3 * (size_t)pix] = nr.x; nrad[3 * (size_t)pix + 1] = nr.y; nrad[3 * (size_t)pix + 2] = nr.z;
+    }
+    /* addIndirectIllumination (PathTracer.h:889-990): all blended values from
+     * the previous image first, then each pixel's blur */
+    for (uint32_t pix = 0; pix < npix; ++pix) {
+        const f3 nc = add3(mul3(ld3(imp + 3 * (size_t)pix), ld3(nrad + 3 * (size_t)pix)),
+                           ld3(residue + 3 * (size_t)pix));
+        const f3 bl = add3(muls3(ld3(fb + 3 * (size_t)pix), old_w), muls3(nc, new_w));
+        contrib[3 * (size_t)pix] = nc.x; contrib[3 * (size_t)pix + 1] = nc.y; contrib[3 * (size_t)pix + 2] = nc.z;
+        blended[3 * (size_t)pix] = bl.x; blended[3 * (size_t)pix + 1] = bl.y; blended[3 * (size_t)pix + 2] = bl.z;
+    }
+    const int rx = cam->res_x, ry = cam->res_y;
+    for (int y = 0; y < ry; ++y) {
+        for (int x = 0; x < rx; ++x) {
+            const size_t pix = (size_t)y * rx + x;
+            const int xl = x > 0 ? x - 1 : 0, xr = x + 1 < rx ? x + 1 : rx - 1;
+            const int yb = y > 0 ? y - 1 : 0, yt = y + 1 < ry ? y + 1 : ry - 1;
+#define BL(px, py) ld3(blended + 3 * ((size_t)(py) * rx + (px)))
+            const f3 top = BL(x, yt);
+            /* the reference's centre tap reads sharedMem[top] (PathTracer.h:964-968) */
+            f3 sum = smul3(1.f, BL(xl, yt));
+            sum = add3(sum, smul3(2.f, top));
+            sum = add3(sum, smul3(1.f, BL(xr, yt)));
+            sum = add3(sum, smul3(2.f, BL(xl, y)));
+            sum = add3(sum, smul3(4.f, top));
+            sum = add3(sum, smul3(2.f, BL(xr, y)));
+            sum = add3(sum, smul3(1.f, BL(xl, yb)));
+            sum = add3(sum, smul3(2.f, BL(x, yb)));
+            sum = add3(sum, smul3(1.f, BL(xr, yb)));
+#undef BL
+            const f3 blurred = divs3(sum, 16.f);
+            const f3 pv = ld3(fb + 3 * pix);
+            const f3 nc = ld3(contrib + 3 * pix);
+            float variance = dot3(nc, rep3(1.f));
+            variance *= variance;
+            f3 out, res;
+            if (variance < fmaxf(0.25f / (float)image_id, 0.05f)) {
+                out = add3(muls3(pv, old_w), muls3(nc, new_w));
+                res = rep3(0.f);
+            } else {
+                const f3 val = min3(add3(blurred, smul3(0.5f, nc)), nc);
+                out = add3(muls3(pv, old_w), muls3(val, new_w));
+                res = sub3(nc, val);
+            }
+            fb[3 * pix] = out.x; fb[3 * pix + 1] = out.y; fb[3 * pix + 2] = out.z;
+            residue[3 * pix] = res.x; residue[3 * pix + 1] = res.y; residue[3 * pix + 2] = res.z;
+        }
+    }
+    free(h); free(hp); free(ls); free(dls); free(blended); free(contrib);
+    if (!end_out) free(end);
+    if (!end_prims_out) free(endp);
+    if (!imp_out) free(imp);
+    return 0;
 }

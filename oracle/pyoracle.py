@@ -49,7 +49,7 @@ class Camera(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("jitter", C.c_int),
                 ("ao_samples", C.c_int), ("ao_dist", C.c_float), ("pt_depth", C.c_int),
-                ("nthreads", C.c_int)]
+                ("nthreads", C.c_int), ("simple_sampler", C.c_int)]
 
 
 def build_lib():
@@ -282,8 +282,8 @@ def camera(pos, dir, up, fov, res_x, res_y):
 
 
 def params(seed=(0x1234, 0x5678), jitter=1, ao_samples=4, ao_dist=2.0, pt_depth=4,
-           nthreads=1):
-    return RenderParams(seed[0], seed[1], jitter, ao_samples, ao_dist, pt_depth, nthreads)
+           nthreads=1, simple_sampler=0):
+    return RenderParams(seed[0], seed[1], jitter, ao_samples, ao_dist, pt_depth, nthreads, simple_sampler)
 
 
 def primary(grid, cam, p, image_id=0):
@@ -325,6 +325,39 @@ def render_pt(grid, cam, p, materials, mat_idx, lights, weights=None, image_id=0
                        None if mi is None else _up(mi), _fp(L), None if w is None else _fp(w),
                        len(L), C.byref(p), image_id, _fp(fb), C.byref(st))
     return fb, st
+
+
+def render_simple(grid, cam, p, image_id=0, fb=None):
+    """renderFrame mode 0 (primary rays + simpleShade); p.simple_sampler is set here."""
+    n = cam.res_x * cam.res_y
+    fb = np.zeros((n, 3), np.float32) if fb is None else np.ascontiguousarray(fb, np.float32).copy()
+    q = RenderParams.from_buffer_copy(p)
+    q.simple_sampler = 1
+    st = Stats()
+    lib().or_render_simple(C.byref(grid.mesh.c), C.byref(grid.c), C.byref(cam), C.byref(q), image_id, _fp(fb),
+                           C.byref(st))
+    return fb, st
+
+
+def render_pt_multi(grid, cam, p, materials, mat_idx, lights, weights=None, image_id=0, fb=None, state=None):
+    """renderFrame mode 1 (the multi-kernel path tracer).  state: (2, npix, 3)
+    new radiance and residue carried between images (zeros with a new camera).
+    Returns fb, state, end planes (7, npix), end prims, importance (npix, 3), stats."""
+    n = cam.res_x * cam.res_y
+    fb = np.zeros((n, 3), np.float32) if fb is None else np.ascontiguousarray(fb, np.float32).copy()
+    state = np.zeros((2, n, 3), np.float32) if state is None else np.ascontiguousarray(state, np.float32).copy()
+    m, mi = _mats(materials, mat_idx)
+    L = np.ascontiguousarray(lights, np.float32).reshape(-1, 15)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    end = np.zeros((7, n), np.float32)
+    endp = np.zeros(n, np.uint32)
+    imp = np.zeros((n, 3), np.float32)
+    st = Stats()
+    lib().or_render_pt_multi(C.byref(grid.mesh.c), C.byref(grid.c), C.byref(cam), _fp(m),
+                             None if mi is None else _up(mi), _fp(L), None if w is None else _fp(w), len(L),
+                             C.byref(p), image_id, _fp(fb), _fp(state), _fp(end), _up(endp), _fp(imp),
+                             C.byref(st))
+    return fb, state, end, endp, imp, st
 
 
 def shading_normal(mesh, prim, org):

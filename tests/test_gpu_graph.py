@@ -8,7 +8,8 @@ import pytest
 
 import pyoracle as O
 from gpu_util import SEED, random_rays
-from happyray_amd import GRID_TWO_LEVEL, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, Engine, scenes
+from happyray_amd import (GRID_TWO_LEVEL, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, MODE_PATH_MULTI, MODE_SIMPLE,
+                          Engine, scenes)
 
 pytestmark = pytest.mark.gpu
 
@@ -34,13 +35,13 @@ def grids_equal(a, b):
     np.testing.assert_array_equal(pa, pb)
 
 
-@pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO])
+@pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_SIMPLE, MODE_PATH_MULTI])
 def test_frame_graph_matches_uncaptured_frames(mode):
     sc, keys = scenes.heightfield_keyframes(quads=64, frames=5, res=(64, 48))
     g = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
     e = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
     org, d = random_rays(sc, 4000, seed=9)
-    kw = dict(jitter=1, pt_depth=3) if mode == MODE_PATH else dict(jitter=1, ao_samples=4, ao_dist=5.0)
+    kw = dict(jitter=1, pt_depth=3) if mode != MODE_AO else dict(jitter=1, ao_samples=4, ao_dist=5.0)
     for i, cur in enumerate([0.0, 0.4, 1.7, 3.9, 2.2, 0.1, 4.5]):
         k1, k2, c = O.frame_ids(cur, 0.4, 5)
         image = i % 3

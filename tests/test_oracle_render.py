@@ -195,3 +195,172 @@ def test_paths_from_camera_rays_equal_the_render():
     rgb, _ = O.path_trace_rays(g, org, d, p, sc.materials, sc.mat_idx, sc.lights)
     assert fb.max() > 0
     np.testing.assert_array_equal(rgb, fb)
+
+
+# ---------------------------------------------------------------------------
+# renderFrame mode 0: the simple integrator
+# ---------------------------------------------------------------------------
+
+def facing_triangle_scene(res=(40, 32)):
+    """One large triangle in the plane z = 5 facing a camera at the origin
+    that looks along +z; the triangle does not cover the whole image."""
+    v = np.array([[-4.0, -3.0, 5.0], [4.5, -2.5, 5.0], [0.5, 3.5, 5.0]])
+    parts = [(v, np.array([[0, 1, 2]], np.uint32), 0)]
+    cam = dict(pos=(0.0, 0.0, 0.0), dir=(0.0, 0.0, 1.0), up=(0.0, 1.0, 0.0), fov=70.0, res=res)
+    return scenes._assemble(parts, [scenes.material((0.5, 0.5, 0.5))], "facing", cam)
+
+
+def test_simple_sampler_offsets():
+    """RegularPixelSampler<2,2> (Camera.h:82-97) for images 0-3: the sample id
+    picks the sub-pixel offsets (1/4, 1/4), (3/4, 1/4), (1/4, 3/4), (3/4, 3/4);
+    from image 4 on mode 0 uses the Gaussian pixel filter (RTEngine.cu:414-423)."""
+    sc = facing_triangle_scene()
+    g, cam = setup(sc)
+    rx, ry = sc.camera["res"]
+    tl, stx, sty = (np.array(a[:], np.float64) for a in (cam.top_left, cam.step_x, cam.step_y))
+    pix = np.arange(rx * ry)
+    x, y = pix % rx, pix // rx
+    offsets = [(0.25, 0.25), (0.75, 0.25), (0.25, 0.75), (0.75, 0.75)]
+    for s, (ox, oy) in enumerate(offsets):
+        d = O.primary_dirs(cam, O.params(jitter=0, simple_sampler=1), s)
+        expect = tl[:, None] + (x + ox)[None] * stx[:, None] + (y + oy)[None] * sty[:, None]
+        np.testing.assert_allclose(d, expect, rtol=0, atol=2e-5)
+    for s in (4, 9):
+        np.testing.assert_array_equal(O.primary_dirs(cam, O.params(jitter=0, simple_sampler=1), s),
+                                      O.primary_dirs(cam, O.params(jitter=1), s))
+
+
+def test_simple_shade_barycentric():
+    """simpleShade (SimpleIntegrator.h:72-116): a hit pixel is the hit point's
+    barycentric coordinates (weights of v0, v1, v2) times |cos| between the
+    unit ray and the face normal; a miss is (0.15, 0.2, 0.3); the running mean
+    weights are 1/(image+1)."""
+    sc = facing_triangle_scene()
+    g, cam = setup(sc)
+    p = O.params()
+    fb, _ = O.render_simple(g, cam, p, image_id=0)
+    h, hp, _ = O.primary(g, cam, O.params(jitter=0, simple_sampler=1), 0)
+    hit = hp < sc.nt
+    assert 0.2 < hit.mean() < 0.9
+    np.testing.assert_array_equal(fb[~hit], np.tile(np.array([0.15, 0.2, 0.3], F), (int((~hit).sum()), 1)))
+    V = sc.verts.T.astype(np.float64)
+    n = np.cross(V[1] - V[0], V[2] - V[0])
+    n /= np.linalg.norm(n)
+    for i in np.nonzero(hit)[0]:
+        q = h[0:3, i].astype(np.float64)
+        A = np.stack([V[0] - V[2], V[1] - V[2]], 1)
+        lam = np.linalg.lstsq(A, q - V[2], rcond=None)[0]
+        d = h[3:6, i].astype(np.float64)
+        c = abs(np.dot(n, d / np.linalg.norm(d)))
+        np.testing.assert_allclose(fb[i], np.array([lam[0], lam[1], 1 - lam[0] - lam[1]]) * c, atol=2e-4)
+    f1, _ = O.render_simple(g, cam, p, image_id=1, fb=fb)
+    n1, _ = O.render_simple(g, cam, p, image_id=1)
+    np.testing.assert_array_equal(f1, (fb * F(0.5) + (n1 / F(0.5)) * F(0.5)).astype(F))
+
+
+# ---------------------------------------------------------------------------
+# renderFrame mode 1: the multi-kernel path tracer
+# ---------------------------------------------------------------------------
+
+def up_plane_scene(kd):
+    """plane_scene with the plane's faces wound to face +y (towards the
+    light), so that computeDirectIllumination's unflipped normal sees it."""
+    v, f = scenes.quad_grid((-50, 0, -50), (100, 0, 0), (0, 0, 100), 4, 4)
+    f = f[:, [0, 2, 1]]
+    tri = np.array([[49.0, 0.37, 49.0], [49.5, 0.37, 49.0], [49.0, 0.9, 49.5]])
+    parts = [(v, f, 0), (tri, np.array([[0, 1, 2]], np.uint32), 1)]
+    mats = [scenes.material(kd), scenes.material((0.1, 0.1, 0.1))]
+    cam = dict(pos=(0.0, 10.0, 0.0), dir=(0.1, -1.0, 0.2), up=(0.0, 0.0, 1.0), fov=60.0, res=(40, 32))
+    return scenes._assemble(parts, mats, "up_plane", cam)
+
+
+def test_path_multi_direct_light_at_absorbed_vertices():
+    """Image 0 of the multi-kernel path tracer over a Lambertian plane under a
+    downward area light (PathTracer.h:374-1215).  At the primary hit the
+    roulette absorbs the path with probability 1 - pi*kd and divides the
+    throughput by that; a path that bounces leaves the scene and adds
+    nothing.  The end vertex of an absorbed path is the primary hit, where the
+    shadow ray carries I*A*cos_l/r^2*0.5 from a uniform light point, the direct
+    light is kd*that*cos_n, and image 0 stores throughput^2 x direct (the
+    throughput multiplies both the new-radiance buffer and the merge).  So
+    every hit pixel is 0 or att^2*kd*0.5*I*f(q) with f = A*cos_n*cos_l/r^2 at a
+    light point q: f lies within its range over the light, the mean of
+    f / E[f] is 1 within 5 standard errors, and the absorbed fraction is
+    1 - pi*kd within 5 binomial standard errors."""
+    kd = 0.05
+    sc = up_plane_scene((kd, kd, kd))
+    light = scenes.area_light((-1.0, 4.0, -1.0), (2.0, 0.0, 0.0), (0.0, 0.0, 2.0), (5.0, 5.0, 5.0))
+    g, cam = setup(sc)
+    p = O.params(seed=(11, 12), nthreads=4)
+    fb, state, end, endp, imp, _ = O.render_pt_multi(g, cam, p, sc.materials, sc.mat_idx, light[None])
+    h, hp, _ = O.primary(g, cam, O.params(seed=(11, 12), jitter=1))
+    on_plane = np.nonzero(hp < sc.nt - 1)[0]
+    assert len(on_plane) > 600
+    albedo = F(kd) * PI
+    att = F(1) / (F(1) - F(0) - albedo)
+    absorbed = fb[on_plane, 0] > 0
+    frac = absorbed.mean()
+    q = 1 - float(albedo)
+    assert abs(frac - q) < 5 * np.sqrt(q * (1 - q) / len(on_plane))
+    # absorbed paths end at the primary hit with throughput att; bounced ones are black
+    ia = on_plane[absorbed]
+    np.testing.assert_array_equal(endp[ia], hp[ia])
+    np.testing.assert_array_equal(end[6, ia], h[6, ia])
+    np.testing.assert_array_equal(imp[ia], np.full((len(ia), 3), att, F))
+    np.testing.assert_array_equal(fb[on_plane[~absorbed]], 0)
+    ratios = []
+    for i in ia[::3]:
+        pnt = h[0:3, i].astype(np.float64)
+        ef, ef2 = irradiance_integrals(pnt, np.array([0, 1.0, 0]), light, m=100)
+        f = fb[i, 0] / (float(att) ** 2 * kd * 0.5 * 5.0)
+        sd = np.sqrt(max(ef2 - ef * ef, 0.0))
+        assert abs(f - ef) < 6 * sd + 1e-4 * ef
+        ratios.append(f / ef)
+    ratios = np.array(ratios)
+    assert abs(ratios.mean() - 1) < 5 * ratios.std() / np.sqrt(len(ratios)) + 1e-3
+
+
+def test_path_multi_merge_conserves_the_contribution():
+    """addIndirectIllumination (PathTracer.h:889-990) over several images:
+    each pixel's new contribution is importance x new radiance + the old
+    residue; the blurred value replaces part of it only where the squared
+    intensity reaches max(0.25/image, 0.05), so fb_new + residue_new / (k+1)
+    equals fb_old * k/(k+1) + contribution / (k+1), the residue is never
+    negative, and it is zero wherever the intensity stays under the threshold."""
+    sc = scenes.small_box()
+    g, cam = setup(sc, O.GRID_TWO_LEVEL, 1.2)
+    p = O.params(seed=(5, 6), nthreads=4)
+    fb = np.zeros((cam.res_x * cam.res_y, 3), F)
+    state = None
+    blurred_any = False
+    for k in range(6):
+        res_old = np.zeros_like(fb) if state is None else state[1].copy()
+        fb_new, state, _, _, imp, _ = O.render_pt_multi(g, cam, p, sc.materials, sc.mat_idx, sc.lights,
+                                                       image_id=k, fb=fb, state=state)
+        nw = F(1) / F(k + 1)
+        ow = F(1) - nw
+        nc = (imp * state[0] + res_old).astype(F)
+        lhs = fb_new.astype(np.float64) + state[1].astype(np.float64) * nw
+        rhs = fb.astype(np.float64) * ow + nc.astype(np.float64) * nw
+        np.testing.assert_allclose(lhs, rhs, rtol=2e-6, atol=1e-7)
+        assert (state[1] >= 0).all()
+        var = nc.astype(np.float64).sum(1) ** 2
+        thr = max(0.25 / k, 0.05) if k > 0 else np.inf
+        low = var < thr * (1 - 1e-5)
+        assert (state[1][low] == 0).all()
+        blurred_any |= bool((state[1] > 0).any())
+        fb = fb_new
+    assert fb.max() > 0 and blurred_any
+
+
+def test_path_multi_without_lights_is_a_no_op():
+    """PathTracer::integrate returns at once without light sources
+    (PathTracer.h:1049-1053)."""
+    sc = scenes.small_box()
+    g, cam = setup(sc)
+    fb0 = np.random.default_rng(1).random((cam.res_x * cam.res_y, 3)).astype(F)
+    st0 = np.random.default_rng(2).random((2, cam.res_x * cam.res_y, 3)).astype(F)
+    fb, st, _, _, _, _ = O.render_pt_multi(g, cam, O.params(), sc.materials, sc.mat_idx,
+                                           np.zeros((0, 15), F), image_id=3, fb=fb0, state=st0)
+    np.testing.assert_array_equal(fb, fb0)
+    np.testing.assert_array_equal(st, st0)
