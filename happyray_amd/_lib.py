@@ -98,7 +98,10 @@ class Request(C.Structure):
                 ("depth", C.c_int), ("on_device", C.c_int), ("org", _vp), ("dir", _vp),
                 ("stride", C.c_uint32), ("t_out", _vp),
                 ("prim_out", _vp), ("occ_out", _vp), ("rgb_out", _vp), ("k1", C.c_uint32),
-                ("k2", C.c_uint32), ("coeff", C.c_float)]
+                ("k2", C.c_uint32), ("coeff", C.c_float), ("layout", C.c_int)]
+
+
+RAYS_SOA, RAYS_RECORDS = 0, 1
 
 
 class RequestResult(C.Structure):
@@ -187,6 +190,10 @@ SIGNATURES = {
     "hr_scene_upload": (C.c_int, [_vp, _vp]),
     "hr_scene_frame": (C.c_int, [_vp, C.c_float, _u32p, _u32p, _f32p]),
     "hr_scene_next_frame": (C.c_int, [_vp, _f32p]),
+    "hr_trace_wire": (C.c_int, [_vp, _f32p, C.c_uint32, _f32p, _u32p]),
+    "hr_occluded_wire": (C.c_int, [_vp, _f32p, C.c_uint32, C.c_float, _u8p]),
+    "hr_render_rays": (C.c_int, [_vp, _f32p, C.c_uint32, _f32p]),
+    "hr_dump_ao_rays": (C.c_int, [_vp, _f32p]),
     "hr_image_to_rgb8": (C.c_int, [_f32p, C.c_int, C.c_int, C.c_float, _u8p]),
     "hr_write_png": (C.c_int, [C.c_char_p, _u8p, C.c_int, C.c_int]),
     "hr_write_screenshot": (C.c_int, [C.c_char_p, _f32p, C.c_int, C.c_int]),

@@ -105,7 +105,9 @@ struct hr_ctx {
     uint32_t pm_npix = 0;
 
     // ray batches
-    DevBuf<float> ray_org, ray_dir, ray_t, ray_hits, ray_rgb;
+    DevBuf<float> ray_org, ray_dir, ray_t, ray_hits, ray_rgb, ray_wire;
+    hr::RenderArgs last_ao_ra{};          // the last AO render's arguments and camera (ray dump)
+    hr::CameraDev last_ao_cam{};
     DevBuf<uint32_t> ray_prim;
     DevBuf<uint8_t> ray_occ;
 
@@ -464,7 +466,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
                          const hr::FrameSel* fs, bool events)
 {
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
-    hr::RenderArgs ra;
+    hr::RenderArgs ra{};
     ra.seed_lo = c->seed_lo;
     ra.seed_hi = c->seed_hi;
     ra.jitter = mode == HR_MODE_PATH_MULTI ? 1 : p->jitter;   // PathTracer's RandomPrimaryRayGenerator
@@ -516,6 +518,9 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
         hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->stream);
         c->last_ao_samples = p->ao_samples;
+        c->last_ao_ra = ra;
+        c->last_ao_ra.stats = nullptr;
+        c->last_ao_cam = c->cam;
     } else if (mode == HR_MODE_PATH) {
         hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
@@ -650,6 +655,7 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     b += c->sort_hist.bytes() + c->lights.bytes() + c->light_cdf.bytes() + c->hits.bytes() + c->fb.bytes();
     b += c->hit_prims.bytes() + c->occ.bytes() + c->ray_org.bytes() + c->ray_dir.bytes() + c->ray_t.bytes();
     b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes() + c->pm.bytes() + c->pm_prims.bytes();
+    b += c->ray_wire.bytes();
     return b;
 }
 
@@ -667,7 +673,7 @@ hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims
         c->stream = keep;
         c->normals_dirty = false;
     }
-    hr::RenderArgs ra;
+    hr::RenderArgs ra{};
     ra.seed_lo = c->seed_lo;
     ra.seed_hi = c->seed_hi;
     ra.jitter = 0;
@@ -764,6 +770,7 @@ hr_status hr_destroy(hr_ctx* c)
     for (auto* b : fbufs) b->release();
     c->pm.release();
     c->pm_prims.release();
+    c->ray_wire.release();
     DevBuf<uint32_t>* ubufs[] = { &c->idx, &c->mat_idx, &c->wave_counts, &c->wave_offsets, &c->scan_chain,
                                   &c->keys, &c->vals,
                                   &c->keys_sorted, &c->top_vals_sorted, &c->prims, &c->leaf_counts,
@@ -1109,13 +1116,26 @@ hr_status hr_get_ao_occlusion(hr_ctx* c, uint8_t* occ)
     return HR_OK;
 }
 
+// 24-byte ray records on the host -> the context's SoA ray planes
+static hr_status upload_wire(hr_ctx* c, const float* rays, uint32_t n)
+{
+    HR_TRY(c->ray_org.reserve(3 * (size_t)n));
+    HR_TRY(c->ray_dir.reserve(3 * (size_t)n));
+    HR_TRY(c->ray_wire.reserve(6 * (size_t)n));
+    HR_TRY(upload(c->ray_wire.p, rays, 6 * (size_t)n * sizeof(float), c->stream));
+    hr::launch_wire_to_soa(c->ray_wire.p, n, c->ray_org.p, c->ray_dir.p, n, c->stream);
+    return HR_OK;
+}
+
+// wire: org holds n 24-byte ray records in host memory and dir is unused
 static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uint32_t n, int anyhit,
-                              float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out, bool device)
+                              float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out, bool device,
+                              bool wire = false)
 {
     HR_TRY(check_ctx(c));
     if (!c->built) return fail(HR_ERR_STATE, "trace before build");
     if (n == 0) return HR_OK;
-    if (!org || !dir) return fail(HR_ERR_ARG, "null ray buffer");
+    if (!org || (!dir && !wire)) return fail(HR_ERR_ARG, "null ray buffer");
     const hr::TraceScene S = trace_scene(c);
     const bool pers = c->tuning.persistent_secondary != 0;
     const uint32_t refill = (uint32_t)c->tuning.refill;
@@ -1134,8 +1154,12 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
     HR_TRY(c->ray_t.reserve(n));
     HR_TRY(c->ray_prim.reserve(n));
     HR_TRY(c->ray_occ.reserve(n));
-    HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
-    HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
+    if (wire) {
+        HR_TRY(upload_wire(c, org, n));
+    } else {
+        HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
+        HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
+    }
     if (pers)
         hr::persist_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
                           c->ray_occ.p, c->pool.p, refill, c->ray_count.p, c->work.p, c->stream);
@@ -1163,6 +1187,59 @@ hr_status hr_occluded(hr_ctx* c, const float* org, const float* dir, uint32_t n,
     if (!occ_out) return fail(HR_ERR_ARG, "null output buffer");
     if (!(tmax > 0.f)) return fail(HR_ERR_ARG, "tmax must be positive");
     return trace_common(c, org, dir, n, 1, tmax, nullptr, nullptr, occ_out, false);
+}
+
+hr_status hr_trace_wire(hr_ctx* c, const float* rays, uint32_t n, float* t_out, uint32_t* prim_out)
+{
+    if (!t_out || !prim_out) return fail(HR_ERR_ARG, "null output buffer");
+    return trace_common(c, rays, nullptr, n, 0, 0.f, t_out, prim_out, nullptr, false, true);
+}
+
+hr_status hr_occluded_wire(hr_ctx* c, const float* rays, uint32_t n, float tmax, uint8_t* occ_out)
+{
+    if (!occ_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (!(tmax > 0.f)) return fail(HR_ERR_ARG, "tmax must be positive");
+    return trace_common(c, rays, nullptr, n, 1, tmax, nullptr, nullptr, occ_out, false, true);
+}
+
+hr_status hr_render_rays(hr_ctx* c, const float* rays, uint32_t n, float* rgb_out)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->built) return fail(HR_ERR_STATE, "render before build");
+    if (!rgb_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (n == 0) return HR_OK;
+    if (!rays) return fail(HR_ERR_ARG, "null ray buffer");
+    HR_TRY(upload_wire(c, rays, n));
+    HR_TRY(c->ray_t.reserve(n));
+    HR_TRY(c->ray_prim.reserve(n));
+    HR_TRY(c->ray_hits.reserve(7 * (size_t)n));
+    HR_TRY(c->ray_rgb.reserve(3 * (size_t)n));
+    // SimpleRayTraverser: trace, then simpleShade at image 0
+    hr::persist_batch(trace_scene(c), c->ray_org.p, c->ray_dir.p, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr,
+                      c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p);
+    hr::fill_u32((uint32_t*)c->ray_rgb.p, 0u, 3 * (size_t)n, nullptr, 1, c->stream);
+    hr::RenderArgs ra{};
+    ra.seed_lo = c->seed_lo;
+    ra.seed_hi = c->seed_hi;
+    hr::launch_simple_shade(trace_scene(c), ra, n, c->ray_hits.p, c->ray_prim.p, c->ray_rgb.p, c->stream);
+    HR_HIP(hipGetLastError());
+    HR_HIP(hipMemcpyAsync(rgb_out, c->ray_rgb.p, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
+}
+
+hr_status hr_dump_ao_rays(hr_ctx* c, float* rays_out)
+{
+    HR_TRY(check_ctx(c));
+    if (!rays_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (!c->last_npix || !c->last_ao_samples) return fail(HR_ERR_STATE, "no AO render");
+    const size_t n = (size_t)c->last_npix * (size_t)(c->last_ao_samples + 1);
+    HR_TRY(c->ray_wire.reserve(6 * n));
+    hr::launch_ao_dump(trace_scene(c), c->last_ao_cam, c->last_ao_ra, c->hits.p, c->hit_prims.p, c->ray_wire.p, c->stream);
+    HR_HIP(hipGetLastError());
+    HR_HIP(hipMemcpyAsync(rays_out, c->ray_wire.p, 6 * n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    return HR_OK;
 }
 
 hr_status hr_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, uint32_t n, float* d_t_out,

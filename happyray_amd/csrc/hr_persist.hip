@@ -240,7 +240,7 @@ __device__ __forceinline__ void set_ray(RayState& r, f3 org, f3 d)
 
 struct GenPrimary {
     CameraDev cam;
-    RenderArgs ra;
+    RenderArgs ra{};
     uint32_t npix;
     float* hits;
     uint32_t* hit_prims;
@@ -270,7 +270,7 @@ struct GenPrimary {
 };
 
 struct GenAO {
-    RenderArgs ra;
+    RenderArgs ra{};
     uint32_t npix;
     const float* hits;
     const uint32_t* hit_prims;

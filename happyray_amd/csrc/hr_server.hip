@@ -310,7 +310,10 @@ size_t scratch_bytes(const hr_request& r)
 {
     const size_t n = r.n;
     size_t b = 0;
-    if (!r.on_device) b += 2 * align256(12 * n);                  // rays
+    if (r.layout == HR_RAYS_RECORDS)                                  // rays: planes (+ staged records)
+        b += 2 * align256(12 * n) + (r.on_device ? 0 : align256(24 * n));
+    else if (!r.on_device)
+        b += 2 * align256(12 * n);
     if (r.kind == HR_REQ_OCCLUDED) {
         if (!r.on_device || !r.occ_out) b += align256(n);
     } else {
@@ -428,7 +431,21 @@ void launch_wave(hr_server* S, Slot& sl, std::vector<Req*>& list)
         const float* org = r.org;
         const float* dir = r.dir;
         uint32_t dev_stride = (uint32_t)stride;
-        if (!r.on_device) {
+        if (r.layout == HR_RAYS_RECORDS) {
+            // 24-byte ray records: deinterleave on the device into SoA planes
+            const float* rec = r.org;
+            if (!r.on_device) {
+                float* w = (float*)take(24 * n);
+                (void)hipMemcpyAsync(w, r.org, 24 * n, hipMemcpyHostToDevice, S->stream);
+                rec = w;
+            }
+            float* o = (float*)take(12 * n);
+            float* d = (float*)take(12 * n);
+            hr::launch_wire_to_soa(rec, (uint32_t)n, o, d, (uint32_t)n, S->stream);
+            org = o;
+            dir = d;
+            dev_stride = (uint32_t)n;
+        } else if (!r.on_device) {
             float* o = (float*)take(12 * n);
             float* d = (float*)take(12 * n);
             for (int k = 0; k < 3; ++k) {
@@ -667,7 +684,9 @@ hr_status hr_server_submit(hr_server* S, const hr_request* r, uint64_t* id)
 {
     if (!S || !r || !id) return fail(HR_ERR_ARG, "null argument");
     if (r->kind < HR_REQ_CLOSEST || r->kind > HR_REQ_REBUILD) return fail(HR_ERR_ARG, "unknown request kind");
-    if (r->kind != HR_REQ_REBUILD && r->n > 0 && (!r->org || !r->dir)) return fail(HR_ERR_ARG, "null rays");
+    if (r->layout != HR_RAYS_SOA && r->layout != HR_RAYS_RECORDS) return fail(HR_ERR_ARG, "unknown ray layout");
+    if (r->kind != HR_REQ_REBUILD && r->n > 0 && (!r->org || (!r->dir && r->layout == HR_RAYS_SOA)))
+        return fail(HR_ERR_ARG, "null rays");
     if (r->kind == HR_REQ_OCCLUDED && !(r->tmax > 0.f)) return fail(HR_ERR_ARG, "tmax must be positive");
     if (r->kind == HR_REQ_PATH && (r->depth < 1 || r->depth > 64)) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
     if (r->n > (1u << 30)) return fail(HR_ERR_ARG, "at most 2^30 rays per request");

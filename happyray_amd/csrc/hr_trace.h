@@ -70,6 +70,14 @@ void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s);
 void launch_simple_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                          const uint32_t* hit_prims, float* fb, hipStream_t s);
 
+// 24-byte ray records (origin xyz, direction xyz) -> SoA planes `stride` floats apart
+void launch_wire_to_soa(const float* rec, uint32_t n, float* org, float* dir, uint32_t stride, hipStream_t s);
+// AORayExporter's ray dump (AORayExporter.h:50-68, :262-300) of the render whose
+// primary hits are in hits / hit_prims: npix camera rays, then ao_samples rays
+// per pixel as AmbientOcclusionRayGenerator makes them, as 24-byte records
+void launch_ao_dump(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, const float* hits,
+                    const uint32_t* hit_prims, float* out, hipStream_t s);
+
 // Buffers of the multi-kernel path tracer (PathTracer.h:1038-1215), npix pixels.
 // rgb buffers hold 3 floats per pixel; new_radiance and residue carry over from
 // one image to the next and are cleared with the camera.

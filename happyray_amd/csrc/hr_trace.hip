@@ -14,6 +14,7 @@
 //   computeAOIllumination            src/RT/Integrator/AOIntegrator.h:64-191
 //   pathTrace                        src/RT/Integrator/PathTracer.h:41-372
 //   simpleShade                      src/RT/Integrator/SimpleIntegrator.h:42-125
+//   dumpRays                         src/RT/Integrator/AORayExporter.h:50-68
 //   uploadObjFrameNormalData         src/Application/WFObjectUploader.h:121-165
 //
 // Departures, shared with the CPU oracle: the counter-based RNG keyed by
@@ -372,6 +373,71 @@ __global__ __launch_bounds__(kBlock) void k_simple_shade(TraceScene S, RenderArg
 }
 
 // ---------------------------------------------------------------------------
+// ray records
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_wire_to_soa(const float* __restrict__ rec, uint32_t n,
+                                                        float* __restrict__ org, float* __restrict__ dir,
+                                                        uint32_t stride)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rec + 6 * (size_t)i;
+    org[i] = r[0];
+    org[(size_t)stride + i] = r[1];
+    org[2 * (size_t)stride + i] = r[2];
+    dir[i] = r[3];
+    dir[(size_t)stride + i] = r[4];
+    dir[2 * (size_t)stride + i] = r[5];
+}
+
+// dumpRays<PrimaryGenerator> then dumpRays<AmbientOcclusionRayGenerator>
+// (AORayExporter.h:50-68): record j < npix is pixel j's camera ray; record
+// npix + j is AO ray j (pixel j / S, sample j % S)
+__global__ __launch_bounds__(kBlock) void k_ao_dump(TraceScene S, CameraDev cam, RenderArgs ra, uint32_t npix,
+                                                    const float* __restrict__ hits,
+                                                    const uint32_t* __restrict__ hit_prims,
+                                                    float* __restrict__ out)
+{
+    const uint32_t S_ = (uint32_t)ra.ao_samples;
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (uint64_t)npix * (S_ + 1u)) return;
+    f3 org, dir;
+    if (j < npix) {
+        org = mk(cam.center[0], cam.center[1], cam.center[2]);
+        dir = primary_direction(cam, ra, (uint32_t)j, npix);
+    } else {
+        const uint32_t a = (uint32_t)(j - npix);
+        const uint32_t pix = a / S_, s = a % S_;
+        // AmbientOcclusionRayGenerator::operator() (RayGenerators.h:272-344)
+        org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+        dir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix], hits[5 * (size_t)npix + pix]);
+        const float th = hits[6 * (size_t)npix + pix];
+        const uint32_t prim = hit_prims[pix];
+        if (th < kFltMax && prim < S.nt) {
+            f3 v0, v1, v2;
+            load_tri(S.rec, prim, v0, v1, v2);
+            f3 n = normalize(cross(v1 - v0, v2 - v0));
+            if (dot(n, dir) > 0.f) n = -n;
+            float u[4];
+            rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamAO + s, u);
+            const f3 h = cosine_dir(u[0], u[1]);
+            f3 tg, bn;
+            local_frame(n, tg, bn);
+            dir = normalize(n * h.z + tg * h.x + bn * h.y);
+            org = org + (0.01f * ra.ao_dist) * dir;
+        }
+    }
+    float* o = out + 6 * j;
+    o[0] = org.x;
+    o[1] = org.y;
+    o[2] = org.z;
+    o[3] = dir.x;
+    o[4] = dir.y;
+    o[5] = dir.z;
+}
+
+// ---------------------------------------------------------------------------
 // per-corner normals
 // ---------------------------------------------------------------------------
 
@@ -491,6 +557,20 @@ void launch_simple_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npi
 {
     hipLaunchKernelGGL(k_simple_shade, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits, hit_prims,
                        fb);
+}
+
+void launch_wire_to_soa(const float* rec, uint32_t n, float* org, float* dir, uint32_t stride, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_wire_to_soa, dim3(blocks_for(n)), dim3(kBlock), 0, s, rec, n, org, dir, stride);
+}
+
+void launch_ao_dump(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, const float* hits,
+                    const uint32_t* hit_prims, float* out, hipStream_t s)
+{
+    const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
+    const size_t n = (size_t)npix * ((size_t)ra.ao_samples + 1);
+    hipLaunchKernelGGL(k_ao_dump, dim3(blocks_for(n)), dim3(kBlock), 0, s, S, cam, ra, npix, hits, hit_prims, out);
 }
 
 void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s)

@@ -231,6 +231,36 @@ class Engine:
               "hr_occluded")
         return occ
 
+    # -- ray records (24 bytes per ray: origin xyz, direction xyz) --------
+    def trace_records(self, records):
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, 6)
+        n = len(r)
+        t = np.zeros(n, np.float32)
+        p = np.zeros(n, np.uint32)
+        check(self._L.hr_trace_wire(self._h, _f(r), n, _f(t), p.ctypes.data_as(_u32p)), "hr_trace_wire")
+        return t, p
+
+    def occluded_records(self, records, tmax):
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, 6)
+        occ = np.zeros(len(r), np.uint8)
+        check(self._L.hr_occluded_wire(self._h, _f(r), len(r), float(tmax), occ.ctypes.data_as(_lib._u8p)),
+              "hr_occluded_wire")
+        return occ
+
+    def render_rays(self, records):
+        """renderFrame mode 0 with input rays: (n, 3) simple shading of each ray's hit."""
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, 6)
+        rgb = np.zeros((len(r), 3), np.float32)
+        check(self._L.hr_render_rays(self._h, _f(r), len(r), _f(rgb)), "hr_render_rays")
+        return rgb
+
+    def dump_ao_rays(self, ao_samples):
+        """The last AO render's rays as AORayExporter dumps them: (npix * (S + 1), 6)."""
+        n = self._res[0] * self._res[1] * (int(ao_samples) + 1)
+        out = np.zeros((n, 6), np.float32)
+        check(self._L.hr_dump_ao_rays(self._h, _f(out)), "hr_dump_ao_rays")
+        return out
+
     def path_trace(self, org, dirs, depth=4):
         """org, dirs: (3, n) float32. Radiance (n, 3) of a path from each ray."""
         o = np.ascontiguousarray(org, np.float32)

@@ -121,6 +121,25 @@ class Server:
                          int(k2), float(coeff))
         return self._submit(r, None, [])
 
+    def submit_records(self, kind, scene, records, tmax=0.0, depth=4):
+        """A request whose payload is 24-byte ray records (origin xyz, direction xyz)."""
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 6)
+        n = len(rec)
+        if kind == REQ_CLOSEST:
+            outs = (np.zeros(n, np.float32), np.zeros(n, np.uint32))
+            ptrs = (outs[0].ctypes.data, outs[1].ctypes.data, None, None)
+        elif kind == REQ_OCCLUDED:
+            outs = np.zeros(n, np.uint8)
+            ptrs = (None, None, outs.ctypes.data, None)
+        elif kind == REQ_PATH:
+            outs = np.zeros((n, 3), np.float32)
+            ptrs = (None, None, None, outs.ctypes.data)
+        else:
+            raise ValueError("ray records carry closest, occluded or path requests")
+        r = _lib.Request(kind, scene, n, float(tmax), int(depth), 0, rec.ctypes.data, None, 0, *ptrs, 0, 0, 0.0,
+                         _lib.RAYS_RECORDS)
+        return self._submit(r, outs, [rec])
+
     def submit_device(self, kind, scene, n, org_ptr, dir_ptr, stride=0, tmax=0.0, depth=4):
         """Rays already in device memory (SoA planes `stride` floats apart);
         outputs are computed and dropped."""

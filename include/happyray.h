@@ -344,7 +344,10 @@ typedef struct {
     float* rgb_out;
     uint32_t k1, k2;           /* rebuild                                               */
     float coeff;
+    int layout;                /* HR_RAYS_SOA, or HR_RAYS_RECORDS: org points to n 24-byte
+                                  ray records (dir and stride unused)                  */
 } hr_request;
+enum { HR_RAYS_SOA = 0, HR_RAYS_RECORDS = 1 };
 typedef struct {
     int status;                /* hr_status of the request                              */
     double latency_ms;         /* submit to completion                                  */
@@ -426,6 +429,23 @@ hr_status hr_scene_upload(hr_ctx* ctx, const hr_scene* scene);
 hr_status hr_scene_frame(const hr_scene* scene, float current, uint32_t* k1, uint32_t* k2,
                          float* coeff);
 hr_status hr_scene_next_frame(const hr_scene* scene, float* current);
+
+/* ---- ray records: the ray file of SimpleRayTraverser::loadRaysFromFile
+ * (src/RT/Integrator/SimpleRayTraverser.h:200-262) and AORayExporter's dump
+ * (src/RT/Integrator/AORayExporter.h:50-68, :262-310) ----
+ * A ray record is 24 bytes: origin x, y, z then direction x, y, z as float32;
+ * records follow each other with no header (a file of b bytes holds b / 24 rays). */
+/* closest hits / occlusion of n host ray records (as hr_trace / hr_occluded) */
+hr_status hr_trace_wire(hr_ctx* ctx, const float* rays, uint32_t n, float* t_out, uint32_t* prim_out);
+hr_status hr_occluded_wire(hr_ctx* ctx, const float* rays, uint32_t n, float tmax, uint8_t* occ_out);
+/* renderFrame mode 0 with input rays (RTEngine.cu:410-413, SimpleRayTraverser::integrate):
+ * the closest hit of each record, then simpleShade at image 0: 3 floats per ray */
+hr_status hr_render_rays(hr_ctx* ctx, const float* rays, uint32_t n, float* rgb_out);
+/* The rays of the last AO render as AORayExporter dumps them: the npix camera rays
+ * (camera position, unnormalized direction), then ao_samples AO rays per pixel
+ * (pixel-major), a missed pixel's records holding its stored hit point and camera
+ * direction; (npix * (ao_samples + 1)) records into rays_out. */
+hr_status hr_dump_ao_rays(hr_ctx* ctx, float* rays_out);
 
 /* ---- image output (host only): Image::gammaCorrect + Image::writePNG
  * (src/Utils/ImagePNG.cpp:64-72, :268-330) ----

@@ -136,6 +136,15 @@ int  or_tri_box_overlap(const float v0[3], const float v1[3], const float v2[3],
  * fb (3 floats per pixel) is the running mean, weight 1/(image_id+1) */
 int  or_render_simple(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
                       const or_render_params* p, int image_id, float* fb, or_stats* stats);
+/* renderFrame mode 0 with input rays (SimpleRayTraverser, RTEngine.cu:410-413):
+ * rays are n 24-byte records (origin xyz, direction xyz); rgb gets the simple
+ * shading of each ray's closest hit at image 0 */
+int  or_render_rays_simple(const or_mesh* mesh, const or_grid* g, const float* rays, uint32_t n,
+                           int nthreads, float* rgb, or_stats* stats);
+/* AORayExporter's dump (AORayExporter.h:50-68): the npix camera rays, then
+ * ao_samples AO rays per pixel, as (npix * (ao_samples + 1)) 24-byte records */
+int  or_dump_ao_rays(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
+                     const or_render_params* p, int image_id, float* out);
 /* renderFrame mode 1: the multi-kernel path tracer (PathTracer.h:374-1215).
  * state holds 6 floats per pixel carried from image to image (new radiance,
  * residue; zero them with the camera); end / end_prims (may be NULL) receive

@@ -465,29 +465,47 @@ typedef struct {
     uint8_t* occ;
 } ao_arg;
 
+/* AmbientOcclusionRayGenerator::operator() (RayGenerators.h:272-344) for
+ * sample s of pixel pix: returns 0 for a missed pixel, whose ray stays the
+ * stored hit point and camera direction */
+static int ao_ray(const or_mesh* m, const or_render_params* p, int image, uint32_t pix, uint32_t s,
+                  const float* hx, const float* hy, const float* hz, const float* hdx, const float* hdy,
+                  const float* hdz, const float* ht, const uint32_t* hprim, f3* org_out, f3* dir_out)
+{
+    const uint32_t prim = hprim[pix];
+    f3 org = mk3(hx[pix], hy[pix], hz[pix]);
+    const f3 pdir = mk3(hdx[pix], hdy[pix], hdz[pix]);
+    *org_out = org;
+    *dir_out = pdir;
+    if (!(ht[pix] < FLT_MAX) || prim >= m->nt) return 0;
+    f3 v0, v1, v2;
+    tri_verts(m, prim, &v0, &v1, &v2);
+    const f3 real_normal = cross3(sub3(v1, v0), sub3(v2, v0));
+    f3 n = norm3(real_normal);
+    if (dot3(n, pdir) > 0.f) n = neg3(n);
+    float u[4];
+    or_rand4(p, pix, (uint32_t)image, OR_STREAM_AO + s, u);
+    const f3 tmp = cosine_sample(u[0], u[1]);
+    f3 tg, bn;
+    local_coords(n, &tg, &bn);
+    const f3 dir = norm3(add3(add3(muls3(n, tmp.z), muls3(tg, tmp.x)), muls3(bn, tmp.y)));
+    *org_out = add3(org, smul3(0.01f * p->ao_dist, dir));
+    *dir_out = dir;
+    return 1;
+}
+
 static void ao_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
 {
     ao_arg* q = (ao_arg*)a;
     const uint32_t S = (uint32_t)q->p->ao_samples;
     for (uint32_t j = b; j < e; ++j) {
         const uint32_t pix = j / S, s = j % S;
-        const uint32_t prim = q->hprim[pix];
-        if (!(q->ht[pix] < FLT_MAX) || prim >= cx->m->nt) { q->occ[j] = 1; continue; }
-        /* AmbientOcclusionRayGenerator::operator() (RayGenerators.h:273-344) */
-        f3 org = mk3(q->hx[pix], q->hy[pix], q->hz[pix]);
-        const f3 pdir = mk3(q->hdx[pix], q->hdy[pix], q->hdz[pix]);
-        f3 v0, v1, v2;
-        tri_verts(cx->m, prim, &v0, &v1, &v2);
-        const f3 real_normal = cross3(sub3(v1, v0), sub3(v2, v0));
-        f3 n = norm3(real_normal);
-        if (dot3(n, pdir) > 0.f) n = neg3(n);
-        float u[4];
-        or_rand4(q->p, pix, (uint32_t)q->image, OR_STREAM_AO + s, u);
-        const f3 tmp = cosine_sample(u[0], u[1]);
-        f3 tg, bn;
-        local_coords(n, &tg, &bn);
-        const f3 dir = norm3(add3(add3(muls3(n, tmp.z), muls3(tg, tmp.x)), muls3(bn, tmp.y)));
-        org = add3(org, smul3(0.01f * q->p->ao_dist, dir));
+        f3 org, dir;
+        if (!ao_ray(cx->m, q->p, q->image, pix, s, q->hx, q->hy, q->hz, q->hdx, q->hdy, q->hdz, q->ht, q->hprim,
+                    &org, &dir)) {
+            q->occ[j] = 1;
+            continue;
+        }
         float t;
         q->occ[j] = (uint8_t)trace_one(cx, org, dir, q->p->ao_dist, 1, q->p->ao_dist, &t, NULL);
     }
@@ -792,6 +810,10 @@ void or_shading_normal(const or_mesh* m, uint32_t prim, const float org[3], floa
 /* renderFrame mode 0: simple integrator                               */
 /* ------------------------------------------------------------------ */
 
+static void simple_shade(const or_mesh* m, uint32_t npix, const float* hx, const float* hy, const float* hz,
+                         const float* hdx, const float* hdy, const float* hdz, const float* ht,
+                         const uint32_t* hp, int image_id, float* fb);
+
 int or_render_simple(const or_mesh* m, const or_grid* g, const or_camera* cam,
                      const or_render_params* p, int image_id, float* fb, or_stats* stats)
 {
@@ -801,7 +823,16 @@ int or_render_simple(const or_mesh* m, const or_grid* g, const or_camera* cam,
     float *hx = h, *hy = h + npix, *hz = h + 2 * (size_t)npix, *hdx = h + 3 * (size_t)npix,
           *hdy = h + 4 * (size_t)npix, *hdz = h + 5 * (size_t)npix, *ht = h + 6 * (size_t)npix;
     or_primary(m, g, cam, p, image_id, hx, hy, hz, hdx, hdy, hdz, ht, hp, stats);
-    /* simpleShade (SimpleIntegrator.h:42-125) */
+    simple_shade(m, npix, hx, hy, hz, hdx, hdy, hdz, ht, hp, image_id, fb);
+    free(h); free(hp);
+    return 0;
+}
+
+/* simpleShade (SimpleIntegrator.h:42-125) over n stored hits */
+static void simple_shade(const or_mesh* m, uint32_t npix, const float* hx, const float* hy, const float* hz,
+                         const float* hdx, const float* hdy, const float* hdz, const float* ht,
+                         const uint32_t* hp, int image_id, float* fb)
+{
     const float new_w = 1.f / (float)(image_id + 1);
     const float old_w = 1.f - new_w;
     for (uint32_t pix = 0; pix < npix; ++pix) {
@@ -827,6 +858,53 @@ int or_render_simple(const or_mesh* m, const or_grid* g, const or_camera* cam,
         o[0] = o[0] * old_w + rad.x * new_w;
         o[1] = o[1] * old_w + rad.y * new_w;
         o[2] = o[2] * old_w + rad.z * new_w;
+    }
+}
+
+int or_render_rays_simple(const or_mesh* m, const or_grid* g, const float* rays, uint32_t n, int nthreads,
+                          float* rgb, or_stats* stats)
+{
+    const size_t N = n ? n : 1;
+    float* soa = (float*)malloc(6 * N * sizeof(float));
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 6; ++k) soa[(size_t)k * n + i] = rays[6 * (size_t)i + k];
+    float* h = (float*)malloc(N * 7 * sizeof(float));
+    uint32_t* hp = (uint32_t*)malloc(N * sizeof(uint32_t));
+    float *hx = h, *hy = h + n, *hz = h + 2 * (size_t)n, *hdx = h + 3 * (size_t)n,
+          *hdy = h + 4 * (size_t)n, *hdz = h + 5 * (size_t)n, *ht = h + 6 * (size_t)n;
+    rays_arg ra = { soa, soa + n, soa + 2 * (size_t)n, soa + 3 * (size_t)n, soa + 4 * (size_t)n,
+                    soa + 5 * (size_t)n, hx, hy, hz, hdx, hdy, hdz, ht, hp };
+    run_tiles(m, g, n, nthreads, rays_tile, &ra, stats);
+    for (size_t i = 0; i < 3 * (size_t)n; ++i) rgb[i] = 0.f;
+    simple_shade(m, n, hx, hy, hz, hdx, hdy, hdz, ht, hp, 0, rgb);
+    free(soa); free(h); free(hp);
+    return 0;
+}
+
+int or_dump_ao_rays(const or_mesh* m, const or_grid* g, const or_camera* cam, const or_render_params* p,
+                    int image_id, float* out)
+{
+    const uint32_t npix = (uint32_t)cam->res_x * (uint32_t)cam->res_y;
+    const uint32_t S = (uint32_t)p->ao_samples;
+    float* h = (float*)malloc((size_t)npix * 7 * sizeof(float));
+    uint32_t* hp = (uint32_t*)malloc((size_t)npix * sizeof(uint32_t));
+    float *hx = h, *hy = h + npix, *hz = h + 2 * (size_t)npix, *hdx = h + 3 * (size_t)npix,
+          *hdy = h + 4 * (size_t)npix, *hdz = h + 5 * (size_t)npix, *ht = h + 6 * (size_t)npix;
+    or_primary(m, g, cam, p, image_id, hx, hy, hz, hdx, hdy, hdz, ht, hp, NULL);
+    /* dumpRays<primary generator> (AORayExporter.h:50-68) */
+    for (uint32_t pix = 0; pix < npix; ++pix) {
+        const f3 d = primary_dir(cam, p, pix, npix, image_id);
+        float* o = out + 6 * (size_t)pix;
+        o[0] = cam->center[0]; o[1] = cam->center[1]; o[2] = cam->center[2];
+        o[3] = d.x; o[4] = d.y; o[5] = d.z;
+    }
+    /* dumpRays<AmbientOcclusionRayGenerator> */
+    for (uint32_t j = 0; j < npix * S; ++j) {
+        f3 org, dir;
+        ao_ray(m, p, image_id, j / S, j % S, hx, hy, hz, hdx, hdy, hdz, ht, hp, &org, &dir);
+        float* o = out + 6 * ((size_t)npix + j);
+        o[0] = org.x; o[1] = org.y; o[2] = org.z;
+        o[3] = dir.x; o[4] = dir.y; o[5] = dir.z;
     }
     free(h); free(hp);
     return 0;

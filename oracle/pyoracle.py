@@ -360,6 +360,24 @@ def render_pt_multi(grid, cam, p, materials, mat_idx, lights, weights=None, imag
     return fb, state, end, endp, imp, st
 
 
+def render_rays_simple(grid, records, nthreads=1):
+    """(n, 3) simple shading of the closest hit of each 24-byte ray record."""
+    r = np.ascontiguousarray(records, np.float32).reshape(-1, 6)
+    rgb = np.zeros((len(r), 3), np.float32)
+    st = Stats()
+    lib().or_render_rays_simple(C.byref(grid.mesh.c), C.byref(grid.c), _fp(r), len(r), nthreads, _fp(rgb),
+                                C.byref(st))
+    return rgb, st
+
+
+def dump_ao_rays(grid, cam, p, image_id=0):
+    """(npix * (ao_samples + 1), 6) ray records as AORayExporter dumps them."""
+    n = cam.res_x * cam.res_y * (p.ao_samples + 1)
+    out = np.zeros((n, 6), np.float32)
+    lib().or_dump_ao_rays(C.byref(grid.mesh.c), C.byref(grid.c), C.byref(cam), C.byref(p), image_id, _fp(out))
+    return out
+
+
 def shading_normal(mesh, prim, org):
     """Path-tracer shading normal of primitive prim seen from hit point org."""
     out = np.zeros(3, np.float32)
