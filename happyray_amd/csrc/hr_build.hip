@@ -155,15 +155,16 @@ __global__ void k_grid_setup(GridParams* __restrict__ gp, BuildStatus* __restric
     }
 }
 
-// After an overflow the grid is unusable: a zero resolution makes every ray
-// miss at its entry test, so kernels that run after the build without a host
-// check (captured frames) read nothing out of bounds.
-__global__ void k_seal(GridParams* gp, const BuildStatus* st)
+// cells[0 .. *n) = (0, 0), grid-stride over the launch: a count kernel clears
+// the cells (or leaves) that the ranges kernel fills after the sort
+__device__ __forceinline__ void zero_cells(uint2* cells, const uint32_t* n)
 {
-    if (threadIdx.x != 0 || st->overflow == 0u) return;
-    gp->res[0] = gp->res[1] = gp->res[2] = 0;
-    gp->num_cells = 0u;
+    if (!cells) return;
+    const uint32_t m = *n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        cells[i] = make_uint2(0u, 0u);
 }
+
 
 
 // ---------------------------------------------------------------------------
@@ -258,11 +259,13 @@ __global__ __launch_bounds__(kBlock) void k_top_pairs(
     const float* __restrict__ vz, uint32_t nt, const GridParams* __restrict__ gp,
     const BuildStatus* __restrict__ st, float4* __restrict__ rec, uint32_t* __restrict__ wave_counts,
     const uint32_t* __restrict__ wave_offsets, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-    uint32_t waves_cap, SceneTable T = SceneTable{})
+    uint32_t waves_cap, SceneTable T = SceneTable{}, uint2* __restrict__ zero = nullptr,
+    const uint32_t* __restrict__ zero_n = nullptr)
 {
     static_assert(!(kExact && kMulti), "batches use bounding-box insertion");
     __shared__ WaveItems sh[kWavesPerBlock];
     if (stop(st)) return;
+    zero_cells(zero, zero_n);
     const int wl = (int)threadIdx.x / kWave;
     const int lane = (int)__lane_id();
     const uint32_t wg = blockIdx.x * kWavesPerBlock + (uint32_t)wl;
@@ -381,9 +384,18 @@ __global__ __launch_bounds__(kBlock) void k_top_pairs(
 // prepareCellRanges (UniformGridBuildKernels.h:384-487) on sorted keys:
 // begin at the first pair of a key, end one past its last pair.  Keys are
 // always < #cells here (no invalid-key tail).
+// Cell ranges of sorted keys.  The build's last launch also seals the grid
+// (seal_gp != nullptr): after an overflow the grid is unusable, and a zero
+// resolution makes every ray miss at its entry test, so kernels that run after
+// the build without a host check (captured frames) read nothing out of bounds.
 __global__ void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_dev,
-                         uint2* __restrict__ cells, const BuildStatus* __restrict__ st)
+                         uint2* __restrict__ cells, const BuildStatus* __restrict__ st,
+                         GridParams* __restrict__ seal_gp = nullptr)
 {
+    if (seal_gp && blockIdx.x == 0 && threadIdx.x == 0 && st->overflow != 0u) {
+        seal_gp->res[0] = seal_gp->res[1] = seal_gp->res[2] = 0;
+        seal_gp->num_cells = 0u;
+    }
     if (stop(st)) return;
     const uint32_t n = *n_dev;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -442,10 +454,12 @@ __global__ __launch_bounds__(kBlock) void k_leaf_pairs(
     const GridParams* __restrict__ gp, const BuildStatus* __restrict__ st,
     const float4* __restrict__ rec, uint32_t* __restrict__ wave_counts,
     const uint32_t* __restrict__ wave_offsets, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-    uint32_t waves_cap, SceneTable T = SceneTable{})
+    uint32_t waves_cap, SceneTable T = SceneTable{}, uint2* __restrict__ zero = nullptr,
+    const uint32_t* __restrict__ zero_n = nullptr)
 {
     __shared__ WaveItems sh[kWavesPerBlock];
     if (stop(st)) return;
+    zero_cells(zero, zero_n);
     const uint32_t n = *n_top;
     const int wl = (int)threadIdx.x / kWave;
     const int lane = (int)__lane_id();
@@ -703,13 +717,15 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
 
     launch_grid_setup(b.params, st, nt, a.lambda_top, b.cells_cap, s);
 
-    // ---- top level / uniform: count, scan, write ----
+    // ---- top level / uniform: count (and clear the cells), scan, write ----
     if (exact)
         hipLaunchKernelGGL((k_top_pairs<true, false>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
-                           a.vy, a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap);
+                           a.vy, a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap,
+                           SceneTable{}, b.cells, &b.params->num_cells);
     else
         hipLaunchKernelGGL((k_top_pairs<false, false>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx,
-                           a.vy, a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap);
+                           a.vy, a.vz, nt, b.params, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap,
+                           SceneTable{}, b.cells, &b.params->num_cells);
     const ScanCheck pairs_check{ st, b.pairs_cap, kOverflowPairs, &st->n_pairs_eff };
     scan_exclusive_1p(b.wave_counts, b.wave_offsets, 1, waves, nullptr, &st->num_pairs, &pairs_check, b.scan, s);
     if (exact)
@@ -724,14 +740,11 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     uint32_t* out_vals = two ? b.top_vals_sorted : b.prims;
     radix_sort_pairs(b.keys, b.vals, b.keys_sorted, out_vals, b.pairs_cap, &st->n_pairs_eff, cell_bits,
                      b.sort, s);
-    fill_u32((uint32_t*)b.cells, 0u, 2 * (size_t)b.cells_cap, &b.params->num_cells, 2, s);
     const uint32_t rblocks = cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256) > 4096 ? 4096
                                  : cdiv(b.pairs_cap > 0 ? b.pairs_cap : 1, 256);
-    hipLaunchKernelGGL(k_ranges, dim3(rblocks), dim3(256), 0, s, b.keys_sorted, &st->n_pairs_eff, b.cells, st);
-    if (!two) {
-        hipLaunchKernelGGL(k_seal, dim3(1), dim3(64), 0, s, b.params, st);
-        return;
-    }
+    hipLaunchKernelGGL(k_ranges, dim3(rblocks), dim3(256), 0, s, b.keys_sorted, &st->n_pairs_eff, b.cells, st,
+                       two ? nullptr : b.params);
+    if (!two) return;
 
     // ---- leaf resolution, leaf begin offsets ----
     const uint32_t cblocks = cdiv(b.cells_cap, 256) > 4096 ? 4096 : cdiv(b.cells_cap, 256);
@@ -746,7 +759,7 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     const uint32_t lblocks = cdiv(lwaves, kWavesPerBlock);
     hipLaunchKernelGGL((k_leaf_pairs<false>), dim3(lblocks), dim3(kBlock), 0, s, b.keys_sorted,
                        b.top_vals_sorted, &st->n_pairs_eff, b.cells, b.params, st, b.rec, b.wave_counts,
-                       nullptr, nullptr, nullptr, b.waves_cap);
+                       nullptr, nullptr, nullptr, b.waves_cap, SceneTable{}, b.leaves, &st->n_leaves_eff);
     const ScanCheck lp_check{ st, b.leaf_pairs_cap, kOverflowLeafPairs, &st->n_leaf_pairs_eff };
     scan_exclusive_1p(b.wave_counts, b.wave_offsets, 1, lwaves, nullptr, &st->num_leaf_pairs, &lp_check, b.scan,
                       s);
@@ -758,12 +771,10 @@ void run_build(const BuildArgs& a, BuildBuffers& b, hipStream_t s)
     const int leaf_bits = bits_for(b.leaves_cap);
     radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.prims, b.leaf_pairs_cap, &st->n_leaf_pairs_eff,
                      leaf_bits, b.sort, s);
-    fill_u32((uint32_t*)b.leaves, 0u, 2 * (size_t)b.leaves_cap, &st->n_leaves_eff, 2, s);
     const uint32_t lrblocks = cdiv(b.leaf_pairs_cap > 0 ? b.leaf_pairs_cap : 1, 256) > 4096 ? 4096
                                   : cdiv(b.leaf_pairs_cap > 0 ? b.leaf_pairs_cap : 1, 256);
     hipLaunchKernelGGL(k_ranges, dim3(lrblocks), dim3(256), 0, s, b.keys_sorted, &st->n_leaf_pairs_eff,
-                       b.leaves, st);
-    hipLaunchKernelGGL(k_seal, dim3(1), dim3(64), 0, s, b.params, st);
+                       b.leaves, st, b.params);
 }
 
 }  // namespace hr

@@ -196,13 +196,13 @@ size_t scan_chain_words(uint32_t n_cap);
 // stable LSD radix sort of (key, value) pairs on the low `bits` key bits
 // (at most 32).  n = *n_dev (<= n_cap).  keys_in / vals_in are overwritten
 // when the number of passes exceeds 2.  The result lands in keys_out /
-// vals_out.  One upsweep counts the digits of every pass; each pass is one
-// scatter launch with decoupled look-back over the tiles.
+// vals_out.  Each pass is a digit count per chunk of tiles, a scan of the
+// counts and an in-order scatter per chunk.
 struct SortTemp {
     uint32_t* keys_tmp;
     uint32_t* vals_tmp;
-    uint32_t* hist;         // sort_hist_words: digit counts of every pass
-    uint32_t* chain;        // sort_chain_words, zeroed when allocated: per-tile digit flags
+    uint32_t* hist;         // sort_hist_words: per-chunk digit counts, then their offsets
+    uint32_t* chain;        // sort_chain_words, zeroed when allocated: the offsets scan's flags
 };
 size_t sort_hist_words(uint32_t n_cap);
 size_t sort_chain_words(uint32_t n_cap);

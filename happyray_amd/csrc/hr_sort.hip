@@ -183,82 +183,108 @@ __global__ void k_fill(uint32_t* __restrict__ p, uint32_t value, size_t n_cap,
 constexpr int kRadix = 256;
 constexpr int kSortItems = kSortTile / kBlock;   // 16
 constexpr int kSortWaves = kBlock / kWave;       // 4
-constexpr int kLookback = 16;                    // tiles read per look-back round trip
 
-// Onesweep-style sort: one upsweep reads the keys once and counts the digits
-// of every pass; each pass is then one scatter kernel whose tiles (ids in
-// launch order) publish their per-digit counts and find the counts of all
-// earlier tiles by decoupled look-back, so a pass needs no histogram kernel
-// and no scan of per-tile histograms.
+// Chunked LSD radix sort.  A pass runs over a fixed grid of G workgroups;
+// workgroup b owns a contiguous chunk of whole 4096-key tiles.
+//  1. k_sort_count: each workgroup counts the digits of its chunk (wave
+//     ballots, no atomics) into counts[digit * G + b];
+//  2. the single-pass scan of that digit-major array gives every chunk's
+//     first output position per digit;
+//  3. k_sort_scatter: each workgroup walks its tiles in order, ranks the keys
+//     of a tile by digit (stable), stages the tile in LDS by digit and writes
+//     each digit's run at the chunk's running offset.
+// Every chunk's tiles keep their order, so the sort is stable.
 
-// global digit counts of every pass: hist[pass * 256 + digit] (zeroed before)
-__global__ __launch_bounds__(kBlock) void k_sort_upsweep(const uint32_t* __restrict__ keys,
-                                                         const uint32_t* __restrict__ n_dev, int passes,
-                                                         uint32_t* __restrict__ hist)
+constexpr int kSortGrid = 512;                   // workgroups per pass (at most)
+
+// rank of each of the wave's keys among the equal digits of the wave (lanes
+// in order), and the wave's counts per digit accumulated into h[digit]
+__device__ __forceinline__ uint32_t wave_digit_rank(uint32_t d, bool valid, uint32_t* h)
 {
-    __shared__ uint32_t h[4][kSortWaves][kRadix];
-    const int w = (int)threadIdx.x / kWave;
-    for (int p = 0; p < 4; ++p)
+    uint64_t match = __ballot(valid);
 #pragma unroll
-        for (int i = 0; i < kSortWaves; ++i) h[p][i][threadIdx.x] = 0u;
-    __syncthreads();
-    const uint32_t n = *n_dev;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const uint32_t k = keys[i];
-        for (int p = 0; p < passes; ++p) atomicAdd(&h[p][w][(k >> (8 * p)) & (kRadix - 1)], 1u);
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        match &= bit ? bal : ~bal;
     }
-    __syncthreads();
-    for (int p = 0; p < passes; ++p) {
-        uint32_t c = 0;
-#pragma unroll
-        for (int i = 0; i < kSortWaves; ++i) c += h[p][i][threadIdx.x];
-        if (c) atomicAdd(&hist[p * kRadix + threadIdx.x], c);
-    }
+    const uint32_t below = (uint32_t)__popcll(match & __lanemask_lt());
+    const uint32_t cnt = h[d];
+    wave_sync();
+    if (valid && below == 0u) h[d] = cnt + (uint32_t)__popcll(match);
+    wave_sync();
+    return cnt + below;
 }
 
-// count of digit d in the keys of tile p (look-back fallback)
-__device__ uint32_t tile_digit_count(const uint32_t* keys, uint32_t p, uint32_t n, int shift, uint32_t d)
+__device__ __forceinline__ void chunk_tiles(uint32_t n, uint32_t b, uint32_t g, uint32_t& t0, uint32_t& t1)
 {
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t per = (tiles + g - 1) / g;
+    t0 = b * per;
+    t1 = t0 + per < tiles ? t0 + per : tiles;
+    if (t0 > t1) t0 = t1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_count(const uint32_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ n_dev, int shift,
+                                                       uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t h[kSortWaves][kRadix];
+    const int w = (int)threadIdx.x / kWave;
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int i = 0; i < kSortWaves; ++i) h[i][threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t n = *n_dev;
+    uint32_t t0, t1;
+    chunk_tiles(n, blockIdx.x, gridDim.x, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t base = t * kSortTile + (uint32_t)w * (kWave * kSortItems) + lane;
+        uint32_t kr[kSortItems];
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const uint32_t i = base + (uint32_t)k * kWave;
+            kr[k] = i < n ? keys[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const uint32_t i = base + (uint32_t)k * kWave;
+            (void)wave_digit_rank((kr[k] >> shift) & (kRadix - 1), i < n, h[w]);
+        }
+    }
+    __syncthreads();
     uint32_t c = 0;
-    const uint32_t b = p * kSortTile;
-    const uint32_t e = b + kSortTile < n ? b + kSortTile : n;
-    for (uint32_t i = b; i < e; ++i) c += ((keys[i] >> shift) & (kRadix - 1)) == d ? 1u : 0u;
-    return c;
+#pragma unroll
+    for (int i = 0; i < kSortWaves; ++i) c += h[i][threadIdx.x];
+    // digit-major: the exclusive scan of this array is each chunk's first
+    // output position per digit
+    counts[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = c;
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(
     const uint32_t* __restrict__ keys_src, const uint32_t* __restrict__ vals_src,
     uint32_t* __restrict__ keys_dst, uint32_t* __restrict__ vals_dst,
-    const uint32_t* __restrict__ n_dev, int shift, const uint32_t* __restrict__ digit_hist,
-    uint32_t* chain)
+    const uint32_t* __restrict__ n_dev, int shift, const uint32_t* __restrict__ counts)
 {
     __shared__ uint32_t skeys[kSortTile];
     __shared__ uint32_t svals[kSortTile];
     __shared__ uint32_t whist[kSortWaves][kRadix];
     __shared__ uint32_t dstart[kRadix];
-    __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t offs[kRadix];
     __shared__ uint32_t lds_scan[kSortWaves];
-    __shared__ uint32_t s_tile, s_epoch;
 
-    unsigned long long* flags = (unsigned long long*)(chain + kChainHead);
-    if (threadIdx.x == 0) {
-        // acquire: the epoch is read before the id is taken, so the workgroup
-        // that takes the last id (and moves the epoch on) comes after it
-        s_epoch = __hip_atomic_load(&chain[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        s_tile = atomicAdd(&chain[0], 1u);
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile, epoch = s_epoch;
     const uint32_t n = *n_dev;
-    const uint32_t base = tile * kSortTile;
-    if (base < n) {
-        const int w = (int)threadIdx.x / kWave;
-        const int lane = (int)__lane_id();
-        const uint64_t lt = __lanemask_lt();
+    uint32_t t0, t1;
+    chunk_tiles(n, blockIdx.x, gridDim.x, t0, t1);
+    if (t0 >= t1) return;
+    const int w = (int)threadIdx.x / kWave;
+    const int lane = (int)__lane_id();
+    offs[threadIdx.x] = counts[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t base = t * kSortTile;
 #pragma unroll
         for (int i = 0; i < kSortWaves; ++i) whist[i][threadIdx.x] = 0u;
         __syncthreads();
-
         uint32_t kr[kSortItems], vr[kSortItems], rank[kSortItems];
 #pragma unroll
         for (int k = 0; k < kSortItems; ++k) {
@@ -266,25 +292,11 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
             const bool valid = i < n;
             kr[k] = valid ? keys_src[i] : 0u;
             vr[k] = valid ? vals_src[i] : 0u;
-            const uint32_t d = (kr[k] >> shift) & (kRadix - 1);
-            uint64_t match = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t bal = __ballot(bit);
-                match &= bit ? bal : ~bal;
-            }
-            const uint32_t below = (uint32_t)__popcll(match & lt);
-            const uint32_t cnt = whist[w][d];
-            wave_sync();
-            if (valid && below == 0u) whist[w][d] = cnt + (uint32_t)__popcll(match);
-            wave_sync();
-            rank[k] = cnt + below;
+            rank[k] = wave_digit_rank((kr[k] >> shift) & (kRadix - 1), valid, whist[w]);
         }
         __syncthreads();
         {
-            // per digit: prefix over waves, tile count, its look-back over the
-            // earlier tiles, and the global start of the digit
+            // per digit: prefix over the waves and the tile's digit starts
             const uint32_t d = threadIdx.x;
             uint32_t tot = 0;
 #pragma unroll
@@ -293,63 +305,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
                 whist[i][d] = tot;
                 tot += c;
             }
-            const uint32_t mask = epoch & 0x3FFFFFFFu;
-            uint32_t excl = 0;
-            if (tile == 0) {
-                __hip_atomic_store(&flags[d], flag_pack(epoch, kPrefix, tot), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __hip_atomic_store(&flags[(size_t)tile * kRadix + d], flag_pack(epoch, kAggregate, tot),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                // Look back kLookback tiles per round trip: the flags of a
-                // window are loaded together, then consumed from the nearest
-                // tile until a prefix (done) or an unpublished tile (wait).
-                // The count travels inside the flag word, so relaxed atomics
-                // suffice.
-                int p = (int)tile - 1;
-                int spins = 0;
-                while (p >= 0) {
-                    uint64_t f[kLookback];
-#pragma unroll
-                    for (int k = 0; k < kLookback; ++k)
-                        f[k] = p - k >= 0 ? __hip_atomic_load(&flags[(size_t)(p - k) * kRadix + d], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT)
-                                          : 0ull;
-                    int used = 0;
-                    bool done = false;
-#pragma unroll
-                    for (int k = 0; k < kLookback; ++k) {
-                        if (done || used < k || p - k < 0) continue;
-                        const uint32_t fe = (uint32_t)(f[k] >> 34);
-                        const uint64_t st = (f[k] >> 32) & 3u;
-                        if (fe == mask && st != 0u) {
-                            excl += (uint32_t)f[k];
-                            used = k + 1;
-                            done = st == kPrefix;
-                        }
-                    }
-                    if (done) break;
-                    p -= used;
-                    if (used > 0) {
-                        spins = 0;
-                        continue;
-                    }
-                    if (++spins > kSpinLimit) {
-                        excl += tile_digit_count(keys_src, (uint32_t)p, n, shift, d);
-                        --p;
-                        spins = 0;
-                        continue;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                __hip_atomic_store(&flags[(size_t)tile * kRadix + d], flag_pack(epoch, kPrefix, excl + tot),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
             uint32_t all;
             dstart[d] = block_excl_scan(tot, lds_scan, all);
-            uint32_t gall;
-            const uint32_t dbase = block_excl_scan(digit_hist[d], lds_scan, gall);
-            gbase[d] = dbase + excl;
         }
         __syncthreads();
 #pragma unroll
@@ -367,16 +324,20 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
         for (uint32_t i = threadIdx.x; i < tile_n; i += kBlock) {
             const uint32_t key = skeys[i];
             const uint32_t d = (key >> shift) & (kRadix - 1);
-            const uint32_t dst = gbase[d] + (i - dstart[d]);
+            const uint32_t dst = offs[d] + (i - dstart[d]);
             if (dst < n) {   // always true when the counts agree; keeps a bad count in bounds
                 keys_dst[dst] = key;
                 vals_dst[dst] = svals[i];
             }
         }
-    }
-    if (tile == gridDim.x - 1 && threadIdx.x == 0) {
-        __hip_atomic_store(&chain[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&chain[1], epoch + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        // advance the chunk's offsets by this tile's digit counts
+        {
+            const uint32_t d = threadIdx.x;
+            const uint32_t next = d + 1 < kRadix ? dstart[d + 1] : tile_n;
+            offs[d] += next - dstart[d];
+        }
+        __syncthreads();
     }
 }
 
@@ -406,23 +367,27 @@ void fill_u32(uint32_t* p, uint32_t value, size_t n_cap, const uint32_t* n_dev, 
                        n_mul);
 }
 
-size_t sort_hist_words(uint32_t n_cap) { return 4 * (size_t)kRadix; }
-
-size_t sort_chain_words(uint32_t n_cap)
+static uint32_t sort_grid(uint32_t n_cap)
 {
-    return kChainHead + 2 * (size_t)kRadix * cdiv(n_cap > 0 ? n_cap : 1, kSortTile);
+    const uint32_t tiles = cdiv(n_cap > 0 ? n_cap : 1, kSortTile);
+    return tiles < (uint32_t)kSortGrid ? tiles : (uint32_t)kSortGrid;
 }
+
+size_t sort_hist_words(uint32_t n_cap) { return 2 * (size_t)kRadix * sort_grid(n_cap); }
+
+size_t sort_chain_words(uint32_t n_cap) { return scan_chain_words((uint32_t)kRadix * sort_grid(n_cap)); }
 
 void radix_sort_pairs(uint32_t* keys_in, uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
                       uint32_t n_cap, const uint32_t* n_dev, int bits, SortTemp& tmp, hipStream_t s)
 {
-    const uint32_t tiles = cdiv(n_cap > 0 ? n_cap : 1, kSortTile);
+    const uint32_t g = sort_grid(n_cap);
+    const uint32_t m = (uint32_t)kRadix * g;
+    uint32_t* counts = tmp.hist;
+    uint32_t* offsets = tmp.hist + m;
+    ScanTemp st{ tmp.chain, scan_chain_words(m) };
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
     if (passes > 4) passes = 4;
-    fill_u32(tmp.hist, 0u, 4 * (size_t)kRadix, nullptr, 1, s);
-    uint32_t ublocks = tiles < 1024 ? tiles : 1024;
-    hipLaunchKernelGGL(k_sort_upsweep, dim3(ublocks), dim3(kBlock), 0, s, keys_in, n_dev, passes, tmp.hist);
     uint32_t* ks = keys_in;
     uint32_t* vs = vals_in;
     for (int p = 0; p < passes; ++p) {
@@ -431,8 +396,9 @@ void radix_sort_pairs(uint32_t* keys_in, uint32_t* vals_in, uint32_t* keys_out, 
         if (p == passes - 1) { kd = keys_out; vd = vals_out; }
         else if ((p & 1) == 0) { kd = tmp.keys_tmp; vd = tmp.vals_tmp; }
         else { kd = keys_in; vd = vals_in; }
-        hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(kBlock), 0, s, ks, vs, kd, vd, n_dev, 8 * p,
-                           tmp.hist + p * kRadix, tmp.chain);
+        hipLaunchKernelGGL(k_sort_count, dim3(g), dim3(kBlock), 0, s, ks, n_dev, 8 * p, counts);
+        scan_exclusive_1p(counts, offsets, 1, m, nullptr, nullptr, nullptr, st, s);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(g), dim3(kBlock), 0, s, ks, vs, kd, vd, n_dev, 8 * p, offsets);
         ks = kd;
         vs = vd;
     }
