@@ -3,7 +3,7 @@ ray tracer: key-frame interpolation, uniform / two-level grid builds, 3D-DDA
 traversal with Moller-Trumbore intersection, ambient occlusion and path
 tracing, behind the C ABI in include/happyray.h.
 """
-from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, MODE_PATH_MULTI,
+from ._lib import (GRID_HIERARCHY, GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, MODE_PATH_MULTI,
                    MODE_SIMPLE, HappyRayError, LIB_PATH, load)
 from .engine import Engine
 from .scenefile import SceneFile
@@ -11,4 +11,4 @@ from .batch import Batch
 from .server import Server
 
 __all__ = ["Engine", "Batch", "Server", "SceneFile", "HappyRayError", "load", "LIB_PATH", "GRID_UNIFORM", "GRID_UNIFORM_EXACT",
-           "GRID_TWO_LEVEL", "MODE_AO", "MODE_PATH", "MODE_PATH_MULTI", "MODE_SIMPLE"]
+           "GRID_TWO_LEVEL", "GRID_HIERARCHY", "MODE_AO", "MODE_PATH", "MODE_PATH_MULTI", "MODE_SIMPLE"]

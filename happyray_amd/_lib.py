@@ -16,6 +16,7 @@ STATUS_NAMES = {0: "HR_OK", 1: "HR_ERR_ARG", 2: "HR_ERR_DEVICE", 3: "HR_ERR_OOM"
 GRID_UNIFORM = 0
 GRID_UNIFORM_EXACT = 1
 GRID_TWO_LEVEL = 2
+GRID_HIERARCHY = 3
 MODE_SIMPLE = 0
 MODE_PATH_MULTI = 1
 MODE_PATH = 2
@@ -36,6 +37,18 @@ class GridInfo(C.Structure):
                 ("res", C.c_int * 3), ("cell_size", C.c_float * 3), ("cell_size_rcp", C.c_float * 3),
                 ("num_cells", C.c_uint32), ("num_leaves", C.c_uint32), ("num_refs", C.c_uint32),
                 ("num_top_pairs", C.c_uint32), ("num_triangles", C.c_uint32)]
+
+
+class Instance(C.Structure):
+    """hr_instance: object id, world-to-object rows m_row_0..2, world box."""
+    _fields_ = [("object", C.c_uint32), ("m", (C.c_float * 4) * 3), ("aabb_min", C.c_float * 3),
+                ("aabb_max", C.c_float * 3)]
+
+
+class ObjectGrid(C.Structure):
+    _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("res", C.c_int * 3),
+                ("cell_size", C.c_float * 3), ("cell_size_rcp", C.c_float * 3), ("first_cell", C.c_uint32),
+                ("num_cells", C.c_uint32)]
 
 
 class RenderParams(C.Structure):
@@ -136,6 +149,10 @@ SIGNATURES = {
     "hr_build": (C.c_int, [_vp]),
     "hr_get_grid_info": (C.c_int, [_vp, C.POINTER(GridInfo)]),
     "hr_get_grid": (C.c_int, [_vp, _u32p, _u32p, _u32p]),
+    "hr_set_instances": (C.c_int, [_vp, _u32p, C.c_uint32, C.POINTER(Instance), C.c_uint32]),
+    "hr_get_object_grids": (C.c_int, [_vp, C.POINTER(ObjectGrid), _u32p]),
+    "hr_trace_instanced": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, _f32p, _u32p, _u32p]),
+    "hr_get_primary_instances": (C.c_int, [_vp, _u32p]),
     "hr_set_camera": (C.c_int, [_vp, _f32p, _f32p, _f32p, C.c_float, C.c_int, C.c_int]),
     "hr_set_lights": (C.c_int, [_vp, _f32p, _f32p, C.c_int]),
     "hr_render": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(RenderParams), _f32p]),

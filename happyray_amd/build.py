@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libhappyray.so")
 CLI = os.path.join(HERE, "happyray")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_trace.hip", "hr_pathmk.hip", "hr_persist.hip", "hr_engine.hip",
+SOURCES = ["hr_sort.hip", "hr_build.hip", "hr_hier.hip", "hr_trace.hip", "hr_pathmk.hip", "hr_persist.hip", "hr_engine.hip",
            "hr_batch.hip", "hr_server.hip"]
 CXX_SOURCES = ["hr_scene.cpp", "hr_image.cpp"]   # host-only C++ (scene files, images)
 ARCH = os.environ.get("HR_OFFLOAD_ARCH", "gfx950")

@@ -57,6 +57,8 @@ struct SceneTable {
     uint32_t* top_base;          // ns + 1: first top-level pair of each scene
     uint32_t* leaf_base;         // ns + 1: first leaf of each scene
     uint32_t* pair_base;         // ns + 1: first leaf-level reference of each scene
+    int object_grids;            // 1: the object grids of an instanced hierarchy, whose cell
+                                 // ranges clamp the first cell to res - 1 and the end to >= 1
 };
 
 void launch_reset(BuildStatus* st, hipStream_t s);

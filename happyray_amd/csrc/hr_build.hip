@@ -31,6 +31,7 @@
 //  * triangles are re-laid once per build as three float4 per triangle
 //    (vertex xyz), the record every later pass and the traversal read.
 #include "hr_build.h"
+#include "hr_hier.h"
 
 namespace hr {
 
@@ -304,7 +305,8 @@ __global__ __launch_bounds__(kBlock) void k_top_pairs(
         }
         f3 lo, hi;
         tri_bounds(v0, v1, v2, lo, hi);
-        box_cells(lo, hi, bmin, csr, rx, ry, rz, mn, mx);
+        if (kMulti && T.object_grids) object_box_cells(lo, hi, bmin, csr, rx, ry, rz, mn, mx);
+        else box_cells(lo, hi, bmin, csr, rx, ry, rz, mn, mx);
         cnt = (uint32_t)((mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]));
     }
 
@@ -848,6 +850,260 @@ void run_build_batch(const BuildArgs& a, BuildBuffers& b, const SceneTable& T, u
     hipLaunchKernelGGL(k_ranges, dim3(lrblocks), dim3(256), 0, s, b.keys_sorted, &st->n_leaf_pairs_eff, b.leaves,
                        st);
     hipLaunchKernelGGL(k_batch_seal, dim3(sblocks), dim3(256), 0, s, T, st);
+}
+
+}  // namespace hr
+
+// ---------------------------------------------------------------------------
+// instanced hierarchy: object grids, then a top grid over instance boxes
+// ---------------------------------------------------------------------------
+//
+// The object grids reuse the pair kernels of batched builds (one "scene" per
+// object, bounding-box insertion with the object-grid clamp, global cell key =
+// first cell of the object + local cell); the instance boxes are then
+// inserted into the top grid.  Both levels go through the stable radix sort,
+// so every cell lists its ids in ascending order.
+
+namespace hr {
+
+namespace {
+
+// resolution along one axis from diagonal * magic: values below 1 (and NaN)
+// give 1; -1 marks a resolution too large for an int
+__device__ __forceinline__ int hier_res(float r)
+{
+    if (r >= 2147483520.f) return -1;
+    return r >= 1.f ? (int)r : 1;
+}
+
+__device__ void hier_grid(f3 lo, f3 hi, float lambda, float count, GridParams* g, uint64_t* ncells)
+{
+    const f3 d = hi - lo;
+    const float volume = d.x * d.y * d.z;
+    const float magic = spec_powf(lambda * count / volume, 0.3333333f);
+    const f3 r = d * magic;
+    int res[3] = { hier_res(r.x), hier_res(r.y), hier_res(r.z) };
+    const bool bad = res[0] < 0 || res[1] < 0 || res[2] < 0;
+    if (bad) res[0] = res[1] = res[2] = 1;
+    const f3 rf = mk((float)res[0], (float)res[1], (float)res[2]);
+    const f3 cs = d / rf, csr = rf / d;
+    g->bmin[0] = lo.x; g->bmin[1] = lo.y; g->bmin[2] = lo.z;
+    g->bmax[0] = hi.x; g->bmax[1] = hi.y; g->bmax[2] = hi.z;
+    g->res[0] = res[0]; g->res[1] = res[1]; g->res[2] = res[2];
+    g->cs[0] = cs.x; g->cs[1] = cs.y; g->cs[2] = cs.z;
+    g->csr[0] = csr.x; g->csr[1] = csr.y; g->csr[2] = csr.z;
+    g->fallback = 0u;
+    g->pad[0] = g->pad[1] = 0u;
+    *ncells = bad ? ~0ull : (uint64_t)res[0] * (uint64_t)res[1] * (uint64_t)res[2];
+}
+
+// top grid: the scene bounds padded by 0.001 |diagonal| (WFObjectUploader.h:
+// 112-114), resolution max(1, diagonal * (lambda * #instances / volume)^0.3333333)
+// (TLGridHierarchySortBuilder.cu:275-291), cell size diagonal / resolution
+// (TLGridHierarchyMemoryManager.h:111-119)
+__global__ void k_hier_top_setup(GridParams* __restrict__ gp, BuildStatus* __restrict__ st, uint32_t ninst,
+                                 float lambda, uint64_t cells_cap)
+{
+    if (threadIdx.x != 0) return;
+    f3 lo = mk(dec_f(st->bounds_enc[0]), dec_f(st->bounds_enc[1]), dec_f(st->bounds_enc[2]));
+    f3 hi = mk(dec_f(st->bounds_enc[3]), dec_f(st->bounds_enc[4]), dec_f(st->bounds_enc[5]));
+    const float diag = len(hi - lo);
+    lo = lo - rep(0.001f * diag);
+    hi = hi + rep(0.001f * diag);
+    uint64_t nc;
+    hier_grid(lo, hi, lambda, (float)ninst, gp, &nc);
+    if (nc > cells_cap || nc > 0xFFFFFFF0ull) {
+        st->overflow |= kOverflowCells;
+        st->num_cells = nc > 0xFFFFFFF0ull ? 0xFFFFFFFFu : (uint32_t)nc;
+        gp->num_cells = 0u;
+    } else {
+        st->num_cells = (uint32_t)nc;
+        gp->num_cells = (uint32_t)nc;
+    }
+}
+
+__global__ void k_obj_reset(uint32_t* __restrict__ enc, uint32_t nobj)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nobj) return;
+    for (int k = 0; k < 3; ++k) {
+        enc[6 * (size_t)i + k] = 0xFFFFFFFFu;
+        enc[6 * (size_t)i + 3 + k] = 0u;
+    }
+}
+
+// bounds of each object's triangles on the first key frame of the current
+// frame (RTEngine.cu:270-289); a wave whose 64 triangles share an object
+// reduces them and issues one atomic per bound
+__global__ __launch_bounds__(kBlock) void k_obj_bounds(const uint32_t* __restrict__ idx,
+                                                       const float* __restrict__ kx, const float* __restrict__ ky,
+                                                       const float* __restrict__ kz, uint32_t nt,
+                                                       const uint32_t* __restrict__ tri_obj,
+                                                       uint32_t* __restrict__ enc)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~(uint32_t)(kWave - 1); base < nt;
+         base += stride) {
+        const uint32_t t = base + __lane_id();
+        const bool valid = t < nt;
+        f3 lo = rep(kFltMax), hi = rep(-kFltMax);
+        uint32_t ob = 0;
+        if (valid) {
+            ob = tri_obj[t];
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t v = idx[3 * (size_t)t + c];
+                const f3 p = mk(kx[v], ky[v], kz[v]);
+                lo = vmin(lo, p);
+                hi = vmax(hi, p);
+            }
+        }
+        const uint32_t last = base + kWave - 1 < nt ? base + kWave - 1 : nt - 1;
+        const uint32_t o0 = tri_obj[base], o1 = tri_obj[last];
+        if (o0 == o1) {
+            lo.x = wave_min(lo.x); lo.y = wave_min(lo.y); lo.z = wave_min(lo.z);
+            hi.x = wave_max(hi.x); hi.y = wave_max(hi.y); hi.z = wave_max(hi.z);
+            if (__lane_id() == 0) {
+                uint32_t* b = enc + 6 * (size_t)o0;
+                atomicMin(b + 0, enc_f(lo.x)); atomicMin(b + 1, enc_f(lo.y)); atomicMin(b + 2, enc_f(lo.z));
+                atomicMax(b + 3, enc_f(hi.x)); atomicMax(b + 4, enc_f(hi.y)); atomicMax(b + 5, enc_f(hi.z));
+            }
+        } else if (valid) {
+            uint32_t* b = enc + 6 * (size_t)ob;
+            atomicMin(b + 0, enc_f(lo.x)); atomicMin(b + 1, enc_f(lo.y)); atomicMin(b + 2, enc_f(lo.z));
+            atomicMax(b + 3, enc_f(hi.x)); atomicMax(b + 4, enc_f(hi.y)); atomicMax(b + 5, enc_f(hi.z));
+        }
+    }
+}
+
+// object grid: bounds padded by 10 EPS (RTEngine.cu:291-296; an object without
+// triangles keeps the inverted FLT_MAX box), resolution from the leaf density
+// and the object's triangle count (RTEngine.cu:312-326, countGridCellsMultiUniformGrid
+// TLGridHierarchySortBuilder.cu:29-68)
+__global__ void k_obj_setup(const uint32_t* __restrict__ enc, const uint32_t* __restrict__ obj_first,
+                            uint32_t nobj, float lambda, GridParams* __restrict__ gps,
+                            uint32_t* __restrict__ cell_counts, BuildStatus* __restrict__ st)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nobj) return;
+    const uint32_t count = obj_first[k + 1] - obj_first[k];
+    f3 lo = rep(kFltMax), hi = rep(-kFltMax);
+    if (count > 0) {
+        const uint32_t* e = enc + 6 * (size_t)k;
+        lo = mk(dec_f(e[0]), dec_f(e[1]), dec_f(e[2]));
+        hi = mk(dec_f(e[3]), dec_f(e[4]), dec_f(e[5]));
+    }
+    const float pad = 10.f * kEps;
+    lo = mk(lo.x - pad, lo.y - pad, lo.z - pad);
+    hi = mk(hi.x + pad, hi.y + pad, hi.z + pad);
+    uint64_t nc;
+    hier_grid(lo, hi, lambda, (float)count, gps + k, &nc);
+    if (nc > 0x0FFFFFFFull) {
+        // one object's grid alone beyond 2^28 cells: no capacity growth helps
+        atomicOr(&st->overflow, kOverflowCells);
+        st->num_cells = 0xFFFFFFFFu;
+        nc = 0;
+    }
+    gps[k].num_cells = (uint32_t)nc;
+    cell_counts[k] = stop(st) ? 0u : (uint32_t)nc;
+}
+
+// countPairs / writeKeysAndValues over instance boxes
+// (UniformGridBuildKernels.h:40-105, :180-252): one lane per instance
+template <bool kWrite>
+__global__ void k_inst_pairs(const InstanceDev* __restrict__ inst, uint32_t ninst,
+                             const GridParams* __restrict__ gp, const BuildStatus* __restrict__ st,
+                             uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    if (stop(st)) {
+        if (!kWrite) counts[i] = 0u;
+        return;
+    }
+    const InstanceDev I = inst[i];
+    const int rx = gp->res[0], ry = gp->res[1], rz = gp->res[2];
+    int mn[3], mx[3];
+    box_cells(mk(I.bmin[0], I.bmin[1], I.bmin[2]), mk(I.bmax[0], I.bmax[1], I.bmax[2]),
+              mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]), mk(gp->csr[0], gp->csr[1], gp->csr[2]), rx, ry, rz, mn,
+              mx);
+    if (!kWrite) {
+        counts[i] = (uint32_t)((mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]));
+        return;
+    }
+    uint32_t pos = offsets[i];
+    const uint32_t sxy = (uint32_t)rx * (uint32_t)ry;
+    for (int z = mn[2]; z < mx[2]; ++z)
+        for (int y = mn[1]; y < mx[1]; ++y)
+            for (int x = mn[0]; x < mx[0]; ++x, ++pos) {
+                keys[pos] = (uint32_t)x + (uint32_t)y * (uint32_t)rx + (uint32_t)z * sxy;
+                vals[pos] = i;
+            }
+}
+
+inline uint32_t range_blocks(uint32_t cap)
+{
+    const uint32_t b = cdiv(cap > 0 ? cap : 1, 256);
+    return b > 4096 ? 4096 : b;
+}
+
+}  // namespace
+
+void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s)
+{
+    BuildStatus* st = b.status;
+    const uint32_t nt = a.nt;
+    const uint32_t nob = a.nobj;
+    const uint32_t oblocks = cdiv(nob > 0 ? nob : 1, 256);
+
+    // ---- grids: top, then one per object (counts scanned into first cells) ----
+    hipLaunchKernelGGL(k_hier_top_setup, dim3(1), dim3(64), 0, s, b.top, st, a.ninst, a.lambda_top, b.cells_cap);
+    hipLaunchKernelGGL(k_obj_reset, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, nob);
+    if (nt > 0) {
+        uint32_t tblocks = cdiv(nt, kBlock);
+        if (tblocks > 2048) tblocks = 2048;
+        hipLaunchKernelGGL(k_obj_bounds, dim3(tblocks), dim3(kBlock), 0, s, a.idx, a.kx, a.ky, a.kz, nt, a.tri_obj,
+                           b.obj_bounds_enc);
+    }
+    hipLaunchKernelGGL(k_obj_setup, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, a.obj_first, nob,
+                       a.lambda_leaf, b.obj_gp, b.obj_cell_counts, st);
+    const ScanCheck cells_check{ st, b.leaves_cap, kOverflowLeaves, &st->n_leaves_eff };
+    scan_exclusive_1p(b.obj_cell_counts, b.obj_cell_base, 1, nob, nullptr, &st->num_leaves, &cells_check, b.scan, s);
+
+    // ---- object references: count (and clear the object cells), scan, write, sort, ranges ----
+    SceneTable T{};
+    T.ns = nob;
+    T.tri_scene = a.tri_obj;
+    T.gps = b.obj_gp;
+    T.cell_base = b.obj_cell_base;
+    T.object_grids = 1;
+    const uint32_t waves = cdiv(nt > 0 ? nt : 1, kWave);
+    const uint32_t blocks = cdiv(waves, kWavesPerBlock);
+    hipLaunchKernelGGL((k_top_pairs<false, false, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx, a.vy, a.vz,
+                       nt, b.top, st, b.rec, b.wave_counts, nullptr, nullptr, nullptr, b.waves_cap, T, b.obj_cells,
+                       &st->n_leaves_eff);
+    const ScanCheck refs_check{ st, b.leaf_pairs_cap, kOverflowLeafPairs, &st->n_leaf_pairs_eff };
+    scan_exclusive_1p(b.wave_counts, b.wave_offsets, 1, waves, nullptr, &st->num_leaf_pairs, &refs_check, b.scan, s);
+    hipLaunchKernelGGL((k_top_pairs<false, true, true>), dim3(blocks), dim3(kBlock), 0, s, a.idx, a.vx, a.vy, a.vz,
+                       nt, b.top, st, b.rec, nullptr, b.wave_offsets, b.keys, b.vals, b.waves_cap, T);
+    radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.prims, b.leaf_pairs_cap, &st->n_leaf_pairs_eff,
+                     bits_for(b.leaves_cap), b.sort, s);
+    hipLaunchKernelGGL(k_ranges, dim3(range_blocks(b.leaf_pairs_cap)), dim3(256), 0, s, b.keys_sorted,
+                       &st->n_leaf_pairs_eff, b.obj_cells, st, nullptr);
+
+    // ---- top references: count, scan, write, sort, clear the top cells, ranges (and seal) ----
+    const uint32_t iblocks = cdiv(a.ninst > 0 ? a.ninst : 1, 256);
+    hipLaunchKernelGGL((k_inst_pairs<false>), dim3(iblocks), dim3(256), 0, s, a.inst, a.ninst, b.top, st,
+                       b.inst_counts, nullptr, nullptr, nullptr);
+    const ScanCheck inst_check{ st, b.pairs_cap, kOverflowPairs, &st->n_pairs_eff };
+    scan_exclusive_1p(b.inst_counts, b.inst_offsets, 1, a.ninst, nullptr, &st->num_pairs, &inst_check, b.scan, s);
+    hipLaunchKernelGGL((k_inst_pairs<true>), dim3(iblocks), dim3(256), 0, s, a.inst, a.ninst, b.top, st, nullptr,
+                       b.inst_offsets, b.keys, b.vals);
+    radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.inst_refs, b.pairs_cap, &st->n_pairs_eff,
+                     bits_for(b.cells_cap), b.sort, s);
+    fill_u32((uint32_t*)b.top_cells, 0u, 2 * (size_t)b.cells_cap, &b.top->num_cells, 2, s);
+    hipLaunchKernelGGL(k_ranges, dim3(range_blocks(b.pairs_cap)), dim3(256), 0, s, b.keys_sorted, &st->n_pairs_eff,
+                       b.top_cells, st, b.top);
 }
 
 }  // namespace hr

@@ -160,6 +160,35 @@ __device__ __forceinline__ void box_cells(f3 lo, f3 hi, f3 origin, f3 rcp, int r
     mx[2] = mx[2] < mn[2] ? mn[2] : mx[2];
 }
 
+// float -> int conversion that saturates (NaN -> the lower bound): defined for
+// every input on both the GPU and the host
+HR_HD int to_int_sat(float x)
+{
+    return (int)fminf(fmaxf(x, -2147483520.f), 2147483520.f);
+}
+
+// cell range of a box in an object grid of the instanced hierarchy
+// (countPairsMultiUniformGrid, TLGridHierarchySortBuilder.cu:131-147): the
+// first cell is clamped to [0, res - 1] and the end to [1, res], so a triangle
+// that has moved outside its object's grid still lands in a border cell
+__device__ __forceinline__ void object_box_cells(f3 lo, f3 hi, f3 origin, f3 rcp, int rx, int ry, int rz,
+                                                 int mn[3], int mx[3])
+{
+    const f3 a = (lo - origin) * rcp;
+    const f3 b = (hi - origin) * rcp + rep(1.f);
+    const int r[3] = { rx, ry, rz };
+    const float av[3] = { a.x, a.y, a.z }, bv[3] = { b.x, b.y, b.z };
+    for (int i = 0; i < 3; ++i) {
+        int lo_i = to_int_sat(av[i]);
+        lo_i = lo_i > 0 ? lo_i : 0;
+        mn[i] = lo_i < r[i] - 1 ? lo_i : r[i] - 1;
+        int hi_i = to_int_sat(bv[i]);
+        hi_i = hi_i < r[i] ? hi_i : r[i];
+        hi_i = hi_i > 1 ? hi_i : 1;
+        mx[i] = hi_i < mn[i] ? mn[i] : hi_i;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host launch helpers (hr_build.hip, hr_sort.hip, hr_trace.hip)
 // ---------------------------------------------------------------------------

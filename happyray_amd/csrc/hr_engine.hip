@@ -21,6 +21,7 @@
 #include "hr_build.h"
 #include "hr_ctx_internal.h"
 #include "hr_error.h"
+#include "hr_hier.h"
 #include "hr_host.h"
 #include "hr_persist.h"
 #include "hr_trace.h"
@@ -83,6 +84,16 @@ struct hr_ctx {
     uint32_t pairs_cap = 0, leaves_cap = 0, leaf_pairs_cap = 0;
     bool built = false;
     hr_grid_info info{};
+
+    // instanced hierarchy (HR_GRID_HIERARCHY): objects are triangle ranges
+    // [obj_first[k], obj_first[k+1]) starting at triangle 0; its build reuses
+    // cells (top grid), top_vals_sorted (instance references), leaves (object
+    // cells) and prims (object references)
+    uint32_t nobj = 0, ninst = 0, obj_tris = 0;
+    DevBuf<uint32_t> tri_obj, obj_first, obj_bounds_enc, obj_cell_counts, obj_cell_base, inst_counts, inst_offsets;
+    DevBuf<hr::InstanceDev> inst;
+    DevBuf<hr::GridParams> obj_gp;
+    DevBuf<uint32_t> hit_inst;
     hr::GridParams h_params{};
 
     // camera / lights
@@ -169,7 +180,8 @@ hr_status validate_indices(const uint32_t* idx, uint32_t nt, uint32_t nv)
 hr_status reserve_build(hr_ctx* c)
 {
     const uint32_t nt = c->nt;
-    const bool two = c->kind == HR_GRID_TWO_LEVEL;
+    const bool hier = c->kind == HR_GRID_HIERARCHY;
+    const bool two = c->kind == HR_GRID_TWO_LEVEL || hier;
     const uint32_t pair_max = std::max(c->pairs_cap, two ? c->leaf_pairs_cap : 0u);
     HR_TRY(c->rec.reserve(3 * (size_t)std::max(nt, 1u)));
     const uint32_t waves = cdiv(std::max<uint64_t>(std::max(nt, c->pairs_cap), 1), 64);
@@ -183,10 +195,19 @@ hr_status reserve_build(hr_ctx* c)
     HR_TRY(c->cells.reserve(c->cells_cap));
     if (two) {
         HR_TRY(c->top_vals_sorted.reserve(c->pairs_cap));
-        HR_TRY(c->leaf_counts.reserve(c->cells_cap));
+        if (!hier) HR_TRY(c->leaf_counts.reserve(c->cells_cap));
         HR_TRY(c->leaves.reserve(c->leaves_cap));
     }
-    const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)c->cells_cap});
+    if (hier) {
+        HR_TRY(c->obj_gp.reserve(c->nobj));
+        HR_TRY(c->obj_bounds_enc.reserve(6 * (size_t)c->nobj));
+        HR_TRY(c->obj_cell_counts.reserve(c->nobj));
+        HR_TRY(c->obj_cell_base.reserve(c->nobj));
+        HR_TRY(c->inst_counts.reserve(c->ninst));
+        HR_TRY(c->inst_offsets.reserve(c->ninst));
+    }
+    const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)c->cells_cap, (size_t)c->nobj,
+                                             (size_t)c->ninst });
     {
         // the single-pass scan's tile flags must start cleared
         const size_t before = c->scan_chain.cap;
@@ -245,6 +266,57 @@ void enqueue_build(hr_ctx* c, const hr::FrameSel* fs = nullptr)
                                c->keyframes.p);
     else
         hr::launch_lerp_bounds(c->src1, c->src2, c->coeff, c->nv, c->verts.p, c->status.p, c->stream);
+    if (c->kind == HR_GRID_HIERARCHY) {
+        hr::HierArgs a;
+        a.idx = c->idx.p;
+        a.vx = c->verts.p;
+        a.vy = c->verts.p + c->nv;
+        a.vz = c->verts.p + 2 * (size_t)c->nv;
+        // object bounds come from the first key frame of the pair (RTEngine.cu:270-289)
+        a.kx = c->src1;
+        a.ky = c->src1 + c->nv;
+        a.kz = c->src1 + 2 * (size_t)c->nv;
+        a.nt = c->obj_tris;
+        a.nobj = c->nobj;
+        a.tri_obj = c->tri_obj.p;
+        a.obj_first = c->obj_first.p;
+        a.ninst = c->ninst;
+        a.inst = c->inst.p;
+        a.lambda_top = c->lambda_top;
+        a.lambda_leaf = c->lambda_leaf;
+        hr::HierBuffers b;
+        b.status = c->status.p;
+        b.top = c->params.p;
+        b.obj_gp = c->obj_gp.p;
+        b.obj_bounds_enc = c->obj_bounds_enc.p;
+        b.obj_cell_counts = c->obj_cell_counts.p;
+        b.obj_cell_base = c->obj_cell_base.p;
+        b.rec = c->rec.p;
+        b.wave_counts = c->wave_counts.p;
+        b.wave_offsets = c->wave_offsets.p;
+        b.waves_cap = (uint32_t)c->wave_counts.cap;
+        b.keys = c->keys.p;
+        b.vals = c->vals.p;
+        b.keys_sorted = c->keys_sorted.p;
+        b.prims = c->prims.p;
+        b.obj_cells = c->leaves.p;
+        b.inst_counts = c->inst_counts.p;
+        b.inst_offsets = c->inst_offsets.p;
+        b.top_cells = c->cells.p;
+        b.inst_refs = c->top_vals_sorted.p;
+        b.cells_cap = c->cells_cap;
+        b.pairs_cap = c->pairs_cap;
+        b.leaves_cap = c->leaves_cap;
+        b.leaf_pairs_cap = c->leaf_pairs_cap;
+        b.scan.chain = c->scan_chain.p;
+        b.scan.chain_cap = c->scan_chain.cap;
+        b.sort.keys_tmp = c->sort_keys_tmp.p;
+        b.sort.vals_tmp = c->sort_vals_tmp.p;
+        b.sort.hist = c->sort_hist.p;
+        b.sort.chain = c->sort_chain.p;
+        hr::run_build_hier(a, b, c->stream);
+        return;
+    }
     hr::BuildArgs a;
     a.kind = c->kind;
     a.lambda_top = c->lambda_top;
@@ -267,7 +339,7 @@ uint32_t grow(uint64_t cur, uint64_t need)
 // grid description after a successful build (c->h_params holds the parameters)
 void set_info(hr_ctx* c, const hr::BuildStatus& st)
 {
-    const bool two = c->kind == HR_GRID_TWO_LEVEL;
+    const bool two = c->kind == HR_GRID_TWO_LEVEL || c->kind == HR_GRID_HIERARCHY;
     hr_grid_info& in = c->info;
     in.kind = c->kind;
     for (int i = 0; i < 3; ++i) {
@@ -296,13 +368,17 @@ void drop_graphs(hr_ctx* c)
 hr_status do_build(hr_ctx* c)
 {
     if (!c->src1 || c->nv == 0) return fail(HR_ERR_STATE, "no geometry uploaded");
-    const bool two = c->kind == HR_GRID_TWO_LEVEL;
+    const bool hier = c->kind == HR_GRID_HIERARCHY;
+    const bool two = c->kind == HR_GRID_TWO_LEVEL || hier;
+    if (hier && c->ninst == 0) return fail(HR_ERR_STATE, "the instanced hierarchy needs hr_set_instances");
+    if (hier && c->obj_tris > c->nt) return fail(HR_ERR_STATE, "the objects cover more triangles than the mesh has");
     // resolution product <= lambda * N (up to rounding) or the 32^3 fallback
+    const uint32_t top_items = hier ? c->ninst : c->nt;
     const uint64_t cells_bound =
-        std::max<uint64_t>(32768, (uint64_t)((double)c->lambda_top * (double)c->nt * 1.001) + 64);
+        std::max<uint64_t>(32768, (uint64_t)((double)c->lambda_top * (double)top_items * 1.001) + 64);
     if (cells_bound > 0xFFFFFFF0ull) return fail(HR_ERR_CAPACITY, "grid density too high");
     if (c->cells_cap < cells_bound) c->cells_cap = cells_bound;
-    if (c->pairs_cap == 0) c->pairs_cap = std::max<uint32_t>(4096, 4 * c->nt);
+    if (c->pairs_cap == 0) c->pairs_cap = std::max<uint32_t>(4096, 4 * top_items);
     if (two) {
         if (c->leaves_cap == 0) c->leaves_cap = std::max<uint32_t>(65536, 8 * c->nt);
         if (c->leaf_pairs_cap == 0) c->leaf_pairs_cap = std::max<uint32_t>(65536, 8 * c->nt);
@@ -343,6 +419,31 @@ hr_status do_build(hr_ctx* c)
         }
     }
     return fail(HR_ERR_CAPACITY, "grid build did not fit after repeated growth");
+}
+
+hr::HierScene hier_scene(hr_ctx* c)
+{
+    hr::HierScene H;
+    H.top = c->params.p;
+    H.top_cells = c->cells.p;
+    H.inst_refs = c->top_vals_sorted.p;
+    H.inst = c->inst.p;
+    H.obj_gp = c->obj_gp.p;
+    H.obj_cell_base = c->obj_cell_base.p;
+    H.obj_cells = c->leaves.p;
+    H.prims = c->prims.p;
+    H.rec = c->rec.p;
+    H.nt = c->nt;
+    return H;
+}
+
+// entry points that the instanced hierarchy does not serve: the reference's
+// hierarchy build renders ambient occlusion only (RTEngine.cu:400-404)
+hr_status no_hierarchy(hr_ctx* c, const char* what)
+{
+    if (c->kind == HR_GRID_HIERARCHY)
+        return fail(HR_ERR_STATE, std::string(what) + " is not available on the instanced hierarchy");
+    return HR_OK;
 }
 
 hr::TraceScene trace_scene(hr_ctx* c)
@@ -397,6 +498,8 @@ hr_status check_render_args(hr_ctx* c, hr_mode mode, int image_id, const hr_rend
     if (mode != HR_MODE_AO && mode != HR_MODE_PATH && mode != HR_MODE_SIMPLE && mode != HR_MODE_PATH_MULTI)
         return fail(HR_ERR_ARG, "unknown render mode");
     if (image_id < 0) return fail(HR_ERR_ARG, "negative image id");
+    if (c->kind == HR_GRID_HIERARCHY && mode != HR_MODE_AO)
+        return fail(HR_ERR_STATE, "the instanced hierarchy renders ambient occlusion only");
     if (mode == HR_MODE_AO && (p->ao_samples < 1 || p->ao_samples > 256))
         return fail(HR_ERR_ARG, "ao_samples must be in [1, 256]");
     if (mode == HR_MODE_PATH && (p->pt_depth < 1 || p->pt_depth > 64))
@@ -412,6 +515,7 @@ hr_status prepare_render(hr_ctx* c, hr_mode mode, const hr_render_params* p)
     HR_TRY(c->hit_prims.reserve(npix));
     HR_TRY(c->fb.reserve(3 * (size_t)npix));
     if (mode == HR_MODE_AO) HR_TRY(c->occ.reserve((size_t)npix * (size_t)p->ao_samples));
+    if (c->kind == HR_GRID_HIERARCHY) HR_TRY(c->hit_inst.reserve(npix));
     if (mode == HR_MODE_PATH_MULTI) {
         const size_t before = c->pm.cap;
         HR_TRY(c->pm.reserve(28 * (size_t)npix));
@@ -491,6 +595,19 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         sh.nidx = c->nidx.p;
     }
     if (events) HR_HIP(hipEventRecord(c->ev_r0, c->stream));
+    if (c->kind == HR_GRID_HIERARCHY) {
+        // TLGHAOIntegrator::integrate (TLGridHierarchyAOIntegrator.h:421-573)
+        hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
+                                  c->fb.p, c->ray_count.p, c->stream);
+        if (events) {
+            HR_HIP(hipEventRecord(c->ev_p1, c->stream));
+            HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+            HR_HIP(hipEventRecord(c->ev_r1, c->stream));
+        }
+        c->last_ao_samples = p->ao_samples;
+        HR_HIP(hipGetLastError());
+        return HR_OK;
+    }
     if (mode == HR_MODE_PATH_MULTI && c->nlights == 0) {
         // PathTracer::integrate returns at once without lights (PathTracer.h:1049-1053)
         if (events) {
@@ -655,7 +772,9 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     b += c->sort_hist.bytes() + c->lights.bytes() + c->light_cdf.bytes() + c->hits.bytes() + c->fb.bytes();
     b += c->hit_prims.bytes() + c->occ.bytes() + c->ray_org.bytes() + c->ray_dir.bytes() + c->ray_t.bytes();
     b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes() + c->pm.bytes() + c->pm_prims.bytes();
-    b += c->ray_wire.bytes();
+    b += c->ray_wire.bytes() + c->tri_obj.bytes() + c->obj_first.bytes() + c->obj_bounds_enc.bytes();
+    b += c->obj_cell_counts.bytes() + c->obj_cell_base.bytes() + c->inst_counts.bytes() + c->inst_offsets.bytes();
+    b += c->inst.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes();
     return b;
 }
 
@@ -663,6 +782,7 @@ hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims
                              hipStream_t s)
 {
     if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
+    HR_TRY(no_hierarchy(c, "path tracing"));
     if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
     if (n == 0) return HR_OK;
     HR_TRY(ensure_default_material(c));
@@ -775,10 +895,14 @@ hr_status hr_destroy(hr_ctx* c)
                                   &c->keys, &c->vals,
                                   &c->keys_sorted, &c->top_vals_sorted, &c->prims, &c->leaf_counts,
                                   &c->sort_chain, &c->sort_keys_tmp, &c->sort_vals_tmp, &c->sort_hist,
-                                  &c->hit_prims, &c->ray_prim };
+                                  &c->hit_prims, &c->ray_prim, &c->tri_obj, &c->obj_first,
+                                  &c->obj_bounds_enc, &c->obj_cell_counts, &c->obj_cell_base, &c->inst_counts,
+                                  &c->inst_offsets, &c->hit_inst };
     for (auto* b : ubufs) b->release();
     c->occ.release();
     c->ray_occ.release();
+    c->inst.release();
+    c->obj_gp.release();
     c->rec.release();
     c->cells.release();
     c->leaves.release();
@@ -931,13 +1055,15 @@ hr_status hr_set_normals(hr_ctx* c, const float* normals, uint32_t nframes, uint
 hr_status hr_set_grid(hr_ctx* c, hr_grid_kind kind, float lambda_top, float lambda_leaf)
 {
     HR_TRY(check_ctx(c));
-    if (kind != HR_GRID_UNIFORM && kind != HR_GRID_UNIFORM_EXACT && kind != HR_GRID_TWO_LEVEL)
+    if (kind != HR_GRID_UNIFORM && kind != HR_GRID_UNIFORM_EXACT && kind != HR_GRID_TWO_LEVEL &&
+        kind != HR_GRID_HIERARCHY)
         return fail(HR_ERR_ARG, "unknown grid kind");
     if (!(lambda_top > 0.f) || !(lambda_top < 1e6f)) return fail(HR_ERR_ARG, "lambda_top out of range");
-    if (kind == HR_GRID_TWO_LEVEL && (!(lambda_leaf > 0.f) || !(lambda_leaf < 1e6f)))
+    if ((kind == HR_GRID_TWO_LEVEL || kind == HR_GRID_HIERARCHY) && (!(lambda_leaf > 0.f) || !(lambda_leaf < 1e6f)))
         return fail(HR_ERR_ARG, "lambda_leaf out of range");
     if (kind != c->kind) {
         c->pairs_cap = c->leaves_cap = c->leaf_pairs_cap = 0;
+        drop_graphs(c);
     }
     c->kind = kind;
     c->lambda_top = lambda_top;
@@ -969,7 +1095,7 @@ hr_status hr_get_grid(hr_ctx* c, uint32_t* cells, uint32_t* leaves, uint32_t* pr
     if (cells)
         HR_HIP(hipMemcpy(cells, c->cells.p, 2 * (size_t)c->info.num_cells * sizeof(uint32_t),
                          hipMemcpyDeviceToHost));
-    if (leaves && c->kind == HR_GRID_TWO_LEVEL && c->info.num_leaves)
+    if (leaves && (c->kind == HR_GRID_TWO_LEVEL || c->kind == HR_GRID_HIERARCHY) && c->info.num_leaves)
         HR_HIP(hipMemcpy(leaves, c->leaves.p, 2 * (size_t)c->info.num_leaves * sizeof(uint32_t),
                          hipMemcpyDeviceToHost));
     if (prims && c->info.num_refs)
@@ -1042,6 +1168,7 @@ hr_status hr_render_frame_graph(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff
                                 const hr_render_params* p, float* rgb_out)
 {
     HR_TRY(check_ctx(c));
+    HR_TRY(no_hierarchy(c, "captured frames"));
     if (c->nframes == 0) return fail(HR_ERR_STATE, "captured frames need resident key frames");
     if (k1 >= c->nframes || k2 >= c->nframes) return fail(HR_ERR_ARG, "key frame index out of range");
     HR_TRY(check_render_args(c, mode, image_id, p));
@@ -1130,17 +1257,21 @@ static hr_status upload_wire(hr_ctx* c, const float* rays, uint32_t n)
 // wire: org holds n 24-byte ray records in host memory and dir is unused
 static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uint32_t n, int anyhit,
                               float tmax, float* t_out, uint32_t* prim_out, uint8_t* occ_out, bool device,
-                              bool wire = false)
+                              bool wire = false, uint32_t* inst_out = nullptr)
 {
     HR_TRY(check_ctx(c));
     if (!c->built) return fail(HR_ERR_STATE, "trace before build");
     if (n == 0) return HR_OK;
     if (!org || (!dir && !wire)) return fail(HR_ERR_ARG, "null ray buffer");
     const hr::TraceScene S = trace_scene(c);
+    const bool hier = c->kind == HR_GRID_HIERARCHY;
     const bool pers = c->tuning.persistent_secondary != 0;
     const uint32_t refill = (uint32_t)c->tuning.refill;
     if (device) {
-        if (pers)
+        if (hier)
+            hr::launch_trace_hier(hier_scene(c), org, dir, n, anyhit, tmax, t_out, prim_out, nullptr, occ_out,
+                                  c->ray_count.p, c->stream);
+        else if (pers)
             hr::persist_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->pool.p, refill,
                               c->ray_count.p, c->work.p, c->stream);
         else
@@ -1160,7 +1291,11 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
         HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
         HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
     }
-    if (pers)
+    if (inst_out) HR_TRY(c->hit_inst.reserve(n));
+    if (hier)
+        hr::launch_trace_hier(hier_scene(c), c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
+                              inst_out ? c->hit_inst.p : nullptr, c->ray_occ.p, c->ray_count.p, c->stream);
+    else if (pers)
         hr::persist_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
                           c->ray_occ.p, c->pool.p, refill, c->ray_count.p, c->work.p, c->stream);
     else
@@ -1172,6 +1307,9 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
         HR_HIP(hipMemcpyAsync(prim_out, c->ray_prim.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
                               c->stream));
     if (occ_out) HR_HIP(hipMemcpyAsync(occ_out, c->ray_occ.p, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    if (inst_out)
+        HR_HIP(hipMemcpyAsync(inst_out, c->hit_inst.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              c->stream));
     HR_HIP(hipStreamSynchronize(c->stream));
     return HR_OK;
 }
@@ -1205,6 +1343,7 @@ hr_status hr_occluded_wire(hr_ctx* c, const float* rays, uint32_t n, float tmax,
 hr_status hr_render_rays(hr_ctx* c, const float* rays, uint32_t n, float* rgb_out)
 {
     HR_TRY(check_ctx(c));
+    HR_TRY(no_hierarchy(c, "input-ray rendering"));
     if (!c->built) return fail(HR_ERR_STATE, "render before build");
     if (!rgb_out) return fail(HR_ERR_ARG, "null output buffer");
     if (n == 0) return HR_OK;
@@ -1231,6 +1370,7 @@ hr_status hr_render_rays(hr_ctx* c, const float* rays, uint32_t n, float* rgb_ou
 hr_status hr_dump_ao_rays(hr_ctx* c, float* rays_out)
 {
     HR_TRY(check_ctx(c));
+    HR_TRY(no_hierarchy(c, "the AO ray dump"));
     if (!rays_out) return fail(HR_ERR_ARG, "null output buffer");
     if (!c->last_npix || !c->last_ao_samples) return fail(HR_ERR_STATE, "no AO render");
     const size_t n = (size_t)c->last_npix * (size_t)(c->last_ao_samples + 1);
@@ -1260,6 +1400,7 @@ hr_status hr_occluded_dev(hr_ctx* c, const float* d_org, const float* d_dir, uin
 hr_status hr_path_trace(hr_ctx* c, const float* org, const float* dir, uint32_t n, int depth, float* rgb_out)
 {
     HR_TRY(check_ctx(c));
+    HR_TRY(no_hierarchy(c, "path tracing"));
     if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
     if (!rgb_out) return fail(HR_ERR_ARG, "null output buffer");
     if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
@@ -1285,6 +1426,7 @@ hr_status hr_path_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, u
                             float* d_rgb_out)
 {
     HR_TRY(check_ctx(c));
+    HR_TRY(no_hierarchy(c, "path tracing"));
     if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
     if (!d_rgb_out) return fail(HR_ERR_ARG, "null output buffer");
     if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
@@ -1366,3 +1508,108 @@ hr_status hr_synchronize(hr_ctx* c)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// instanced hierarchy
+// ---------------------------------------------------------------------------
+
+hr_status hr_set_instances(hr_ctx* c, const uint32_t* obj_first, uint32_t nobj, const hr_instance* inst,
+                           uint32_t ninst)
+{
+    HR_TRY(check_ctx(c));
+    if (nobj == 0 || !obj_first) return fail(HR_ERR_ARG, "at least one object is needed");
+    if (ninst == 0 || !inst) return fail(HR_ERR_ARG, "at least one instance is needed");
+    if (obj_first[0] != 0) return fail(HR_ERR_ARG, "the first object starts at triangle 0");
+    for (uint32_t k = 0; k < nobj; ++k)
+        if (obj_first[k + 1] < obj_first[k]) return fail(HR_ERR_ARG, "object triangle ranges must not decrease");
+    if (obj_first[nobj] > 0x3FFFFFFFu) return fail(HR_ERR_CAPACITY, "too many triangles");
+    std::vector<hr::InstanceDev> h(ninst);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const hr_instance& a = inst[i];
+        if (a.object >= nobj) return fail(HR_ERR_ARG, "instance object id out of range");
+        for (int r = 0; r < 3; ++r) {
+            bool ok = std::isfinite(a.aabb_min[r]) && std::isfinite(a.aabb_max[r]);
+            for (int k = 0; k < 4; ++k) ok = ok && std::isfinite(a.m[r][k]);
+            if (!ok) return fail(HR_ERR_ARG, "instance transform and box must be finite");
+        }
+        hr::InstanceDev& d = h[i];
+        for (int k = 0; k < 3; ++k) {
+            d.bmin[k] = a.aabb_min[k];
+            d.bmax[k] = a.aabb_max[k];
+            // GeometryInstanceMatrix::setTransformation (TwoLevelGridHierarchy.h:618-632)
+            d.irot0[k] = a.m[k][0];
+            d.irot1[k] = a.m[k][1];
+            d.irot2[k] = a.m[k][2];
+            d.it[k] = a.m[k][3];
+        }
+        d.object = a.object;
+        d.pad = 0;
+    }
+    const uint32_t ntri = obj_first[nobj];
+    std::vector<uint32_t> tri_obj(std::max(ntri, 1u), 0u);
+    for (uint32_t k = 0; k < nobj; ++k)
+        for (uint32_t t = obj_first[k]; t < obj_first[k + 1]; ++t) tri_obj[t] = k;
+    HR_TRY(c->inst.reserve(ninst));
+    HR_TRY(c->obj_first.reserve((size_t)nobj + 1));
+    HR_TRY(c->tri_obj.reserve(tri_obj.size()));
+    HR_TRY(upload(c->inst.p, h.data(), (size_t)ninst * sizeof(hr::InstanceDev), c->stream));
+    HR_TRY(upload(c->obj_first.p, obj_first, ((size_t)nobj + 1) * sizeof(uint32_t), c->stream));
+    HR_TRY(upload(c->tri_obj.p, tri_obj.data(), tri_obj.size() * sizeof(uint32_t), c->stream));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    c->nobj = nobj;
+    c->ninst = ninst;
+    c->obj_tris = ntri;
+    if (c->kind == HR_GRID_HIERARCHY) c->built = false;
+    return HR_OK;
+}
+
+hr_status hr_get_object_grids(hr_ctx* c, hr_object_grid* grids, uint32_t* inst_refs)
+{
+    HR_TRY(check_ctx(c));
+    if (c->kind != HR_GRID_HIERARCHY) return fail(HR_ERR_STATE, "the grid is not an instanced hierarchy");
+    if (!c->built) return fail(HR_ERR_STATE, "no grid built");
+    HR_HIP(hipStreamSynchronize(c->stream));
+    if (grids) {
+        std::vector<hr::GridParams> gp(c->nobj);
+        std::vector<uint32_t> base(c->nobj);
+        HR_HIP(hipMemcpy(gp.data(), c->obj_gp.p, (size_t)c->nobj * sizeof(hr::GridParams), hipMemcpyDeviceToHost));
+        HR_HIP(hipMemcpy(base.data(), c->obj_cell_base.p, (size_t)c->nobj * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < c->nobj; ++k) {
+            hr_object_grid& o = grids[k];
+            for (int i = 0; i < 3; ++i) {
+                o.bmin[i] = gp[k].bmin[i];
+                o.bmax[i] = gp[k].bmax[i];
+                o.res[i] = gp[k].res[i];
+                o.cell_size[i] = gp[k].cs[i];
+                o.cell_size_rcp[i] = gp[k].csr[i];
+            }
+            o.first_cell = base[k];
+            o.num_cells = gp[k].num_cells;
+        }
+    }
+    if (inst_refs && c->info.num_top_pairs)
+        HR_HIP(hipMemcpy(inst_refs, c->top_vals_sorted.p, (size_t)c->info.num_top_pairs * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    return HR_OK;
+}
+
+hr_status hr_trace_instanced(hr_ctx* c, const float* org, const float* dir, uint32_t n, float* t_out,
+                             uint32_t* prim_out, uint32_t* inst_out)
+{
+    HR_TRY(check_ctx(c));
+    if (c->kind != HR_GRID_HIERARCHY) return fail(HR_ERR_STATE, "the grid is not an instanced hierarchy");
+    if (!t_out || !prim_out || !inst_out) return fail(HR_ERR_ARG, "null output buffer");
+    return trace_common(c, org, dir, n, 0, 0.f, t_out, prim_out, nullptr, false, false, inst_out);
+}
+
+hr_status hr_get_primary_instances(hr_ctx* c, uint32_t* inst)
+{
+    HR_TRY(check_ctx(c));
+    if (c->kind != HR_GRID_HIERARCHY) return fail(HR_ERR_STATE, "the grid is not an instanced hierarchy");
+    if (!c->last_npix || !c->hit_inst.p) return fail(HR_ERR_STATE, "nothing rendered");
+    if (!inst) return fail(HR_ERR_ARG, "null output pointer");
+    HR_HIP(hipStreamSynchronize(c->stream));
+    HR_HIP(hipMemcpy(inst, c->hit_inst.p, (size_t)c->last_npix * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return HR_OK;
+}

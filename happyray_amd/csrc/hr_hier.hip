@@ -1,0 +1,381 @@
+// hr_hier.hip -- traversal and AO integrator of the instanced two-level grid
+// hierarchy for gfx950 (its build is in hr_build.hip; hr_hier.h lists the
+// reference behaviour followed).
+//
+// Traversal: one ray per lane.  The top grid is stepped as the reference's
+// TLGridHierarchyTraverser does (entry EPS past the box, no clamp of the
+// entry cell); for every instance of a cell the world ray is clipped to the
+// instance box, moved into object coordinates and run through the object's
+// uniform grid to its closest hit, which counts only inside the box interval.
+// An any-hit ray (AO) stops after the first top cell that leaves it with a
+// distance below 0.9999, as the reference's shadow rays do, and is occluded
+// when its final distance is below tmax.
+#include "hr_hier.h"
+#include "hr_traverse.h"
+
+namespace hr {
+
+namespace {
+
+constexpr int kBlock = 256;
+using namespace trav;
+
+inline uint32_t cdiv_u(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+
+// GeometryInstanceMatrix::transformRay (TwoLevelGridHierarchy.h:650-665)
+__device__ __forceinline__ void instance_ray(const InstanceDev& I, f3 org, f3 rcp, f3& org_t, f3& rcp_t)
+{
+    const f3 r0 = mk(I.irot0[0], I.irot0[1], I.irot0[2]);
+    const f3 r1 = mk(I.irot1[0], I.irot1[1], I.irot1[2]);
+    const f3 r2 = mk(I.irot2[0], I.irot2[1], I.irot2[2]);
+    const f3 tr = mk(I.it[0], I.it[1], I.it[2]);
+    org_t = r0 * org.x + r1 * org.y + r2 * org.z + tr;
+    const f3 dt = r0 / rcp.x + r1 / rcp.y + r2 / rcp.z;
+    rcp_t = mk(1.f / dt.x, 1.f / dt.y, 1.f / dt.z);
+}
+
+// transformNormalToGlobal (TwoLevelGridHierarchy.h:677-694): the transpose of
+// the world-to-object rotation
+__device__ __forceinline__ f3 instance_normal(const InstanceDev& I, f3 n)
+{
+    return mk(I.irot0[0], I.irot1[0], I.irot2[0]) * n.x + mk(I.irot0[1], I.irot1[1], I.irot2[1]) * n.y +
+           mk(I.irot0[2], I.irot1[2], I.irot2[2]) * n.z;
+}
+
+// GeometryInstanceTraverser (TLGridHierarchyTraverser.h:45-231) for the
+// instances of one top cell: clip the world ray to the instance box, move it
+// into object coordinates, step the object grid (entry EPS past its box, the
+// entry cell clamped on the side the ray leaves by), keep the closest hit
+// that lies inside the instance box interval
+__device__ void hier_cell(const HierScene& H, uint32_t b, uint32_t e, f3 org, f3 rcp, float& t, uint32_t& best,
+                          uint32_t& inst, Work& wk)
+{
+    TraceScene S{};
+    S.prims = H.prims;
+    S.rec = H.rec;
+    S.nt = H.nt;
+    for (uint32_t it = b; it < e; ++it) {
+        const uint32_t ii = H.inst_refs[it];
+        const InstanceDev I = H.inst[ii];
+        const f3 t1 = (mk(I.bmin[0], I.bmin[1], I.bmin[2]) - org) * rcp;
+        f3 tm = (mk(I.bmax[0], I.bmax[1], I.bmax[2]) - org) * rcp;
+        const f3 tn = vmin(t1, tm);
+        tm = vmax(t1, tm);
+        const float te1 = fmaxf(fmaxf(tn.x, tn.y), tn.z);
+        const float tx1 = fminf(fminf(tm.x, tm.y), tm.z);
+        if (!(tx1 > te1 && tx1 >= 0.f && te1 < t)) continue;
+        f3 ot, rt;
+        instance_ray(I, org, rcp, ot, rt);
+        const f3 dt = mk(1.f / rt.x, 1.f / rt.y, 1.f / rt.z);
+        const GridRegs G = load_grid(H.obj_gp + I.object);
+        const uint32_t base = H.obj_cell_base[I.object];
+        float rayT = t;
+        uint32_t hit = H.nt;
+        const f3 g1 = (G.bmin - ot) * rt;
+        f3 gf = (G.bmax - ot) * rt;
+        const f3 gn = vmin(g1, gf);
+        gf = vmax(g1, gf);
+        const float tentry = fmaxf(fmaxf(gn.x, gn.y), gn.z);
+        const float texit = fminf(fminf(gf.x, gf.y), gf.z);
+        bool flag = texit > tentry && texit >= 0.f && tentry < rayT;
+        const f3 entry = (tentry >= 0.f) ? ot + rep(tentry + kEps) / rt : ot;
+        f3 cf = (entry - G.bmin) * G.csr;
+        cf = mk(floorf(cf.x), floorf(cf.y), floorf(cf.z));
+        cf.x = rt.x > 0.f ? fminf((float)G.rx - 1.f, cf.x) : fmaxf(cf.x, 0.f);
+        cf.y = rt.y > 0.f ? fminf((float)G.ry - 1.f, cf.y) : fmaxf(cf.y, 0.f);
+        cf.z = rt.z > 0.f ? fminf((float)G.rz - 1.f, cf.z) : fmaxf(cf.z, 0.f);
+        float tx = ((cf.x + (rt.x > 0.f ? 1.f : 0.f)) * G.cs.x + G.bmin.x - ot.x) * rt.x;
+        float ty = ((cf.y + (rt.y > 0.f ? 1.f : 0.f)) * G.cs.y + G.bmin.y - ot.y) * rt.y;
+        float tz = ((cf.z + (rt.z > 0.f ? 1.f : 0.f)) * G.cs.z + G.bmin.z - ot.z) * rt.z;
+        int cx = to_int_sat(cf.x), cy = to_int_sat(cf.y), cz = to_int_sat(cf.z);
+        const int sx = rt.x > 0.f ? 1 : -1, sy = rt.y > 0.f ? 1 : -1, sz = rt.z > 0.f ? 1 : -1;
+        flag = flag && (sx > 0 ? cx < G.rx : cx > -1) && (sy > 0 ? cy < G.ry : cy > -1) &&
+               (sz > 0 ? cz < G.rz : cz > -1);
+        const float ddx = G.cs.x * fabsf(rt.x), ddy = G.cs.y * fabsf(rt.y), ddz = G.cs.z * fabsf(rt.z);
+        const uint32_t sxy = (uint32_t)G.rx * (uint32_t)G.ry;
+        while (flag) {
+            ++wk.steps;
+            if (cx < G.rx && cx > -1 && cy < G.ry && cy > -1 && cz < G.rz && cz > -1) {
+                const uint2 cell =
+                    H.obj_cells[base + (uint32_t)cx + (uint32_t)cy * (uint32_t)G.rx + (uint32_t)cz * sxy];
+                isect_range<false>(S, cell.x, cell.y, ot, dt, rayT, hit, wk);
+            }
+            const int d = min_dim(tx, ty, tz);
+            if (d == 0) {
+                flag = rayT > tx;
+                cx += sx; tx += ddx;
+                flag = flag && (sx > 0 ? cx < G.rx : cx > -1);
+            } else if (d == 1) {
+                flag = rayT > ty;
+                cy += sy; ty += ddy;
+                flag = flag && (sy > 0 ? cy < G.ry : cy > -1);
+            } else {
+                flag = rayT > tz;
+                cz += sz; tz += ddz;
+                flag = flag && (sz > 0 ? cz < G.rz : cz > -1);
+            }
+        }
+        if (hit != H.nt && rayT < t && rayT >= te1 && rayT < tx1) {
+            t = rayT;
+            best = hit;
+            inst = ii;
+        }
+    }
+}
+
+// a shadow ray of the hierarchy stops after the top cell in which its
+// distance fell below this (TLGridHierarchyTraverser.h:262-267, :366-372)
+constexpr float kShadowStop = 0.9999f;
+
+// TLGridHierarchyTraverser::operator() (TLGridHierarchyTraverser.h:235-400):
+// the top grid is entered EPS past its box without clamping the entry cell;
+// cells outside the grid read as empty
+template <bool kAny>
+__device__ uint32_t trace_hier(const HierScene& H, f3 org, f3 d, float& t, f3& dir_out, uint32_t& inst, Work& wk)
+{
+    const f3 rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    dir_out = mk(1.f / rcp.x, 1.f / rcp.y, 1.f / rcp.z);
+    uint32_t best = H.nt;
+    inst = 0xFFFFFFFFu;
+    const GridRegs G = load_grid(H.top);
+    bool flag = !(kAny && t < kShadowStop);
+    const f3 t1 = (G.bmin - org) * rcp;
+    f3 tfar = (G.bmax - org) * rcp;
+    const f3 tnear = vmin(t1, tfar);
+    tfar = vmax(t1, tfar);
+    const float tentry = fmaxf(fmaxf(tnear.x, tnear.y), tnear.z);
+    const float texit = fminf(fminf(tfar.x, tfar.y), tfar.z);
+    flag = flag && texit > tentry && texit >= 0.f;
+    const f3 entry = (tentry >= 0.f) ? org + rep(tentry + kEps) / rcp : org;
+    f3 cf = (entry - G.bmin) * G.csr;
+    cf = mk(floorf(cf.x), floorf(cf.y), floorf(cf.z));
+    float tx = ((cf.x + (rcp.x > 0.f ? 1.f : 0.f)) * G.cs.x + G.bmin.x - org.x) * rcp.x;
+    float ty = ((cf.y + (rcp.y > 0.f ? 1.f : 0.f)) * G.cs.y + G.bmin.y - org.y) * rcp.y;
+    float tz = ((cf.z + (rcp.z > 0.f ? 1.f : 0.f)) * G.cs.z + G.bmin.z - org.z) * rcp.z;
+    int cx = to_int_sat(cf.x), cy = to_int_sat(cf.y), cz = to_int_sat(cf.z);
+    const int sx = rcp.x > 0.f ? 1 : -1, sy = rcp.y > 0.f ? 1 : -1, sz = rcp.z > 0.f ? 1 : -1;
+    flag = flag && (sx > 0 ? cx < G.rx : cx > -1) && (sy > 0 ? cy < G.ry : cy > -1) &&
+           (sz > 0 ? cz < G.rz : cz > -1);
+    const float ddx = G.cs.x * fabsf(rcp.x), ddy = G.cs.y * fabsf(rcp.y), ddz = G.cs.z * fabsf(rcp.z);
+    const uint32_t sxy = (uint32_t)G.rx * (uint32_t)G.ry;
+    while (flag) {
+        ++wk.steps;
+        if (cx < G.rx && cx > -1 && cy < G.ry && cy > -1 && cz < G.rz && cz > -1) {
+            const uint2 cell = H.top_cells[(uint32_t)cx + (uint32_t)cy * (uint32_t)G.rx + (uint32_t)cz * sxy];
+            hier_cell(H, cell.x, cell.y, org, rcp, t, best, inst, wk);
+        }
+        if (kAny && t < kShadowStop) flag = false;
+        const int dm = min_dim(tx, ty, tz);
+        if (dm == 0) {
+            flag = flag && t > tx;
+            cx += sx; tx += ddx;
+            flag = flag && (sx > 0 ? cx < G.rx : cx > -1);
+        } else if (dm == 1) {
+            flag = flag && t > ty;
+            cy += sy; ty += ddy;
+            flag = flag && (sy > 0 ? cy < G.ry : cy > -1);
+        } else {
+            flag = flag && t > tz;
+            cz += sz; tz += ddz;
+            flag = flag && (sz > 0 ? cz < G.rz : cz > -1);
+        }
+    }
+    return best;
+}
+
+template <bool kAny>
+__global__ __launch_bounds__(kBlock) void k_trace_hier(HierScene H, const float* __restrict__ org,
+                                                       const float* __restrict__ dir, uint32_t n, float tmax,
+                                                       float* __restrict__ t_out, uint32_t* __restrict__ prim_out,
+                                                       uint32_t* __restrict__ inst_out,
+                                                       uint8_t* __restrict__ occ_out, unsigned long long* ray_count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    count_rays(ray_count, i < n);
+    if (i >= n) return;
+    Work wk;
+    const f3 o = mk(org[i], org[(size_t)n + i], org[2 * (size_t)n + i]);
+    const f3 d = mk(dir[i], dir[(size_t)n + i], dir[2 * (size_t)n + i]);
+    float t = kFltMax;
+    f3 d2;
+    uint32_t inst;
+    const uint32_t best = trace_hier<kAny>(H, o, d, t, d2, inst, wk);
+    if (kAny) {
+        occ_out[i] = t < tmax ? 1u : 0u;
+        if (t_out) t_out[i] = t;
+    } else {
+        t_out[i] = t;
+        prim_out[i] = t < kFltMax ? best : H.nt;
+        if (inst_out) inst_out[i] = t < kFltMax ? inst : 0xFFFFFFFFu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TLGHAOIntegrator
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_primary_hier(HierScene H, CameraDev cam, RenderArgs ra, uint32_t npix,
+                                                         float* __restrict__ hits, uint32_t* __restrict__ hit_prims,
+                                                         uint32_t* __restrict__ hit_inst,
+                                                         unsigned long long* ray_count)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    count_rays(ray_count, pix < npix);
+    if (pix >= npix) return;
+    Work wk;
+    const f3 d0 = primary_direction(cam, ra, pix, npix);
+    const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
+    float t = kFltMax;
+    f3 dir;
+    uint32_t inst;
+    const uint32_t best = trace_hier<false>(H, org, d0, t, dir, inst, wk);
+    const f3 hp = org + (t - kEps) * dir;
+    hits[pix] = hp.x;
+    hits[(size_t)npix + pix] = hp.y;
+    hits[2 * (size_t)npix + pix] = hp.z;
+    hits[3 * (size_t)npix + pix] = dir.x;
+    hits[4 * (size_t)npix + pix] = dir.y;
+    hits[5 * (size_t)npix + pix] = dir.z;
+    hits[6 * (size_t)npix + pix] = t;
+    hit_prims[pix] = t < kFltMax ? best : H.nt;
+    hit_inst[pix] = t < kFltMax ? inst : 0xFFFFFFFFu;
+}
+
+// the hit's face normal in world coordinates (TLGridHierarchyAOIntegrator.h:316-357)
+__device__ __forceinline__ f3 hier_normal(const HierScene& H, uint32_t prim, uint32_t inst)
+{
+    f3 v0, v1, v2;
+    load_tri(H.rec, prim, v0, v1, v2);
+    f3 n = normalize(cross(v1 - v0, v2 - v0));
+    if (inst != 0xFFFFFFFFu) n = instance_normal(H.inst[inst], n);
+    return n;
+}
+
+// AmbientOcclusionRayGenerator of the hierarchy (TLGridHierarchyAOIntegrator.h:296-380)
+__global__ __launch_bounds__(kBlock) void k_ao_rays_hier(HierScene H, RenderArgs ra, uint32_t npix,
+                                                         const float* __restrict__ hits,
+                                                         const uint32_t* __restrict__ hit_prims,
+                                                         const uint32_t* __restrict__ hit_inst,
+                                                         uint8_t* __restrict__ occ, unsigned long long* ray_count)
+{
+    const uint32_t S_ = (uint32_t)ra.ao_samples;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = j < npix * S_;
+    const uint32_t pix = in ? j / S_ : 0u;
+    const uint32_t s = in ? j % S_ : 0u;
+    const uint32_t prim = in ? hit_prims[pix] : H.nt;
+    const float th = in ? hits[6 * (size_t)npix + pix] : kFltMax;
+    const bool active = in && th < kFltMax && prim < H.nt;
+    count_rays(ray_count, active);
+    if (!in) return;
+    if (!active) {
+        occ[j] = 1u;
+        return;
+    }
+    Work wk;
+    const f3 pdir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix], hits[5 * (size_t)npix + pix]);
+    f3 org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+    f3 n = hier_normal(H, prim, hit_inst[pix]);
+    if (dot(n, pdir) > 0.f) n = -n;
+    float u[4];
+    rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamAO + s, u);
+    const f3 h = cosine_dir(u[0], u[1]);
+    f3 tg, bn;
+    local_frame(n, tg, bn);
+    const f3 dir = normalize(n * h.z + tg * h.x + bn * h.y);
+    org = org + (0.0001f * ra.ao_dist) * n;
+    float t = kFltMax;
+    f3 d2;
+    uint32_t inst;
+    trace_hier<true>(H, org, dir, t, d2, inst, wk);
+    occ[j] = t < ra.ao_dist ? 1u : 0u;
+}
+
+// computeAOIlluminationTLGH (TLGridHierarchyAOIntegrator.h:64-204)
+__global__ __launch_bounds__(kBlock) void k_ao_shade_hier(HierScene H, RenderArgs ra, uint32_t npix,
+                                                          const float* __restrict__ hits,
+                                                          const uint32_t* __restrict__ hit_prims,
+                                                          const uint32_t* __restrict__ hit_inst,
+                                                          const uint8_t* __restrict__ occ, Shading sh,
+                                                          float* __restrict__ fb)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    const uint32_t S_ = (uint32_t)ra.ao_samples;
+    const float th = hits[6 * (size_t)npix + pix];
+    const uint32_t prim = hit_prims[pix];
+    const bool hit = th < kFltMax;
+    f3 n = rep(0.f), kd = rep(0.f), pdir = rep(0.f);
+    if (hit && prim < H.nt) {
+        n = hier_normal(H, prim, hit_inst[pix]);
+        const float* m = sh.mats + 12 * (size_t)(sh.mat_idx ? sh.mat_idx[prim] : 0u);
+        kd = mk(m[0], m[1], m[2]) * 3.14159f;
+        pdir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix], hits[5 * (size_t)npix + pix]);
+    }
+    f3 rad = rep(0.f);
+    for (uint32_t s = 0; s < S_; ++s) {
+        f3 v;
+        if (hit) {
+            v = occ[(size_t)pix * S_ + s] ? rep(0.f) : rep(1.f);
+            if (v.x + v.y + v.z > 0.f && prim < H.nt) {
+                v.x *= kd.x;
+                v.y *= kd.y;
+                v.z *= kd.z;
+                const float sin_zn = fmaxf(0.f, sqrtf(1.f - n.z));
+                const float cos_en = fabsf(dot(pdir, n));
+                v.x *= (0.7f * cos_en + 0.3f * sin_zn);
+                v.y *= (0.7f * cos_en + 0.2f * sin_zn);
+                v.z *= (0.9f * cos_en);
+            }
+        } else {
+            v = rep(1.f);
+        }
+        rad = rad + v * 1.f / (float)S_;
+    }
+    const float nw = 1.f / (float)(image_of(ra) + 1);
+    const float ow = 1.f - nw;
+    float* o = fb + 3 * (size_t)pix;
+    o[0] = o[0] * ow + rad.x * nw;
+    o[1] = o[1] * ow + rad.y * nw;
+    o[2] = o[2] * ow + rad.z * nw;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+
+void launch_trace_hier(const HierScene& H, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
+                       float* t_out, uint32_t* prim_out, uint32_t* inst_out, uint8_t* occ_out,
+                       unsigned long long* ray_count, hipStream_t s)
+{
+    if (n == 0) return;
+    const dim3 g(cdiv_u(n, kBlock)), b(kBlock);
+    if (anyhit)
+        hipLaunchKernelGGL(k_trace_hier<true>, g, b, 0, s, H, org, dir, n, tmax, t_out, prim_out, inst_out, occ_out,
+                           ray_count);
+    else
+        hipLaunchKernelGGL(k_trace_hier<false>, g, b, 0, s, H, org, dir, n, tmax, t_out, prim_out, inst_out, occ_out,
+                           ray_count);
+}
+
+void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const RenderArgs& ra, float* hits,
+                           uint32_t* hit_prims, uint32_t* hit_inst, uint8_t* occ, const Shading& sh, float* fb,
+                           unsigned long long* ray_count, hipStream_t s)
+{
+    const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
+    hipLaunchKernelGGL(k_primary_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, cam, ra, npix, hits,
+                       hit_prims, hit_inst, ray_count);
+    const size_t nrays = (size_t)npix * (size_t)ra.ao_samples;
+    hipLaunchKernelGGL(k_ao_rays_hier, dim3(cdiv_u(nrays, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits, hit_prims,
+                       hit_inst, occ, ray_count);
+    hipLaunchKernelGGL(k_ao_shade_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits, hit_prims,
+                       hit_inst, occ, sh, fb);
+}
+
+}  // namespace hr

@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, MODE_PATH_MULTI,
+from ._lib import (GRID_HIERARCHY, GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, MODE_PATH_MULTI,
                    MODE_SIMPLE, check)
 
 _f32p = C.POINTER(C.c_float)
@@ -151,6 +151,63 @@ class Engine:
         check(self._L.hr_get_grid(self._h, _u(cells), _u(leaves) if gi.num_leaves else None,
                                   _u(prims) if gi.num_refs else None), "hr_get_grid")
         return gi, cells, leaves, prims
+
+    # -- instanced hierarchy ----------------------------------------------
+    def set_instances(self, obj_first, objects, matrices, aabb_min, aabb_max):
+        """Objects are triangle ranges [obj_first[k], obj_first[k+1]); instance i
+        shows object objects[i] through the world-to-object rows matrices[i]
+        (3x4, the instances file's m_row_0..2) inside the world box
+        [aabb_min[i], aabb_max[i]] (WFObject::loadInstances,
+        src/Application/WFObjectReader.cpp:762-824)."""
+        first = np.ascontiguousarray(obj_first, np.uint32)
+        objects = np.asarray(objects, np.uint32).reshape(-1)
+        m = np.asarray(matrices, np.float32).reshape(-1, 3, 4)
+        lo = np.asarray(aabb_min, np.float32).reshape(-1, 3)
+        hi = np.asarray(aabb_max, np.float32).reshape(-1, 3)
+        n = objects.shape[0]
+        arr = (_lib.Instance * max(n, 1))()
+        for i in range(n):
+            arr[i].object = int(objects[i])
+            for r in range(3):
+                for c in range(4):
+                    arr[i].m[r][c] = float(m[i, r, c])
+                arr[i].aabb_min[r] = float(lo[i, r])
+                arr[i].aabb_max[r] = float(hi[i, r])
+        check(self._L.hr_set_instances(self._h, _u(first), first.shape[0] - 1, arr, n), "hr_set_instances")
+        self._nobj = first.shape[0] - 1
+
+    def object_grids(self):
+        """(list of dicts per object grid, instance references of the top cells)."""
+        gi = self.grid_info()
+        grids = (_lib.ObjectGrid * self._nobj)()
+        refs = np.zeros(gi.num_top_pairs, np.uint32)
+        check(self._L.hr_get_object_grids(self._h, grids, _u(refs) if gi.num_top_pairs else None),
+              "hr_get_object_grids")
+        out = []
+        for g in grids:
+            out.append(dict(bmin=np.array(g.bmin[:], np.float32), bmax=np.array(g.bmax[:], np.float32),
+                            res=list(g.res), cell_size=np.array(g.cell_size[:], np.float32),
+                            cell_size_rcp=np.array(g.cell_size_rcp[:], np.float32),
+                            first_cell=g.first_cell, num_cells=g.num_cells))
+        return out, refs
+
+    def trace_instanced(self, org, dirs):
+        """Closest hits on the instanced hierarchy: (t, prim, instance)."""
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[1]
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.uint32)
+        inst = np.zeros(n, np.uint32)
+        check(self._L.hr_trace_instanced(self._h, _f(o), _f(d), n, _f(t), _u(prim), _u(inst)),
+              "hr_trace_instanced")
+        return t, prim, inst
+
+    def primary_instances(self):
+        n = self._res[0] * self._res[1]
+        inst = np.zeros(n, np.uint32)
+        check(self._L.hr_get_primary_instances(self._h, _u(inst)), "hr_get_primary_instances")
+        return inst
 
     def get_bounding_box_diagonal_length(self):
         gi = self.grid_info()
@@ -303,5 +360,5 @@ class Engine:
         check(self._L.hr_synchronize(self._h), "hr_synchronize")
 
 
-__all__ = ["Engine", "GRID_UNIFORM", "GRID_UNIFORM_EXACT", "GRID_TWO_LEVEL", "MODE_AO",
+__all__ = ["Engine", "GRID_HIERARCHY", "GRID_UNIFORM", "GRID_UNIFORM_EXACT", "GRID_TWO_LEVEL", "MODE_AO",
            "MODE_PATH", "MODE_PATH_MULTI", "MODE_SIMPLE"]

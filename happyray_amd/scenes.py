@@ -395,3 +395,79 @@ def served_workload(seed=5, n_scenes=64, tri_range=(10000, 2000000), n_requests=
         n = int(np.exp(rng.uniform(np.log(ray_range[0]), np.log(ray_range[1]))))
         reqs.append((float(t[i]), kind, s, n, 0, 0, 0, 0.0))
     return out_scenes, pools, reqs
+
+
+# ---------------------------------------------------------------------------
+# instanced scenes (the TLGRIDHIERARCHY build)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Instancing:
+    obj_first: np.ndarray        # (nobj + 1,) uint32: object k owns triangles [obj_first[k], obj_first[k+1])
+    objects: np.ndarray          # (ninst,) uint32
+    matrices: np.ndarray         # (ninst, 3, 4) float32: world-to-object rows (m_row_0..2)
+    aabb_min: np.ndarray         # (ninst, 3) float32: world box of the instance
+    aabb_max: np.ndarray
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def instanced_scene(seed=6, n_inst=64, subdiv=2, n_soup=200, extent=20.0, res=(64, 48)):
+    """Three objects: 0 a room box of side `extent` centred at the origin
+    (8x8 quads per face), 1 a unit icosphere, 2 a triangle soup in [-1, 1]^3.
+    Instance 0 shows the room unmoved; the others place spheres and soups with
+    a random rotation, a scale in [0.5, 1.5] and a translation inside the room.
+    An instance box is the world bounds of its transformed vertices padded by
+    1e-3.  Returns (Scene, Instancing); the mesh holds every object once, in
+    object coordinates, so its bounds (the top grid's) are the room's."""
+    rng = np.random.default_rng(seed)
+    h = extent / 2.0
+    E = extent
+    room_faces = [((-h, -h, -h), (0, E, 0), (0, 0, E)), ((h, -h, -h), (0, 0, E), (0, E, 0)),
+                  ((-h, -h, -h), (0, 0, E), (E, 0, 0)), ((-h, h, -h), (E, 0, 0), (0, 0, E)),
+                  ((-h, -h, -h), (E, 0, 0), (0, E, 0)), ((-h, -h, h), (0, E, 0), (E, 0, 0))]
+    rv, rf, off = [], [], 0
+    for o, du, dv in room_faces:
+        v, f = quad_grid(o, du, dv, 8, 8)
+        rv.append(v)
+        rf.append(f + off)
+        off += len(v)
+    room_v, room_f = np.concatenate(rv), np.concatenate(rf)
+    sph_v, sph_f = icosphere(subdiv)
+    c = rng.uniform(-0.7, 0.7, size=(n_soup, 1, 3))
+    soup_v = np.clip(c + rng.uniform(-0.3, 0.3, size=(n_soup, 3, 3)), -1.0, 1.0).reshape(-1, 3)
+    soup_f = np.arange(3 * n_soup, dtype=np.uint32).reshape(-1, 3)
+    mats = [material((0.5, 0.5, 0.5)), material((0.7, 0.3, 0.2)), material((0.2, 0.4, 0.7))]
+    cam = dict(pos=(-0.9 * h, -0.8 * h, 0.85 * h), dir=(1.0, 0.9, -0.8), up=(0.0, 0.0, 1.0), fov=70.0,
+               res=res)
+    sc = _assemble([(room_v, room_f, 0), (sph_v, sph_f, 1), (soup_v, soup_f, 2)], mats, "instanced", cam)
+    counts = [len(room_f), len(sph_f), len(soup_f)]
+    obj_first = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    obj_verts = [room_v, sph_v, soup_v]
+    objects = [0]
+    mats_w2o = [np.hstack([np.eye(3), np.zeros((3, 1))])]
+    lo = [room_v.min(0) - 1e-3]
+    hi = [room_v.max(0) + 1e-3]
+    for i in range(1, n_inst):
+        k = 1 + (i % 2)
+        R = _rotation(rng)
+        s = rng.uniform(0.5, 1.5)
+        t = rng.uniform(-0.75 * h, 0.75 * h, size=3)
+        A = R.T / s                              # world -> object
+        M = np.hstack([A, (-A @ t)[:, None]])
+        w = (obj_verts[k] * s) @ R.T + t          # object -> world
+        objects.append(k)
+        mats_w2o.append(M)
+        lo.append(w.min(0) - 1e-3)
+        hi.append(w.max(0) + 1e-3)
+    inst = Instancing(obj_first=obj_first, objects=np.array(objects, np.uint32),
+                      matrices=np.array(mats_w2o, np.float32), aabb_min=np.array(lo, np.float32),
+                      aabb_max=np.array(hi, np.float32))
+    return sc, inst

@@ -42,7 +42,11 @@ typedef enum {
 typedef enum {
     HR_GRID_UNIFORM = 0,       /* uniform grid, bounding-box insertion          */
     HR_GRID_UNIFORM_EXACT = 1, /* uniform grid, exact triangle/box insertion    */
-    HR_GRID_TWO_LEVEL = 2      /* two-level grid                                */
+    HR_GRID_TWO_LEVEL = 2,     /* two-level grid                                */
+    HR_GRID_HIERARCHY = 3      /* instanced two-level grid hierarchy: a uniform grid per
+                                  object under a uniform grid of instance boxes
+                                  (the reference's TLGRIDHIERARCHY build,
+                                  src/RT/RTEngine.cu:260-349); renders AO only    */
 } hr_grid_kind;
 
 /* RTEngine::renderFrame modes (src/RT/RTEngine.cu:398-444) */
@@ -151,6 +155,54 @@ hr_status hr_set_frame(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coeff);
  * geometry drops the normals. */
 hr_status hr_set_normals(hr_ctx* ctx, const float* normals, uint32_t nframes, uint32_t nn,
                          const uint32_t* nidx, uint32_t nt);
+
+/* ---- instanced hierarchy (HR_GRID_HIERARCHY) ----
+ * One instance of an object: the world-to-object transform as the rows
+ * m_row_0..2 of the instances file (WFObject::loadInstances,
+ * src/Application/WFObjectReader.cpp:762-824; the fourth column is the
+ * translation) and the instance's world-space box (AABB_min / AABB_max). */
+typedef struct {
+    uint32_t object;
+    float m[3][4];
+    float aabb_min[3], aabb_max[3];
+} hr_instance;
+
+/* Objects and instances of the current mesh (replaces RTEngine::upload's
+ * TLGRIDHIERARCHY part, src/RT/RTEngine.cu:260-349).  Object k owns the
+ * triangles [obj_first[k], obj_first[k+1]) (obj_first has nobj + 1 entries,
+ * obj_first[0] == 0, non-decreasing; triangles past obj_first[nobj] are not
+ * in any object and never hit).  Object grids are bounded by the object's
+ * triangles on the first key frame of the current frame, padded by 1e-4
+ * (RTEngine.cu:264-326); the top grid spans the padded scene bounds and holds
+ * the instance boxes. */
+hr_status hr_set_instances(hr_ctx* ctx, const uint32_t* obj_first, uint32_t nobj, const hr_instance* inst,
+                           uint32_t ninst);
+
+/* one object's uniform grid after an HR_GRID_HIERARCHY build; its cells are
+ * entries [first_cell, first_cell + num_cells) of hr_get_grid's leaves array */
+typedef struct {
+    float bmin[3], bmax[3];
+    int res[3];
+    float cell_size[3], cell_size_rcp[3];
+    uint32_t first_cell;
+    uint32_t num_cells;
+} hr_object_grid;
+
+/* grids: nobj entries (may be NULL); inst_refs: the instance id of each top-cell
+ * reference, hr_grid_info.num_top_pairs entries (may be NULL).  With this kind,
+ * hr_get_grid returns the top cells (begin, end into inst_refs), the object
+ * cells as leaves (begin, end into prims) and the object references as prims. */
+hr_status hr_get_object_grids(hr_ctx* ctx, hr_object_grid* grids, uint32_t* inst_refs);
+
+/* closest hits with the instance hit (TLGridHierarchyTraverser::getBestHitInstance,
+ * src/RT/Algorithm/TLGridHierarchyTraverser.h:242; 0xFFFFFFFF on a miss).
+ * hr_trace and hr_occluded work on the hierarchy too: an occlusion ray stops
+ * once its distance is below 0.9999 (TLGridHierarchyTraverser.h:262-267,
+ * :366-372) and is occluded when its final distance is below tmax. */
+hr_status hr_trace_instanced(hr_ctx* ctx, const float* org, const float* dir, uint32_t n, float* t_out,
+                             uint32_t* prim_out, uint32_t* inst_out);
+/* instance hit by each pixel's primary ray in the last HR_GRID_HIERARCHY render */
+hr_status hr_get_primary_instances(hr_ctx* ctx, uint32_t* inst);
 
 /* ---- build: setGridDensities + buildAccStruct (src/RT/RTEngine.cu:354-368, :449-453) ---- */
 hr_status hr_set_grid(hr_ctx* ctx, hr_grid_kind kind, float lambda_top, float lambda_leaf);

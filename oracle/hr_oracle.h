@@ -38,6 +38,7 @@ extern "C" {
 #define OR_GRID_UNIFORM        0  /* uniform grid, bounding-box insertion   */
 #define OR_GRID_UNIFORM_EXACT  1  /* uniform grid, exact triangle insertion */
 #define OR_GRID_TWO_LEVEL      2  /* two-level grid                          */
+#define OR_GRID_HIERARCHY      3  /* instanced two-level grid hierarchy      */
 
 typedef struct {
     const float* vx;   /* SoA vertex coordinates, nv each */
@@ -194,6 +195,58 @@ int  or_render_pt(const or_mesh* mesh, const or_grid* g, const or_camera* cam,
                   const or_light* lights, const float* light_weights, int nlights,
                   const or_render_params* p, int image_id,
                   float* fb_rgb, or_stats* stats);
+
+/* ---- instanced hierarchy (or_hier.c) ---- */
+/* one instance: world-to-object rows m_row_0..2 of the instances file and the
+ * instance's world box (WFObjectReader.cpp:762-824) */
+typedef struct {
+    uint32_t object;
+    float m[3][4];
+    float aabb_min[3], aabb_max[3];
+} or_instance;
+
+typedef struct {
+    float bmin[3], bmax[3];
+    int res[3];
+    float cell_size[3], cell_size_rcp[3];
+    uint32_t first_cell, num_cells;
+} or_object_grid;
+
+typedef struct {
+    or_grid top;             /* cells: (begin, end) into inst_refs; num_leaves = object
+                                cells, num_refs = object references, num_top_pairs =
+                                instance references */
+    uint32_t nobj, ninst;
+    or_object_grid* objs;
+    uint32_t num_obj_cells;
+    uint32_t* obj_cells;     /* 2 * num_obj_cells (begin, end) into prims */
+    uint32_t num_obj_refs;
+    uint32_t* prims;         /* triangle ids */
+    uint32_t num_inst_refs;
+    uint32_t* inst_refs;     /* instance ids */
+    const or_instance* inst; /* the caller's instances (kept, not copied) */
+} or_hier;
+
+/* m: current-frame mesh; kx, ky, kz: the first key frame's vertices (object
+ * bounds); bmin, bmax: the padded scene bounds of or_lerp_bounds; object k
+ * owns triangles [obj_first[k], obj_first[k+1]).  Returns 0, or -2 when a
+ * resolution or count exceeds 32 bits, -1 on allocation failure. */
+int  or_build_hier(const or_mesh* m, const float* kx, const float* ky, const float* kz,
+                   const float bmin[3], const float bmax[3], const uint32_t* obj_first, uint32_t nobj,
+                   const or_instance* inst, uint32_t ninst, float lambda_top, float lambda_leaf,
+                   or_hier* out);
+void or_hier_free(or_hier* h);
+/* closest hits (t, prim, instance; prim = nt and instance 0xFFFFFFFF on a
+ * miss) or occlusion (distance below tmax after the 0.9999 shadow stop) */
+int  or_trace_hier(const or_mesh* m, const or_hier* h, const float* ox, const float* oy, const float* oz,
+                   const float* dx, const float* dy, const float* dz, uint32_t n, int anyhit, float tmax,
+                   float* t_out, uint32_t* prim_out, uint32_t* inst_out, uint8_t* occ_out);
+/* TLGHAOIntegrator (TLGridHierarchyAOIntegrator.h:421-573) at image image_id:
+ * fb is the running mean; occ_out (ao_samples per pixel), hits7 (7 SoA
+ * planes), hprim_out and hinst_out may be NULL */
+int  or_render_ao_hier(const or_mesh* m, const or_hier* h, const or_camera* cam, const or_material* mats,
+                       const uint32_t* mat_idx, const or_render_params* p, int image_id, float* fb,
+                       uint8_t* occ_out, float* hits7, uint32_t* hprim_out, uint32_t* hinst_out);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,19 @@ static inline void tri_bbox(f3 v0, f3 v1, f3 v2, f3* lo, f3* hi)
     *lo = a; *hi = b;
 }
 
+/* CosineHemisphereSampler (HemisphereSamplers.hpp:58-71) */
+static inline f3 cosine_sample(float u1, float u2)
+{
+    f3 r;
+    float s, c;
+    or_sincosf(2.f * OR_PI * u1, &s, &c);
+    r.y = s; r.x = c;
+    r.x *= sqrtf(1 - u2);
+    r.y *= sqrtf(1 - u2);
+    r.z = sqrtf(u2);
+    return r;
+}
+
 /* RNG stream identifiers of the counter-based generator */
 #define OR_STREAM_PRIMARY 0u
 #define OR_STREAM_AO      16u   /* + sample index */

@@ -446,19 +446,6 @@ int or_primary(const or_mesh* m, const or_grid* g, const or_camera* cam,
 /* ambient occlusion                                                   */
 /* ------------------------------------------------------------------ */
 
-/* CosineHemisphereSampler (HemisphereSamplers.hpp:58-71) */
-static f3 cosine_sample(float u1, float u2)
-{
-    f3 r;
-    float s, c;
-    or_sincosf(2.f * OR_PI * u1, &s, &c);
-    r.y = s; r.x = c;
-    r.x *= sqrtf(1 - u2);
-    r.y *= sqrtf(1 - u2);
-    r.z = sqrtf(u2);
-    return r;
-}
-
 typedef struct {
     const or_render_params* p; int image; uint32_t npix;
     const float *hx, *hy, *hz, *hdx, *hdy, *hdz, *ht; const uint32_t* hprim;

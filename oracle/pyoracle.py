@@ -52,6 +52,24 @@ class RenderParams(C.Structure):
                 ("nthreads", C.c_int), ("simple_sampler", C.c_int)]
 
 
+class Instance(C.Structure):
+    _fields_ = [("object", C.c_uint32), ("m", (C.c_float * 4) * 3), ("aabb_min", C.c_float * 3),
+                ("aabb_max", C.c_float * 3)]
+
+
+class ObjectGrid(C.Structure):
+    _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("res", C.c_int * 3),
+                ("cell_size", C.c_float * 3), ("cell_size_rcp", C.c_float * 3), ("first_cell", C.c_uint32),
+                ("num_cells", C.c_uint32)]
+
+
+class Hier(C.Structure):
+    _fields_ = [("top", Grid), ("nobj", C.c_uint32), ("ninst", C.c_uint32), ("objs", C.POINTER(ObjectGrid)),
+                ("num_obj_cells", C.c_uint32), ("obj_cells", _u32p), ("num_obj_refs", C.c_uint32),
+                ("prims", _u32p), ("num_inst_refs", C.c_uint32), ("inst_refs", _u32p),
+                ("inst", C.POINTER(Instance))]
+
+
 def build_lib():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
@@ -101,6 +119,13 @@ def lib():
         L.or_render_pt.argtypes = [C.POINTER(Mesh), C.POINTER(Grid), C.POINTER(Camera),
                                    _f32p, _u32p, _f32p, _f32p, C.c_int,
                                    C.POINTER(RenderParams), C.c_int, _f32p, C.POINTER(Stats)]
+        L.or_build_hier.argtypes = [C.POINTER(Mesh), _f32p, _f32p, _f32p, _f32p, _f32p, _u32p, C.c_uint32,
+                                    C.POINTER(Instance), C.c_uint32, C.c_float, C.c_float, C.POINTER(Hier)]
+        L.or_hier_free.argtypes = [C.POINTER(Hier)]
+        L.or_trace_hier.argtypes = [C.POINTER(Mesh), C.POINTER(Hier)] + [_f32p] * 6 + [
+            C.c_uint32, C.c_int, C.c_float, _f32p, _u32p, _u32p, _u8p]
+        L.or_render_ao_hier.argtypes = [C.POINTER(Mesh), C.POINTER(Hier), C.POINTER(Camera), _f32p, _u32p,
+                                        C.POINTER(RenderParams), C.c_int, _f32p, _u8p, _f32p, _u32p, _u32p]
         _lib = L
     return _lib
 
@@ -408,3 +433,110 @@ def primary_dirs(cam, p, image_id=0):
     d = np.zeros((3, n), np.float32)
     lib().or_primary_dirs(C.byref(cam), C.byref(p), image_id, _fp(d[0]), _fp(d[1]), _fp(d[2]))
     return d
+
+
+# ---------------------------------------------------------------------------
+# instanced hierarchy
+# ---------------------------------------------------------------------------
+
+def instances_struct(objects, matrices, aabb_min, aabb_max):
+    objects = np.asarray(objects, np.uint32).reshape(-1)
+    m = np.asarray(matrices, np.float32).reshape(-1, 3, 4)
+    lo = np.asarray(aabb_min, np.float32).reshape(-1, 3)
+    hi = np.asarray(aabb_max, np.float32).reshape(-1, 3)
+    arr = (Instance * max(len(objects), 1))()
+    for i in range(len(objects)):
+        arr[i].object = int(objects[i])
+        for r in range(3):
+            for c in range(4):
+                arr[i].m[r][c] = float(m[i, r, c])
+            arr[i].aabb_min[r] = float(lo[i, r])
+            arr[i].aabb_max[r] = float(hi[i, r])
+    return arr
+
+
+class OHier:
+    """Owns an or_hier built over mesh (current frame) with object bounds from
+    the key-frame vertices key (3, nv) and the padded scene bounds."""
+
+    def __init__(self, mesh, key, bmin, bmax, obj_first, objects, matrices, aabb_min, aabb_max,
+                 lambda_top=1.2, lambda_leaf=2.2):
+        self.mesh = mesh
+        self.key = np.ascontiguousarray(key, np.float32).reshape(3, -1)
+        self.kx, self.ky, self.kz = (np.ascontiguousarray(self.key[i]) for i in range(3))
+        self.obj_first = np.ascontiguousarray(obj_first, np.uint32)
+        self.inst = instances_struct(objects, matrices, aabb_min, aabb_max)
+        self.ninst = len(np.asarray(objects).reshape(-1))
+        self.c = Hier()
+        self.status = lib().or_build_hier(C.byref(mesh.c), _fp(self.kx), _fp(self.ky), _fp(self.kz),
+                                          _fp(_f3(bmin)), _fp(_f3(bmax)), _up(self.obj_first),
+                                          len(self.obj_first) - 1, self.inst, self.ninst, lambda_top,
+                                          lambda_leaf, C.byref(self.c))
+
+    def __del__(self):
+        try:
+            lib().or_hier_free(C.byref(self.c))
+        except Exception:
+            pass
+
+    def top_params(self):
+        g = self.c.top
+        return {"bmin": np.array(g.bmin[:], np.float32), "bmax": np.array(g.bmax[:], np.float32),
+                "res": tuple(g.res), "cell_size": np.array(g.cell_size[:], np.float32),
+                "cell_size_rcp": np.array(g.cell_size_rcp[:], np.float32)}
+
+    def top_cells(self):
+        n = 2 * self.c.top.num_cells
+        return np.ctypeslib.as_array(self.c.top.cells, shape=(n,)).copy().reshape(-1, 2)
+
+    def inst_refs(self):
+        n = self.c.num_inst_refs
+        return np.zeros(0, np.uint32) if n == 0 else np.ctypeslib.as_array(self.c.inst_refs, shape=(n,)).copy()
+
+    def obj_cells(self):
+        n = 2 * self.c.num_obj_cells
+        return np.zeros((0, 2), np.uint32) if n == 0 else \
+            np.ctypeslib.as_array(self.c.obj_cells, shape=(n,)).copy().reshape(-1, 2)
+
+    def prims(self):
+        n = self.c.num_obj_refs
+        return np.zeros(0, np.uint32) if n == 0 else np.ctypeslib.as_array(self.c.prims, shape=(n,)).copy()
+
+    def objects(self):
+        out = []
+        for k in range(self.c.nobj):
+            g = self.c.objs[k]
+            out.append(dict(bmin=np.array(g.bmin[:], np.float32), bmax=np.array(g.bmax[:], np.float32),
+                            res=list(g.res), cell_size=np.array(g.cell_size[:], np.float32),
+                            cell_size_rcp=np.array(g.cell_size_rcp[:], np.float32),
+                            first_cell=g.first_cell, num_cells=g.num_cells))
+        return out
+
+
+def trace_hier(h, org, dirs, anyhit=False, tmax=3.402823466e38):
+    o = np.ascontiguousarray(org, np.float32).reshape(3, -1)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(3, -1)
+    n = o.shape[1]
+    ox, oy, oz = (np.ascontiguousarray(o[i]) for i in range(3))
+    dx, dy, dz = (np.ascontiguousarray(d[i]) for i in range(3))
+    t = np.zeros(n, np.float32)
+    prim = np.zeros(n, np.uint32)
+    inst = np.zeros(n, np.uint32)
+    occ = np.zeros(n, np.uint8)
+    lib().or_trace_hier(C.byref(h.mesh.c), C.byref(h.c), _fp(ox), _fp(oy), _fp(oz), _fp(dx), _fp(dy), _fp(dz), n,
+                        1 if anyhit else 0, tmax, _fp(t), _up(prim), _up(inst), occ.ctypes.data_as(_u8p))
+    return (occ, t) if anyhit else (t, prim, inst)
+
+
+def render_ao_hier(h, cam, p, materials, mat_idx=None, image_id=0, fb=None):
+    n = cam.res_x * cam.res_y
+    fb = np.zeros((n, 3), np.float32) if fb is None else np.ascontiguousarray(fb, np.float32).copy()
+    m, mi = _mats(materials, mat_idx)
+    occ = np.zeros(n * p.ao_samples, np.uint8)
+    hits = np.zeros((7, n), np.float32)
+    hp = np.zeros(n, np.uint32)
+    hi = np.zeros(n, np.uint32)
+    lib().or_render_ao_hier(C.byref(h.mesh.c), C.byref(h.c), C.byref(cam), _fp(m),
+                            None if mi is None else _up(mi), C.byref(p), image_id, _fp(fb),
+                            occ.ctypes.data_as(_u8p), _fp(hits), _up(hp), _up(hi))
+    return fb, occ, hits, hp, hi

@@ -1,0 +1,152 @@
+"""GPU parity of the instanced two-level grid hierarchy (HR_GRID_HIERARCHY)
+against the oracle (or_hier.c) on the same seeded scene: object grids, top
+grid, every cell list and reference bit-exact; closest hits (distance,
+triangle, instance), occlusion, and the hierarchy's AO frames exact."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import SEED
+from happyray_amd import Engine, GRID_HIERARCHY, HappyRayError, MODE_AO, MODE_PATH, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def inst_scene():
+    return scenes.instanced_scene(seed=6, n_inst=96, n_soup=300, subdiv=3, res=(96, 72))
+
+
+def _gpu(sc, ins, lt=1.2, ll=2.2, v2=None, coeff=1.0):
+    e = Engine(seed=SEED)
+    if v2 is None:
+        e.upload_scene(sc)
+    else:
+        e.upload(sc.verts, v2, coeff, sc.idx)
+        e.upload_materials(sc.materials, sc.mat_idx)
+    e.set_instances(ins.obj_first, ins.objects, ins.matrices, ins.aabb_min, ins.aabb_max)
+    e.set_grid(GRID_HIERARCHY, lt, ll)
+    e.build()
+    return e
+
+
+def _oracle(sc, ins, lt=1.2, ll=2.2, v2=None, coeff=1.0):
+    v, bmin, bmax = O.lerp_bounds(sc.verts, sc.verts if v2 is None else v2, coeff)
+    mesh = O.OMesh(v[0], v[1], v[2], sc.idx)
+    h = O.OHier(mesh, sc.verts, bmin, bmax, ins.obj_first, ins.objects, ins.matrices, ins.aabb_min,
+                ins.aabb_max, lt, ll)
+    assert h.status == 0
+    return h
+
+
+def _assert_build_equal(e, h):
+    info, top, obj_cells, prims = e.grid()
+    grids, refs = e.object_grids()
+    p = h.top_params()
+    assert tuple(info.res) == p["res"]
+    for k in ("bmin", "bmax", "cell_size", "cell_size_rcp"):
+        np.testing.assert_array_equal(np.array(getattr(info, k)[:], np.float32), p[k])
+    assert info.num_cells == h.c.top.num_cells
+    assert info.num_leaves == h.c.num_obj_cells
+    assert info.num_refs == h.c.num_obj_refs
+    assert info.num_top_pairs == h.c.num_inst_refs
+    np.testing.assert_array_equal(top, h.top_cells())
+    np.testing.assert_array_equal(refs, h.inst_refs())
+    for g, o in zip(grids, h.objects()):
+        assert list(g["res"]) == list(o["res"])
+        assert g["first_cell"] == o["first_cell"] and g["num_cells"] == o["num_cells"]
+        for k in ("bmin", "bmax", "cell_size", "cell_size_rcp"):
+            np.testing.assert_array_equal(g[k], o[k])
+    np.testing.assert_array_equal(obj_cells, h.obj_cells())
+    np.testing.assert_array_equal(prims, h.prims())
+
+
+@pytest.mark.parametrize("lt,ll", [(1.2, 2.2), (8.0, 12.0)])
+def test_hier_build_parity(inst_scene, lt, ll):
+    print(f"hierarchy build parity lt={lt} ll={ll}", flush=True)
+    sc, ins = inst_scene
+    e = _gpu(sc, ins, lt, ll)
+    _assert_build_equal(e, _oracle(sc, ins, lt, ll))
+    e.close()
+
+
+def test_hier_build_parity_interpolated(inst_scene):
+    """Object bounds from the first key frame, triangles from the interpolated
+    frame: triangles that left their object grid land in its border cells."""
+    print("hierarchy build parity, interpolated frame", flush=True)
+    sc, ins = inst_scene
+    rng = np.random.default_rng(9)
+    v2 = (sc.verts * np.float32(1.15) + rng.normal(0, 0.05, sc.verts.shape)).astype(np.float32)
+    e = _gpu(sc, ins, v2=v2, coeff=0.6)
+    _assert_build_equal(e, _oracle(sc, ins, v2=v2, coeff=0.6))
+    e.close()
+
+
+def _rays(n, seed, h=10.0):
+    rng = np.random.default_rng(seed)
+    org = rng.uniform(-0.9 * h, 0.9 * h, size=(3, n)).astype(np.float32)
+    d = rng.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    d[:, :16] = np.eye(3)[:, np.arange(16) % 3]      # axis-parallel rays
+    return org, d.astype(np.float32)
+
+
+def test_hier_trace_parity(inst_scene):
+    print("hierarchy trace parity", flush=True)
+    sc, ins = inst_scene
+    e = _gpu(sc, ins)
+    h = _oracle(sc, ins)
+    org, dirs = _rays(20000, 3)
+    t, prim, inst = e.trace_instanced(org, dirs)
+    to, po, io = O.trace_hier(h, org, dirs)
+    np.testing.assert_array_equal(t, to)
+    np.testing.assert_array_equal(prim, po)
+    np.testing.assert_array_equal(inst, io)
+    t2, p2 = e.trace(org, dirs)
+    np.testing.assert_array_equal(t2, to)
+    np.testing.assert_array_equal(p2, po)
+    for tmax in (0.5, 2.0, 1e30):
+        occ = e.occluded(org, dirs, tmax)
+        oo, _ = O.trace_hier(h, org, dirs, anyhit=True, tmax=tmax)
+        np.testing.assert_array_equal(occ, oo)
+    e.close()
+
+
+def test_hier_ao_parity(inst_scene):
+    print("hierarchy AO parity", flush=True)
+    sc, ins = inst_scene
+    e = _gpu(sc, ins)
+    h = _oracle(sc, ins)
+    c = sc.camera
+    rx, ry = c["res"]
+    e.set_camera(c["pos"], c["dir"], c["up"], c["fov"], rx, ry)
+    cam = O.camera(c["pos"], c["dir"], c["up"], c["fov"], rx, ry)
+    fbo = None
+    for image in range(3):
+        p = O.params(seed=SEED, ao_samples=4, ao_dist=1.7)
+        fbo, occo, hitso, hpo, hio = O.render_ao_hier(h, cam, p, sc.materials, sc.mat_idx, image, fbo)
+        fb = e.render_frame(image, MODE_AO, jitter=1, ao_samples=4, ao_dist=1.7)
+        planes, prims = e.primary_hits()
+        np.testing.assert_array_equal(planes, hitso)
+        np.testing.assert_array_equal(prims, hpo)
+        np.testing.assert_array_equal(e.primary_instances(), hio)
+        np.testing.assert_array_equal(e.ao_occlusion(4), occo)
+        np.testing.assert_array_equal(np.asarray(fb).reshape(-1, 3), fbo)
+    e.close()
+
+
+def test_hier_rejects_other_modes_and_missing_instances(inst_scene):
+    print("hierarchy error paths", flush=True)
+    sc, ins = inst_scene
+    e = Engine(seed=SEED)
+    e.upload_scene(sc)
+    e.set_grid(GRID_HIERARCHY, 1.2, 2.2)
+    with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
+        e.build()
+    e.set_instances(ins.obj_first, ins.objects, ins.matrices, ins.aabb_min, ins.aabb_max)
+    e.build()
+    c = sc.camera
+    e.set_camera(c["pos"], c["dir"], c["up"], c["fov"], 16, 16)
+    with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
+        e.render_frame(0, MODE_PATH)
+    e.close()
