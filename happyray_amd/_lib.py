@@ -36,7 +36,8 @@ class GridInfo(C.Structure):
     _fields_ = [("kind", C.c_int), ("bmin", C.c_float * 3), ("bmax", C.c_float * 3),
                 ("res", C.c_int * 3), ("cell_size", C.c_float * 3), ("cell_size_rcp", C.c_float * 3),
                 ("num_cells", C.c_uint32), ("num_leaves", C.c_uint32), ("num_refs", C.c_uint32),
-                ("num_top_pairs", C.c_uint32), ("num_triangles", C.c_uint32)]
+                ("num_top_pairs", C.c_uint32), ("num_triangles", C.c_uint32),
+                ("num_objects", C.c_uint32), ("num_instances", C.c_uint32)]
 
 
 class Instance(C.Structure):
@@ -79,7 +80,8 @@ class SceneInfo(C.Structure):
                 ("num_triangles", C.c_uint32), ("num_materials", C.c_uint32),
                 ("num_lights", C.c_uint32), ("frame_step", C.c_float), ("has_camera", C.c_int),
                 ("cam_pos", C.c_float * 3), ("cam_dir", C.c_float * 3), ("cam_up", C.c_float * 3),
-                ("cam_fov", C.c_float), ("cam_res_x", C.c_int), ("cam_res_y", C.c_int)]
+                ("cam_fov", C.c_float), ("cam_res_x", C.c_int), ("cam_res_y", C.c_int),
+                ("num_objects", C.c_uint32), ("num_instances", C.c_uint32)]
 
 
 class MeshDesc(C.Structure):
@@ -204,6 +206,7 @@ SIGNATURES = {
     "hr_scene_get_triangles": (C.c_int, [_vp, _u32p, _u32p, _u32p]),
     "hr_scene_get_materials": (C.c_int, [_vp, _f32p]),
     "hr_scene_get_lights": (C.c_int, [_vp, _f32p, _f32p]),
+    "hr_scene_get_instances": (C.c_int, [_vp, _u32p, C.POINTER(Instance)]),
     "hr_scene_upload": (C.c_int, [_vp, _vp]),
     "hr_scene_frame": (C.c_int, [_vp, C.c_float, _u32p, _u32p, _f32p]),
     "hr_scene_next_frame": (C.c_int, [_vp, _f32p]),

@@ -566,6 +566,8 @@ hr_status hr_batch_get_grid_info(hr_batch* b, uint32_t s, hr_grid_info* out)
     out->num_refs = b->h_pair_base[s + 1] - b->h_pair_base[s];
     out->num_top_pairs = b->h_top_base[s + 1] - b->h_top_base[s];
     out->num_triangles = b->h_tbase[s + 1] - b->h_tbase[s];
+    out->num_objects = 0;
+    out->num_instances = 0;
     return HR_OK;
 }
 

@@ -61,7 +61,11 @@ int main(int argc, char* argv[])
     if (hr_scene_upload(ctx, scene)) return fail("uploading the scene");
     // initScene uploads key frame 0 twice with coefficient 1
     if (hr_set_frame(ctx, 0, 0, 1.f)) return fail("selecting frame 0");
-    if (hr_set_grid(ctx, HR_GRID_TWO_LEVEL, 1.2f, 2.2f)) return fail("grid settings");
+    // a scene with an instances file renders through the instanced hierarchy,
+    // as the reference's TLGRIDHIERARCHY build does when there is more than one
+    // instance (RTEngine.cu:263-264); that build renders ambient occlusion only
+    const bool hierarchy = info.num_instances > 1;
+    if (hr_set_grid(ctx, hierarchy ? HR_GRID_HIERARCHY : HR_GRID_TWO_LEVEL, 1.2f, 2.2f)) return fail("grid settings");
     if (hr_build(ctx)) return fail("building the grid");
     int res_x = 512, res_y = 512;
     if (info.has_camera) {
@@ -82,6 +86,10 @@ int main(int argc, char* argv[])
     int mode = 0;
     std::cin >> mode;
     mode = mode < 0 ? 0 : (mode > 3 ? 3 : mode);
+    if (hierarchy && mode != 3) {
+        std::cout << "(the instanced hierarchy renders ambient occlusion)\n";
+        mode = 3;
+    }
     std::cout << "samples per pixel: ";
     int spp = 1;
     std::cin >> spp;

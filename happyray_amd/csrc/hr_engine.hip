@@ -354,6 +354,9 @@ void set_info(hr_ctx* c, const hr::BuildStatus& st)
     in.num_refs = two ? st.num_leaf_pairs : st.num_pairs;
     in.num_top_pairs = two ? st.num_pairs : 0u;
     in.num_triangles = c->nt;
+    const bool hier = c->kind == HR_GRID_HIERARCHY;
+    in.num_objects = hier ? c->nobj : 0u;
+    in.num_instances = hier ? c->ninst : 0u;
     c->built = true;
 }
 

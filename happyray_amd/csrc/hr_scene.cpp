@@ -375,6 +375,8 @@ bool read_obj(const std::string& path, Obj& o)
     std::string line;
     size_t lineno = 0;
     std::vector<uint32_t> cv, cn;
+    size_t obj_begin = 0;
+    o.obj_first.push_back(0);
     while (std::getline(in, line)) {
         ++lineno;
         line = cut_comment(line);
@@ -440,6 +442,10 @@ bool read_obj(const std::string& path, Obj& o)
                 Face f{ { cv[0], cv[k], cv[k + 1] }, { cn[0], cn[k], cn[k + 1] }, cur_mat };
                 o.faces.push_back(f);
             }
+        } else if (c0 == 'o' && (c[1] == 0 || std::isspace((unsigned char)c[1]))) {
+            // a new object starts; an empty one is not recorded (WFObjectReader.cpp:516-530)
+            if (obj_begin < o.faces.size()) o.obj_first.push_back((uint32_t)o.faces.size());
+            obj_begin = o.faces.size();
         } else if (strncasecmp(c, "usemtl", 6) == 0) {
             const std::string name = trim(std::string(c + 6));
             if (name.empty()) {
@@ -458,8 +464,10 @@ bool read_obj(const std::string& path, Obj& o)
             const std::string name = trim(std::string(c + 6));
             if (!name.empty()) libs.push_back(name);
         }
-        // o, g, s and unknown entities carry nothing the renderer uses
+        // g, s and unknown entities carry nothing the renderer uses
     }
+    // the last object always closes (WFObjectReader.cpp:583)
+    o.obj_first.push_back((uint32_t)o.faces.size());
     const uint32_t nv = (uint32_t)(o.verts.size() / 3);
     for (const Face& f : o.faces)
         for (int k = 0; k < 3; ++k)
@@ -596,6 +604,63 @@ bool read_lights(const std::string& path, std::vector<Light>& out, std::string& 
     return true;
 }
 
+bool read_instances(const std::string& path, std::vector<Instance>& out, std::string& err)
+{
+    std::ifstream in(path.c_str());
+    if (!in) {
+        err = "cannot open instances file " + path;
+        return false;
+    }
+    out.clear();
+    std::string line;
+    int lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        const size_t cr = line.find('\r');
+        if (cr != std::string::npos) line = line.substr(0, cr);
+        line = cut_comment(line);
+        std::stringstream ss(line);
+        std::string key;
+        if (!(ss >> key)) continue;
+        const std::string where = path + ":" + std::to_string(lineno);
+        if (key == "obj_id") {
+            // a new instance: identity transform, box unbounded until AABB lines
+            long long id = -1;
+            if (!(ss >> id) || id < 0 || id > 0xFFFFFFFFll) {
+                err = where + ": invalid object id";
+                return false;
+            }
+            Instance n;
+            n.object = (uint32_t)id;
+            out.push_back(n);
+            continue;
+        }
+        int row = -1;
+        if (key == "m_row_0") row = 0;
+        else if (key == "m_row_1") row = 1;
+        else if (key == "m_row_2") row = 2;
+        const bool box_lo = key == "AABB_min", box_hi = key == "AABB_max";
+        if (row < 0 && !box_lo && !box_hi) continue;   // unknown keys are ignored
+        if (out.empty()) {
+            err = where + ": " + key + " before the first obj_id";
+            return false;
+        }
+        Instance& I = out.back();
+        float v[4];
+        const int nval = row >= 0 ? 4 : 3;
+        for (int k = 0; k < nval; ++k)
+            if (!(ss >> v[k])) {
+                err = where + ": " + key + " needs " + std::to_string(nval) + " numbers";
+                return false;
+            }
+        if (row >= 0)
+            for (int k = 0; k < 4; ++k) I.m[row][k] = v[k];
+        else
+            for (int k = 0; k < 3; ++k) (box_lo ? I.lo : I.hi)[k] = v[k];
+    }
+    return true;
+}
+
 bool load_scene(const std::string& cfg_path, Scene& sc)
 {
     Config cfg;
@@ -651,6 +716,16 @@ bool load_scene(const std::string& cfg_path, Scene& sc)
                 for (const Light& l : sc.lights) insert_light_geometry(l, sc.frames[i]);
     }
     for (const Light& l : sc.lights) sc.light_weights.push_back(light_weight(l));
+    if (!cfg.instances_file.empty()) {
+        if (!read_instances(resolve(base, cfg.instances_file), sc.instances, sc.error)) return false;
+        const size_t nobj = sc.frames[0].obj_first.size() - 1;
+        for (const Instance& I : sc.instances)
+            if (I.object >= nobj) {
+                sc.error = "instance of object " + std::to_string(I.object) + " but the scene has " +
+                           std::to_string(nobj) + " objects";
+                return false;
+            }
+    }
     return true;
 }
 
@@ -762,6 +837,27 @@ hr_status hr_scene_get_info(const hr_scene* sc, hr_scene_info* out)
     out->cam_fov = s.camera.fov;
     out->cam_res_x = s.camera.res_x;
     out->cam_res_y = s.camera.res_y;
+    out->num_objects = (uint32_t)(f.obj_first.size() - 1);
+    out->num_instances = (uint32_t)s.instances.size();
+    return HR_OK;
+}
+
+hr_status hr_scene_get_instances(const hr_scene* sc, uint32_t* obj_first, hr_instance* inst)
+{
+    if (hr_status st = check_scene(sc)) return st;
+    const hr::scene::Scene& s = sc->s;
+    if (obj_first)
+        std::copy(s.frames[0].obj_first.begin(), s.frames[0].obj_first.end(), obj_first);
+    if (inst)
+        for (size_t i = 0; i < s.instances.size(); ++i) {
+            const hr::scene::Instance& I = s.instances[i];
+            inst[i].object = I.object;
+            for (int r = 0; r < 3; ++r) {
+                for (int k = 0; k < 4; ++k) inst[i].m[r][k] = I.m[r][k];
+                inst[i].aabb_min[r] = I.lo[r];
+                inst[i].aabb_max[r] = I.hi[r];
+            }
+        }
     return HR_OK;
 }
 
@@ -851,6 +947,14 @@ hr_status hr_scene_upload(hr_ctx* ctx, const hr_scene* sc)
     if (s.camera.present) {
         const hr::scene::Camera& c = s.camera;
         if (hr_status st = hr_set_camera(ctx, c.pos, c.dir, c.up, c.fov, c.res_x, c.res_y)) return st;
+    }
+    if (!s.instances.empty()) {
+        std::vector<hr_instance> inst(s.instances.size());
+        hr_scene_get_instances(sc, nullptr, inst.data());
+        const std::vector<uint32_t>& of = s.frames[0].obj_first;
+        if (hr_status st = hr_set_instances(ctx, of.data(), (uint32_t)(of.size() - 1), inst.data(),
+                                            (uint32_t)inst.size()))
+            return st;
     }
     return HR_OK;
 }

@@ -31,12 +31,20 @@ struct Obj {                  // WFObject (triangles only)
     std::vector<float> normals;    // xyz per normal
     std::vector<Face> faces;
     std::vector<Mat> mats;
+    std::vector<uint32_t> obj_first;   // object k: faces [obj_first[k], obj_first[k+1]) ('o' lines)
     size_t skipped_lines = 0;      // malformed lines, skipped as the reference does
     std::string error;
 };
 
 struct Light {                // AreaLightSource
     float position[3], normal[3], intensity[3], edge1[3], edge2[3];
+};
+
+struct Instance {             // WFObject::Instance (src/Application/WFObject.hpp:136-146)
+    uint32_t object = 0;
+    float m[3][4] = { { 1.f, 0.f, 0.f, 0.f }, { 0.f, 1.f, 0.f, 0.f }, { 0.f, 0.f, 1.f, 0.f } };
+    float lo[3] = { -3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f };
+    float hi[3] = { 3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f };
 };
 
 struct Camera {               // CameraManager (src/Application/CameraManager.cpp:96-168)
@@ -65,6 +73,7 @@ struct Scene {
     Camera camera;
     std::vector<Light> lights;
     std::vector<float> light_weights;
+    std::vector<Instance> instances;  // the scene.cfg's instances file
     std::string error;
 };
 
@@ -72,6 +81,8 @@ bool read_config(const std::string& path, Config& out, std::string& err);
 bool read_obj(const std::string& path, Obj& out);
 bool read_camera(const std::string& path, Camera& out, std::string& err);
 bool read_lights(const std::string& path, std::vector<Light>& out, std::string& err);
+// WFObject::loadInstances (src/Application/WFObjectReader.cpp:762-824)
+bool read_instances(const std::string& path, std::vector<Instance>& out, std::string& err);
 // SceneLoader::loadScene (src/Application/SceneLoader.cpp:212-332)
 bool load_scene(const std::string& cfg_path, Scene& out);
 // PhongMaterial rows (12 floats) after setupPhongCoefficients

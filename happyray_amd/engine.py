@@ -174,17 +174,16 @@ class Engine:
                 arr[i].aabb_min[r] = float(lo[i, r])
                 arr[i].aabb_max[r] = float(hi[i, r])
         check(self._L.hr_set_instances(self._h, _u(first), first.shape[0] - 1, arr, n), "hr_set_instances")
-        self._nobj = first.shape[0] - 1
 
     def object_grids(self):
         """(list of dicts per object grid, instance references of the top cells)."""
         gi = self.grid_info()
-        grids = (_lib.ObjectGrid * self._nobj)()
+        grids = (_lib.ObjectGrid * max(gi.num_objects, 1))()
         refs = np.zeros(gi.num_top_pairs, np.uint32)
         check(self._L.hr_get_object_grids(self._h, grids, _u(refs) if gi.num_top_pairs else None),
               "hr_get_object_grids")
         out = []
-        for g in grids:
+        for g in grids[:gi.num_objects]:
             out.append(dict(bmin=np.array(g.bmin[:], np.float32), bmax=np.array(g.bmax[:], np.float32),
                             res=list(g.res), cell_size=np.array(g.cell_size[:], np.float32),
                             cell_size_rcp=np.array(g.cell_size_rcp[:], np.float32),

@@ -112,6 +112,22 @@ class SceneFile:
               "hr_scene_get_lights")
         return L[:n], w[:n]
 
+    def instances(self):
+        """(obj_first (num_objects + 1,), objects, matrices (n, 3, 4), aabb_min, aabb_max)
+        as hr_set_instances / Engine.set_instances take them."""
+        from . import _lib
+        nobj = self.info.num_objects
+        n = self.info.num_instances
+        first = np.zeros(nobj + 1, np.uint32)
+        arr = (_lib.Instance * max(n, 1))()
+        check(self._L.hr_scene_get_instances(self._h, first.ctypes.data_as(_u32p), arr),
+              "hr_scene_get_instances")
+        objects = np.array([arr[i].object for i in range(n)], np.uint32)
+        m = np.array([[list(arr[i].m[r]) for r in range(3)] for i in range(n)], np.float32).reshape(n, 3, 4)
+        lo = np.array([list(arr[i].aabb_min) for i in range(n)], np.float32).reshape(n, 3)
+        hi = np.array([list(arr[i].aabb_max) for i in range(n)], np.float32).reshape(n, 3)
+        return first, objects, m, lo, hi
+
     def frame(self, current):
         k1, k2, c = C.c_uint32(), C.c_uint32(), C.c_float()
         check(self._L.hr_scene_frame(self._h, float(current), C.byref(k1), C.byref(k2), C.byref(c)),

@@ -76,10 +76,13 @@ typedef struct {
     int res[3];
     float cell_size[3], cell_size_rcp[3];
     uint32_t num_cells;      /* (top-level) cells                         */
-    uint32_t num_leaves;     /* two-level: leaf cells                     */
+    uint32_t num_leaves;     /* two-level: leaf cells; hierarchy: object cells */
     uint32_t num_refs;       /* primitive references (length of prims)   */
-    uint32_t num_top_pairs;  /* two-level: top-level (cell, prim) pairs   */
+    uint32_t num_top_pairs;  /* two-level: top-level (cell, prim) pairs;
+                                hierarchy: (top cell, instance) pairs     */
     uint32_t num_triangles;
+    uint32_t num_objects;    /* hierarchy: objects, else 0                */
+    uint32_t num_instances;  /* hierarchy: instances, else 0              */
 } hr_grid_info;
 
 /* AreaLightSource (src/RT/Primitive/LightSource.hpp:38-99) */
@@ -455,6 +458,8 @@ typedef struct {
     float cam_pos[3], cam_dir[3], cam_up[3];   /* after the camera file's rotation      */
     float cam_fov;
     int cam_res_x, cam_res_y;
+    uint32_t num_objects;         /* objects of the first key frame ('o' lines)          */
+    uint32_t num_instances;       /* entries of the scene.cfg's instances file           */
 } hr_scene_info;
 hr_status hr_scene_load(const char* cfg_path, hr_scene** out);
 /* one OBJ file as a static scene (AnimationManager::read(const char*) + light setup) */
@@ -473,6 +478,13 @@ hr_status hr_scene_get_triangles(const hr_scene* scene, uint32_t* idx, uint32_t*
 hr_status hr_scene_get_materials(const hr_scene* scene, float* mats);
 /* area lights and their sampling weights (area * |intensity|) */
 hr_status hr_scene_get_lights(const hr_scene* scene, hr_area_light* lights, float* weights);
+/* objects as triangle ranges (num_objects + 1 entries, see hr_set_instances) and the
+ * instances of the scene.cfg's "instances" file (WFObject::loadInstances,
+ * src/Application/WFObjectReader.cpp:762-824: an obj_id line starts an instance with
+ * the identity transform; m_row_0..2 set the world-to-object rows, AABB_min/AABB_max
+ * its box, which is unbounded until given); either may be NULL.  hr_scene_upload
+ * passes them to hr_set_instances when the scene has instances. */
+hr_status hr_scene_get_instances(const hr_scene* scene, uint32_t* obj_first, hr_instance* inst);
 /* Make the scene a context's geometry: key frames resident (frame 0 current),
  * normals, materials, lights and, when the scene has one, the camera. */
 hr_status hr_scene_upload(hr_ctx* ctx, const hr_scene* scene);

@@ -5,9 +5,10 @@ import os
 import numpy as np
 
 
-def write_obj(path, verts, faces, normals=None, mtllib=None, groups=None, extra=()):
+def write_obj(path, verts, faces, normals=None, mtllib=None, groups=None, extra=(), objects=()):
     """verts (nv, 3); faces: list of corner lists, each corner an int (1-based
-    vertex) or (v, n) pair written as v//n; groups: list of (first_face, material)."""
+    vertex) or (v, n) pair written as v//n; groups: list of (first_face, material);
+    objects: face indices before which an "o <k>" line starts a new object."""
     lines = []
     if mtllib:
         lines.append(f"mtllib {mtllib}")
@@ -18,13 +19,20 @@ def write_obj(path, verts, faces, normals=None, mtllib=None, groups=None, extra=
             lines.append("vn %.9g %.9g %.9g" % tuple(n))
     lines.extend(extra)
     starts = dict(groups or [])
+    obj_at = {}
+    for k, f0 in enumerate(objects):
+        obj_at.setdefault(f0, []).append(k)
     for i, f in enumerate(faces):
+        for k in obj_at.get(i, ()):
+            lines.append(f"o {k}")
         if i in starts:
             lines.append(f"usemtl {starts[i]}")
         corners = []
         for c in f:
             corners.append(f"{c[0]}//{c[1]}" if isinstance(c, tuple) else str(c))
         lines.append("f " + " ".join(corners))
+    for k in obj_at.get(len(faces), ()):
+        lines.append(f"o {k}")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
@@ -119,4 +127,37 @@ def animated_heightfield_scene(root, q=24, frames=3, step=0.4, res=(96, 64)):
     write_cfg(cfg, ["# animated height field", ("frame prefix", "anim/hf"), ("frame suffix", ".obj"),
                     ("number of frames", frames), ("frame step", step),
                     ("camera", "camera.cfg"), ("light", "light.cfg")])
+    return cfg
+
+
+def write_instances(path, objects, matrices, aabb_min=None, aabb_max=None, comments=True):
+    """The instances file of WFObject::loadInstances: per instance an obj_id
+    line, m_row_0..2 (world-to-object rows with the translation last) and the
+    AABB_min / AABB_max lines (omitted where aabb_min / aabb_max is None)."""
+    lines = []
+    for i, k in enumerate(objects):
+        if comments:
+            lines.append("#new instance#")
+        lines.append(f"obj_id {int(k)}")
+        for r in range(3):
+            lines.append(f"m_row_{r} " + " ".join("%.9g" % x for x in matrices[i][r]))
+        if aabb_min is not None:
+            lines.append("AABB_min " + " ".join("%.9g" % x for x in aabb_min[i]))
+        if aabb_max is not None:
+            lines.append("AABB_max " + " ".join("%.9g" % x for x in aabb_max[i]))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def instanced_scene_files(root, sc, ins):
+    """scene.cfg + one OBJ whose objects are the Instancing's triangle ranges
+    ("o" lines), the instances file and a camera.cfg.  Returns the scene.cfg path."""
+    faces = [[int(a) + 1, int(b) + 1, int(c) + 1] for a, b, c in sc.idx]
+    write_obj(os.path.join(root, "inst.obj"), sc.verts.T, faces, objects=list(ins.obj_first[:-1]))
+    write_instances(os.path.join(root, "instances.txt"), ins.objects, ins.matrices, ins.aabb_min,
+                    ins.aabb_max)
+    c = sc.camera
+    write_camera(os.path.join(root, "camera.cfg"), c["pos"], c["dir"], c["up"], fov=c["fov"], res=c["res"])
+    cfg = os.path.join(root, "scene.cfg")
+    write_cfg(cfg, [("obj", "inst.obj"), ("camera", "camera.cfg"), ("instances", "instances.txt")])
     return cfg

@@ -150,3 +150,34 @@ def test_hier_rejects_other_modes_and_missing_instances(inst_scene):
     with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
         e.render_frame(0, MODE_PATH)
     e.close()
+
+
+def test_hier_scene_file(tmp_path):
+    """A scene.cfg with an OBJ split by 'o' lines and an instances file, loaded
+    and uploaded by the library, against the oracle built from the parsed data."""
+    print("hierarchy from scene files", flush=True)
+    from happyray_amd import SceneFile
+    from scene_files import instanced_scene_files
+    sc, ins = scenes.instanced_scene(seed=8, n_inst=40, n_soup=150, res=(48, 36))
+    sf = SceneFile.load(instanced_scene_files(str(tmp_path), sc, ins))
+    e = Engine(seed=SEED)
+    e.load_scene(sf)
+    e.set_grid(GRID_HIERARCHY, 1.2, 2.2)
+    e.build()
+    v = sf.vertices(0)
+    idx, _, mat_idx = sf.triangles()
+    first, objects, m, lo, hi = sf.instances()
+    assert len(objects) == 40 and len(first) == 4
+    _, bmin, bmax = O.lerp_bounds(v, v, 0.0)
+    h = O.OHier(O.OMesh(v[0], v[1], v[2], idx), v, bmin, bmax, first, objects, m, lo, hi)
+    assert h.status == 0
+    _assert_build_equal(e, h)
+    c = sf.camera
+    cam = O.camera(c["pos"], c["dir"], c["up"], c["fov"], c["res"][0], c["res"][1])
+    p = O.params(seed=SEED, ao_samples=4, ao_dist=1.0)
+    fbo, occo, _, _, hio = O.render_ao_hier(h, cam, p, sf.materials(), mat_idx)
+    fb = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=1.0)
+    np.testing.assert_array_equal(e.primary_instances(), hio)
+    np.testing.assert_array_equal(e.ao_occlusion(4), occo)
+    np.testing.assert_array_equal(np.asarray(fb).reshape(-1, 3), fbo)
+    e.close()

@@ -10,8 +10,8 @@ import numpy as np
 import pytest
 
 import pyoracle as O
-from scene_files import (animated_heightfield_scene, write_camera, write_cfg, write_lights,
-                         write_mtl, write_obj)
+from scene_files import (animated_heightfield_scene, write_camera, write_cfg, write_instances,
+                         write_lights, write_mtl, write_obj)
 
 PI = np.float32(math.pi)
 SQ = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)]
@@ -275,3 +275,59 @@ def test_scene_vertices_and_normals_roundtrip(hrlib, tmp_path):
     assert idx.shape == (2 * 9 + 2, 3)                  # + the light quad
     np.testing.assert_array_equal(idx[:18], nidx[:18])  # corners written as v//v
     assert set(mi[:9]) == {1} and set(mi[9:18]) == {2}
+
+
+def test_objects_and_instances_file(hrlib, tmp_path):
+    """'o' lines split the faces into objects (an empty object is not recorded,
+    the last one always closes); an instances file sets each instance's rows and
+    box, a missing box stays unbounded (WFObjectReader.cpp:516-583, :762-824)."""
+    faces = [[1, 2, 3], [1, 3, 4], [1, 2, 4], [2, 3, 4], [1, 2, 3]]
+    # objects start before faces 0 (twice: the first is empty), 2 and 4
+    write_obj(tmp_path / "o.obj", SQ, faces, objects=[0, 0, 2, 4])
+    m0 = [[1, 0, 0, 0.5], [0, 2, 0, -1], [0, 0, 0.5, 2]]
+    m1 = [[0, 1, 0, 0], [-1, 0, 0, 0], [0, 0, 1, 0.25]]
+    write_instances(tmp_path / "inst.txt", [2, 0], [m0, m1])
+    with open(tmp_path / "inst.txt", "a") as fh:
+        fh.write("AABB_min -1 -2 -3   # trailing comment\nAABB_max 1 2 3\nunknown_key 7\n")
+    write_cfg(tmp_path / "s.cfg", [("obj", "o.obj"), ("instances", "inst.txt")])
+    sf = load(hrlib, tmp_path / "s.cfg")
+    assert sf.info.num_objects == 3 and sf.info.num_instances == 2
+    first, objects, m, lo, hi = sf.instances()
+    # 5 faces of the file plus the default light quad's 2, outside every object
+    assert sf.info.num_triangles == 7
+    np.testing.assert_array_equal(first, [0, 2, 4, 5])
+    np.testing.assert_array_equal(objects, [2, 0])
+    np.testing.assert_array_equal(m[0], np.float32(m0))
+    np.testing.assert_array_equal(m[1], np.float32(m1))
+    big = np.float32(3.402823466e38)
+    np.testing.assert_array_equal(lo[0], [-big] * 3)
+    np.testing.assert_array_equal(hi[0], [big] * 3)
+    np.testing.assert_array_equal(lo[1], [-1, -2, -3])
+    np.testing.assert_array_equal(hi[1], [1, 2, 3])
+    # without 'o' lines the whole file is one object
+    write_obj(tmp_path / "p.obj", SQ, faces)
+    write_cfg(tmp_path / "p.cfg", [("obj", "p.obj")])
+    sp = load(hrlib, tmp_path / "p.cfg")
+    assert sp.info.num_objects == 1 and sp.info.num_instances == 0
+    np.testing.assert_array_equal(sp.instances()[0], [0, 5])
+
+
+def test_instances_file_errors(hrlib, tmp_path):
+    from happyray_amd._lib import STATUS_NAMES
+    SceneFile, _ = hrlib
+    names = {v: k for k, v in STATUS_NAMES.items()}
+    write_obj(tmp_path / "o.obj", SQ, [[1, 2, 3]])
+    eye = [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]]
+    write_instances(tmp_path / "bad_id.txt", [1], [eye])          # one object only
+    write_cfg(tmp_path / "a.cfg", [("obj", "o.obj"), ("instances", "bad_id.txt")])
+    assert status_of(hrlib, lambda: SceneFile.load(str(tmp_path / "a.cfg"))) == names["HR_ERR_PARSE"]
+    with open(tmp_path / "early.txt", "w") as fh:
+        fh.write("m_row_0 1 0 0 0\nobj_id 0\n")
+    write_cfg(tmp_path / "b.cfg", [("obj", "o.obj"), ("instances", "early.txt")])
+    assert status_of(hrlib, lambda: SceneFile.load(str(tmp_path / "b.cfg"))) == names["HR_ERR_PARSE"]
+    with open(tmp_path / "short.txt", "w") as fh:
+        fh.write("obj_id 0\nm_row_1 0 1 0\n")
+    write_cfg(tmp_path / "c.cfg", [("obj", "o.obj"), ("instances", "short.txt")])
+    assert status_of(hrlib, lambda: SceneFile.load(str(tmp_path / "c.cfg"))) == names["HR_ERR_PARSE"]
+    write_cfg(tmp_path / "d.cfg", [("obj", "o.obj"), ("instances", "absent.txt")])
+    assert status_of(hrlib, lambda: SceneFile.load(str(tmp_path / "d.cfg"))) == names["HR_ERR_IO"]
