@@ -44,14 +44,17 @@ def parse():
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--scenes", type=int, default=256, help="batched workload: number of scenes")
     ap.add_argument("--requests", type=int, default=2000, help="served workload: number of requests")
-    ap.add_argument("--workload", default="soup_ao", choices=["soup_ao", "keyframes_pt", "batched", "served"],
+    ap.add_argument("--workload", default="soup_ao",
+                    choices=["soup_ao", "keyframes_pt", "batched", "served", "instanced_ao"],
                     help="soup_ao: 1M-triangle two-level rebuild + AO frame (default); "
                          "keyframes_pt: 708x708-quad heightfield over 30 key frames, "
                          "interpolation + rebuild + path tracing per frame, captured as HIP graphs; "
                          "batched: 256 scenes of 1k-200k triangles with 4k-1M rays each, one segmented "
                          "build + mixed persistent traces after autotuning, against per-scene launches; "
                          "served: Poisson requests over 64 resident scenes through the request server, "
-                         "against sequential per-request launches")
+                         "against sequential per-request launches; "
+                         "instanced_ao: 10,000 instances of a 20k-triangle sphere and a 100k-triangle soup "
+                         "in a room, instanced hierarchy rebuild + AO frame (4 AO rays per hit)")
     return ap.parse_args()
 
 
@@ -472,6 +475,87 @@ def main_served(args):
     print(json.dumps(out), flush=True)
 
 
+def main_instanced(args):
+    """Config: the instanced hierarchy (the reference's TLGRIDHIERARCHY build).
+    Objects: a room of extent 100 (768 triangles), a unit icosphere (20,480
+    triangles) and a 100,000-triangle soup in [-1, 1]^3 (corners within 0.02 of
+    their centre); 10,000 instances (the
+    room once, spheres and soups with random rotation, scale 0.5-1.5 and
+    position).  One step = hierarchy rebuild (object grids + top grid) + the
+    hierarchy's AO frame: 1024x1024 primary rays + 4 AO rays per hit
+    (NUMAMBIENTOCCLUSIONSAMPLES_TLGH, TLGridHierarchyAOIntegrator.h:47)."""
+    from happyray_amd import Engine, GRID_HIERARCHY, MODE_AO, scenes
+    sc, ins = scenes.instanced_scene(seed=12, n_inst=10000, subdiv=5, n_soup=100000, extent=100.0,
+                                     res=(1024, 1024), soup_size=0.02)
+    e = Engine(seed=(0x1234, 0x5678))
+    e.upload_scene(sc)
+    e.set_instances(ins.obj_first, ins.objects, ins.matrices, ins.aabb_min, ins.aabb_max)
+    e.set_grid(GRID_HIERARCHY, 1.2, 2.2)
+    e.set_camera_dict(sc.camera)
+    e.build()
+    ao_dist = 0.05 * e.get_bounding_box_diagonal_length()      # RTEngine.cu:261
+    S = 4
+
+    def step(i):
+        e.build()
+        e.render_frame(i, MODE_AO, jitter=1, ao_samples=S, ao_dist=ao_dist, download=False)
+
+    for i in range(args.warmup):
+        step(i)
+    e.synchronize()
+    r0 = e.timing().rays_traced
+    build_ms, render_ms = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i)
+        tm = e.timing()
+        build_ms.append(tm.build_ms)
+        render_ms.append(tm.render_ms)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    rays = e.timing().rays_traced - r0
+    info = e.grid_info()
+    out = {
+        "metric": METRIC, "value": round(rays / wall / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "instanced: 10,000 instances of a 20,480-triangle icosphere and a "
+                               "100,000-triangle soup in a 768-triangle room of extent 100; instanced "
+                               "hierarchy top 1.2 leaf 2.2; 1024x1024 primary + 4 AO rays per hit; "
+                               "step = hierarchy rebuild + AO frame",
+                   "mesh_triangles": sc.nt, "instances": int(len(ins.objects)),
+                   "rays_per_step": int(rays // args.steps), "top_res": list(info.res),
+                   "object_cells": info.num_leaves, "object_refs": info.num_refs,
+                   "instance_refs": info.num_top_pairs, "ao_dist": round(ao_dist, 4)},
+        "rebuild_ms": round(float(np.mean(build_ms)), 4),
+        "render_ms": round(float(np.mean(render_ms)), 4),
+        "render_mrays_s": round(rays / args.steps / (float(np.mean(render_ms)) * 1e-3) / 1e6, 3),
+    }
+    e.close()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        v = sc.verts
+        _, bmin, bmax = O.lerp_bounds(v, v, 1.0)
+        t0 = time.perf_counter()
+        h = O.OHier(O.OMesh(v[0], v[1], v[2], sc.idx), v, bmin, bmax, ins.obj_first, ins.objects, ins.matrices,
+                    ins.aabb_min, ins.aabb_max)
+        build_s = time.perf_counter() - t0
+        c = sc.camera
+        rx, ry = 128, 128
+        oc = O.camera(c["pos"], c["dir"], c["up"], c["fov"], rx, ry)
+        p = O.params(seed=(0x1234, 0x5678), jitter=1, ao_samples=S, ao_dist=ao_dist)
+        t0 = time.perf_counter()
+        _, occ, hits, _, _ = O.render_ao_hier(h, oc, p, sc.materials, sc.mat_idx)
+        render_s = time.perf_counter() - t0
+        nr = rx * ry + int((hits[6] < 3.4e38).sum()) * S
+        out["cpu_baseline"] = {
+            "value": round(nr / render_s / 1e6, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
+            "sample": f"the same view at {rx}x{ry} ({nr} rays, {render_s:.2f} s on 1 thread); "
+                      f"hierarchy build on 1 thread: {build_s * 1e3:.1f} ms"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -484,6 +568,8 @@ def main():
         return main_batched(args)
     if args.workload == "served":
         return main_served(args)
+    if args.workload == "instanced_ao":
+        return main_instanced(args)
     from happyray_amd import Engine, GRID_TWO_LEVEL, MODE_AO, scenes
 
     scene = scenes.soup_in_room()

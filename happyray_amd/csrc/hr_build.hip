@@ -932,46 +932,73 @@ __global__ void k_obj_reset(uint32_t* __restrict__ enc, uint32_t nobj)
     }
 }
 
+__device__ __forceinline__ void flush_bounds(uint32_t* enc, uint32_t ob, f3 lo, f3 hi)
+{
+    uint32_t* b = enc + 6 * (size_t)ob;
+    atomicMin(b + 0, enc_f(lo.x)); atomicMin(b + 1, enc_f(lo.y)); atomicMin(b + 2, enc_f(lo.z));
+    atomicMax(b + 3, enc_f(hi.x)); atomicMax(b + 4, enc_f(hi.y)); atomicMax(b + 5, enc_f(hi.z));
+}
+
 // bounds of each object's triangles on the first key frame of the current
-// frame (RTEngine.cu:270-289); a wave whose 64 triangles share an object
-// reduces them and issues one atomic per bound
+// frame (RTEngine.cu:270-289).  Each block takes a contiguous run of
+// per_block triangles; a thread keeps running bounds while its triangles stay
+// in one object (objects are contiguous triangle ranges) and flushes them at
+// a change; a block whose triangles all share one object reduces its bounds
+// and issues six atomics.
 __global__ __launch_bounds__(kBlock) void k_obj_bounds(const uint32_t* __restrict__ idx,
                                                        const float* __restrict__ kx, const float* __restrict__ ky,
                                                        const float* __restrict__ kz, uint32_t nt,
                                                        const uint32_t* __restrict__ tri_obj,
-                                                       uint32_t* __restrict__ enc)
+                                                       uint32_t* __restrict__ enc, uint32_t per_block)
 {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~(uint32_t)(kWave - 1); base < nt;
-         base += stride) {
-        const uint32_t t = base + __lane_id();
-        const bool valid = t < nt;
-        f3 lo = rep(kFltMax), hi = rep(-kFltMax);
-        uint32_t ob = 0;
-        if (valid) {
-            ob = tri_obj[t];
-            for (int c = 0; c < 3; ++c) {
-                const uint32_t v = idx[3 * (size_t)t + c];
-                const f3 p = mk(kx[v], ky[v], kz[v]);
-                lo = vmin(lo, p);
-                hi = vmax(hi, p);
-            }
+    __shared__ float red[6][kWavesPerBlock];
+    __shared__ uint32_t obj0;
+    __shared__ int uniform;
+    const uint32_t none = 0xFFFFFFFFu;
+    const uint32_t b0 = blockIdx.x * per_block;
+    const uint32_t b1 = nt - b0 < per_block ? nt : b0 + per_block;
+    uint32_t cur = none;
+    f3 lo = rep(kFltMax), hi = rep(-kFltMax);
+    for (uint32_t t = b0 + threadIdx.x; t < b1; t += kBlock) {
+        const uint32_t ob = tri_obj[t];
+        if (ob != cur) {
+            if (cur != none) flush_bounds(enc, cur, lo, hi);
+            cur = ob;
+            lo = rep(kFltMax);
+            hi = rep(-kFltMax);
         }
-        const uint32_t last = base + kWave - 1 < nt ? base + kWave - 1 : nt - 1;
-        const uint32_t o0 = tri_obj[base], o1 = tri_obj[last];
-        if (o0 == o1) {
-            lo.x = wave_min(lo.x); lo.y = wave_min(lo.y); lo.z = wave_min(lo.z);
-            hi.x = wave_max(hi.x); hi.y = wave_max(hi.y); hi.z = wave_max(hi.z);
-            if (__lane_id() == 0) {
-                uint32_t* b = enc + 6 * (size_t)o0;
-                atomicMin(b + 0, enc_f(lo.x)); atomicMin(b + 1, enc_f(lo.y)); atomicMin(b + 2, enc_f(lo.z));
-                atomicMax(b + 3, enc_f(hi.x)); atomicMax(b + 4, enc_f(hi.y)); atomicMax(b + 5, enc_f(hi.z));
-            }
-        } else if (valid) {
-            uint32_t* b = enc + 6 * (size_t)ob;
-            atomicMin(b + 0, enc_f(lo.x)); atomicMin(b + 1, enc_f(lo.y)); atomicMin(b + 2, enc_f(lo.z));
-            atomicMax(b + 3, enc_f(hi.x)); atomicMax(b + 4, enc_f(hi.y)); atomicMax(b + 5, enc_f(hi.z));
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t v = idx[3 * (size_t)t + c];
+            const f3 p = mk(kx[v], ky[v], kz[v]);
+            lo = vmin(lo, p);
+            hi = vmax(hi, p);
         }
+    }
+    if (threadIdx.x == 0) {
+        obj0 = cur;          // thread 0 has triangles whenever the block has any
+        uniform = 1;
+    }
+    __syncthreads();
+    if (cur != none && cur != obj0) uniform = 0;
+    __syncthreads();
+    if (obj0 == none) return;
+    if (!uniform) {
+        if (cur != none) flush_bounds(enc, cur, lo, hi);
+        return;
+    }
+    const float v[6] = { wave_min(lo.x), wave_min(lo.y), wave_min(lo.z), wave_max(hi.x), wave_max(hi.y),
+                         wave_max(hi.z) };
+    const int w = (int)threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int k = 0; k < 6; ++k) red[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        f3 l = mk(red[0][0], red[1][0], red[2][0]), h = mk(red[3][0], red[4][0], red[5][0]);
+        for (int i = 1; i < kWavesPerBlock; ++i) {
+            l = vmin(l, mk(red[0][i], red[1][i], red[2][i]));
+            h = vmax(h, mk(red[3][i], red[4][i], red[5][i]));
+        }
+        flush_bounds(enc, obj0, l, h);
     }
 }
 
@@ -1007,38 +1034,58 @@ __global__ void k_obj_setup(const uint32_t* __restrict__ enc, const uint32_t* __
     cell_counts[k] = stop(st) ? 0u : (uint32_t)nc;
 }
 
-// countPairs / writeKeysAndValues over instance boxes
-// (UniformGridBuildKernels.h:40-105, :180-252): one lane per instance
-template <bool kWrite>
-__global__ void k_inst_pairs(const InstanceDev* __restrict__ inst, uint32_t ninst,
-                             const GridParams* __restrict__ gp, const BuildStatus* __restrict__ st,
-                             uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+// countPairs over instance boxes (UniformGridBuildKernels.h:40-105): one lane
+// per instance
+__global__ void k_inst_count(const InstanceDev* __restrict__ inst, uint32_t ninst, const GridParams* __restrict__ gp,
+                             const BuildStatus* __restrict__ st, uint32_t* __restrict__ counts)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     if (stop(st)) {
-        if (!kWrite) counts[i] = 0u;
+        counts[i] = 0u;
         return;
     }
     const InstanceDev I = inst[i];
-    const int rx = gp->res[0], ry = gp->res[1], rz = gp->res[2];
     int mn[3], mx[3];
     box_cells(mk(I.bmin[0], I.bmin[1], I.bmin[2]), mk(I.bmax[0], I.bmax[1], I.bmax[2]),
-              mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]), mk(gp->csr[0], gp->csr[1], gp->csr[2]), rx, ry, rz, mn,
-              mx);
-    if (!kWrite) {
-        counts[i] = (uint32_t)((mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]));
-        return;
+              mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]), mk(gp->csr[0], gp->csr[1], gp->csr[2]), gp->res[0],
+              gp->res[1], gp->res[2], mn, mx);
+    counts[i] = (uint32_t)((mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]));
+}
+
+// writeKeysAndValues over instance boxes (UniformGridBuildKernels.h:180-252),
+// one lane per pair: pair j belongs to the last instance whose first pair is
+// <= j and is its (j - first)-th cell in x, then y, then z order -- the same
+// sequence a serial loop per instance writes, without one lane walking a box
+// that covers the whole grid
+__global__ void k_inst_write(const InstanceDev* __restrict__ inst, uint32_t ninst, const GridParams* __restrict__ gp,
+                             const BuildStatus* __restrict__ st, const uint32_t* __restrict__ offsets,
+                             const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals)
+{
+    if (stop(st)) return;
+    const uint32_t n = *n_dev;
+    const int rx = gp->res[0], ry = gp->res[1], rz = gp->res[2];
+    const f3 org = mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]);
+    const f3 csr = mk(gp->csr[0], gp->csr[1], gp->csr[2]);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ninst;      // offsets[lo] <= j < offsets[hi] (offsets[ninst] = n)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (offsets[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const InstanceDev I = inst[lo];
+        int mn[3], mx[3];
+        box_cells(mk(I.bmin[0], I.bmin[1], I.bmin[2]), mk(I.bmax[0], I.bmax[1], I.bmax[2]), org, csr, rx, ry, rz,
+                  mn, mx);
+        const uint32_t k = j - offsets[lo];
+        const uint32_t dx = (uint32_t)(mx[0] - mn[0]), dy = (uint32_t)(mx[1] - mn[1]);
+        const uint32_t q = k / dx;
+        const uint32_t x = (uint32_t)mn[0] + k % dx, y = (uint32_t)mn[1] + q % dy, z = (uint32_t)mn[2] + q / dy;
+        keys[j] = x + y * (uint32_t)rx + z * (uint32_t)rx * (uint32_t)ry;
+        vals[j] = lo;
     }
-    uint32_t pos = offsets[i];
-    const uint32_t sxy = (uint32_t)rx * (uint32_t)ry;
-    for (int z = mn[2]; z < mx[2]; ++z)
-        for (int y = mn[1]; y < mx[1]; ++y)
-            for (int x = mn[0]; x < mx[0]; ++x, ++pos) {
-                keys[pos] = (uint32_t)x + (uint32_t)y * (uint32_t)rx + (uint32_t)z * sxy;
-                vals[pos] = i;
-            }
 }
 
 inline uint32_t range_blocks(uint32_t cap)
@@ -1060,10 +1107,11 @@ void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s)
     hipLaunchKernelGGL(k_hier_top_setup, dim3(1), dim3(64), 0, s, b.top, st, a.ninst, a.lambda_top, b.cells_cap);
     hipLaunchKernelGGL(k_obj_reset, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, nob);
     if (nt > 0) {
-        uint32_t tblocks = cdiv(nt, kBlock);
-        if (tblocks > 2048) tblocks = 2048;
-        hipLaunchKernelGGL(k_obj_bounds, dim3(tblocks), dim3(kBlock), 0, s, a.idx, a.kx, a.ky, a.kz, nt, a.tri_obj,
-                           b.obj_bounds_enc);
+        uint32_t tblocks = cdiv(nt, 512);
+        if (tblocks > 512) tblocks = 512;
+        const uint32_t per_block = cdiv(cdiv(nt, tblocks), kBlock) * kBlock;
+        hipLaunchKernelGGL(k_obj_bounds, dim3(cdiv(nt, per_block)), dim3(kBlock), 0, s, a.idx, a.kx, a.ky, a.kz, nt,
+                           a.tri_obj, b.obj_bounds_enc, per_block);
     }
     hipLaunchKernelGGL(k_obj_setup, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, a.obj_first, nob,
                        a.lambda_leaf, b.obj_gp, b.obj_cell_counts, st);
@@ -1093,12 +1141,11 @@ void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s)
 
     // ---- top references: count, scan, write, sort, clear the top cells, ranges (and seal) ----
     const uint32_t iblocks = cdiv(a.ninst > 0 ? a.ninst : 1, 256);
-    hipLaunchKernelGGL((k_inst_pairs<false>), dim3(iblocks), dim3(256), 0, s, a.inst, a.ninst, b.top, st,
-                       b.inst_counts, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_inst_count, dim3(iblocks), dim3(256), 0, s, a.inst, a.ninst, b.top, st, b.inst_counts);
     const ScanCheck inst_check{ st, b.pairs_cap, kOverflowPairs, &st->n_pairs_eff };
     scan_exclusive_1p(b.inst_counts, b.inst_offsets, 1, a.ninst, nullptr, &st->num_pairs, &inst_check, b.scan, s);
-    hipLaunchKernelGGL((k_inst_pairs<true>), dim3(iblocks), dim3(256), 0, s, a.inst, a.ninst, b.top, st, nullptr,
-                       b.inst_offsets, b.keys, b.vals);
+    hipLaunchKernelGGL(k_inst_write, dim3(range_blocks(b.pairs_cap)), dim3(256), 0, s, a.inst, a.ninst, b.top, st,
+                       b.inst_offsets, &st->n_pairs_eff, b.keys, b.vals);
     radix_sort_pairs(b.keys, b.vals, b.keys_sorted, b.inst_refs, b.pairs_cap, &st->n_pairs_eff,
                      bits_for(b.cells_cap), b.sort, s);
     fill_u32((uint32_t*)b.top_cells, 0u, 2 * (size_t)b.cells_cap, &b.top->num_cells, 2, s);

@@ -51,8 +51,31 @@ __device__ __forceinline__ f3 instance_normal(const InstanceDev& I, f3 n)
 // into object coordinates, step the object grid (entry EPS past its box, the
 // entry cell clamped on the side the ray leaves by), keep the closest hit
 // that lies inside the instance box interval
+//
+// An instance that spans several top cells is met again in each of them.  A
+// second visit cannot change the result: the first one already kept the
+// closest hit of the whole object below the ray's distance at the time, the
+// distance has only shrunk since, and the box tests that skip an instance
+// fail again for a shorter ray.  So each ray remembers the instances it has
+// visited in a small direct-mapped table (id & 7) and skips them.
+struct Mailbox {
+    uint32_t e[8];
+};
+
+__device__ __forceinline__ bool mailbox_seen(Mailbox& m, uint32_t id)
+{
+    const uint32_t k = id & 7u;
+    uint32_t v = 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) v = j == k ? m.e[j] : v;
+    if (v == id) return true;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) m.e[j] = j == k ? id : m.e[j];
+    return false;
+}
+
 __device__ void hier_cell(const HierScene& H, uint32_t b, uint32_t e, f3 org, f3 rcp, float& t, uint32_t& best,
-                          uint32_t& inst, Work& wk)
+                          uint32_t& inst, Mailbox& mb, Work& wk)
 {
     TraceScene S{};
     S.prims = H.prims;
@@ -60,6 +83,7 @@ __device__ void hier_cell(const HierScene& H, uint32_t b, uint32_t e, f3 org, f3
     S.nt = H.nt;
     for (uint32_t it = b; it < e; ++it) {
         const uint32_t ii = H.inst_refs[it];
+        if (mailbox_seen(mb, ii)) continue;
         const InstanceDev I = H.inst[ii];
         const f3 t1 = (mk(I.bmin[0], I.bmin[1], I.bmin[2]) - org) * rcp;
         f3 tm = (mk(I.bmax[0], I.bmax[1], I.bmax[2]) - org) * rcp;
@@ -133,16 +157,23 @@ constexpr float kShadowStop = 0.9999f;
 
 // TLGridHierarchyTraverser::operator() (TLGridHierarchyTraverser.h:235-400):
 // the top grid is entered EPS past its box without clamping the entry cell;
-// cells outside the grid read as empty
+// cells outside the grid read as empty.  A shadow ray stops after the top
+// cell that leaves its distance below tterm: 0.9999 in the reference; an
+// occlusion test against tmax >= 0.9999 may stop once the distance is below
+// tmax, since the distance only shrinks and the ray is occluded either way.
 template <bool kAny>
-__device__ uint32_t trace_hier(const HierScene& H, f3 org, f3 d, float& t, f3& dir_out, uint32_t& inst, Work& wk)
+__device__ uint32_t trace_hier(const HierScene& H, f3 org, f3 d, float& t, float tterm, f3& dir_out, uint32_t& inst,
+                               Work& wk)
 {
+    Mailbox mb;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mb.e[j] = 0xFFFFFFFFu;
     const f3 rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
     dir_out = mk(1.f / rcp.x, 1.f / rcp.y, 1.f / rcp.z);
     uint32_t best = H.nt;
     inst = 0xFFFFFFFFu;
     const GridRegs G = load_grid(H.top);
-    bool flag = !(kAny && t < kShadowStop);
+    bool flag = !(kAny && t < tterm);
     const f3 t1 = (G.bmin - org) * rcp;
     f3 tfar = (G.bmax - org) * rcp;
     const f3 tnear = vmin(t1, tfar);
@@ -166,9 +197,9 @@ __device__ uint32_t trace_hier(const HierScene& H, f3 org, f3 d, float& t, f3& d
         ++wk.steps;
         if (cx < G.rx && cx > -1 && cy < G.ry && cy > -1 && cz < G.rz && cz > -1) {
             const uint2 cell = H.top_cells[(uint32_t)cx + (uint32_t)cy * (uint32_t)G.rx + (uint32_t)cz * sxy];
-            hier_cell(H, cell.x, cell.y, org, rcp, t, best, inst, wk);
+            hier_cell(H, cell.x, cell.y, org, rcp, t, best, inst, mb, wk);
         }
-        if (kAny && t < kShadowStop) flag = false;
+        if (kAny && t < tterm) flag = false;
         const int dm = min_dim(tx, ty, tz);
         if (dm == 0) {
             flag = flag && t > tx;
@@ -203,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_hier(HierScene H, const float*
     float t = kFltMax;
     f3 d2;
     uint32_t inst;
-    const uint32_t best = trace_hier<kAny>(H, o, d, t, d2, inst, wk);
+    const uint32_t best = trace_hier<kAny>(H, o, d, t, fmaxf(kShadowStop, tmax), d2, inst, wk);
     if (kAny) {
         occ_out[i] = t < tmax ? 1u : 0u;
         if (t_out) t_out[i] = t;
@@ -232,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_primary_hier(HierScene H, CameraDev 
     float t = kFltMax;
     f3 dir;
     uint32_t inst;
-    const uint32_t best = trace_hier<false>(H, org, d0, t, dir, inst, wk);
+    const uint32_t best = trace_hier<false>(H, org, d0, t, 0.f, dir, inst, wk);
     const f3 hp = org + (t - kEps) * dir;
     hits[pix] = hp.x;
     hits[(size_t)npix + pix] = hp.y;
@@ -291,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_rays_hier(HierScene H, RenderArgs
     float t = kFltMax;
     f3 d2;
     uint32_t inst;
-    trace_hier<true>(H, org, dir, t, d2, inst, wk);
+    trace_hier<true>(H, org, dir, t, fmaxf(kShadowStop, ra.ao_dist), d2, inst, wk);
     occ[j] = t < ra.ao_dist ? 1u : 0u;
 }
 

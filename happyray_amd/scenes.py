@@ -419,13 +419,14 @@ def _rotation(rng):
                      [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
 
 
-def instanced_scene(seed=6, n_inst=64, subdiv=2, n_soup=200, extent=20.0, res=(64, 48)):
+def instanced_scene(seed=6, n_inst=64, subdiv=2, n_soup=200, extent=20.0, res=(64, 48), soup_size=0.3):
     """Three objects: 0 a room box of side `extent` centred at the origin
-    (8x8 quads per face), 1 a unit icosphere, 2 a triangle soup in [-1, 1]^3.
+    (8x8 quads per face), 1 a unit icosphere, 2 a triangle soup in [-1, 1]^3
+    (corners within +-soup_size of a centre).
     Instance 0 shows the room unmoved; the others place spheres and soups with
     a random rotation, a scale in [0.5, 1.5] and a translation inside the room.
-    An instance box is the world bounds of its transformed vertices padded by
-    1e-3.  Returns (Scene, Instancing); the mesh holds every object once, in
+    An instance box is the world bounds of the 8 transformed corners of its
+    object's box, padded by 1e-3 (it holds every transformed triangle).  Returns (Scene, Instancing); the mesh holds every object once, in
     object coordinates, so its bounds (the top grid's) are the room's."""
     rng = np.random.default_rng(seed)
     h = extent / 2.0
@@ -442,7 +443,7 @@ def instanced_scene(seed=6, n_inst=64, subdiv=2, n_soup=200, extent=20.0, res=(6
     room_v, room_f = np.concatenate(rv), np.concatenate(rf)
     sph_v, sph_f = icosphere(subdiv)
     c = rng.uniform(-0.7, 0.7, size=(n_soup, 1, 3))
-    soup_v = np.clip(c + rng.uniform(-0.3, 0.3, size=(n_soup, 3, 3)), -1.0, 1.0).reshape(-1, 3)
+    soup_v = np.clip(c + rng.uniform(-soup_size, soup_size, size=(n_soup, 3, 3)), -1.0, 1.0).reshape(-1, 3)
     soup_f = np.arange(3 * n_soup, dtype=np.uint32).reshape(-1, 3)
     mats = [material((0.5, 0.5, 0.5)), material((0.7, 0.3, 0.2)), material((0.2, 0.4, 0.7))]
     cam = dict(pos=(-0.9 * h, -0.8 * h, 0.85 * h), dir=(1.0, 0.9, -0.8), up=(0.0, 0.0, 1.0), fov=70.0,
@@ -455,6 +456,11 @@ def instanced_scene(seed=6, n_inst=64, subdiv=2, n_soup=200, extent=20.0, res=(6
     mats_w2o = [np.hstack([np.eye(3), np.zeros((3, 1))])]
     lo = [room_v.min(0) - 1e-3]
     hi = [room_v.max(0) + 1e-3]
+    corners = []
+    for ov in obj_verts:
+        a, b = ov.min(0), ov.max(0)
+        corners.append(np.array([[(a, b)[i][0], (a, b)[j][1], (a, b)[k][2]]
+                                 for i in (0, 1) for j in (0, 1) for k in (0, 1)]))
     for i in range(1, n_inst):
         k = 1 + (i % 2)
         R = _rotation(rng)
@@ -462,7 +468,7 @@ def instanced_scene(seed=6, n_inst=64, subdiv=2, n_soup=200, extent=20.0, res=(6
         t = rng.uniform(-0.75 * h, 0.75 * h, size=3)
         A = R.T / s                              # world -> object
         M = np.hstack([A, (-A @ t)[:, None]])
-        w = (obj_verts[k] * s) @ R.T + t          # object -> world
+        w = (corners[k] * s) @ R.T + t           # object -> world
         objects.append(k)
         mats_w2o.append(M)
         lo.append(w.min(0) - 1e-3)
