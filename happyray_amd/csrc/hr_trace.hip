@@ -186,140 +186,42 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = pix < npix;
     const GridRegs G = load_grid(S.gp);
-    f3 radiance = rep(0.f);
-    float ray_t = in ? hits[6 * (size_t)npix + pix] : kFltMax;
-    uint32_t prim = in ? hit_prims[pix] : S.nt;
+    PathState P;
+    P.radiance = rep(0.f);
+    P.ray_t = in ? hits[6 * (size_t)npix + pix] : kFltMax;
+    P.prim = in ? hit_prims[pix] : S.nt;
     uint32_t traced = 0;
     Work wk;
-    if (ray_t < kFltMax && prim < S.nt) {
-        f3 org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
-        f3 dir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix],
-                    hits[5 * (size_t)npix + pix]);
-        f3 att = rep(1.f);
-        bool last_spec = true;
-        for (int depth = 0; depth < ra.pt_depth; ++depth) {
-            float u[4], w[4];
-            rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPT + 2u * (uint32_t)depth, u);
-            rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPT + 2u * (uint32_t)depth + 1u,
-                  w);
-            f3 v0, v1, v2;
-            load_tri(S.rec, prim, v0, v1, v2);
-            f3 n = shading_normal(sh, prim, org, v0, v1, v2);
-            float cos_ne = -dot(n, dir);
-            const float* m = sh.mats + 12 * (size_t)(sh.mat_idx ? sh.mat_idx[prim] : 0u);
-            const f3 kd = mk(m[0], m[1], m[2]);
-            const f3 ks = mk(m[4], m[5], m[6]);
-            const f3 ke = mk(m[8], m[9], m[10]);
-            const float se = m[7];
-            const float ior = m[3] - 10.f;
-            bool end = false;
-            if (ior > 0.f) {
-                // transparent (PathTracer.h:133-183)
-                last_spec = true;
-                float n1 = 1.000293f, n2 = ior;
-                if (cos_ne < 0.f) {
-                    n1 = ior;
-                    n2 = 1.000293f;
-                    n = -n;
-                    cos_ne = -cos_ne;
-                    att.x *= spec_expf(-ray_t * kd.x * kPi);
-                    att.y *= spec_expf(-ray_t * kd.y * kPi);
-                    att.z *= spec_expf(-ray_t * kd.z * kPi);
+    if (P.ray_t < kFltMax && P.prim < S.nt) {
+        P.org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+        P.dir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix], hits[5 * (size_t)npix + pix]);
+        P.att = rep(1.f);
+        P.last_spec = true;
+        for (P.depth = 0; P.depth < ra.pt_depth; ++P.depth) {
+            PathVertex V;
+            pt_vertex(S, sh, ra, pix, P, V);
+            if (V.opaque) {
+                if (V.shadow) {
+                    float st = 0.9999f;
+                    f3 dd;
+                    trace_ray<kTwo, true>(G, S, P.org, V.sdir, st, 0.9999f, dd, wk);
+                    ++traced;
+                    pt_light(P, V, st);
                 }
-                const float f0 = (n1 - n2) * (n1 - n2) / ((n1 + n2) * (n1 + n2));
-                const float fres = f0 + (1.f - f0) * spec_powf(1.f - fabsf(cos_ne), 5.f);
-                const float nr = n1 / n2;
-                const float sin2 = nr * nr * (1.f - cos_ne * cos_ne);
-                if (sin2 >= 1.f || w[2] < fres) {
-                    org = org + (5.f * kEps) * n;
-                    dir = dir + (2 * cos_ne) * n;
-                    if (sin2 >= 1.f) {
-                        if (w[3] < 0.5f) att = att * 2.f;
-                        else end = true;
-                    }
-                } else {
-                    org = org - (5.f * kEps) * n;
-                    const float ns = nr * cos_ne - sqrtf(1.f - sin2);
-                    dir = nr * dir + ns * n;
-                }
-            } else {
-                // opaque (PathTracer.h:184-313)
-                if (cos_ne < 0.f) { n = -n; cos_ne = -cos_ne; }
-                org = org + (5.f * kEps) * n;
-                if (last_spec) radiance = radiance + att * ke;
-                last_spec = false;
-                if (sh.nlights > 0) {
-                    // AreaLightSourceCollection::getLight (LightSource.hpp:116-128)
-                    int li = sh.nlights - 1;
-                    for (int i = 0; i < sh.nlights - 1; ++i)
-                        if (u[0] < sh.light_cdf[i] / sh.light_cdf[sh.nlights - 1]) { li = i; break; }
-                    const float* L = sh.lights + 15 * (size_t)li;
-                    const f3 lpos = mk(L[0], L[1], L[2]), lnrm = mk(L[3], L[4], L[5]);
-                    const f3 lint = mk(L[6], L[7], L[8]), le1 = mk(L[9], L[10], L[11]),
-                             le2 = mk(L[12], L[13], L[14]);
-                    const f3 lp = lpos + u[1] * le1 + u[2] * le2;
-                    const f3 sdir = lp - org;
-                    const float d2r = 1.f / dot(sdir, sdir);
-                    const f3 sdn = sdir * sqrtf(d2r);
-                    const float cos_nl = dot(n, sdn);
-                    const float cos_hl = fmaxf(0.f, dot(normalize(sdn - dir), n));
-                    const float cos_ln = dot(-lnrm, sdn);
-                    if (cos_nl > 0.f && cos_ln > 0.f) {
-                        float st = 0.9999f;
-                        f3 dd;
-                        trace_ray<kTwo, true>(G, S, org, sdir, st, 0.9999f, dd, wk);
-                        ++traced;
-                        if (!(st < 0.9999f)) {
-                            const float area = len(cross(le1, le2));
-                            const f3 ls = ((lint * area) * d2r) * cos_ln;
-                            radiance = radiance + ((att * kd) * ls) * cos_nl;
-                            if (!last_spec)
-                                radiance = radiance +
-                                           (((att * ks) * fmaxf(0.f, spec_powf(cos_hl, se))) * ls) * cos_nl;
-                        }
-                    }
-                }
-                if (depth + 1 >= ra.pt_depth) break;
-                const float albedo_d = hmax(kd) * kPi;
-                const float albedo_s = hmax(ks) * 2.f * kPi / (se + 2);
-                const float rnum = u[3];
-                if (rnum < albedo_d) {
-                    att = ((att * kd) * kPi) / albedo_d;
-                    const f3 rd = cosine_dir(w[0], w[1]);
-                    f3 tg, bn;
-                    local_frame(n, tg, bn);
-                    dir = tg * rd.x + bn * rd.y + n * rd.z;
-                } else if (rnum < albedo_d + albedo_s) {
-                    // PowerCosineHemisphereSampler (HemisphereSamplers.hpp:78-94)
-                    f3 rd;
-                    rd.z = spec_powf(w[1], 1.f / (se + 1.f));
-                    float s_, c_;
-                    spec_sincosf(2.f * kPi * w[0], s_, c_);
-                    rd.x = c_ * sqrtf(1 - rd.z * rd.z);
-                    rd.y = s_ * sqrtf(1 - rd.z * rd.z);
-                    const f3 upv = dir + (2.f * cos_ne) * n;
-                    f3 rt, fw;
-                    local_frame(upv, rt, fw);
-                    dir = rt * rd.x + fw * rd.y + upv * rd.z;
-                    last_spec = true;
-                    const float cnl = dot(dir, n);
-                    if (cnl < 0.f) end = true;
-                    else att = (((att * cnl) * ks) * kPi) / ((se + 1.f) * 0.5f * albedo_s);
-                } else {
-                    end = true;
-                }
+                if (P.depth + 1 >= ra.pt_depth) break;
+                pt_bounce(P, V);
             }
-            if (end || depth + 1 >= ra.pt_depth) break;
+            if (V.end || P.depth + 1 >= ra.pt_depth) break;
             // extension ray (PathTracer.h:318-332)
             float t = kFltMax;
             f3 d2;
-            const uint32_t hit = trace_ray<kTwo, false>(G, S, org, dir, t, 0.f, d2, wk);
+            const uint32_t hit = trace_ray<kTwo, false>(G, S, P.org, P.dir, t, 0.f, d2, wk);
             ++traced;
-            dir = d2;
+            P.dir = d2;
             if (!(t < kFltMax)) break;
-            org = org + (t - kEps) * dir;
-            prim = hit;
-            ray_t = t;
+            P.org = P.org + (t - kEps) * P.dir;
+            P.prim = hit;
+            P.ray_t = t;
         }
     }
     if (ray_count && traced) atomicAdd(ray_count, (unsigned long long)traced);
@@ -328,9 +230,9 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
     const float nw = 1.f / (float)(image_of(ra) + 1);
     const float ow = 1.f - nw;
     float* o = fb + 3 * (size_t)pix;
-    o[0] = o[0] * ow + radiance.x * nw;
-    o[1] = o[1] * ow + radiance.y * nw;
-    o[2] = o[2] * ow + radiance.z * nw;
+    o[0] = o[0] * ow + P.radiance.x * nw;
+    o[1] = o[1] * ow + P.radiance.y * nw;
+    o[2] = o[2] * ow + P.radiance.z * nw;
 }
 
 // ---------------------------------------------------------------------------

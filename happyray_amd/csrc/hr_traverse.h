@@ -322,5 +322,164 @@ __device__ __forceinline__ f3 cosine_dir(float u1, float u2)
     return r;
 }
 
+// ---------------------------------------------------------------------------
+// path tracer vertex (PathTracer.h:41-372), shared by the one-lane-per-pixel
+// megakernel and the persistent path kernel so both apply the same operations
+// ---------------------------------------------------------------------------
+
+struct PathState {
+    f3 org, dir, att, radiance;
+    float ray_t;
+    uint32_t prim;
+    int depth;
+    bool last_spec;
+};
+
+// what a vertex leaves for its shadow ray and its bounce
+struct PathVertex {
+    bool opaque, end, shadow;
+    f3 n;
+    float cos_ne;
+    f3 kd, ks;
+    float se, u3, w0, w1;
+    f3 sdir;           // shadow ray direction (to the light point), from P.org
+    f3 add_d, add_s;   // next-event terms added when the shadow ray is unoccluded
+};
+
+// one path vertex up to its shadow ray: random numbers, material, the
+// transparent branch in full (PathTracer.h:133-183), the opaque branch's
+// offset, emission and light sample (PathTracer.h:184-260)
+__device__ __forceinline__ void pt_vertex(const TraceScene& S, const Shading& sh, const RenderArgs& ra, uint32_t pix,
+                                          PathState& P, PathVertex& V)
+{
+    float u[4], w[4];
+    rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPT + 2u * (uint32_t)P.depth, u);
+    rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPT + 2u * (uint32_t)P.depth + 1u, w);
+    f3 v0, v1, v2;
+    load_tri(S.rec, P.prim, v0, v1, v2);
+    f3 n = shading_normal(sh, P.prim, P.org, v0, v1, v2);
+    float cos_ne = -dot(n, P.dir);
+    const float* m = sh.mats + 12 * (size_t)(sh.mat_idx ? sh.mat_idx[P.prim] : 0u);
+    const f3 kd = mk(m[0], m[1], m[2]);
+    const f3 ks = mk(m[4], m[5], m[6]);
+    const f3 ke = mk(m[8], m[9], m[10]);
+    const float se = m[7];
+    const float ior = m[3] - 10.f;
+    V.end = false;
+    V.shadow = false;
+    V.opaque = !(ior > 0.f);
+    if (!V.opaque) {
+        P.last_spec = true;
+        float n1 = 1.000293f, n2 = ior;
+        if (cos_ne < 0.f) {
+            n1 = ior;
+            n2 = 1.000293f;
+            n = -n;
+            cos_ne = -cos_ne;
+            P.att.x *= spec_expf(-P.ray_t * kd.x * kPi);
+            P.att.y *= spec_expf(-P.ray_t * kd.y * kPi);
+            P.att.z *= spec_expf(-P.ray_t * kd.z * kPi);
+        }
+        const float f0 = (n1 - n2) * (n1 - n2) / ((n1 + n2) * (n1 + n2));
+        const float fres = f0 + (1.f - f0) * spec_powf(1.f - fabsf(cos_ne), 5.f);
+        const float nr = n1 / n2;
+        const float sin2 = nr * nr * (1.f - cos_ne * cos_ne);
+        if (sin2 >= 1.f || w[2] < fres) {
+            P.org = P.org + (5.f * kEps) * n;
+            P.dir = P.dir + (2 * cos_ne) * n;
+            if (sin2 >= 1.f) {
+                if (w[3] < 0.5f) P.att = P.att * 2.f;
+                else V.end = true;
+            }
+        } else {
+            P.org = P.org - (5.f * kEps) * n;
+            const float ns = nr * cos_ne - sqrtf(1.f - sin2);
+            P.dir = nr * P.dir + ns * n;
+        }
+        return;
+    }
+    if (cos_ne < 0.f) { n = -n; cos_ne = -cos_ne; }
+    P.org = P.org + (5.f * kEps) * n;
+    if (P.last_spec) P.radiance = P.radiance + P.att * ke;
+    P.last_spec = false;
+    V.n = n;
+    V.cos_ne = cos_ne;
+    V.kd = kd;
+    V.ks = ks;
+    V.se = se;
+    V.u3 = u[3];
+    V.w0 = w[0];
+    V.w1 = w[1];
+    if (sh.nlights > 0) {
+        // AreaLightSourceCollection::getLight (LightSource.hpp:116-128)
+        int li = sh.nlights - 1;
+        for (int i = 0; i < sh.nlights - 1; ++i)
+            if (u[0] < sh.light_cdf[i] / sh.light_cdf[sh.nlights - 1]) { li = i; break; }
+        const float* L = sh.lights + 15 * (size_t)li;
+        const f3 lpos = mk(L[0], L[1], L[2]), lnrm = mk(L[3], L[4], L[5]);
+        const f3 lint = mk(L[6], L[7], L[8]), le1 = mk(L[9], L[10], L[11]), le2 = mk(L[12], L[13], L[14]);
+        const f3 lp = lpos + u[1] * le1 + u[2] * le2;
+        const f3 sdir = lp - P.org;
+        const float d2r = 1.f / dot(sdir, sdir);
+        const f3 sdn = sdir * sqrtf(d2r);
+        const float cos_nl = dot(n, sdn);
+        const float cos_hl = fmaxf(0.f, dot(normalize(sdn - P.dir), n));
+        const float cos_ln = dot(-lnrm, sdn);
+        if (cos_nl > 0.f && cos_ln > 0.f) {
+            V.shadow = true;
+            V.sdir = sdir;
+            const float area = len(cross(le1, le2));
+            const f3 ls = ((lint * area) * d2r) * cos_ln;
+            V.add_d = ((P.att * kd) * ls) * cos_nl;
+            V.add_s = (((P.att * ks) * fmaxf(0.f, spec_powf(cos_hl, se))) * ls) * cos_nl;
+        }
+    }
+}
+
+// the shadow ray's outcome: the light sample counts when nothing is closer
+// than 0.9999 (PathTracer.h:244-258)
+__device__ __forceinline__ void pt_light(PathState& P, const PathVertex& V, float shadow_t)
+{
+    if (!(shadow_t < 0.9999f)) {
+        P.radiance = P.radiance + V.add_d;
+        if (!P.last_spec) P.radiance = P.radiance + V.add_s;
+    }
+}
+
+// Russian roulette and the bounce of an opaque vertex (PathTracer.h:262-313)
+__device__ __forceinline__ void pt_bounce(PathState& P, PathVertex& V)
+{
+    const f3 kd = V.kd, ks = V.ks, n = V.n;
+    const float se = V.se;
+    const float albedo_d = hmax(kd) * kPi;
+    const float albedo_s = hmax(ks) * 2.f * kPi / (se + 2);
+    const float rnum = V.u3;
+    if (rnum < albedo_d) {
+        P.att = ((P.att * kd) * kPi) / albedo_d;
+        const f3 rd = cosine_dir(V.w0, V.w1);
+        f3 tg, bn;
+        local_frame(n, tg, bn);
+        P.dir = tg * rd.x + bn * rd.y + n * rd.z;
+    } else if (rnum < albedo_d + albedo_s) {
+        // PowerCosineHemisphereSampler (HemisphereSamplers.hpp:78-94)
+        f3 rd;
+        rd.z = spec_powf(V.w1, 1.f / (se + 1.f));
+        float s_, c_;
+        spec_sincosf(2.f * kPi * V.w0, s_, c_);
+        rd.x = c_ * sqrtf(1 - rd.z * rd.z);
+        rd.y = s_ * sqrtf(1 - rd.z * rd.z);
+        const f3 upv = P.dir + (2.f * V.cos_ne) * n;
+        f3 rt, fw;
+        local_frame(upv, rt, fw);
+        P.dir = rt * rd.x + fw * rd.y + upv * rd.z;
+        P.last_spec = true;
+        const float cnl = dot(P.dir, n);
+        if (cnl < 0.f) V.end = true;
+        else P.att = (((P.att * cnl) * ks) * kPi) / ((se + 1.f) * 0.5f * albedo_s);
+    } else {
+        V.end = true;
+    }
+}
+
 }  // namespace trav
 }  // namespace hr
