@@ -213,7 +213,7 @@ SIGNATURES = {
     "hr_trace_wire": (C.c_int, [_vp, _f32p, C.c_uint32, _f32p, _u32p]),
     "hr_occluded_wire": (C.c_int, [_vp, _f32p, C.c_uint32, C.c_float, _u8p]),
     "hr_render_rays": (C.c_int, [_vp, _f32p, C.c_uint32, _f32p]),
-    "hr_dump_ao_rays": (C.c_int, [_vp, _f32p]),
+    "hr_dump_ao_rays": (C.c_int, [_vp, _f32p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "hr_image_to_rgb8": (C.c_int, [_f32p, C.c_int, C.c_int, C.c_float, _u8p]),
     "hr_write_png": (C.c_int, [C.c_char_p, _u8p, C.c_int, C.c_int]),
     "hr_write_screenshot": (C.c_int, [C.c_char_p, _f32p, C.c_int, C.c_int]),

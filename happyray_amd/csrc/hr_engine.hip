@@ -93,7 +93,11 @@ struct hr_ctx {
     DevBuf<uint32_t> tri_obj, obj_first, obj_bounds_enc, obj_cell_counts, obj_cell_base, inst_counts, inst_offsets;
     DevBuf<hr::InstanceDev> inst;
     DevBuf<hr::GridParams> obj_gp;
-    DevBuf<uint32_t> hit_inst;
+    DevBuf<uint32_t> hit_inst;        // instance of each pixel's primary hit (hierarchy renders)
+    DevBuf<uint32_t> ray_inst;        // instance of each traced ray's hit (hr_trace_instanced)
+    uint32_t hier_npix = 0;           // pixels of the last render when it was a hierarchy render
+    bool ao_dump_ok = false;          // the last render was an AO render of a grid (hits, last_ao_*)
+    uint32_t occ_npix = 0;            // pixels of the last render when it was an AO render
     hr::GridParams h_params{};
 
     // camera / lights
@@ -598,6 +602,9 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         sh.nidx = c->nidx.p;
     }
     if (events) HR_HIP(hipEventRecord(c->ev_r0, c->stream));
+    c->hier_npix = c->kind == HR_GRID_HIERARCHY ? npix : 0u;
+    c->ao_dump_ok = false;
+    c->occ_npix = mode == HR_MODE_AO ? npix : 0u;
     if (c->kind == HR_GRID_HIERARCHY) {
         // TLGHAOIntegrator::integrate (TLGridHierarchyAOIntegrator.h:421-573)
         hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
@@ -641,6 +648,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         c->last_ao_ra = ra;
         c->last_ao_ra.stats = nullptr;
         c->last_ao_cam = c->cam;
+        c->ao_dump_ok = !fs;
     } else if (mode == HR_MODE_PATH) {
         hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
@@ -777,7 +785,7 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes() + c->pm.bytes() + c->pm_prims.bytes();
     b += c->ray_wire.bytes() + c->tri_obj.bytes() + c->obj_first.bytes() + c->obj_bounds_enc.bytes();
     b += c->obj_cell_counts.bytes() + c->obj_cell_base.bytes() + c->inst_counts.bytes() + c->inst_offsets.bytes();
-    b += c->inst.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes();
+    b += c->inst.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes() + c->ray_inst.bytes();
     return b;
 }
 
@@ -900,7 +908,7 @@ hr_status hr_destroy(hr_ctx* c)
                                   &c->sort_chain, &c->sort_keys_tmp, &c->sort_vals_tmp, &c->sort_hist,
                                   &c->hit_prims, &c->ray_prim, &c->tri_obj, &c->obj_first,
                                   &c->obj_bounds_enc, &c->obj_cell_counts, &c->obj_cell_base, &c->inst_counts,
-                                  &c->inst_offsets, &c->hit_inst };
+                                  &c->inst_offsets, &c->hit_inst, &c->ray_inst };
     for (auto* b : ubufs) b->release();
     c->occ.release();
     c->ray_occ.release();
@@ -1239,10 +1247,11 @@ hr_status hr_get_primary_hits(hr_ctx* c, float* planes7, uint32_t* prims)
 hr_status hr_get_ao_occlusion(hr_ctx* c, uint8_t* occ)
 {
     HR_TRY(check_ctx(c));
-    if (!c->last_npix || !c->last_ao_samples) return fail(HR_ERR_STATE, "no AO render");
+    if (!c->occ_npix || c->occ_npix != c->last_npix || !c->last_ao_samples)
+        return fail(HR_ERR_STATE, "the last render was not an AO render");
     if (!occ) return fail(HR_ERR_ARG, "null output pointer");
     HR_HIP(hipStreamSynchronize(c->stream));
-    HR_HIP(hipMemcpy(occ, c->occ.p, (size_t)c->last_npix * (size_t)c->last_ao_samples, hipMemcpyDeviceToHost));
+    HR_HIP(hipMemcpy(occ, c->occ.p, (size_t)c->occ_npix * (size_t)c->last_ao_samples, hipMemcpyDeviceToHost));
     return HR_OK;
 }
 
@@ -1294,10 +1303,10 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
         HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
         HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
     }
-    if (inst_out) HR_TRY(c->hit_inst.reserve(n));
+    if (inst_out) HR_TRY(c->ray_inst.reserve(n));
     if (hier)
         hr::launch_trace_hier(hier_scene(c), c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
-                              inst_out ? c->hit_inst.p : nullptr, c->ray_occ.p, c->ray_count.p, c->stream);
+                              inst_out ? c->ray_inst.p : nullptr, c->ray_occ.p, c->ray_count.p, c->stream);
     else if (pers)
         hr::persist_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
                           c->ray_occ.p, c->pool.p, refill, c->ray_count.p, c->work.p, c->stream);
@@ -1311,7 +1320,7 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
                               c->stream));
     if (occ_out) HR_HIP(hipMemcpyAsync(occ_out, c->ray_occ.p, (size_t)n, hipMemcpyDeviceToHost, c->stream));
     if (inst_out)
-        HR_HIP(hipMemcpyAsync(inst_out, c->hit_inst.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+        HR_HIP(hipMemcpyAsync(inst_out, c->ray_inst.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
                               c->stream));
     HR_HIP(hipStreamSynchronize(c->stream));
     return HR_OK;
@@ -1370,13 +1379,16 @@ hr_status hr_render_rays(hr_ctx* c, const float* rays, uint32_t n, float* rgb_ou
     return HR_OK;
 }
 
-hr_status hr_dump_ao_rays(hr_ctx* c, float* rays_out)
+hr_status hr_dump_ao_rays(hr_ctx* c, float* rays_out, uint64_t max_rays, uint64_t* count)
 {
     HR_TRY(check_ctx(c));
     HR_TRY(no_hierarchy(c, "the AO ray dump"));
-    if (!rays_out) return fail(HR_ERR_ARG, "null output buffer");
-    if (!c->last_npix || !c->last_ao_samples) return fail(HR_ERR_STATE, "no AO render");
-    const size_t n = (size_t)c->last_npix * (size_t)(c->last_ao_samples + 1);
+    if (!c->ao_dump_ok) return fail(HR_ERR_STATE, "the last render was not an AO render");
+    const size_t npix = (size_t)c->last_ao_cam.res_x * (size_t)c->last_ao_cam.res_y;
+    const size_t n = npix * (size_t)(c->last_ao_ra.ao_samples + 1);
+    if (count) *count = n;
+    if (!rays_out) return HR_OK;
+    if (max_rays < n) return fail(HR_ERR_ARG, "output holds fewer records than the dump");
     HR_TRY(c->ray_wire.reserve(6 * n));
     hr::launch_ao_dump(trace_scene(c), c->last_ao_cam, c->last_ao_ra, c->hits.p, c->hit_prims.p, c->ray_wire.p, c->stream);
     HR_HIP(hipGetLastError());
@@ -1610,7 +1622,8 @@ hr_status hr_get_primary_instances(hr_ctx* c, uint32_t* inst)
 {
     HR_TRY(check_ctx(c));
     if (c->kind != HR_GRID_HIERARCHY) return fail(HR_ERR_STATE, "the grid is not an instanced hierarchy");
-    if (!c->last_npix || !c->hit_inst.p) return fail(HR_ERR_STATE, "nothing rendered");
+    if (!c->last_npix || c->hier_npix != c->last_npix || c->hit_inst.cap < c->last_npix)
+        return fail(HR_ERR_STATE, "the last render was not a hierarchy render");
     if (!inst) return fail(HR_ERR_ARG, "null output pointer");
     HR_HIP(hipStreamSynchronize(c->stream));
     HR_HIP(hipMemcpy(inst, c->hit_inst.p, (size_t)c->last_npix * sizeof(uint32_t), hipMemcpyDeviceToHost));

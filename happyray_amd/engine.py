@@ -310,11 +310,16 @@ class Engine:
         check(self._L.hr_render_rays(self._h, _f(r), len(r), _f(rgb)), "hr_render_rays")
         return rgb
 
-    def dump_ao_rays(self, ao_samples):
-        """The last AO render's rays as AORayExporter dumps them: (npix * (S + 1), 6)."""
-        n = self._res[0] * self._res[1] * (int(ao_samples) + 1)
+    def dump_ao_rays(self, ao_samples=None):
+        """The last AO render's rays as AORayExporter dumps them: (npix * (S + 1), 6).
+        ao_samples, when given, is checked against the dump's size."""
+        cnt = C.c_uint64()
+        check(self._L.hr_dump_ao_rays(self._h, None, 0, C.byref(cnt)), "hr_dump_ao_rays")
+        n = cnt.value
+        if ao_samples is not None and n != self._res[0] * self._res[1] * (int(ao_samples) + 1):
+            raise ValueError("the last AO render used another sample count or resolution")
         out = np.zeros((n, 6), np.float32)
-        check(self._L.hr_dump_ao_rays(self._h, _f(out)), "hr_dump_ao_rays")
+        check(self._L.hr_dump_ao_rays(self._h, _f(out), n, None), "hr_dump_ao_rays")
         return out
 
     def path_trace(self, org, dirs, depth=4):

@@ -244,7 +244,8 @@ hr_status hr_render_frame_graph(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coe
 /* Primary-ray hit buffer of the last render (SimpleRayBuffer, RayBuffers.h:34-195):
  * 7 SoA float planes (hit point xyz, direction xyz, t) and primitive ids. */
 hr_status hr_get_primary_hits(hr_ctx* ctx, float* planes7, uint32_t* prims);
-/* AO occlusion flags of the last AO render (1 = occluded), ao_samples per pixel. */
+/* AO occlusion flags of the last render, which must be an AO render (HR_ERR_STATE
+ * otherwise): 1 = occluded, ao_samples per pixel. */
 hr_status hr_get_ao_occlusion(hr_ctx* ctx, uint8_t* occ);
 
 /* ---- ray batches: the trace<> kernel with a RayLoader generator
@@ -508,8 +509,11 @@ hr_status hr_render_rays(hr_ctx* ctx, const float* rays, uint32_t n, float* rgb_
 /* The rays of the last AO render as AORayExporter dumps them: the npix camera rays
  * (camera position, unnormalized direction), then ao_samples AO rays per pixel
  * (pixel-major), a missed pixel's records holding its stored hit point and camera
- * direction; (npix * (ao_samples + 1)) records into rays_out. */
-hr_status hr_dump_ao_rays(hr_ctx* ctx, float* rays_out);
+ * direction; (npix * (ao_samples + 1)) records into rays_out, which holds
+ * max_rays records (HR_ERR_ARG when fewer; rays_out NULL: *count only).  *count
+ * (may be NULL) receives the number of records.  The last render must have
+ * been an AO render of a grid (HR_ERR_STATE otherwise). */
+hr_status hr_dump_ao_rays(hr_ctx* ctx, float* rays_out, uint64_t max_rays, uint64_t* count);
 
 /* ---- image output (host only): Image::gammaCorrect + Image::writePNG
  * (src/Utils/ImagePNG.cpp:64-72, :268-330) ----

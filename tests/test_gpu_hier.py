@@ -7,7 +7,7 @@ import pytest
 
 import pyoracle as O
 from gpu_util import SEED
-from happyray_amd import Engine, GRID_HIERARCHY, HappyRayError, MODE_AO, MODE_PATH, scenes
+from happyray_amd import Engine, GRID_HIERARCHY, GRID_TWO_LEVEL, HappyRayError, MODE_AO, MODE_PATH, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -149,6 +149,24 @@ def test_hier_rejects_other_modes_and_missing_instances(inst_scene):
     e.set_camera(c["pos"], c["dir"], c["up"], c["fov"], 16, 16)
     with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
         e.render_frame(0, MODE_PATH)
+    # instance outputs: traces keep their own buffer, a render of another grid
+    # kind leaves no primary instances to read
+    e.render_frame(0, MODE_AO, ao_samples=2, ao_dist=1.0)
+    pi = e.primary_instances()
+    org, dirs = _rays(5000, 8)
+    e.trace_instanced(org, dirs)
+    np.testing.assert_array_equal(e.primary_instances(), pi)
+    with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
+        e.dump_ao_rays(2)
+    e.set_grid(GRID_TWO_LEVEL, 1.2, 2.2)
+    e.build()
+    e.render_frame(0, MODE_AO, ao_samples=2, ao_dist=1.0)
+    with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
+        e.primary_instances()
+    assert e.dump_ao_rays(2).shape[0] > 0
+    e.render_frame(1, MODE_PATH)
+    with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
+        e.dump_ao_rays(2)
     e.close()
 
 
