@@ -35,6 +35,7 @@ struct RayState {
     uint32_t cell;      // packed word of the current top cell (two-level)
     uint32_t lbeg;      // its first leaf
     uint32_t it, end;   // candidate range still to test
+    uint32_t pn;        // the primitive id at it, loaded one test ahead
     bool in_leaf;
 };
 
@@ -50,6 +51,7 @@ __device__ __forceinline__ void leaf_load(const TraceScene& S, RayState& r, Work
     const uint2 leaf = S.leaves[r.lbeg + (uint32_t)r.lx + r0 * (uint32_t)r.ly + r0 * r1 * (uint32_t)r.lz];
     r.it = leaf.x;
     r.end = leaf.y;
+    if (r.it < r.end) r.pn = S.prims[r.it];
 }
 
 // visit the current top cell: non-empty cells start their leaf DDA at
@@ -96,6 +98,7 @@ __device__ __forceinline__ void uniform_load(const GridRegs& G, const TraceScene
     const uint2 cell = S.cells[cell_index(G, r.cx, r.cy, r.cz)];
     r.it = cell.x;
     r.end = cell.y;
+    if (r.it < r.end) r.pn = S.prims[r.it];
 }
 
 // one ray/triangle test; true when an any-hit ray is finished
@@ -103,8 +106,10 @@ template <bool kAny>
 __device__ __forceinline__ bool test_one(const TraceScene& S, RayState& r, Work& wk)
 {
     ++wk.tests;
-    const uint32_t p = S.prims[r.it];
+    const uint32_t p = r.pn;
     ++r.it;
+    // the next candidate's id is fetched while this triangle is loaded and tested
+    if (r.it < r.end) r.pn = S.prims[r.it];
     f3 v0, v1, v2;
     load_tri(S.rec, p, v0, v1, v2);
     const f3 e1 = v1 - v0;

@@ -45,9 +45,12 @@ template <bool kAny>
 __device__ __forceinline__ bool isect_range(const TraceScene& S, uint32_t b, uint32_t e, f3 org, f3 dir,
                                             float& t, uint32_t& best, Work& wk)
 {
+    uint32_t pn = b < e ? S.prims[b] : 0u;
     for (uint32_t it = b; it < e; ++it) {
         ++wk.tests;
-        const uint32_t p = S.prims[it];
+        const uint32_t p = pn;
+        // the next candidate's id is fetched while this triangle is loaded and tested
+        if (it + 1 < e) pn = S.prims[it + 1];
         f3 v0, v1, v2;
         load_tri(S.rec, p, v0, v1, v2);
         const f3 e1 = v1 - v0;
