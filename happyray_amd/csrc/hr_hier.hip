@@ -126,7 +126,7 @@ __device__ void hier_cell(const HierScene& H, uint32_t b, uint32_t e, f3 org, f3
             if (cx < G.rx && cx > -1 && cy < G.ry && cy > -1 && cz < G.rz && cz > -1) {
                 const uint2 cell =
                     H.obj_cells[base + (uint32_t)cx + (uint32_t)cy * (uint32_t)G.rx + (uint32_t)cz * sxy];
-                isect_range<false>(S, cell.x, cell.y, ot, dt, rayT, hit, wk);
+                isect_range<false, false>(S, cell.x, cell.y, ot, dt, rayT, hit, wk);
             }
             const int d = min_dim(tx, ty, tz);
             if (d == 0) {

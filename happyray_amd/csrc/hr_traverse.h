@@ -41,18 +41,53 @@ struct Work {
     uint32_t steps = 0, tests = 0;
 };
 
-template <bool kAny>
+// candidates [b, e) of one cell or leaf.  kPipe: software pipeline -- the
+// next candidate's triangle and the id after it are in flight while the
+// current triangle is tested; otherwise only the next id is fetched ahead
+// (fewer live registers, for callers deep in a loop nest)
+template <bool kAny, bool kPipe = true>
 __device__ __forceinline__ bool isect_range(const TraceScene& S, uint32_t b, uint32_t e, f3 org, f3 dir,
                                             float& t, uint32_t& best, Work& wk)
 {
-    uint32_t pn = b < e ? S.prims[b] : 0u;
+    if (b >= e) return false;
+    if (!kPipe) {
+        uint32_t pn = S.prims[b];
+        for (uint32_t it = b; it < e; ++it) {
+            ++wk.tests;
+            const uint32_t p = pn;
+            if (it + 1 < e) pn = S.prims[it + 1];
+            f3 v0, v1, v2;
+            load_tri(S.rec, p, v0, v1, v2);
+            const f3 e1 = v1 - v0;
+            const f3 e2 = v2 - v0;
+            const f3 pv = cross(dir, e2);
+            const float det_rcp = 1.f / dot(e1, pv);
+            f3 tv = org - v0;
+            const float alpha = det_rcp * dot(tv, pv);
+            tv = cross(tv, e1);
+            const float beta = det_rcp * dot(tv, dir);
+            const float dist = det_rcp * dot(e2, tv);
+            if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps && dist < t) {
+                t = dist;
+                best = p;
+                if (kAny) return true;
+            }
+        }
+        return false;
+    }
+    uint32_t p = S.prims[b];
+    f3 n0, n1, n2;
+    load_tri(S.rec, p, n0, n1, n2);
+    uint32_t pn = b + 1 < e ? S.prims[b + 1] : 0u;
     for (uint32_t it = b; it < e; ++it) {
         ++wk.tests;
-        const uint32_t p = pn;
-        // the next candidate's id is fetched while this triangle is loaded and tested
-        if (it + 1 < e) pn = S.prims[it + 1];
-        f3 v0, v1, v2;
-        load_tri(S.rec, p, v0, v1, v2);
+        const uint32_t pc = p;
+        const f3 v0 = n0, v1 = n1, v2 = n2;
+        if (it + 1 < e) {
+            p = pn;
+            load_tri(S.rec, p, n0, n1, n2);
+            if (it + 2 < e) pn = S.prims[it + 2];
+        }
         const f3 e1 = v1 - v0;
         const f3 e2 = v2 - v0;
         const f3 pv = cross(dir, e2);
@@ -64,7 +99,7 @@ __device__ __forceinline__ bool isect_range(const TraceScene& S, uint32_t b, uin
         const float dist = det_rcp * dot(e2, tv);
         if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps && dist < t) {
             t = dist;
-            best = p;
+            best = pc;
             if (kAny) return true;
         }
     }
