@@ -64,8 +64,10 @@ struct SceneTable {
 void launch_reset(BuildStatus* st, hipStream_t s);
 // fs != nullptr: the key frames are blocks fs->k1, fs->k2 of keys and the
 // coefficient is fs->coeff, all read on the device
+// part: device scratch of 6 * kBoundsParts words for the per-block partial bounds
+constexpr uint32_t kBoundsParts = 1024;
 void launch_lerp_bounds(const float* k1, const float* k2, float c, uint32_t nv, float* out,
-                        BuildStatus* st, hipStream_t s, const FrameSel* fs = nullptr,
+                        BuildStatus* st, uint32_t* part, hipStream_t s, const FrameSel* fs = nullptr,
                         const float* keys = nullptr);
 void launch_grid_setup(GridParams* gp, BuildStatus* st, uint32_t nprims, float lambda,
                        uint64_t cells_cap, hipStream_t s);

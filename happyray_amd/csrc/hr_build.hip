@@ -64,7 +64,7 @@ __global__ void k_reset(BuildStatus* st)
 __global__ __launch_bounds__(kBlock) void k_lerp_bounds(
     const float* __restrict__ k1, const float* __restrict__ k2, float c, const FrameSel* __restrict__ fs,
     const float* __restrict__ keys, uint32_t nv, float* __restrict__ ox, float* __restrict__ oy,
-    float* __restrict__ oz, BuildStatus* __restrict__ st)
+    float* __restrict__ oz, uint32_t* __restrict__ part)
 {
     __shared__ float red[6][kWavesPerBlock];
     if (fs) {
@@ -100,8 +100,40 @@ __global__ __launch_bounds__(kBlock) void k_lerp_bounds(
         float v = red[threadIdx.x][0];
         for (int i = 1; i < kWavesPerBlock; ++i)
             v = threadIdx.x < 3 ? fminf(v, red[threadIdx.x][i]) : fmaxf(v, red[threadIdx.x][i]);
-        if (threadIdx.x < 3) atomicMin(&st->bounds_enc[threadIdx.x], enc_f(v));
-        else atomicMax(&st->bounds_enc[threadIdx.x], enc_f(v));
+        // one partial per block: same-address device atomics from every block
+        // serialise at the memory side, so the blocks' bounds are combined by
+        // k_bounds_final instead
+        part[threadIdx.x * gridDim.x + blockIdx.x] = enc_f(v);
+    }
+}
+
+// the scene bounds from the lerp's per-block partials (one workgroup)
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const uint32_t* __restrict__ part, uint32_t nparts,
+                                                         BuildStatus* __restrict__ st)
+{
+    __shared__ uint32_t red[6][kWavesPerBlock];
+    uint32_t v[6] = { 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u };
+    for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x)
+        for (int k = 0; k < 6; ++k) {
+            const uint32_t e = part[k * nparts + b];
+            v[k] = k < 3 ? (e < v[k] ? e : v[k]) : (e > v[k] ? e : v[k]);
+        }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) {
+            const uint32_t o = __shfl_xor(v[k], d, kWave);
+            v[k] = k < 3 ? (o < v[k] ? o : v[k]) : (o > v[k] ? o : v[k]);
+        }
+    const int w = (int)threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int k = 0; k < 6; ++k) red[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = (int)threadIdx.x;
+        uint32_t r = red[k][0];
+        for (int i = 1; i < kWavesPerBlock; ++i) r = k < 3 ? (red[k][i] < r ? red[k][i] : r) : (red[k][i] > r ? red[k][i] : r);
+        st->bounds_enc[k] = r;
     }
 }
 
@@ -689,12 +721,13 @@ inline int bits_for(uint64_t n)
 // ---------------------------------------------------------------------------
 
 void launch_lerp_bounds(const float* k1, const float* k2, float c, uint32_t nv, float* out,
-                        BuildStatus* st, hipStream_t s, const FrameSel* fs, const float* keys)
+                        BuildStatus* st, uint32_t* part, hipStream_t s, const FrameSel* fs, const float* keys)
 {
     uint32_t blocks = cdiv(nv > 0 ? nv : 1, kBlock);
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > kBoundsParts) blocks = kBoundsParts;
     hipLaunchKernelGGL(k_lerp_bounds, dim3(blocks), dim3(kBlock), 0, s, k1, k2, c, fs, keys, nv, out, out + nv,
-                       out + 2 * (size_t)nv, st);
+                       out + 2 * (size_t)nv, part);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, s, part, blocks, st);
 }
 
 void launch_reset(BuildStatus* st, hipStream_t s)
@@ -944,12 +977,14 @@ __device__ __forceinline__ void flush_bounds(uint32_t* enc, uint32_t ob, f3 lo, 
 // per_block triangles; a thread keeps running bounds while its triangles stay
 // in one object (objects are contiguous triangle ranges) and flushes them at
 // a change; a block whose triangles all share one object reduces its bounds
-// and issues six atomics.
+// into its partial, which k_obj_bounds_final combines per object (same-address
+// device atomics from many blocks serialise at the memory side).
 __global__ __launch_bounds__(kBlock) void k_obj_bounds(const uint32_t* __restrict__ idx,
                                                        const float* __restrict__ kx, const float* __restrict__ ky,
                                                        const float* __restrict__ kz, uint32_t nt,
                                                        const uint32_t* __restrict__ tri_obj,
-                                                       uint32_t* __restrict__ enc, uint32_t per_block)
+                                                       uint32_t* __restrict__ enc, uint32_t per_block,
+                                                       uint32_t* __restrict__ part)
 {
     __shared__ float red[6][kWavesPerBlock];
     __shared__ uint32_t obj0;
@@ -983,6 +1018,7 @@ __global__ __launch_bounds__(kBlock) void k_obj_bounds(const uint32_t* __restric
     __syncthreads();
     if (obj0 == none) return;
     if (!uniform) {
+        if (threadIdx.x == 0) part[7 * (size_t)blockIdx.x] = none;
         if (cur != none) flush_bounds(enc, cur, lo, hi);
         return;
     }
@@ -998,7 +1034,45 @@ __global__ __launch_bounds__(kBlock) void k_obj_bounds(const uint32_t* __restric
             l = vmin(l, mk(red[0][i], red[1][i], red[2][i]));
             h = vmax(h, mk(red[3][i], red[4][i], red[5][i]));
         }
-        flush_bounds(enc, obj0, l, h);
+        uint32_t* q = part + 7 * (size_t)blockIdx.x;
+        q[0] = obj0;
+        q[1] = enc_f(l.x); q[2] = enc_f(l.y); q[3] = enc_f(l.z);
+        q[4] = enc_f(h.x); q[5] = enc_f(h.y); q[6] = enc_f(h.z);
+    }
+}
+
+// one wave per object: the partials of the blocks inside its triangle range
+__global__ __launch_bounds__(kWave) void k_obj_bounds_final(const uint32_t* __restrict__ obj_first, uint32_t nobj,
+                                                             uint32_t per_block, const uint32_t* __restrict__ part,
+                                                             uint32_t* __restrict__ enc)
+{
+    const uint32_t k = blockIdx.x;
+    if (k >= nobj) return;
+    const uint32_t t0 = obj_first[k], t1 = obj_first[k + 1];
+    if (t0 >= t1) return;
+    const uint32_t b0 = t0 / per_block, b1 = (t1 - 1) / per_block;
+    uint32_t v[6] = { 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u };
+    bool any = false;
+    for (uint32_t b = b0 + __lane_id(); b <= b1; b += kWave) {
+        const uint32_t* q = part + 7 * (size_t)b;
+        if (q[0] != k) continue;
+        any = true;
+        for (int i = 0; i < 3; ++i) {
+            v[i] = q[1 + i] < v[i] ? q[1 + i] : v[i];
+            v[3 + i] = q[4 + i] > v[3 + i] ? q[4 + i] : v[3 + i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) {
+            const uint32_t o = __shfl_xor(v[i], d, kWave);
+            v[i] = i < 3 ? (o < v[i] ? o : v[i]) : (o > v[i] ? o : v[i]);
+        }
+    if (__ballot(any) != 0ull && __lane_id() == 0) {
+        uint32_t* e = enc + 6 * (size_t)k;
+        atomicMin(e + 0, v[0]); atomicMin(e + 1, v[1]); atomicMin(e + 2, v[2]);
+        atomicMax(e + 3, v[3]); atomicMax(e + 4, v[4]); atomicMax(e + 5, v[5]);
     }
 }
 
@@ -1108,10 +1182,12 @@ void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s)
     hipLaunchKernelGGL(k_obj_reset, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, nob);
     if (nt > 0) {
         uint32_t tblocks = cdiv(nt, 512);
-        if (tblocks > 512) tblocks = 512;
+        if (tblocks > kObjBoundsBlocks) tblocks = kObjBoundsBlocks;
         const uint32_t per_block = cdiv(cdiv(nt, tblocks), kBlock) * kBlock;
         hipLaunchKernelGGL(k_obj_bounds, dim3(cdiv(nt, per_block)), dim3(kBlock), 0, s, a.idx, a.kx, a.ky, a.kz, nt,
-                           a.tri_obj, b.obj_bounds_enc, per_block);
+                           a.tri_obj, b.obj_bounds_enc, per_block, b.obj_part);
+        hipLaunchKernelGGL(k_obj_bounds_final, dim3(nob), dim3(kWave), 0, s, a.obj_first, nob, per_block, b.obj_part,
+                           b.obj_bounds_enc);
     }
     hipLaunchKernelGGL(k_obj_setup, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, a.obj_first, nob,
                        a.lambda_leaf, b.obj_gp, b.obj_cell_counts, st);

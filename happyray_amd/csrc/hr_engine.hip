@@ -75,6 +75,7 @@ struct hr_ctx {
     // build
     DevBuf<hr::GridParams> params;
     DevBuf<hr::BuildStatus> status;
+    DevBuf<uint32_t> bounds_part;     // per-block partial bounds of the interpolation
     hr::BuildStatus* h_status = nullptr;  // pinned
     DevBuf<float4> rec;
     DevBuf<uint32_t> wave_counts, wave_offsets, keys, vals, keys_sorted, top_vals_sorted, prims, leaf_counts;
@@ -90,7 +91,8 @@ struct hr_ctx {
     // cells (top grid), top_vals_sorted (instance references), leaves (object
     // cells) and prims (object references)
     uint32_t nobj = 0, ninst = 0, obj_tris = 0;
-    DevBuf<uint32_t> tri_obj, obj_first, obj_bounds_enc, obj_cell_counts, obj_cell_base, inst_counts, inst_offsets;
+    DevBuf<uint32_t> tri_obj, obj_first, obj_bounds_enc, obj_cell_counts, obj_cell_base, inst_counts, inst_offsets,
+        obj_part;
     DevBuf<hr::InstanceDev> inst;
     DevBuf<hr::GridParams> obj_gp;
     DevBuf<uint32_t> hit_inst;        // instance of each pixel's primary hit (hierarchy renders)
@@ -205,6 +207,7 @@ hr_status reserve_build(hr_ctx* c)
     if (hier) {
         HR_TRY(c->obj_gp.reserve(c->nobj));
         HR_TRY(c->obj_bounds_enc.reserve(6 * (size_t)c->nobj));
+        HR_TRY(c->obj_part.reserve(7 * (size_t)hr::kObjBoundsBlocks));
         HR_TRY(c->obj_cell_counts.reserve(c->nobj));
         HR_TRY(c->obj_cell_base.reserve(c->nobj));
         HR_TRY(c->inst_counts.reserve(c->ninst));
@@ -266,10 +269,11 @@ void enqueue_build(hr_ctx* c, const hr::FrameSel* fs = nullptr)
 {
     hr::launch_reset(c->status.p, c->stream);
     if (fs)
-        hr::launch_lerp_bounds(nullptr, nullptr, 0.f, c->nv, c->verts.p, c->status.p, c->stream, fs,
+        hr::launch_lerp_bounds(nullptr, nullptr, 0.f, c->nv, c->verts.p, c->status.p, c->bounds_part.p, c->stream, fs,
                                c->keyframes.p);
     else
-        hr::launch_lerp_bounds(c->src1, c->src2, c->coeff, c->nv, c->verts.p, c->status.p, c->stream);
+        hr::launch_lerp_bounds(c->src1, c->src2, c->coeff, c->nv, c->verts.p, c->status.p, c->bounds_part.p,
+                               c->stream);
     if (c->kind == HR_GRID_HIERARCHY) {
         hr::HierArgs a;
         a.idx = c->idx.p;
@@ -293,6 +297,7 @@ void enqueue_build(hr_ctx* c, const hr::FrameSel* fs = nullptr)
         b.top = c->params.p;
         b.obj_gp = c->obj_gp.p;
         b.obj_bounds_enc = c->obj_bounds_enc.p;
+        b.obj_part = c->obj_part.p;
         b.obj_cell_counts = c->obj_cell_counts.p;
         b.obj_cell_base = c->obj_cell_base.p;
         b.rec = c->rec.p;
@@ -689,7 +694,7 @@ std::vector<unsigned char> graph_signature(hr_ctx* c, hr_mode mode, const hr_ren
     put(v, c->nt);
     put(v, c->nv);
     put(v, c->nframes);
-    const void* ptrs[] = { c->keyframes.p, c->verts.p, c->idx.p, c->rec.p, c->wave_counts.p, c->keys.p,
+    const void* ptrs[] = { c->bounds_part.p, c->keyframes.p, c->verts.p, c->idx.p, c->rec.p, c->wave_counts.p, c->keys.p,
                            c->vals.p, c->keys_sorted.p, c->top_vals_sorted.p, c->prims.p, c->cells.p,
                            c->leaves.p, c->leaf_counts.p, c->wave_offsets.p, c->scan_chain.p,
                            c->sort_chain.p, c->sort_keys_tmp.p,
@@ -785,7 +790,8 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes() + c->pm.bytes() + c->pm_prims.bytes();
     b += c->ray_wire.bytes() + c->tri_obj.bytes() + c->obj_first.bytes() + c->obj_bounds_enc.bytes();
     b += c->obj_cell_counts.bytes() + c->obj_cell_base.bytes() + c->inst_counts.bytes() + c->inst_offsets.bytes();
-    b += c->inst.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes() + c->ray_inst.bytes();
+    b += c->inst.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes() + c->ray_inst.bytes() + c->obj_part.bytes();
+    b += c->bounds_part.bytes();
     return b;
 }
 
@@ -858,6 +864,7 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
         if ((s = c->params.reserve(1)) != HR_OK) break;
         if ((s = c->status.reserve(1)) != HR_OK) break;
+        if ((s = c->bounds_part.reserve(6 * (size_t)hr::kBoundsParts)) != HR_OK) break;
         if ((s = c->ray_count.reserve(1)) != HR_OK) break;
         if ((s = c->work.reserve(6)) != HR_OK) break;
         if ((s = c->pool.reserve(1)) != HR_OK) break;
@@ -908,7 +915,7 @@ hr_status hr_destroy(hr_ctx* c)
                                   &c->sort_chain, &c->sort_keys_tmp, &c->sort_vals_tmp, &c->sort_hist,
                                   &c->hit_prims, &c->ray_prim, &c->tri_obj, &c->obj_first,
                                   &c->obj_bounds_enc, &c->obj_cell_counts, &c->obj_cell_base, &c->inst_counts,
-                                  &c->inst_offsets, &c->hit_inst, &c->ray_inst };
+                                  &c->inst_offsets, &c->hit_inst, &c->ray_inst, &c->obj_part, &c->bounds_part };
     for (auto* b : ubufs) b->release();
     c->occ.release();
     c->ray_occ.release();

@@ -59,6 +59,7 @@ struct HierBuffers {
     GridParams* top;
     GridParams* obj_gp;             // nobj
     uint32_t* obj_bounds_enc;       // 6 per object
+    uint32_t* obj_part;             // 7 * kObjBoundsBlocks: per-block partial object bounds
     uint32_t* obj_cell_counts;      // nobj
     uint32_t* obj_cell_base;        // nobj: first cell of each object in obj_cells
     float4* rec;
@@ -77,6 +78,8 @@ struct HierBuffers {
     ScanTemp scan;
     SortTemp sort;
 };
+
+constexpr uint32_t kObjBoundsBlocks = 512;   // workgroups of the object-bounds pass (at most)
 
 // everything after the lerp / bounds: object grids, then the top grid
 void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s);
