@@ -100,50 +100,71 @@ __global__ __launch_bounds__(kBlock) void k_scan_1p(const uint32_t* __restrict__
         }
         uint32_t agg;
         const uint32_t ex = block_excl_scan(s, lds, agg);
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < kWave) {
+            // decoupled look-back by the first wave: lane i reads the flag of
+            // tile (p_hi - i), so 64 predecessors are examined per round
+            const int lane = (int)__lane_id();
+            const uint32_t ep = epoch & 0x3FFFFFFFu;
             uint32_t excl = 0;
             if (tile == 0) {
-                __hip_atomic_store(&flags[0], flag_pack(epoch, kPrefix, agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0)
+                    __hip_atomic_store(&flags[0], flag_pack(epoch, kPrefix, agg), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                __hip_atomic_store(&flags[tile], flag_pack(epoch, kAggregate, agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                int p = (int)tile - 1;
+                if (lane == 0)
+                    __hip_atomic_store(&flags[tile], flag_pack(epoch, kAggregate, agg), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                int p_hi = (int)tile - 1;
                 int spins = 0;
-                while (p >= 0) {
-                    // the sum travels inside the flag word: relaxed atomics suffice
-                    const uint64_t f = __hip_atomic_load(&flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const uint32_t fe = (uint32_t)(f >> 34);
-                    const uint64_t st = (f >> 32) & 3u;
-                    if (fe == (epoch & 0x3FFFFFFFu) && st == kPrefix) {
-                        excl += (uint32_t)f;
-                        break;
+                while (true) {
+                    const int p = p_hi - lane;
+                    uint64_t st = 0;
+                    uint32_t val = 0;
+                    if (p >= 0) {
+                        // the sum travels inside the flag word: relaxed atomics suffice
+                        const uint64_t f = __hip_atomic_load(&flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if ((uint32_t)(f >> 34) == ep) {
+                            st = (f >> 32) & 3u;
+                            val = (uint32_t)f;
+                        }
                     }
-                    if (fe == (epoch & 0x3FFFFFFFu) && st == kAggregate) {
-                        excl += (uint32_t)f;
-                        --p;
+                    if (p >= 0 && st == 0 && spins > kSpinLimit) {
+                        // bounded wait: recompute the missing tile's sum from the input
+                        st = kAggregate;
+                        val = tile_sum(in, (uint32_t)p, n);
+                    }
+                    const uint64_t pre = __ballot(p >= 0 && st == kPrefix);
+                    const uint64_t ready = __ballot(p < 0 || st != 0);
+                    // lanes up to the nearest prefix (or the whole window) must have published
+                    const int last = pre ? (int)__builtin_ctzll(pre) : kWave - 1;
+                    const uint64_t need = last == kWave - 1 ? ~0ull : ((1ull << (last + 1)) - 1ull);
+                    if ((ready & need) == need) {
+                        uint32_t v = (lane <= last && p >= 0) ? val : 0u;
+#pragma unroll
+                        for (int d = kWave / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+                        excl += v;
+                        if (pre || p_hi - (kWave - 1) <= 0) break;
+                        p_hi -= kWave;
                         spins = 0;
                         continue;
                     }
-                    if (++spins > kSpinLimit) {
-                        excl += tile_sum(in, (uint32_t)p, n);
-                        --p;
-                        spins = 0;
-                        continue;
-                    }
+                    ++spins;
                     __builtin_amdgcn_s_sleep(1);
                 }
-                __hip_atomic_store(&flags[tile], flag_pack(epoch, kPrefix, excl + agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0)
+                    __hip_atomic_store(&flags[tile], flag_pack(epoch, kPrefix, excl + agg), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
-            s_excl = excl;
-            // the tile holding the last element knows the total
-            if ((n - 1) / kScanTile == tile) {
-                const uint32_t total = excl + agg;
-                if (total_dev) *total_dev = total;
-                if (check.st) {
-                    if (total > check.cap) atomicOr(&check.st->overflow, check.bit);
-                    *check.eff = check.st->overflow ? 0u : total;
+            if (lane == 0) {
+                s_excl = excl;
+                // the tile holding the last element knows the total
+                if ((n - 1) / kScanTile == tile) {
+                    const uint32_t total = excl + agg;
+                    if (total_dev) *total_dev = total;
+                    if (check.st) {
+                        if (total > check.cap) atomicOr(&check.st->overflow, check.bit);
+                        *check.eff = check.st->overflow ? 0u : total;
+                    }
                 }
             }
         }
