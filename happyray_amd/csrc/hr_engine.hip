@@ -135,7 +135,7 @@ struct hr_ctx {
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
-    hr_tuning tuning{ 0, 1, 48 };
+    hr_tuning tuning{ 0, 1, 48, 12 };
 
     // captured frames: a rebuild graph and a render graph, replayed per frame
     hipGraphExec_t g_build = nullptr, g_render = nullptr;
@@ -613,7 +613,8 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     if (c->kind == HR_GRID_HIERARCHY) {
         // TLGHAOIntegrator::integrate (TLGridHierarchyAOIntegrator.h:421-573)
         hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
-                                  c->fb.p, c->ray_count.p, c->stream);
+                                  c->fb.p, c->tuning.persistent_primary != 0, c->tuning.persistent_secondary != 0,
+                                  c->pool.p, (uint32_t)c->tuning.hier_refill, c->ray_count.p, c->stream);
         if (events) {
             HR_HIP(hipEventRecord(c->ev_p1, c->stream));
             HR_HIP(hipEventRecord(c->ev_p2, c->stream));
@@ -1510,6 +1511,7 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
         (t->persistent_secondary != 0 && t->persistent_secondary != 1))
         return fail(HR_ERR_ARG, "persistent flags must be 0 or 1");
     if (t->refill < 1 || t->refill > 64) return fail(HR_ERR_ARG, "refill must be in [1, 64]");
+    if (t->hier_refill < 1 || t->hier_refill > 64) return fail(HR_ERR_ARG, "hier_refill must be in [1, 64]");
     c->tuning = *t;
     return HR_OK;
 }

@@ -375,6 +375,389 @@ __global__ __launch_bounds__(kBlock) void k_ao_shade_hier(HierScene H, RenderArg
     o[2] = o[2] * ow + rad.z * nw;
 }
 
+// ---------------------------------------------------------------------------
+// persistent AO rays: the traversal of trace_hier<true> as a flat state
+// machine -- one unit per loop iteration (one triangle test, one object-grid
+// step, one instance set-up, one top-grid step) -- with waves refilling idle
+// lanes from a global counter, as the uniform / two-level persistent kernels
+// do.  The units are those of trace_hier in its order, so every ray's
+// occlusion bit is the one-ray-per-lane kernel's.
+// ---------------------------------------------------------------------------
+
+struct HierRay {
+    f3 org, rcp;
+    float t, tterm;
+    float tstop;                 // an accepted hit closer than this finishes the ray
+    uint32_t best, inst;
+    Mailbox mb;
+    int cx, cy, cz;              // top grid
+    float tx, ty, tz;
+    uint32_t it, end;            // instance references of the current top cell
+    bool in_inst;                // an instance's object grid is being stepped
+    uint32_t ii, base, hit;
+    float te1, tx1, rayT;
+    f3 ot, rt, dt;               // object-space origin, reciprocal direction, direction
+    f3 ocs;                      // object cell size
+    int rx, ry, rz;
+    int ocx, ocy, ocz;
+    float otx, oty, otz;
+    uint32_t pit, pend;          // triangle references of the current object cell
+};
+
+__device__ __forceinline__ void hp_top_cell(const HierScene& H, const GridRegs& G, HierRay& r)
+{
+    r.it = r.end = 0u;
+    if (r.cx < G.rx && r.cx > -1 && r.cy < G.ry && r.cy > -1 && r.cz < G.rz && r.cz > -1) {
+        const uint2 cell =
+            H.top_cells[(uint32_t)r.cx + (uint32_t)r.cy * (uint32_t)G.rx + (uint32_t)r.cz * ((uint32_t)G.rx * (uint32_t)G.ry)];
+        r.it = cell.x;
+        r.end = cell.y;
+    }
+}
+
+__device__ __forceinline__ void hp_obj_cell(const HierScene& H, HierRay& r)
+{
+    r.pit = r.pend = 0u;
+    if (r.ocx < r.rx && r.ocx > -1 && r.ocy < r.ry && r.ocy > -1 && r.ocz < r.rz && r.ocz > -1) {
+        const uint2 cell = H.obj_cells[r.base + (uint32_t)r.ocx + (uint32_t)r.ocy * (uint32_t)r.rx +
+                                       (uint32_t)r.ocz * ((uint32_t)r.rx * (uint32_t)r.ry)];
+        r.pit = cell.x;
+        r.pend = cell.y;
+    }
+}
+
+// set up the top-grid DDA of a generated ray; false when it does not start
+__device__ __forceinline__ bool hp_begin(const HierScene& H, const GridRegs& G, HierRay& r)
+{
+    r.best = H.nt;
+    r.inst = 0xFFFFFFFFu;
+    r.in_inst = false;
+    r.pit = r.pend = 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r.mb.e[j] = 0xFFFFFFFFu;
+    bool flag = !(r.t < r.tterm);
+    const f3 t1 = (G.bmin - r.org) * r.rcp;
+    f3 tfar = (G.bmax - r.org) * r.rcp;
+    const f3 tnear = vmin(t1, tfar);
+    tfar = vmax(t1, tfar);
+    const float tentry = fmaxf(fmaxf(tnear.x, tnear.y), tnear.z);
+    const float texit = fminf(fminf(tfar.x, tfar.y), tfar.z);
+    flag = flag && texit > tentry && texit >= 0.f;
+    const f3 entry = (tentry >= 0.f) ? r.org + rep(tentry + kEps) / r.rcp : r.org;
+    f3 cf = (entry - G.bmin) * G.csr;
+    cf = mk(floorf(cf.x), floorf(cf.y), floorf(cf.z));
+    r.tx = ((cf.x + (r.rcp.x > 0.f ? 1.f : 0.f)) * G.cs.x + G.bmin.x - r.org.x) * r.rcp.x;
+    r.ty = ((cf.y + (r.rcp.y > 0.f ? 1.f : 0.f)) * G.cs.y + G.bmin.y - r.org.y) * r.rcp.y;
+    r.tz = ((cf.z + (r.rcp.z > 0.f ? 1.f : 0.f)) * G.cs.z + G.bmin.z - r.org.z) * r.rcp.z;
+    r.cx = to_int_sat(cf.x);
+    r.cy = to_int_sat(cf.y);
+    r.cz = to_int_sat(cf.z);
+    flag = flag && (r.rcp.x > 0.f ? r.cx < G.rx : r.cx > -1) && (r.rcp.y > 0.f ? r.cy < G.ry : r.cy > -1) &&
+           (r.rcp.z > 0.f ? r.cz < G.rz : r.cz > -1);
+    if (!flag) return false;
+    hp_top_cell(H, G, r);
+    return true;
+}
+
+// one unit of work; true when the ray is finished
+__device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, HierRay& r)
+{
+    if (r.pit < r.pend) {
+        // one triangle of the object cell (MollerTrumboreIntersectionTest)
+        const uint32_t p = H.prims[r.pit];
+        ++r.pit;
+        f3 v0, v1, v2;
+        load_tri(H.rec, p, v0, v1, v2);
+        const f3 e1 = v1 - v0;
+        const f3 e2 = v2 - v0;
+        const f3 pv = cross(r.dt, e2);
+        const float det_rcp = 1.f / dot(e1, pv);
+        f3 tv = r.ot - v0;
+        const float alpha = det_rcp * dot(tv, pv);
+        tv = cross(tv, e1);
+        const float beta = det_rcp * dot(tv, r.dt);
+        const float dist = det_rcp * dot(e2, tv);
+        if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps && dist < r.rayT) {
+            r.rayT = dist;
+            r.hit = p;
+        }
+        return false;
+    }
+    if (r.in_inst) {
+        // the object DDA's step after a cell
+        const int d = min_dim(r.otx, r.oty, r.otz);
+        bool flag;
+        if (d == 0) {
+            flag = r.rayT > r.otx;
+            r.ocx += r.rt.x > 0.f ? 1 : -1;
+            r.otx += r.ocs.x * fabsf(r.rt.x);
+            flag = flag && (r.rt.x > 0.f ? r.ocx < r.rx : r.ocx > -1);
+        } else if (d == 1) {
+            flag = r.rayT > r.oty;
+            r.ocy += r.rt.y > 0.f ? 1 : -1;
+            r.oty += r.ocs.y * fabsf(r.rt.y);
+            flag = flag && (r.rt.y > 0.f ? r.ocy < r.ry : r.ocy > -1);
+        } else {
+            flag = r.rayT > r.otz;
+            r.ocz += r.rt.z > 0.f ? 1 : -1;
+            r.otz += r.ocs.z * fabsf(r.rt.z);
+            flag = flag && (r.rt.z > 0.f ? r.ocz < r.rz : r.ocz > -1);
+        }
+        if (flag) {
+            hp_obj_cell(H, r);
+            return false;
+        }
+        r.in_inst = false;
+        if (r.hit != H.nt && r.rayT < r.t && r.rayT >= r.te1 && r.rayT < r.tx1) {
+            r.t = r.rayT;
+            r.best = r.hit;
+            r.inst = r.ii;
+            // the distance only shrinks from here: an occlusion test against
+            // tstop is decided
+            if (r.t < r.tstop) return true;
+        }
+        return false;
+    }
+    // the next instance of the top cell that the ray meets
+    while (r.it < r.end) {
+        const uint32_t ii = H.inst_refs[r.it];
+        ++r.it;
+        if (mailbox_seen(r.mb, ii)) continue;
+        const InstanceDev I = H.inst[ii];
+        const f3 b1 = (mk(I.bmin[0], I.bmin[1], I.bmin[2]) - r.org) * r.rcp;
+        f3 bm = (mk(I.bmax[0], I.bmax[1], I.bmax[2]) - r.org) * r.rcp;
+        const f3 bn = vmin(b1, bm);
+        bm = vmax(b1, bm);
+        const float te1 = fmaxf(fmaxf(bn.x, bn.y), bn.z);
+        const float tx1 = fminf(fminf(bm.x, bm.y), bm.z);
+        if (!(tx1 > te1 && tx1 >= 0.f && te1 < r.t)) continue;
+        f3 ot, rt;
+        instance_ray(I, r.org, r.rcp, ot, rt);
+        const GridRegs O = load_grid(H.obj_gp + I.object);
+        const f3 g1 = (O.bmin - ot) * rt;
+        f3 gf = (O.bmax - ot) * rt;
+        const f3 gn = vmin(g1, gf);
+        gf = vmax(g1, gf);
+        const float tentry = fmaxf(fmaxf(gn.x, gn.y), gn.z);
+        const float texit = fminf(fminf(gf.x, gf.y), gf.z);
+        bool flag = texit > tentry && texit >= 0.f && tentry < r.t;
+        const f3 entry = (tentry >= 0.f) ? ot + rep(tentry + kEps) / rt : ot;
+        f3 cf = (entry - O.bmin) * O.csr;
+        cf = mk(floorf(cf.x), floorf(cf.y), floorf(cf.z));
+        cf.x = rt.x > 0.f ? fminf((float)O.rx - 1.f, cf.x) : fmaxf(cf.x, 0.f);
+        cf.y = rt.y > 0.f ? fminf((float)O.ry - 1.f, cf.y) : fmaxf(cf.y, 0.f);
+        cf.z = rt.z > 0.f ? fminf((float)O.rz - 1.f, cf.z) : fmaxf(cf.z, 0.f);
+        const int ocx = to_int_sat(cf.x), ocy = to_int_sat(cf.y), ocz = to_int_sat(cf.z);
+        flag = flag && (rt.x > 0.f ? ocx < O.rx : ocx > -1) && (rt.y > 0.f ? ocy < O.ry : ocy > -1) &&
+               (rt.z > 0.f ? ocz < O.rz : ocz > -1);
+        if (!flag) continue;          // no object cell is stepped: no hit from this instance
+        r.ii = ii;
+        r.te1 = te1;
+        r.tx1 = tx1;
+        r.ot = ot;
+        r.rt = rt;
+        r.dt = mk(1.f / rt.x, 1.f / rt.y, 1.f / rt.z);
+        r.ocs = O.cs;
+        r.rx = O.rx;
+        r.ry = O.ry;
+        r.rz = O.rz;
+        r.base = H.obj_cell_base[I.object];
+        r.otx = ((cf.x + (rt.x > 0.f ? 1.f : 0.f)) * O.cs.x + O.bmin.x - ot.x) * rt.x;
+        r.oty = ((cf.y + (rt.y > 0.f ? 1.f : 0.f)) * O.cs.y + O.bmin.y - ot.y) * rt.y;
+        r.otz = ((cf.z + (rt.z > 0.f ? 1.f : 0.f)) * O.cs.z + O.bmin.z - ot.z) * rt.z;
+        r.ocx = ocx;
+        r.ocy = ocy;
+        r.ocz = ocz;
+        r.rayT = r.t;
+        r.hit = H.nt;
+        r.in_inst = true;
+        hp_obj_cell(H, r);
+        return false;
+    }
+    // the top cell is done: the shadow stop, then the top DDA's step
+    if (r.t < r.tterm) return true;
+    const int dm = min_dim(r.tx, r.ty, r.tz);
+    bool flag;
+    if (dm == 0) {
+        flag = r.t > r.tx;
+        r.cx += r.rcp.x > 0.f ? 1 : -1;
+        r.tx += G.cs.x * fabsf(r.rcp.x);
+        flag = flag && (r.rcp.x > 0.f ? r.cx < G.rx : r.cx > -1);
+    } else if (dm == 1) {
+        flag = r.t > r.ty;
+        r.cy += r.rcp.y > 0.f ? 1 : -1;
+        r.ty += G.cs.y * fabsf(r.rcp.y);
+        flag = flag && (r.rcp.y > 0.f ? r.cy < G.ry : r.cy > -1);
+    } else {
+        flag = r.t > r.tz;
+        r.cz += r.rcp.z > 0.f ? 1 : -1;
+        r.tz += G.cs.z * fabsf(r.rcp.z);
+        flag = flag && (r.rcp.z > 0.f ? r.cz < G.rz : r.cz > -1);
+    }
+    if (!flag) return true;
+    hp_top_cell(H, G, r);
+    return false;
+}
+
+// AmbientOcclusionRayGenerator of the hierarchy (TLGridHierarchyAOIntegrator.h:296-380)
+// for the persistent kernel: ray j = sample j % S of pixel j / S
+struct GenAOHier {
+    RenderArgs ra;
+    uint32_t npix;
+    const float* hits;
+    const uint32_t* hit_prims;
+    const uint32_t* hit_inst;
+    uint8_t* occ;
+
+    __device__ __forceinline__ uint32_t count() const { return npix * (uint32_t)ra.ao_samples; }
+
+    // false: ray j has no traversal (its result is written); *traced: it counts as a ray
+    __device__ __forceinline__ bool gen(const HierScene& H, const GridRegs& G, uint32_t j, HierRay& r,
+                                        bool& traced) const
+    {
+        const uint32_t S_ = (uint32_t)ra.ao_samples;
+        const uint32_t pix = j / S_, s = j % S_;
+        const uint32_t prim = hit_prims[pix];
+        const float th = hits[6 * (size_t)npix + pix];
+        traced = th < kFltMax && prim < H.nt;
+        if (!traced) {
+            occ[j] = 1u;
+            return false;
+        }
+        const f3 pdir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix], hits[5 * (size_t)npix + pix]);
+        f3 o = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+        f3 n = hier_normal(H, prim, hit_inst[pix]);
+        if (dot(n, pdir) > 0.f) n = -n;
+        float u[4];
+        rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamAO + s, u);
+        const f3 h = cosine_dir(u[0], u[1]);
+        f3 tg, bn;
+        local_frame(n, tg, bn);
+        const f3 dir = normalize(n * h.z + tg * h.x + bn * h.y);
+        r.org = o + (0.0001f * ra.ao_dist) * n;
+        r.rcp = mk(1.f / dir.x, 1.f / dir.y, 1.f / dir.z);
+        r.t = kFltMax;
+        r.tterm = fmaxf(kShadowStop, ra.ao_dist);
+        r.tstop = ra.ao_dist;
+        if (hp_begin(H, G, r)) return true;
+        finish(H, j, r);
+        return false;
+    }
+
+    __device__ __forceinline__ void finish(const HierScene&, uint32_t j, const HierRay& r) const
+    {
+        occ[j] = r.t < ra.ao_dist ? 1u : 0u;
+    }
+};
+
+// camera rays of the hierarchy (k_primary_hier's outputs)
+struct GenPrimaryHier {
+    CameraDev cam;
+    RenderArgs ra;
+    uint32_t npix;
+    float* hits;
+    uint32_t* hit_prims;
+    uint32_t* hit_inst;
+
+    __device__ __forceinline__ uint32_t count() const { return npix; }
+
+    __device__ __forceinline__ bool gen(const HierScene& H, const GridRegs& G, uint32_t j, HierRay& r,
+                                        bool& traced) const
+    {
+        traced = true;
+        const f3 d0 = primary_direction(cam, ra, j, npix);
+        r.org = mk(cam.center[0], cam.center[1], cam.center[2]);
+        r.rcp = mk(1.f / d0.x, 1.f / d0.y, 1.f / d0.z);
+        r.t = kFltMax;
+        r.tterm = 0.f;
+        r.tstop = -1.f;
+        if (hp_begin(H, G, r)) return true;
+        finish(H, j, r);
+        return false;
+    }
+
+    __device__ __forceinline__ void finish(const HierScene& H, uint32_t pix, const HierRay& r) const
+    {
+        const f3 dir = mk(1.f / r.rcp.x, 1.f / r.rcp.y, 1.f / r.rcp.z);
+        const f3 hp = r.org + (r.t - kEps) * dir;
+        hits[pix] = hp.x;
+        hits[(size_t)npix + pix] = hp.y;
+        hits[2 * (size_t)npix + pix] = hp.z;
+        hits[3 * (size_t)npix + pix] = dir.x;
+        hits[4 * (size_t)npix + pix] = dir.y;
+        hits[5 * (size_t)npix + pix] = dir.z;
+        hits[6 * (size_t)npix + pix] = r.t;
+        hit_prims[pix] = r.t < kFltMax ? r.best : H.nt;
+        hit_inst[pix] = r.t < kFltMax ? r.inst : 0xFFFFFFFFu;
+    }
+};
+
+// Waves fetch rays from *pool when `refill` of their lanes are idle (or all
+// are); every wave leaves the loop once the pool is exhausted and its lanes
+// are idle, so the grid drains.
+template <class Gen>
+__global__ __launch_bounds__(kBlock) void k_hier_persist(HierScene H, Gen g, uint32_t* __restrict__ pool,
+                                                         uint32_t refill, unsigned long long* ray_count)
+{
+    const GridRegs G = load_grid(H.top);
+    const uint64_t lt = __lanemask_lt();
+    const uint32_t nrays = g.count();
+    HierRay r;
+    uint32_t j = 0, started = 0;
+    bool live = false, exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill || idle == ~0ull)) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(pool, nidle);
+            base = __shfl(base, 0, kWave);
+            if (base + nidle >= nrays) exhausted = true;
+            if (!live) {
+                j = base + (uint32_t)__popcll(idle & lt);
+                if (j < nrays) {
+                    bool traced = false;
+                    live = g.gen(H, G, j, r, traced);
+                    started += traced ? 1u : 0u;
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (live && hp_step(H, G, r)) {
+            live = false;
+            g.finish(H, j, r);
+        }
+    }
+    if (ray_count) {
+        unsigned long long c = started;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if (__lane_id() == 0 && c) atomicAdd(ray_count, c);
+    }
+}
+
+template <class Gen>
+void launch_hier_persist(const HierScene& H, const Gen& g, uint32_t nrays, uint32_t* pool, uint32_t refill,
+                         unsigned long long* ray_count, hipStream_t s)
+{
+    static uint32_t blocks_max = 0;
+    if (!blocks_max) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_hier_persist<Gen>, kBlock, 0);
+        blocks_max = (uint32_t)((cus > 0 ? cus : 256) * (per > 0 ? per : 1));
+    }
+    uint32_t blocks = cdiv_u(nrays, kBlock);
+    if (blocks > blocks_max) blocks = blocks_max;
+    if (!blocks) return;
+    fill_u32(pool, 0u, 1, nullptr, 1, s);
+    hipLaunchKernelGGL(k_hier_persist<Gen>, dim3(blocks), dim3(kBlock), 0, s, H, g, pool, refill, ray_count);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -397,14 +780,25 @@ void launch_trace_hier(const HierScene& H, const float* org, const float* dir, u
 
 void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const RenderArgs& ra, float* hits,
                            uint32_t* hit_prims, uint32_t* hit_inst, uint8_t* occ, const Shading& sh, float* fb,
+                           bool pers_primary, bool pers_ao, uint32_t* pool, uint32_t refill,
                            unsigned long long* ray_count, hipStream_t s)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
-    hipLaunchKernelGGL(k_primary_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, cam, ra, npix, hits,
-                       hit_prims, hit_inst, ray_count);
+    if (pers_primary) {
+        const GenPrimaryHier g{ cam, ra, npix, hits, hit_prims, hit_inst };
+        launch_hier_persist(H, g, npix, pool, refill, ray_count, s);
+    } else {
+        hipLaunchKernelGGL(k_primary_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, cam, ra, npix, hits,
+                           hit_prims, hit_inst, ray_count);
+    }
     const size_t nrays = (size_t)npix * (size_t)ra.ao_samples;
-    hipLaunchKernelGGL(k_ao_rays_hier, dim3(cdiv_u(nrays, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits, hit_prims,
-                       hit_inst, occ, ray_count);
+    if (pers_ao) {
+        const GenAOHier g{ ra, npix, hits, hit_prims, hit_inst, occ };
+        launch_hier_persist(H, g, (uint32_t)nrays, pool, refill, ray_count, s);
+    } else {
+        hipLaunchKernelGGL(k_ao_rays_hier, dim3(cdiv_u(nrays, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits,
+                           hit_prims, hit_inst, occ, ray_count);
+    }
     hipLaunchKernelGGL(k_ao_shade_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits, hit_prims,
                        hit_inst, occ, sh, fb);
 }

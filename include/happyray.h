@@ -279,6 +279,8 @@ typedef struct {
     int persistent_secondary;  /* AO rays and ray batches: 1 (default) persistent, 0 not */
     int refill;                /* idle lanes of a 64-lane wave that trigger a fetch,
                                   1..64 (default 48)                                     */
+    int hier_refill;           /* the same for the AO rays of the instanced hierarchy,
+                                  whose lengths vary more: 1..64 (default 12)            */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

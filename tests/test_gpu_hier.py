@@ -112,10 +112,15 @@ def test_hier_trace_parity(inst_scene):
     e.close()
 
 
-def test_hier_ao_parity(inst_scene):
-    print("hierarchy AO parity", flush=True)
+@pytest.mark.parametrize("pp,pers,refill", [(0, 0, 12), (1, 1, 1), (0, 1, 12), (1, 1, 64)])
+def test_hier_ao_parity(inst_scene, pp, pers, refill):
+    """AO frames of the hierarchy, with the camera and AO rays in the
+    one-ray-per-lane kernels and in the persistent kernel at several refill
+    thresholds."""
+    print(f"hierarchy AO parity persistent primary={pp} ao={pers} refill={refill}", flush=True)
     sc, ins = inst_scene
     e = _gpu(sc, ins)
+    e.set_tuning(pp, pers, hier_refill=refill)
     h = _oracle(sc, ins)
     c = sc.camera
     rx, ry = c["res"]
