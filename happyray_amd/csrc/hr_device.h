@@ -208,7 +208,7 @@ struct ScanCheck {
 };
 
 constexpr uint32_t kScanTile = 2048;     // 256 threads x 8
-constexpr uint32_t kSortTile = 4096;     // 256 threads x 16
+constexpr uint32_t kSortTile = 2048;     // 256 threads x 8
 
 // Exclusive scan of n uint32 (n_cap, or *n_dev when n_dev != nullptr) into
 // out[i * out_stride], with decoupled look-back: one launch; tiles take

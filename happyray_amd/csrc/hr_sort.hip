@@ -202,7 +202,7 @@ __global__ void k_fill(uint32_t* __restrict__ p, uint32_t value, size_t n_cap,
 // ---------------------------------------------------------------------------
 
 constexpr int kRadix = 256;
-constexpr int kSortItems = kSortTile / kBlock;   // 16
+constexpr int kSortItems = kSortTile / kBlock;   // 8
 constexpr int kSortWaves = kBlock / kWave;       // 4
 
 // Chunked LSD radix sort.  A pass runs over a fixed grid of G workgroups;
@@ -216,7 +216,7 @@ constexpr int kSortWaves = kBlock / kWave;       // 4
 //     each digit's run at the chunk's running offset.
 // Every chunk's tiles keep their order, so the sort is stable.
 
-constexpr int kSortGrid = 512;                   // workgroups per pass (at most)
+constexpr int kSortGrid = 1024;                  // workgroups per pass (at most)
 
 // rank of each of the wave's keys among the equal digits of the wave (lanes
 // in order), and the wave's counts per digit accumulated into h[digit]
