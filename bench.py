@@ -30,7 +30,7 @@ FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_STEP = 20
 FLOPS_PER_TEST = 50
 BYTES_PER_STEP = 8          # one (begin, end) cell or leaf
-BYTES_PER_TEST = 4 + 48     # primitive id + triangle record
+BYTES_PER_TEST = 4 + 36     # primitive id + triangle record (9 floats)
 
 
 def parse():

@@ -96,7 +96,7 @@ hr_status check_batch(hr_batch* b)
 hr_status reserve_build(hr_batch* b)
 {
     const uint32_t pair_max = std::max(b->pairs_cap, b->leaf_pairs_cap);
-    HR_TRY(b->rec.reserve(3 * (size_t)std::max(b->nt, 1u)));
+    HR_TRY(b->rec.reserve((9 * (size_t)std::max(b->nt, 1u) + 3) / 4));   // 9 floats per triangle
     const uint32_t waves = cdiv(std::max<uint64_t>(std::max(b->nt, b->pairs_cap), 1), 64);
     const uint32_t waves_cap = (waves + 3) / 4 * 4 + 4;
     HR_TRY(b->wave_counts.reserve(waves_cap));

@@ -20,7 +20,7 @@ struct BuildArgs {
 struct BuildBuffers {
     GridParams* params;
     BuildStatus* status;
-    float4* rec;                // 3 * nt triangle records
+    float4* rec;                // triangle records: 9 floats per triangle
     uint32_t* wave_counts;      // per-wave pair counts
     uint32_t* wave_offsets;     // their exclusive scan
     uint32_t* keys;             // unsorted keys / values (pairs, then leaf pairs)

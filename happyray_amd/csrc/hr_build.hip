@@ -329,9 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_top_pairs(
             v0 = mk(vx[i0], vy[i0], vz[i0]);
             v1 = mk(vx[i1], vy[i1], vz[i1]);
             v2 = mk(vx[i2], vy[i2], vz[i2]);
-            rec[3 * (size_t)p + 0] = make_float4(v0.x, v0.y, v0.z, 0.f);
-            rec[3 * (size_t)p + 1] = make_float4(v1.x, v1.y, v1.z, 0.f);
-            rec[3 * (size_t)p + 2] = make_float4(v2.x, v2.y, v2.z, 0.f);
+            float* f = reinterpret_cast<float*>(rec) + 9 * (size_t)p;
+            f[0] = v0.x; f[1] = v0.y; f[2] = v0.z;
+            f[3] = v1.x; f[4] = v1.y; f[5] = v1.z;
+            f[6] = v2.x; f[7] = v2.y; f[8] = v2.z;
         } else {
             load_tri(rec, p, v0, v1, v2);
         }

@@ -118,16 +118,15 @@ __device__ __forceinline__ int wave_owner(const uint32_t* excl, uint32_t j)
     return o;
 }
 
-// triangle record: three float4 (vertex xyz, unused w) per triangle
+// triangle record: nine packed floats (v0, v1, v2) per triangle, in a float4
+// buffer of ceil(9 nt / 4) elements
 __device__ __forceinline__ void load_tri(const float4* __restrict__ rec, uint32_t p, f3& v0, f3& v1,
                                          f3& v2)
 {
-    const float4 a = rec[3 * (size_t)p + 0];
-    const float4 b = rec[3 * (size_t)p + 1];
-    const float4 c = rec[3 * (size_t)p + 2];
-    v0 = mk(a.x, a.y, a.z);
-    v1 = mk(b.x, b.y, b.z);
-    v2 = mk(c.x, c.y, c.z);
+    const float* f = reinterpret_cast<const float*>(rec) + 9 * (size_t)p;
+    v0 = mk(f[0], f[1], f[2]);
+    v1 = mk(f[3], f[4], f[5]);
+    v2 = mk(f[6], f[7], f[8]);
 }
 
 // BBoxExtractor<Triangle> (src/RT/Primitive/BBox.hpp:96-110)

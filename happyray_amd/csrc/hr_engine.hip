@@ -189,7 +189,7 @@ hr_status reserve_build(hr_ctx* c)
     const bool hier = c->kind == HR_GRID_HIERARCHY;
     const bool two = c->kind == HR_GRID_TWO_LEVEL || hier;
     const uint32_t pair_max = std::max(c->pairs_cap, two ? c->leaf_pairs_cap : 0u);
-    HR_TRY(c->rec.reserve(3 * (size_t)std::max(nt, 1u)));
+    HR_TRY(c->rec.reserve((9 * (size_t)std::max(nt, 1u) + 3) / 4));   // 9 floats per triangle
     const uint32_t waves = cdiv(std::max<uint64_t>(std::max(nt, c->pairs_cap), 1), 64);
     const uint32_t waves_cap = (waves + 3) / 4 * 4 + 4;
     HR_TRY(c->wave_counts.reserve(waves_cap));

@@ -206,9 +206,10 @@ constexpr int kSortItems = kSortTile / kBlock;   // 8
 constexpr int kSortWaves = kBlock / kWave;       // 4
 
 // Chunked LSD radix sort.  A pass runs over a fixed grid of G workgroups;
-// workgroup b owns a contiguous chunk of whole 4096-key tiles.
-//  1. k_sort_count: each workgroup counts the digits of its chunk (wave
-//     ballots, no atomics) into counts[digit * G + b];
+// workgroup b owns a contiguous chunk of whole 2048-key tiles.
+//  1. k_sort_count: each workgroup counts the digits of its chunk (one LDS
+//     histogram per wave; counts do not depend on the order of the adds)
+//     into counts[digit * G + b];
 //  2. the single-pass scan of that digit-major array gives every chunk's
 //     first output position per digit;
 //  3. k_sort_scatter: each workgroup walks its tiles in order, ranks the keys
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_count(const uint32_t* __restric
 #pragma unroll
         for (int k = 0; k < kSortItems; ++k) {
             const uint32_t i = base + (uint32_t)k * kWave;
-            (void)wave_digit_rank((kr[k] >> shift) & (kRadix - 1), i < n, h[w]);
+            if (i < n) atomicAdd(&h[w][(kr[k] >> shift) & (kRadix - 1)], 1u);
         }
     }
     __syncthreads();
