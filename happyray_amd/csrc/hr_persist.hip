@@ -364,11 +364,12 @@ struct GenBatch {
 // the persistent kernel
 // ---------------------------------------------------------------------------
 
+// at most 64 VGPRs, so eight waves per SIMD hide the gathers' latency
+// (AO 3.19 -> 3.12 ms against the compiler's 70 VGPRs and seven waves)
 template <bool kTwo, bool kAny, class Gen>
-__global__ __launch_bounds__(kBlock) void k_persist(TraceScene S, Gen gen, uint32_t nrays,
-                                                    uint32_t* __restrict__ pool, uint32_t refill,
-                                                    unsigned long long* ray_count,
-                                                    unsigned long long* stats)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_persist(
+    TraceScene S, Gen gen, uint32_t nrays, uint32_t* __restrict__ pool, uint32_t refill,
+    unsigned long long* ray_count, unsigned long long* stats)
 {
     const GridRegs G = load_grid(S.gp);
     const uint64_t lt = __lanemask_lt();

@@ -40,11 +40,11 @@ using namespace trav;
 // primary rays
 // ---------------------------------------------------------------------------
 
+// six waves per SIMD (at most 80 VGPRs): 0.695 -> 0.682 ms for 1024x1024 camera rays
 template <bool kTwo>
-__global__ __launch_bounds__(kBlock) void k_primary(TraceScene S, CameraDev cam, RenderArgs ra,
-                                                    uint32_t npix, float* __restrict__ hits,
-                                                    uint32_t* __restrict__ hit_prims,
-                                                    unsigned long long* ray_count)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_primary(
+    TraceScene S, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
+    uint32_t* __restrict__ hit_prims, unsigned long long* ray_count)
 {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = pix < npix;
