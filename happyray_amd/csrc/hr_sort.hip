@@ -210,9 +210,10 @@ constexpr int kSortWaves = kBlock / kWave;       // 4
 //  1. k_sort_count: each workgroup counts the digits of its chunk (one LDS
 //     histogram per wave; counts do not depend on the order of the adds)
 //     into counts[digit * G + b];
-//  2. the single-pass scan of that digit-major array gives every chunk's
-//     first output position per digit;
-//  3. k_sort_scatter: each workgroup walks its tiles in order, ranks the keys
+//  2. k_sort_rowscan: one workgroup per digit scans that digit's row (each
+//     chunk's first slot among the digit's keys) and writes the digit total;
+//  3. k_sort_scatter: each workgroup adds the totals of the smaller digits to
+//     its row entries, then walks its tiles in order, ranks the keys
 //     of a tile by digit (stable), stages the tile in LDS by digit and writes
 //     each digit's run at the chunk's running offset.
 // Every chunk's tiles keep their order, so the sort is stable.
@@ -283,10 +284,39 @@ __global__ __launch_bounds__(kBlock) void k_sort_count(const uint32_t* __restric
     counts[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = c;
 }
 
+// one workgroup per digit d: the exclusive scan of counts[d * G .. d * G + G)
+// in place (each chunk's first slot among the digit's keys) and the digit's
+// total; rows are independent, so there is no chain between workgroups
+__global__ __launch_bounds__(kBlock) void k_sort_rowscan(uint32_t* __restrict__ counts, uint32_t g,
+                                                         uint32_t* __restrict__ totals)
+{
+    __shared__ uint32_t lds_scan[kSortWaves];
+    constexpr uint32_t kPer = (uint32_t)kSortGrid / kBlock;
+    uint32_t* row = counts + (size_t)blockIdx.x * g;
+    uint32_t v[kPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t i = threadIdx.x * kPer + k;
+        v[k] = i < g ? row[i] : 0u;
+        s += v[k];
+    }
+    uint32_t all;
+    uint32_t run = block_excl_scan(s, lds_scan, all);
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t i = threadIdx.x * kPer + k;
+        if (i < g) row[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = all;
+}
+
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(
     const uint32_t* __restrict__ keys_src, const uint32_t* __restrict__ vals_src,
     uint32_t* __restrict__ keys_dst, uint32_t* __restrict__ vals_dst,
-    const uint32_t* __restrict__ n_dev, int shift, const uint32_t* __restrict__ counts)
+    const uint32_t* __restrict__ n_dev, int shift, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ totals)
 {
     __shared__ uint32_t skeys[kSortTile];
     __shared__ uint32_t svals[kSortTile];
@@ -301,7 +331,12 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
     if (t0 >= t1) return;
     const int w = (int)threadIdx.x / kWave;
     const int lane = (int)__lane_id();
-    offs[threadIdx.x] = counts[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+    {
+        // the digit's first slot: the keys of all smaller digits
+        uint32_t all;
+        const uint32_t base = block_excl_scan(totals[threadIdx.x], lds_scan, all);
+        offs[threadIdx.x] = base + counts[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+    }
     for (uint32_t t = t0; t < t1; ++t) {
         const uint32_t base = t * kSortTile;
 #pragma unroll
@@ -405,8 +440,7 @@ void radix_sort_pairs(uint32_t* keys_in, uint32_t* vals_in, uint32_t* keys_out, 
     const uint32_t g = sort_grid(n_cap);
     const uint32_t m = (uint32_t)kRadix * g;
     uint32_t* counts = tmp.hist;
-    uint32_t* offsets = tmp.hist + m;
-    ScanTemp st{ tmp.chain, scan_chain_words(m) };
+    uint32_t* totals = tmp.hist + m;     // kRadix words of the second half
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
     if (passes > 4) passes = 4;
@@ -419,8 +453,9 @@ void radix_sort_pairs(uint32_t* keys_in, uint32_t* vals_in, uint32_t* keys_out, 
         else if ((p & 1) == 0) { kd = tmp.keys_tmp; vd = tmp.vals_tmp; }
         else { kd = keys_in; vd = vals_in; }
         hipLaunchKernelGGL(k_sort_count, dim3(g), dim3(kBlock), 0, s, ks, n_dev, 8 * p, counts);
-        scan_exclusive_1p(counts, offsets, 1, m, nullptr, nullptr, nullptr, st, s);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(g), dim3(kBlock), 0, s, ks, vs, kd, vd, n_dev, 8 * p, offsets);
+        hipLaunchKernelGGL(k_sort_rowscan, dim3(kRadix), dim3(kBlock), 0, s, counts, g, totals);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(g), dim3(kBlock), 0, s, ks, vs, kd, vd, n_dev, 8 * p, counts,
+                           totals);
         ks = kd;
         vs = vd;
     }
