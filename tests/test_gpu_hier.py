@@ -172,6 +172,11 @@ def test_hier_rejects_other_modes_and_missing_instances(inst_scene):
     e.render_frame(1, MODE_PATH)
     with pytest.raises(HappyRayError, match="HR_ERR_STATE"):
         e.dump_ao_rays(2)
+    # the hierarchy's refill threshold is validated like the others
+    for bad in (0, 65):
+        with pytest.raises(HappyRayError, match="HR_ERR_ARG"):
+            e.set_tuning(0, 1, 48, bad)
+    assert e.tuning().hier_refill == 12
     e.close()
 
 
