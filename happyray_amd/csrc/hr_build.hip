@@ -1112,7 +1112,9 @@ __global__ void k_obj_setup(const uint32_t* __restrict__ enc, const uint32_t* __
 // countPairs over instance boxes (UniformGridBuildKernels.h:40-105): one lane
 // per instance
 __global__ void k_inst_count(const InstanceDev* __restrict__ inst, uint32_t ninst, const GridParams* __restrict__ gp,
-                             const BuildStatus* __restrict__ st, uint32_t* __restrict__ counts)
+                             const BuildStatus* __restrict__ st, uint32_t* __restrict__ counts,
+                             const GridParams* __restrict__ obj_gp, const uint32_t* __restrict__ obj_base,
+                             InstGrid* __restrict__ inst_grid)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
@@ -1121,6 +1123,20 @@ __global__ void k_inst_count(const InstanceDev* __restrict__ inst, uint32_t nins
         return;
     }
     const InstanceDev I = inst[i];
+    {
+        // the instance's object grid, for the traversal
+        const GridParams& O = obj_gp[I.object];
+        InstGrid q;
+        for (int k = 0; k < 3; ++k) {
+            q.bmin[k] = O.bmin[k];
+            q.bmax[k] = O.bmax[k];
+            q.cs[k] = O.cs[k];
+            q.csr[k] = O.csr[k];
+            q.res[k] = O.res[k];
+        }
+        q.base = obj_base[I.object];
+        inst_grid[i] = q;
+    }
     int mn[3], mx[3];
     box_cells(mk(I.bmin[0], I.bmin[1], I.bmin[2]), mk(I.bmax[0], I.bmax[1], I.bmax[2]),
               mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]), mk(gp->csr[0], gp->csr[1], gp->csr[2]), gp->res[0],
@@ -1218,7 +1234,8 @@ void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s)
 
     // ---- top references: count, scan, write, sort, clear the top cells, ranges (and seal) ----
     const uint32_t iblocks = cdiv(a.ninst > 0 ? a.ninst : 1, 256);
-    hipLaunchKernelGGL(k_inst_count, dim3(iblocks), dim3(256), 0, s, a.inst, a.ninst, b.top, st, b.inst_counts);
+    hipLaunchKernelGGL(k_inst_count, dim3(iblocks), dim3(256), 0, s, a.inst, a.ninst, b.top, st, b.inst_counts,
+                       b.obj_gp, b.obj_cell_base, b.inst_grid);
     const ScanCheck inst_check{ st, b.pairs_cap, kOverflowPairs, &st->n_pairs_eff };
     scan_exclusive_1p(b.inst_counts, b.inst_offsets, 1, a.ninst, nullptr, &st->num_pairs, &inst_check, b.scan, s);
     hipLaunchKernelGGL(k_inst_write, dim3(range_blocks(b.pairs_cap)), dim3(256), 0, s, a.inst, a.ninst, b.top, st,

@@ -94,6 +94,7 @@ struct hr_ctx {
     DevBuf<uint32_t> tri_obj, obj_first, obj_bounds_enc, obj_cell_counts, obj_cell_base, inst_counts, inst_offsets,
         obj_part;
     DevBuf<hr::InstanceDev> inst;
+    DevBuf<hr::InstGrid> inst_grid;
     DevBuf<hr::GridParams> obj_gp;
     DevBuf<uint32_t> hit_inst;        // instance of each pixel's primary hit (hierarchy renders)
     DevBuf<uint32_t> ray_inst;        // instance of each traced ray's hit (hr_trace_instanced)
@@ -212,6 +213,7 @@ hr_status reserve_build(hr_ctx* c)
         HR_TRY(c->obj_cell_base.reserve(c->nobj));
         HR_TRY(c->inst_counts.reserve(c->ninst));
         HR_TRY(c->inst_offsets.reserve(c->ninst));
+        HR_TRY(c->inst_grid.reserve(c->ninst));
     }
     const size_t scan_n = std::max<size_t>({ (size_t)waves_cap, (size_t)c->cells_cap, (size_t)c->nobj,
                                              (size_t)c->ninst });
@@ -300,6 +302,7 @@ void enqueue_build(hr_ctx* c, const hr::FrameSel* fs = nullptr)
         b.obj_part = c->obj_part.p;
         b.obj_cell_counts = c->obj_cell_counts.p;
         b.obj_cell_base = c->obj_cell_base.p;
+        b.inst_grid = c->inst_grid.p;
         b.rec = c->rec.p;
         b.wave_counts = c->wave_counts.p;
         b.wave_offsets = c->wave_offsets.p;
@@ -440,6 +443,7 @@ hr::HierScene hier_scene(hr_ctx* c)
     H.top_cells = c->cells.p;
     H.inst_refs = c->top_vals_sorted.p;
     H.inst = c->inst.p;
+    H.inst_grid = c->inst_grid.p;
     H.obj_gp = c->obj_gp.p;
     H.obj_cell_base = c->obj_cell_base.p;
     H.obj_cells = c->leaves.p;
@@ -791,7 +795,7 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     b += c->ray_prim.bytes() + c->ray_occ.bytes() + c->ray_hits.bytes() + c->pm.bytes() + c->pm_prims.bytes();
     b += c->ray_wire.bytes() + c->tri_obj.bytes() + c->obj_first.bytes() + c->obj_bounds_enc.bytes();
     b += c->obj_cell_counts.bytes() + c->obj_cell_base.bytes() + c->inst_counts.bytes() + c->inst_offsets.bytes();
-    b += c->inst.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes() + c->ray_inst.bytes() + c->obj_part.bytes();
+    b += c->inst.bytes() + c->inst_grid.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes() + c->ray_inst.bytes() + c->obj_part.bytes();
     b += c->bounds_part.bytes();
     return b;
 }
@@ -921,6 +925,7 @@ hr_status hr_destroy(hr_ctx* c)
     c->occ.release();
     c->ray_occ.release();
     c->inst.release();
+    c->inst_grid.release();
     c->obj_gp.release();
     c->rec.release();
     c->cells.release();

@@ -27,12 +27,22 @@ struct InstanceDev {
     uint32_t pad;
 };
 
+// Per instance: its object's grid and first object cell, gathered by the
+// build so that the traversal loads them beside the instance rather than
+// after it (the object id is no longer on the load path).
+struct InstGrid {
+    float bmin[3], bmax[3], cs[3], csr[3];
+    int res[3];
+    uint32_t base;
+};
+
 // Everything the hierarchy traversal reads, by device pointer.
 struct HierScene {
     const GridParams* top;          // uniform grid over the instance boxes
     const uint2* top_cells;         // (begin, end) into inst_refs
     const uint32_t* inst_refs;      // instance index per top-level reference
     const InstanceDev* inst;
+    const InstGrid* inst_grid;      // per instance: its object's grid
     const GridParams* obj_gp;       // per object: its uniform grid
     const uint32_t* obj_cell_base;  // per object: its first cell in obj_cells
     const uint2* obj_cells;         // every object's cells, (begin, end) into prims
@@ -62,6 +72,7 @@ struct HierBuffers {
     uint32_t* obj_part;             // 7 * kObjBoundsBlocks: per-block partial object bounds
     uint32_t* obj_cell_counts;      // nobj
     uint32_t* obj_cell_base;        // nobj: first cell of each object in obj_cells
+    InstGrid* inst_grid;            // ninst: each instance's object grid
     float4* rec;
     uint32_t *wave_counts, *wave_offsets;
     uint32_t waves_cap;

@@ -62,6 +62,19 @@ struct Mailbox {
     uint32_t e[8];
 };
 
+__device__ __forceinline__ GridRegs inst_grid_regs(const InstGrid& q)
+{
+    GridRegs g;
+    g.bmin = mk(q.bmin[0], q.bmin[1], q.bmin[2]);
+    g.bmax = mk(q.bmax[0], q.bmax[1], q.bmax[2]);
+    g.cs = mk(q.cs[0], q.cs[1], q.cs[2]);
+    g.csr = mk(q.csr[0], q.csr[1], q.csr[2]);
+    g.rx = q.res[0];
+    g.ry = q.res[1];
+    g.rz = q.res[2];
+    return g;
+}
+
 __device__ __forceinline__ bool mailbox_seen(Mailbox& m, uint32_t id)
 {
     const uint32_t k = id & 7u;
@@ -85,6 +98,7 @@ __device__ void hier_cell(const HierScene& H, uint32_t b, uint32_t e, f3 org, f3
         const uint32_t ii = H.inst_refs[it];
         if (mailbox_seen(mb, ii)) continue;
         const InstanceDev I = H.inst[ii];
+        const InstGrid Q = H.inst_grid[ii];
         const f3 t1 = (mk(I.bmin[0], I.bmin[1], I.bmin[2]) - org) * rcp;
         f3 tm = (mk(I.bmax[0], I.bmax[1], I.bmax[2]) - org) * rcp;
         const f3 tn = vmin(t1, tm);
@@ -95,8 +109,8 @@ __device__ void hier_cell(const HierScene& H, uint32_t b, uint32_t e, f3 org, f3
         f3 ot, rt;
         instance_ray(I, org, rcp, ot, rt);
         const f3 dt = mk(1.f / rt.x, 1.f / rt.y, 1.f / rt.z);
-        const GridRegs G = load_grid(H.obj_gp + I.object);
-        const uint32_t base = H.obj_cell_base[I.object];
+        const GridRegs G = inst_grid_regs(Q);
+        const uint32_t base = Q.base;
         float rayT = t;
         uint32_t hit = H.nt;
         const f3 g1 = (G.bmin - ot) * rt;
@@ -524,6 +538,7 @@ __device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, H
         ++r.it;
         if (mailbox_seen(r.mb, ii)) continue;
         const InstanceDev I = H.inst[ii];
+        const InstGrid Q = H.inst_grid[ii];
         const f3 b1 = (mk(I.bmin[0], I.bmin[1], I.bmin[2]) - r.org) * r.rcp;
         f3 bm = (mk(I.bmax[0], I.bmax[1], I.bmax[2]) - r.org) * r.rcp;
         const f3 bn = vmin(b1, bm);
@@ -533,7 +548,7 @@ __device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, H
         if (!(tx1 > te1 && tx1 >= 0.f && te1 < r.t)) continue;
         f3 ot, rt;
         instance_ray(I, r.org, r.rcp, ot, rt);
-        const GridRegs O = load_grid(H.obj_gp + I.object);
+        const GridRegs O = inst_grid_regs(Q);
         const f3 g1 = (O.bmin - ot) * rt;
         f3 gf = (O.bmax - ot) * rt;
         const f3 gn = vmin(g1, gf);
@@ -561,7 +576,7 @@ __device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, H
         r.rx = O.rx;
         r.ry = O.ry;
         r.rz = O.rz;
-        r.base = H.obj_cell_base[I.object];
+        r.base = Q.base;
         r.otx = ((cf.x + (rt.x > 0.f ? 1.f : 0.f)) * O.cs.x + O.bmin.x - ot.x) * rt.x;
         r.oty = ((cf.y + (rt.y > 0.f ? 1.f : 0.f)) * O.cs.y + O.bmin.y - ot.y) * rt.y;
         r.otz = ((cf.z + (rt.z > 0.f ? 1.f : 0.f)) * O.cs.z + O.bmin.z - ot.z) * rt.z;
