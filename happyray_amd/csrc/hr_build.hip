@@ -1198,7 +1198,7 @@ void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s)
     hipLaunchKernelGGL(k_hier_top_setup, dim3(1), dim3(64), 0, s, b.top, st, a.ninst, a.lambda_top, b.cells_cap);
     hipLaunchKernelGGL(k_obj_reset, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, nob);
     if (nt > 0) {
-        uint32_t tblocks = cdiv(nt, 512);
+        uint32_t tblocks = cdiv(nt, 256);   // one triangle per thread (0.197 -> 0.162 ms of rebuild against two)
         if (tblocks > kObjBoundsBlocks) tblocks = kObjBoundsBlocks;
         const uint32_t per_block = cdiv(cdiv(nt, tblocks), kBlock) * kBlock;
         hipLaunchKernelGGL(k_obj_bounds, dim3(cdiv(nt, per_block)), dim3(kBlock), 0, s, a.idx, a.kx, a.ky, a.kz, nt,

@@ -90,7 +90,7 @@ struct HierBuffers {
     SortTemp sort;
 };
 
-constexpr uint32_t kObjBoundsBlocks = 512;   // workgroups of the object-bounds pass (at most)
+constexpr uint32_t kObjBoundsBlocks = 2048;   // workgroups of the object-bounds pass (at most)
 
 // everything after the lerp / bounds: object grids, then the top grid
 void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s);
