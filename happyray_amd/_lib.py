@@ -64,7 +64,7 @@ class Timing(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("persistent_primary", C.c_int), ("persistent_secondary", C.c_int),
-                ("refill", C.c_int), ("hier_refill", C.c_int)]
+                ("refill", C.c_int), ("hier_refill", C.c_int), ("hier_bounds_blocks", C.c_int)]
 
 
 class WorkCounts(C.Structure):

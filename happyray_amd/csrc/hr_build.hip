@@ -1199,7 +1199,9 @@ void run_build_hier(const HierArgs& a, HierBuffers& b, hipStream_t s)
     hipLaunchKernelGGL(k_obj_reset, dim3(oblocks), dim3(256), 0, s, b.obj_bounds_enc, nob);
     if (nt > 0) {
         uint32_t tblocks = cdiv(nt, 256);   // one triangle per thread (0.197 -> 0.162 ms of rebuild against two)
-        if (tblocks > kObjBoundsBlocks) tblocks = kObjBoundsBlocks;
+        const uint32_t cap = (a.bounds_blocks >= 1 && a.bounds_blocks <= kObjBoundsBlocks) ? a.bounds_blocks
+                                                                                        : kObjBoundsBlocks;
+        if (tblocks > cap) tblocks = cap;
         const uint32_t per_block = cdiv(cdiv(nt, tblocks), kBlock) * kBlock;
         hipLaunchKernelGGL(k_obj_bounds, dim3(cdiv(nt, per_block)), dim3(kBlock), 0, s, a.idx, a.kx, a.ky, a.kz, nt,
                            a.tri_obj, b.obj_bounds_enc, per_block, b.obj_part);

@@ -136,7 +136,7 @@ struct hr_ctx {
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
-    hr_tuning tuning{ 0, 1, 48, 12 };
+    hr_tuning tuning{ 0, 1, 48, 12, 0 };
 
     // captured frames: a rebuild graph and a render graph, replayed per frame
     hipGraphExec_t g_build = nullptr, g_render = nullptr;
@@ -294,6 +294,7 @@ void enqueue_build(hr_ctx* c, const hr::FrameSel* fs = nullptr)
         a.inst = c->inst.p;
         a.lambda_top = c->lambda_top;
         a.lambda_leaf = c->lambda_leaf;
+        a.bounds_blocks = (uint32_t)c->tuning.hier_bounds_blocks;
         hr::HierBuffers b;
         b.status = c->status.p;
         b.top = c->params.p;
@@ -518,6 +519,10 @@ hr_status check_render_args(hr_ctx* c, hr_mode mode, int image_id, const hr_rend
         return fail(HR_ERR_STATE, "the instanced hierarchy renders ambient occlusion only");
     if (mode == HR_MODE_AO && (p->ao_samples < 1 || p->ao_samples > 256))
         return fail(HR_ERR_ARG, "ao_samples must be in [1, 256]");
+    // AO ray ids are 32-bit (pixel * samples + sample)
+    if (mode == HR_MODE_AO &&
+        (uint64_t)c->cam.res_x * (uint64_t)c->cam.res_y * (uint64_t)p->ao_samples > 0xFFFFFFFFull)
+        return fail(HR_ERR_ARG, "pixels * ao_samples exceeds 2^32 - 1 AO rays");
     if (mode == HR_MODE_PATH && (p->pt_depth < 1 || p->pt_depth > 64))
         return fail(HR_ERR_ARG, "pt_depth must be in [1, 64]");
     return HR_OK;
@@ -976,6 +981,10 @@ hr_status hr_upload_mesh(hr_ctx* c, const float* xyz1, const float* xyz2, float 
     }
     c->src1 = c->key_a.p;
     c->coeff = coeff;
+    // the resident key frames (and graphs recorded over them) describe the
+    // previous mesh: frame selection needs hr_set_keyframes again
+    c->nframes = 0;
+    drop_graphs(c);
     c->geo_frames = c->src2 == c->src1 ? 1u : 2u;
     c->k1 = 0;
     c->k2 = c->geo_frames - 1u;
@@ -1517,6 +1526,8 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
         return fail(HR_ERR_ARG, "persistent flags must be 0 or 1");
     if (t->refill < 1 || t->refill > 64) return fail(HR_ERR_ARG, "refill must be in [1, 64]");
     if (t->hier_refill < 1 || t->hier_refill > 64) return fail(HR_ERR_ARG, "hier_refill must be in [1, 64]");
+    if (t->hier_bounds_blocks < 0 || t->hier_bounds_blocks > (int)hr::kObjBoundsBlocks)
+        return fail(HR_ERR_ARG, "hier_bounds_blocks must be in [0, 2048]");
     c->tuning = *t;
     return HR_OK;
 }

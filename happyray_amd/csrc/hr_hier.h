@@ -62,6 +62,7 @@ struct HierArgs {
     uint32_t ninst;
     const InstanceDev* inst;
     float lambda_top, lambda_leaf;
+    uint32_t bounds_blocks;         // cap on the object-bounds workgroups (0: kObjBoundsBlocks)
 };
 
 struct HierBuffers {

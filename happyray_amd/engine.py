@@ -346,8 +346,10 @@ class Engine:
         check(self._L.hr_get_timing(self._h, C.byref(t)), "hr_get_timing")
         return t
 
-    def set_tuning(self, persistent_primary=0, persistent_secondary=1, refill=48, hier_refill=12):
-        t = _lib.Tuning(int(persistent_primary), int(persistent_secondary), int(refill), int(hier_refill))
+    def set_tuning(self, persistent_primary=0, persistent_secondary=1, refill=48, hier_refill=12,
+                   hier_bounds_blocks=0):
+        t = _lib.Tuning(int(persistent_primary), int(persistent_secondary), int(refill), int(hier_refill),
+                        int(hier_bounds_blocks))
         check(self._L.hr_set_tuning(self._h, C.byref(t)), "hr_set_tuning")
 
     def tuning(self):

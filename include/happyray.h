@@ -281,6 +281,10 @@ typedef struct {
                                   1..64 (default 48)                                     */
     int hier_refill;           /* the same for the AO rays of the instanced hierarchy,
                                   whose lengths vary more: 1..64 (default 12)            */
+    int hier_bounds_blocks;    /* workgroups of the hierarchy build's object-bounds pass:
+                                  0 (default) one triangle per thread on up to 2048
+                                  workgroups; 1..2048 caps the count, so that each
+                                  thread walks a run of several triangles              */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

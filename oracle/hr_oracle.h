@@ -173,6 +173,25 @@ int  or_path_trace_rays(const or_mesh* mesh, const or_grid* g,
  * (PathTracer.h:109-135), exposed for tests */
 void or_shading_normal(const or_mesh* mesh, uint32_t prim, const float org[3], float out[3]);
 
+/* The path tracers' sampling steps, exposed for tests (the renderers call the
+ * same functions).  or_pt_glass: one vertex on a transparent material of index
+ * ior (PathTracer.h:133-183; mode1 != 0: :556-622), updating org, dir and the
+ * throughput att; returns 0 reflected, 1 refracted, 2 ended.
+ * or_power_cosine_dir: PowerCosineHemisphereSampler about the unit axis upv
+ * (HemisphereSamplers.hpp:78-94).  or_cosine_dir: CosineHemisphereSampler
+ * about the unit normal n (:58-71) in getLocalCoordinates' frame.
+ * or_pt_glossy: the megakernel's glossy bounce (PathTracer.h:284-305) for
+ * cos_ne = -n.dir >= 0; returns 0 when the path ends.  or_pm_merge: mode 1's
+ * addIndirectIllumination (PathTracer.h:889-990) on 3-float-per-pixel
+ * buffers; updates residue and fb. */
+int  or_pt_glass(float org[3], float dir[3], float att[3], const float n[3], float cos_ne, float ior,
+                 float ray_t, const float kd[3], float w_fres, float w_tir, int mode1);
+void or_power_cosine_dir(float w0, float w1, float se, const float upv[3], float out[3]);
+void or_cosine_dir(float w0, float w1, const float n[3], float out[3]);
+int  or_pt_glossy(float dir[3], float att[3], const float n[3], float cos_ne, const float ks[3], float se,
+                  float w0, float w1);
+void or_pm_merge(const float* imp, const float* nrad, float* residue, float* fb, int image_id, int rx, int ry);
+
 /* ---- traversal (or_trace.c) ---- */
 int  or_trace(const or_mesh* mesh, const or_grid* g,
               const float* ox, const float* oy, const float* oz,

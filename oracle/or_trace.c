@@ -31,8 +31,18 @@
  * intersection arithmetic (numpy float32), ids equal except equal-t ties;
  * any-hit against brute force; known-answer rays through a unit right
  * triangle; rays starting inside the grid, axis-parallel and grazing rays;
- * an unoccluded plane renders the closed-form AO shade; the closed-box
- * white furnace gives E[L] = Le * sum_{k<d} a^k within a statistical bound.
+ * an unoccluded plane renders the closed-form AO shade.  Path tracer: a
+ * closed-box white furnace (every wall emits Le, albedo a per channel, the
+ * walls as area lights) has mean radiance Le * sum_{k<=d} a^k at depth cap d
+ * and traces 1 + sum_{k<d} A^k * 5/6 + sum_{1<=k<d} A^k rays per path
+ * (A = max albedo: the roulette); the glass step (or_pt_glass) against
+ * Snell's law, the mirror law, Schlick's reflectance frequency, total
+ * internal reflection and Beer absorption; a glass slab transmits
+ * (1 - F0)^2 of the backdrop's emission; the power-cosine lobe
+ * (or_power_cosine_dir) against its moments (n+1)/(n+2) and (n+1)/(n+3);
+ * the glossy bounce (or_pt_glossy) conserves ks / max(ks) at normal
+ * incidence; the mode-1 merge (or_pm_merge) against the 3x3 weights read
+ * from addIndirectIllumination with delta inputs.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -566,6 +576,120 @@ int or_render_ao(const or_mesh* m, const or_grid* g, const or_camera* cam,
 /* path tracing                                                        */
 /* ------------------------------------------------------------------ */
 
+/* One vertex on a transparent material (PathTracer.h:133-183 in the megakernel,
+ * :556-622 in the multi-kernel tracer).  The ray arrives along dir at org after
+ * travelling ray_t; n is the shading normal, cos_ne = -n.dir.  Inside the
+ * material (cos_ne < 0) the normal is turned and the throughput absorbs
+ * exp(-ray_t * kd * pi).  Schlick's reflectance with n_air = 1.000293 picks
+ * reflection when w_fres < F, or always under total internal reflection,
+ * where the throughput doubles when w_tir < 0.5 and the path ends otherwise.
+ * refr_off: the refracted origin moves this far against the normal (5 EPS in
+ * the megakernel, 10 EPS in mode 1); tir_ge: total reflection is sin2 >= 1
+ * (megakernel) or sin2 > 1 (mode 1) when choosing the branch.
+ * Returns 0 reflected, 1 refracted, 2 ended. */
+static int glass_vertex(f3* org, f3* dir, f3* att, f3 n, float cos_ne, float ior, float ray_t, f3 kd,
+                        float w_fres, float w_tir, float refr_off, int tir_ge)
+{
+    float n1 = 1.000293f, n2 = ior;
+    if (cos_ne < 0.f) {
+        n1 = ior; n2 = 1.000293f;
+        n = neg3(n);
+        cos_ne = -cos_ne;
+        att->x *= or_expf(-ray_t * kd.x * OR_PI);
+        att->y *= or_expf(-ray_t * kd.y * OR_PI);
+        att->z *= or_expf(-ray_t * kd.z * OR_PI);
+    }
+    const float f0 = (n1 - n2) * (n1 - n2) / ((n1 + n2) * (n1 + n2));
+    const float fres = f0 + (1.f - f0) * or_powf(1.f - fabsf(cos_ne), 5.f);
+    const float nr = n1 / n2;
+    const float sin2 = nr * nr * (1.f - cos_ne * cos_ne);
+    const int tir = tir_ge ? sin2 >= 1.f : sin2 > 1.f;
+    if (tir || w_fres < fres) {
+        *org = add3(*org, smul3(5.f * OR_EPS, n));
+        *dir = add3(*dir, smul3(2 * cos_ne, n));
+        if (sin2 >= 1.f) {
+            if (w_tir < 0.5f) *att = muls3(*att, 2.f);
+            else return 2;
+        }
+        return 0;
+    }
+    *org = sub3(*org, smul3(refr_off, n));
+    const float ns = nr * cos_ne - sqrtf(1.f - sin2);
+    *dir = add3(smul3(nr, *dir), smul3(ns, n));
+    return 1;
+}
+
+/* PowerCosineHemisphereSampler (HemisphereSamplers.hpp:78-94): a direction of
+ * density (e + 1)/(2 pi) cos^e about the unit axis upv, in the frame of
+ * getLocalCoordinates. */
+static f3 power_cosine_dir(float w0, float w1, float se, f3 upv)
+{
+    f3 rd;
+    rd.z = or_powf(w1, 1.f / (se + 1.f));
+    float s_, c_;
+    or_sincosf(2.f * OR_PI * w0, &s_, &c_);
+    rd.y = s_; rd.x = c_;
+    rd.x *= sqrtf(1 - rd.z * rd.z);
+    rd.y *= sqrtf(1 - rd.z * rd.z);
+    f3 rt, fw;
+    local_coords(upv, &rt, &fw);
+    return add3(add3(muls3(rt, rd.x), muls3(fw, rd.y)), muls3(upv, rd.z));
+}
+
+/* The megakernel's glossy bounce (PathTracer.h:284-305): a power-cosine
+ * direction about the mirror direction of dir in n (cos_ne = -n.dir >= 0);
+ * the path ends when it points below the surface, else the throughput takes
+ * cos(dir', n) * ks * pi / ((e + 1) / 2 * albedo_s).  Returns 0 when ended. */
+static int glossy_bounce(f3* dir, f3* att, f3 n, float cos_ne, f3 ks, float se, float albedo_s,
+                         float w0, float w1)
+{
+    const f3 upv = add3(*dir, smul3(2.f * cos_ne, n));
+    *dir = power_cosine_dir(w0, w1, se, upv);
+    const float cnl = dot3(*dir, n);
+    if (cnl < 0.f) return 0;
+    /* PathTracer.h:303-304 */
+    *att = divs3(muls3(mul3(muls3(*att, cnl), ks), OR_PI), ((se + 1.f) * 0.5f * albedo_s));
+    return 1;
+}
+
+int or_pt_glass(float org[3], float dir[3], float att[3], const float n[3], float cos_ne, float ior,
+                float ray_t, const float kd[3], float w_fres, float w_tir, int mode1)
+{
+    f3 o = ld3(org), d = ld3(dir), a = ld3(att);
+    const int r = glass_vertex(&o, &d, &a, ld3(n), cos_ne, ior, ray_t, ld3(kd), w_fres, w_tir,
+                               mode1 ? 10.f * OR_EPS : 5.f * OR_EPS, !mode1);
+    org[0] = o.x; org[1] = o.y; org[2] = o.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    att[0] = a.x; att[1] = a.y; att[2] = a.z;
+    return r;
+}
+
+void or_power_cosine_dir(float w0, float w1, float se, const float upv[3], float out[3])
+{
+    const f3 d = power_cosine_dir(w0, w1, se, ld3(upv));
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+void or_cosine_dir(float w0, float w1, const float n[3], float out[3])
+{
+    const f3 rd = cosine_sample(w0, w1);
+    f3 tg, bn;
+    local_coords(ld3(n), &tg, &bn);
+    const f3 d = add3(add3(muls3(tg, rd.x), muls3(bn, rd.y)), muls3(ld3(n), rd.z));
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+int or_pt_glossy(float dir[3], float att[3], const float n[3], float cos_ne, const float ks[3], float se,
+                 float w0, float w1)
+{
+    f3 d = ld3(dir), a = ld3(att);
+    const float albedo_s = maxc3(ld3(ks)) * 2.f * OR_PI / (se + 2);
+    const int r = glossy_bounce(&d, &a, ld3(n), cos_ne, ld3(ks), se, albedo_s, w0, w1);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    att[0] = a.x; att[1] = a.y; att[2] = a.z;
+    return r;
+}
+
 typedef struct {
     const or_material* mats; const uint32_t* mat_idx;
     const or_light* lights; const float* cdf; int nlights;
@@ -610,31 +734,8 @@ static void pt_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
                 if (ior > 0.f) {
                     /* PathTracer.h:133-183 (transparent) */
                     last_spec = 1;
-                    float n1 = 1.000293f, n2 = ior;
-                    if (cos_ne < 0.f) {
-                        n1 = ior; n2 = 1.000293f;
-                        n = neg3(n);
-                        cos_ne = -cos_ne;
-                        att.x *= or_expf(-ray_t * kd.x * OR_PI);
-                        att.y *= or_expf(-ray_t * kd.y * OR_PI);
-                        att.z *= or_expf(-ray_t * kd.z * OR_PI);
-                    }
-                    const float f0 = (n1 - n2) * (n1 - n2) / ((n1 + n2) * (n1 + n2));
-                    const float fres = f0 + (1.f - f0) * or_powf(1.f - fabsf(cos_ne), 5.f);
-                    const float nr = n1 / n2;
-                    const float sin2 = nr * nr * (1.f - cos_ne * cos_ne);
-                    if (sin2 >= 1.f || w[2] < fres) {
-                        org = add3(org, smul3(5.f * OR_EPS, n));
-                        dir = add3(dir, smul3(2 * cos_ne, n));
-                        if (sin2 >= 1.f) {
-                            if (w[3] < 0.5f) att = muls3(att, 2.f);
-                            else break;
-                        }
-                    } else {
-                        org = sub3(org, smul3(5.f * OR_EPS, n));
-                        const float ns = nr * cos_ne - sqrtf(1.f - sin2);
-                        dir = add3(smul3(nr, dir), smul3(ns, n));
-                    }
+                    if (glass_vertex(&org, &dir, &att, n, cos_ne, ior, ray_t, kd, w[2], w[3], 5.f * OR_EPS, 1) == 2)
+                        break;
                 } else {
                     /* PathTracer.h:184-313 (opaque) */
                     if (cos_ne < 0.f) { n = neg3(n); cos_ne = -cos_ne; }
@@ -677,24 +778,8 @@ static void pt_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
                         local_coords(n, &tg, &bn);
                         dir = add3(add3(muls3(tg, rd.x), muls3(bn, rd.y)), muls3(n, rd.z));
                     } else if (rnum < albedo_d + albedo_s) {
-                        /* PowerCosineHemisphereSampler (HemisphereSamplers.hpp:78-94) */
-                        f3 rd;
-                        rd.z = or_powf(w[1], 1.f / (se + 1.f));
-                        float s_, c_;
-                        or_sincosf(2.f * OR_PI * w[0], &s_, &c_);
-                        rd.y = s_; rd.x = c_;
-                        rd.x *= sqrtf(1 - rd.z * rd.z);
-                        rd.y *= sqrtf(1 - rd.z * rd.z);
-                        const f3 upv = add3(dir, smul3(2.f * cos_ne, n));
-                        f3 rt, fw;
-                        local_coords(upv, &rt, &fw);
-                        dir = add3(add3(muls3(rt, rd.x), muls3(fw, rd.y)), muls3(upv, rd.z));
                         last_spec = 1;
-                        const float cnl = dot3(dir, n);
-                        if (cnl < 0.f) break;
-                        /* PathTracer.h:303-304 */
-                        att = divs3(muls3(mul3(muls3(att, cnl), ks), OR_PI),
-                                    ((se + 1.f) * 0.5f * albedo_s));
+                        if (!glossy_bounce(&dir, &att, n, cos_ne, ks, se, albedo_s, w[0], w[1])) break;
                     } else {
                         break;
                     }
@@ -974,37 +1059,11 @@ static void pm_path_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
             const float ior = mt->ior_w - 10.f;
             if (ior > 0.f) {
                 /* transparent (PathTracer.h:556-622) */
-                const f3 kd = ld3(mt->kd);
-                float n1 = 1.000293f, n2 = ior;
-                if (dnd < 0.f) {
-                    n1 = ior; n2 = 1.000293f;
-                    nrm = neg3(nrm);
-                    dnd = -dnd;
-                    att.x *= or_expf(-t * kd.x * OR_PI);
-                    att.y *= or_expf(-t * kd.y * OR_PI);
-                    att.z *= or_expf(-t * kd.z * OR_PI);
-                }
-                const float f0 = (n1 - n2) * (n1 - n2) / ((n1 + n2) * (n1 + n2));
-                const float fres = f0 + (1.f - f0) * or_powf(1.f - fabsf(dnd), 5.f);
-                const float ni = n1 / n2;
-                const float sin2 = ni * ni * (1 - dnd * dnd);
-                if (sin2 > 1.f || ra[1] < fres) {
-                    org = add3(org, smul3(5.f * OR_EPS, nrm));
-                    dir = add3(dir, smul3(2 * dnd, nrm));
-                    if (sin2 >= 1.f) {
-                        if (ra[2] < 0.5f) {
-                            att = muls3(att, 2.f);
-                        } else {
-                            att = rep3(0.f);
-                            e_hp = add3(org, smul3(t - OR_EPS, dir));
-                            e_dir = dir; e_t = t; e_prim = prim;
-                            break;
-                        }
-                    }
-                } else {
-                    const float nf = ni * dnd - sqrtf(1.f - sin2);
-                    org = sub3(org, smul3(10.f * OR_EPS, nrm));
-                    dir = add3(smul3(ni, dir), smul3(nf, nrm));
+                if (glass_vertex(&org, &dir, &att, nrm, dnd, ior, t, ld3(mt->kd), ra[1], ra[2], 10.f * OR_EPS, 0) == 2) {
+                    att = rep3(0.f);
+                    e_hp = add3(org, smul3(t - OR_EPS, dir));
+                    e_dir = dir; e_t = t; e_prim = prim;
+                    break;
                 }
             } else if (diffuse) {
                 /* PathTracer.h:623-641 */
@@ -1019,17 +1078,7 @@ static void pm_path_tile(void* a, ctx_t* cx, uint32_t b, uint32_t e)
                  * (HemisphereSamplers.hpp:78-94) */
                 const float se = mt->spec_exp;
                 if (dnd < 0.f) { nrm = neg3(nrm); dnd = -dnd; }
-                f3 rd;
-                rd.z = or_powf(rb[1], 1.f / (se + 1.f));
-                float s_, c_;
-                or_sincosf(2.f * OR_PI * rb[0], &s_, &c_);
-                rd.y = s_; rd.x = c_;
-                rd.x *= sqrtf(1 - rd.z * rd.z);
-                rd.y *= sqrtf(1 - rd.z * rd.z);
-                const f3 up = add3(dir, smul3(2.f * dnd, nrm));
-                f3 rt, fw;
-                local_coords(up, &rt, &fw);
-                dir = add3(add3(muls3(rt, rd.x), muls3(fw, rd.y)), muls3(up, rd.z));
+                dir = power_cosine_dir(rb[0], rb[1], se, add3(dir, smul3(2.f * dnd, nrm)));
                 org = add3(org, smul3(5.f * OR_EPS, nrm));
                 if (dot3(dir, nrm) < OR_EPS) att = muls3(att, 0.f);
                 else att = muls3(att, dot3(dir, nrm));
@@ -1145,8 +1194,6 @@ int or_render_pt_multi(const or_mesh* m, const or_grid* g, const or_camera* cam,
     float* imp = imp_out ? imp_out : (float*)malloc(n * 3 * sizeof(float));
     float* ls = (float*)malloc(n * 3 * sizeof(float));
     float* dls = (float*)malloc(n * 3 * sizeof(float));
-    float* blended = (float*)malloc(n * 3 * sizeof(float));
-    float* contrib = (float*)malloc(n * 3 * sizeof(float));
     float* nrad = state;
     float* residue = state + 3 * n;
     float *hx = h, *hy = h + n, *hz = h + 2 * n, *hdx = h + 3 * n, *hdy = h + 4 * n, *hdz = h + 5 * n,
@@ -1199,8 +1246,27 @@ int or_render_pt_multi(const or_mesh* m, const or_grid* g, const or_camera* cam,
         const f3 nr = add3(muls3(ld3(nrad + 3 * (size_t)pix), old_w), muls3(mul3(rad, att), new_w));
         nrad[3 * (size_t)pix] = nr.x; nrad[3 * (size_t)pix + 1] = nr.y; nrad[3 * (size_t)pix + 2] = nr.z;
     }
-    /* addIndirectIllumination (PathTracer.h:889-990): all blended values from
-     * the previous image first, then each pixel's blur */
+    or_pm_merge(imp, nrad, residue, fb, image_id, cam->res_x, cam->res_y);
+    free(h); free(hp); free(ls); free(dls);
+    if (!end_out) free(end);
+    if (!end_prims_out) free(endp);
+    if (!imp_out) free(imp);
+    return 0;
+}
+
+/* addIndirectIllumination (PathTracer.h:889-990): every pixel's new
+ * contribution imp * nrad + residue blended into the running mean of fb; a
+ * pixel whose contribution is bright ((r + g + b)^2 at or above
+ * max(0.25 / image, 0.05)) takes min(blur + contribution / 2, contribution)
+ * and keeps the rest as residue.  All blended values are formed first, then
+ * each pixel's 3x3 blur of them (neighbours clamped to the image). */
+void or_pm_merge(const float* imp, const float* nrad, float* residue, float* fb, int image_id, int rx, int ry)
+{
+    const uint32_t npix = (uint32_t)rx * (uint32_t)ry;
+    const float new_w = 1.f / (float)(image_id + 1);
+    const float old_w = 1.f - new_w;
+    float* blended = (float*)malloc((size_t)(npix ? npix : 1) * 3 * sizeof(float));
+    float* contrib = (float*)malloc((size_t)(npix ? npix : 1) * 3 * sizeof(float));
     for (uint32_t pix = 0; pix < npix; ++pix) {
         const f3 nc = add3(mul3(ld3(imp + 3 * (size_t)pix), ld3(nrad + 3 * (size_t)pix)),
                            ld3(residue + 3 * (size_t)pix));
@@ -1208,7 +1274,6 @@ int or_render_pt_multi(const or_mesh* m, const or_grid* g, const or_camera* cam,
         contrib[3 * (size_t)pix] = nc.x; contrib[3 * (size_t)pix + 1] = nc.y; contrib[3 * (size_t)pix + 2] = nc.z;
         blended[3 * (size_t)pix] = bl.x; blended[3 * (size_t)pix + 1] = bl.y; blended[3 * (size_t)pix + 2] = bl.z;
     }
-    const int rx = cam->res_x, ry = cam->res_y;
     for (int y = 0; y < ry; ++y) {
         for (int x = 0; x < rx; ++x) {
             const size_t pix = (size_t)y * rx + x;
@@ -1245,9 +1310,5 @@ int or_render_pt_multi(const or_mesh* m, const or_grid* g, const or_camera* cam,
             residue[3 * pix] = res.x; residue[3 * pix + 1] = res.y; residue[3 * pix + 2] = res.z;
         }
     }
-    free(h); free(hp); free(ls); free(dls); free(blended); free(contrib);
-    if (!end_out) free(end);
-    if (!end_prims_out) free(endp);
-    if (!imp_out) free(imp);
-    return 0;
+    free(blended); free(contrib);
 }

@@ -119,6 +119,14 @@ def lib():
         L.or_render_pt.argtypes = [C.POINTER(Mesh), C.POINTER(Grid), C.POINTER(Camera),
                                    _f32p, _u32p, _f32p, _f32p, C.c_int,
                                    C.POINTER(RenderParams), C.c_int, _f32p, C.POINTER(Stats)]
+        L.or_pt_glass.restype = C.c_int
+        L.or_pt_glass.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float, _f32p,
+                                  C.c_float, C.c_float, C.c_int]
+        L.or_power_cosine_dir.argtypes = [C.c_float, C.c_float, C.c_float, _f32p, _f32p]
+        L.or_cosine_dir.argtypes = [C.c_float, C.c_float, _f32p, _f32p]
+        L.or_pt_glossy.restype = C.c_int
+        L.or_pt_glossy.argtypes = [_f32p, _f32p, _f32p, C.c_float, _f32p, C.c_float, C.c_float, C.c_float]
+        L.or_pm_merge.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_int]
         L.or_build_hier.argtypes = [C.POINTER(Mesh), _f32p, _f32p, _f32p, _f32p, _f32p, _u32p, C.c_uint32,
                                     C.POINTER(Instance), C.c_uint32, C.c_float, C.c_float, C.POINTER(Hier)]
         L.or_hier_free.argtypes = [C.POINTER(Hier)]
@@ -425,6 +433,43 @@ def path_trace_rays(grid, org, dirs, p, materials, mat_idx, lights, weights=None
                              _fp(L), None if w is None else _fp(w), len(L), C.byref(p), _fp(rgb),
                              C.byref(st))
     return rgb, st
+
+
+def pt_glass(org, dirs, att, n, cos_ne, ior, ray_t, kd, w_fres, w_tir, mode1=False):
+    """One glass vertex (or_pt_glass): returns (status, org, dir, att);
+    status 0 reflected, 1 refracted, 2 ended."""
+    o, d, a = _f3(org).copy(), _f3(dirs).copy(), _f3(att).copy()
+    r = lib().or_pt_glass(_fp(o), _fp(d), _fp(a), _fp(_f3(n)), cos_ne, ior, ray_t, _fp(_f3(kd)),
+                          w_fres, w_tir, int(bool(mode1)))
+    return r, o, d, a
+
+
+def power_cosine_dir(w0, w1, se, upv):
+    out = np.zeros(3, np.float32)
+    lib().or_power_cosine_dir(w0, w1, se, _fp(_f3(upv)), _fp(out))
+    return out
+
+
+def cosine_dir(w0, w1, n):
+    out = np.zeros(3, np.float32)
+    lib().or_cosine_dir(w0, w1, _fp(_f3(n)), _fp(out))
+    return out
+
+
+def pt_glossy(dirs, att, n, cos_ne, ks, se, w0, w1):
+    """The megakernel's glossy bounce: returns (alive, dir, att)."""
+    d, a = _f3(dirs).copy(), _f3(att).copy()
+    r = lib().or_pt_glossy(_fp(d), _fp(a), _fp(_f3(n)), cos_ne, _fp(_f3(ks)), se, w0, w1)
+    return r, d, a
+
+
+def pm_merge(imp, nrad, residue, fb, image_id, rx, ry):
+    """Mode 1's merge step on (npix, 3) buffers: returns (fb, residue)."""
+    fb = np.ascontiguousarray(fb, np.float32).copy()
+    residue = np.ascontiguousarray(residue, np.float32).copy()
+    lib().or_pm_merge(_fp(np.ascontiguousarray(imp, np.float32)), _fp(np.ascontiguousarray(nrad, np.float32)),
+                      _fp(residue), _fp(fb), image_id, rx, ry)
+    return fb, residue
 
 
 def primary_dirs(cam, p, image_id=0):

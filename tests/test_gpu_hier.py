@@ -17,8 +17,10 @@ def inst_scene():
     return scenes.instanced_scene(seed=6, n_inst=96, n_soup=300, subdiv=3, res=(96, 72))
 
 
-def _gpu(sc, ins, lt=1.2, ll=2.2, v2=None, coeff=1.0):
+def _gpu(sc, ins, lt=1.2, ll=2.2, v2=None, coeff=1.0, bounds_blocks=0):
     e = Engine(seed=SEED)
+    if bounds_blocks:
+        e.set_tuning(hier_bounds_blocks=bounds_blocks)
     if v2 is None:
         e.upload_scene(sc)
     else:
@@ -61,11 +63,13 @@ def _assert_build_equal(e, h):
     np.testing.assert_array_equal(prims, h.prims())
 
 
-@pytest.mark.parametrize("lt,ll", [(1.2, 2.2), (8.0, 12.0)])
-def test_hier_build_parity(inst_scene, lt, ll):
-    print(f"hierarchy build parity lt={lt} ll={ll}", flush=True)
+# bounds_blocks 1 and 3 give each thread of the object-bounds pass a run of
+# about 10 and 4 triangles (2,348 triangles), so runs cross object boundaries
+@pytest.mark.parametrize("lt,ll,bounds_blocks", [(1.2, 2.2, 0), (8.0, 12.0, 0), (1.2, 2.2, 1), (8.0, 12.0, 3)])
+def test_hier_build_parity(inst_scene, lt, ll, bounds_blocks):
+    print(f"hierarchy build parity lt={lt} ll={ll} bounds_blocks={bounds_blocks}", flush=True)
     sc, ins = inst_scene
-    e = _gpu(sc, ins, lt, ll)
+    e = _gpu(sc, ins, lt, ll, bounds_blocks=bounds_blocks)
     _assert_build_equal(e, _oracle(sc, ins, lt, ll))
     e.close()
 

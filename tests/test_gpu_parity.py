@@ -141,6 +141,44 @@ def test_path_tracing_parity(cluster, kind, lt, ll):
     e.close()
 
 
+def mixed_material_cluster(res=128):
+    """The cluster with its spheres in turn diffuse, glossy (Phong lobe),
+    clear glass and absorbing glass (IOR 1.5), so that paths take every
+    branch of the path tracers (PathTracer.h:133-313)."""
+    sc = scenes.icosphere_cluster(res=res)
+    mats = np.concatenate([sc.materials, np.stack([
+        scenes.material((0.05, 0.05, 0.05), ks=(0.25, 0.25, 0.2), spec_exp=30.0),
+        scenes.material((0.0, 0.0, 0.0), ior=1.5),
+        scenes.material((0.08, 0.02, 0.01), ior=1.5)])]).astype(np.float32)
+    nm = len(sc.materials)
+    mi = sc.mat_idx.copy()
+    per = 5120
+    for s in range(16):
+        mi[s * per:(s + 1) * per] = (mi[s * per], nm, nm + 1, nm + 2)[s % 4]
+    sc.materials, sc.mat_idx = mats, mi
+    return sc
+
+
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)])
+def test_path_tracing_parity_glass_and_glossy(kind, lt, ll):
+    sc = mixed_material_cluster()
+    print(f"PT parity glass/glossy kind={kind}", flush=True)
+    e = gpu_engine(sc, kind, lt, ll)
+    e.set_camera_dict(sc.camera)
+    e.set_lights(sc.lights)
+    og = oracle_grid(sc, kind, lt, ll)
+    cam = O.camera(**{k: sc.camera[k] for k in ("pos", "dir", "up", "fov")},
+                   res_x=sc.camera["res"][0], res_y=sc.camera["res"][1])
+    for image in (0, 1):
+        p = O.params(seed=SEED, jitter=1, pt_depth=8, nthreads=8)
+        fb_g = e.render_frame(image, MODE_PATH, jitter=1, pt_depth=8)
+        fb_o, _ = O.render_pt(og, cam, p, sc.materials, sc.mat_idx, sc.lights, image_id=image,
+                              fb=None if image == 0 else fb_o)
+        assert fb_o.max() > 0
+        assert_image_match(fb_g, fb_o)
+    e.close()
+
+
 def test_keyframe_interpolation_parity():
     print("keyframe parity", flush=True)
     sc, keys = scenes.heightfield_keyframes(quads=64, frames=5, res=(64, 48))
