@@ -166,6 +166,7 @@ SIGNATURES = {
     "hr_occluded_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_float, _vp]),
     "hr_path_trace": (C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_int, _f32p]),
     "hr_path_trace_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, _vp]),
+    "hr_path_trace_dev_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, _vp]),
     "hr_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "hr_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -145,7 +146,55 @@ struct hr_ctx {
     hr::FrameSel* h_frame = nullptr;         // pinned: source of the frame selection
     hr::GridParams* h_params_pin = nullptr;  // pinned: grid parameters after a replay
     bool graph_timed = false;
+
+    // this device's copy of the allocation-free entry point's scratch array
+    unsigned char* noalloc = nullptr;
 };
+
+// ---------------------------------------------------------------------------
+// Scratch of hr_path_trace_dev_noalloc: a fixed array in the code object's
+// device memory, so that the entry point needs no allocation.  Each device
+// holds its own copy for the life of the process; its address comes from
+// hipGetSymbolAddress once per device (the array's name is not a device
+// address in host code).  Layout:
+//   [0, 4)                    persistent kernel's ray-fetch counter
+//   [64, 112)                 default Phong material (no uploaded materials)
+//   [256, 256 + 4 M)          closest-hit distances
+//   [.. + 4 M)                primitive ids
+//   [.. + 28 M)               7-plane hit records (plane stride = the call's n)
+// with M = kNoAllocMaxRays.  Every call on a device shares its copy.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint32_t kNoAllocMaxRays = 1u << 21;
+constexpr size_t kNoAllocHead = 256;
+constexpr size_t kNoAllocBytes = kNoAllocHead + 36 * (size_t)kNoAllocMaxRays;
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kNoAllocDevices] = {};
+
+// the scratch address on `device`, which must be current; looked up (and the
+// default material written) on the device's first call
+hr_status noalloc_scratch(int device, unsigned char** out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kNoAllocDevices) return fail(HR_ERR_ARG, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(HR_ERR_DEVICE, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+        const float m[12] = { 0.2f, 0.2f, 0.2f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f };
+        const hipError_t e2 = hipMemcpy((unsigned char*)p + 64, m, sizeof m, hipMemcpyHostToDevice);
+        if (e2 != hipSuccess) return fail(HR_ERR_DEVICE, hipGetErrorString(e2));
+        g_noalloc_ptr[device] = (unsigned char*)p;
+    }
+    *out = g_noalloc_ptr[device];
+    return HR_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -805,14 +854,17 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     return b;
 }
 
-hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims, uint32_t n, int depth, float* rgb,
-                             hipStream_t s)
+namespace {
+// paths from a hit buffer; default_mat: where the default material lies when
+// the context has none (nullptr: allocate it in the context)
+hr_status path_trace_hits(hr_ctx* c, const float* hits, const uint32_t* prims, uint32_t n, int depth, float* rgb,
+                          hipStream_t s, const float* default_mat)
 {
     if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
     HR_TRY(no_hierarchy(c, "path tracing"));
     if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
     if (n == 0) return HR_OK;
-    HR_TRY(ensure_default_material(c));
+    if (!default_mat || c->mats.p) HR_TRY(ensure_default_material(c));
     if (c->has_normals && c->normals_dirty) {
         hipStream_t keep = c->stream;
         c->stream = s;
@@ -831,7 +883,7 @@ hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims
     ra.stats = nullptr;
     ra.frame = nullptr;
     hr::Shading sh;
-    sh.mats = c->mats.p;
+    sh.mats = c->mats.p ? c->mats.p : default_mat;
     sh.mat_idx = c->has_mat_idx ? c->mat_idx.p : nullptr;
     sh.lights = c->lights.p;
     sh.light_cdf = c->light_cdf.p;
@@ -842,6 +894,13 @@ hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims
     hr::launch_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, c->ray_count.p, s);
     HR_HIP(hipGetLastError());
     return HR_OK;
+}
+}  // namespace
+
+hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims, uint32_t n, int depth, float* rgb,
+                             hipStream_t s)
+{
+    return path_trace_hits(c, hits, prims, n, depth, rgb, s, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -885,6 +944,7 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         e = hipHostMalloc((void**)&c->h_params_pin, sizeof(hr::GridParams), hipHostMallocDefault);
         if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
         if ((s = c->frame_sel.reserve(1)) != HR_OK) break;
+        if ((s = noalloc_scratch(c->device, &c->noalloc)) != HR_OK) break;
         hipEvent_t* evs[6] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1, &c->ev_p1, &c->ev_p2 };
         for (hipEvent_t* ev : evs) {
             e = hipEventCreate(ev);
@@ -1475,6 +1535,30 @@ hr_status hr_path_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, u
     hr::persist_batch(trace_scene(c), d_org, d_dir, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr, c->pool.p,
                       (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p);
     return hr::ctx_path_trace(c, c->ray_hits.p, c->ray_prim.p, n, depth, d_rgb_out, c->stream);
+}
+
+hr_status hr_path_trace_dev_noalloc(hr_ctx* c, const float* d_org, const float* d_dir, uint32_t n, int depth,
+                                    float* d_rgb_out)
+{
+    HR_TRY(check_ctx(c));
+    HR_TRY(no_hierarchy(c, "path tracing"));
+    if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
+    if (!d_rgb_out) return fail(HR_ERR_ARG, "null output buffer");
+    if (depth < 1 || depth > 64) return fail(HR_ERR_ARG, "depth must be in [1, 64]");
+    if (n > kNoAllocMaxRays)
+        return fail(HR_ERR_ARG, "hr_path_trace_dev_noalloc takes at most 2097152 rays per call");
+    if (n == 0) return HR_OK;
+    if (!d_org || !d_dir) return fail(HR_ERR_ARG, "null ray buffer");
+    if (!c->noalloc) return fail(HR_ERR_STATE, "no scratch address for this device");
+    unsigned char* base = c->noalloc;
+    uint32_t* pool = (uint32_t*)base;                       // cleared by persist_batch's fill kernel
+    const float* default_mat = (const float*)(base + 64);
+    float* t = (float*)(base + kNoAllocHead);
+    uint32_t* prim = (uint32_t*)(t + kNoAllocMaxRays);
+    float* hits = (float*)(prim + kNoAllocMaxRays);         // 7 planes of stride n
+    hr::persist_batch(trace_scene(c), d_org, d_dir, n, 0, 0.f, t, prim, nullptr, pool, (uint32_t)c->tuning.refill,
+                      c->ray_count.p, c->work.p, c->stream, hits);
+    return path_trace_hits(c, hits, prim, n, depth, d_rgb_out, c->stream, default_mat);
 }
 
 hr_status hr_get_timing(hr_ctx* c, hr_timing* out)

@@ -334,6 +334,12 @@ class Engine:
     def path_trace_dev(self, org, dirs, n, depth, rgb_out):
         check(self._L.hr_path_trace_dev(self._h, org, dirs, n, int(depth), rgb_out), "hr_path_trace_dev")
 
+    def path_trace_dev_noalloc(self, org, dirs, n, depth, rgb_out):
+        """path_trace_dev without any device allocation (at most 2,097,152 rays;
+        the scratch is shared by every context on the device)."""
+        check(self._L.hr_path_trace_dev_noalloc(self._h, org, dirs, n, int(depth), rgb_out),
+              "hr_path_trace_dev_noalloc")
+
     def trace_dev(self, org, dirs, n, t_out, prim_out):
         """Closest hit of n rays whose SoA buffers are device pointers (ints)."""
         check(self._L.hr_trace_dev(self._h, org, dirs, n, t_out, prim_out), "hr_trace_dev")

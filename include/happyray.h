@@ -270,6 +270,18 @@ hr_status hr_path_trace(hr_ctx* ctx, const float* org, const float* dir, uint32_
 /* the same with device buffers; returns before the work finishes (hr_synchronize) */
 hr_status hr_path_trace_dev(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n, int depth,
                             float* d_rgb_out);
+/* No counterpart in the reference.  hr_path_trace_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and results, but the hit
+ * records and the ray-fetch counter live in a fixed scratch array in the
+ * library's device code (about 72 MiB per device, held until the process
+ * ends), so the call allocates nothing.  n is at most 2,097,152 rays
+ * (HR_ERR_ARG above).  Every context on a device shares that device's array:
+ * calls that overlap on one device (other contexts' streams, other threads)
+ * race and must be ordered by the caller.  The context must have been built;
+ * returns before the work finishes (hr_synchronize). */
+hr_status hr_path_trace_dev_noalloc(hr_ctx* ctx, const float* d_org, const float* d_dir, uint32_t n, int depth,
+                                    float* d_rgb_out);
 
 /* Kernel choices of a context (no reference counterpart: the reference fixes
  * its launch shapes at compile time, src/RT/Algorithm/RayTracingKernels.h:34-43). */
