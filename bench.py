@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
+    ap.add_argument("--gather-calibration", action="store_true",
+                    help="also run a 1 GiB gather of 128-byte rows in random order (counter calibration)")
     ap.add_argument("--scenes", type=int, default=256, help="batched workload: number of scenes")
     ap.add_argument("--requests", type=int, default=2000, help="served workload: number of requests")
     ap.add_argument("--workload", default="soup_ao",
@@ -71,6 +73,39 @@ def copy_calibration(nbytes=256 << 20):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
+
+
+def gather_calibration(rows=1 << 23, row_floats=32):
+    """One gather of known bytes: `rows` rows of 128 bytes (one cache line
+    each, 1 GiB in all, beyond the 256 MB last-level cache) read in a random
+    permutation by torch's index_select, so that a profiler's read counters
+    can be scaled for scattered line-sized reads as the copy scales them for
+    streaming reads.  Reads rows * 128 bytes of rows plus 8 bytes of index
+    per row; writes rows * 128 bytes."""
+    import torch
+    src = torch.ones((rows, row_floats), dtype=torch.float32, device="cuda")
+    idx = torch.randperm(rows, device="cuda")
+    for _ in range(2):
+        out = torch.index_select(src, 0, idx)
+    torch.cuda.synchronize()
+    del src, idx, out
+
+
+def host_cpu():
+    """Core count and model of the host the baseline runs on."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return n, model
 
 
 def roofline_ao(ws, npix_hit, ao_rays, kernel_ms):
@@ -117,11 +152,19 @@ def cpu_baseline(scene, cam, ao_dist, threads):
     t0 = time.perf_counter()
     _, _, st = O.render_ao(g, c, p, scene.materials, scene.mat_idx)
     render_s = time.perf_counter() - t0
+    # the same frame on one thread
+    p1 = O.params(seed=(0x1234, 0x5678), jitter=1, ao_samples=8, ao_dist=ao_dist, nthreads=1)
+    t0 = time.perf_counter()
+    _, _, st1 = O.render_ao(g, c, p1, scene.materials, scene.mat_idx)
+    one_s = time.perf_counter() - t0
+    nproc, model = host_cpu()
     return {"value": round(st.rays / render_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
             "sample": f"one full AO frame of the same workload ({st.rays} rays, "
-                      f"{render_s:.2f} s on {threads} threads); two-level build of all "
+                      f"{render_s:.2f} s on {threads} threads, {one_s:.2f} s on 1 thread); two-level build of all "
                       f"{scene.nt} triangles on 1 thread: {build_s * 1e3:.1f} ms",
+            "value_1thread": round(st1.rays / one_s / 1e6, 3),
+            "nproc": nproc, "cpu_model": model,
             "build_ms_1thread": round(build_s * 1e3, 2)}
 
 
@@ -651,6 +694,8 @@ def main():
     e.close()
     if args.copy_calibration:
         copy_calibration()
+    if args.gather_calibration:
+        gather_calibration()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cam, ao_dist, args.cpu_threads)
     print(json.dumps(out), flush=True)

@@ -137,7 +137,7 @@ struct hr_ctx {
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
-    hr_tuning tuning{ 0, 1, 48, 12, 0 };
+    hr_tuning tuning{ 0, 1, 48, 12, 0, 0, 32, 16, 1 };
 
     // captured frames: a rebuild graph and a render graph, replayed per frame
     hipGraphExec_t g_build = nullptr, g_render = nullptr;
@@ -695,7 +695,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     const uint32_t refill = (uint32_t)c->tuning.refill;
     if (c->tuning.persistent_primary)
         hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
-                            c->stream);
+                            c->stream, (uint32_t)c->tuning.fuse_units);
     else
         hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
     if (events) HR_HIP(hipEventRecord(c->ev_p1, c->stream));
@@ -703,7 +703,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     if (mode == HR_MODE_AO) {
         if (pers)
             hr::persist_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->pool.p, refill, c->ray_count.p,
-                           c->stream);
+                           c->stream, (uint32_t)c->tuning.fuse_units);
         else
             hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
@@ -714,7 +714,12 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         c->last_ao_cam = c->cam;
         c->ao_dump_ok = !fs;
     } else if (mode == HR_MODE_PATH) {
-        hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
+        if (c->tuning.persistent_path)
+            hr::persist_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->pool.p,
+                           (uint32_t)c->tuning.path_refill, (uint32_t)c->tuning.path_shade, c->ray_count.p,
+                           c->stream);
+        else
+            hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
     } else if (mode == HR_MODE_SIMPLE) {
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
@@ -858,7 +863,7 @@ namespace {
 // paths from a hit buffer; default_mat: where the default material lies when
 // the context has none (nullptr: allocate it in the context)
 hr_status path_trace_hits(hr_ctx* c, const float* hits, const uint32_t* prims, uint32_t n, int depth, float* rgb,
-                          hipStream_t s, const float* default_mat)
+                          hipStream_t s, const float* default_mat, uint32_t* pool = nullptr)
 {
     if (!c->built) return fail(HR_ERR_STATE, "path trace before build");
     HR_TRY(no_hierarchy(c, "path tracing"));
@@ -891,7 +896,11 @@ hr_status path_trace_hits(hr_ctx* c, const float* hits, const uint32_t* prims, u
     sh.normals = c->has_normals ? c->normals.p : nullptr;
     sh.nidx = c->has_normals ? c->nidx.p : nullptr;
     hr::fill_u32((uint32_t*)rgb, 0u, 3 * (size_t)n, nullptr, 1, s);
-    hr::launch_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, c->ray_count.p, s);
+    if (c->tuning.persistent_path)
+        hr::persist_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, pool ? pool : c->pool.p,
+                       (uint32_t)c->tuning.path_refill, (uint32_t)c->tuning.path_shade, c->ray_count.p, s);
+    else
+        hr::launch_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, c->ray_count.p, s);
     HR_HIP(hipGetLastError());
     return HR_OK;
 }
@@ -1367,7 +1376,7 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
                                   c->ray_count.p, c->stream);
         else if (pers)
             hr::persist_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->pool.p, refill,
-                              c->ray_count.p, c->work.p, c->stream);
+                              c->ray_count.p, c->work.p, c->stream, nullptr, (uint32_t)c->tuning.fuse_units);
         else
             hr::launch_trace_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->ray_count.p,
                                    c->work.p, c->stream);
@@ -1391,7 +1400,8 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
                               inst_out ? c->ray_inst.p : nullptr, c->ray_occ.p, c->ray_count.p, c->stream);
     else if (pers)
         hr::persist_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
-                          c->ray_occ.p, c->pool.p, refill, c->ray_count.p, c->work.p, c->stream);
+                          c->ray_occ.p, c->pool.p, refill, c->ray_count.p, c->work.p, c->stream, nullptr,
+                          (uint32_t)c->tuning.fuse_units);
     else
         hr::launch_trace_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
                                c->ray_occ.p, c->ray_count.p, c->work.p, c->stream);
@@ -1449,7 +1459,8 @@ hr_status hr_render_rays(hr_ctx* c, const float* rays, uint32_t n, float* rgb_ou
     HR_TRY(c->ray_rgb.reserve(3 * (size_t)n));
     // SimpleRayTraverser: trace, then simpleShade at image 0
     hr::persist_batch(trace_scene(c), c->ray_org.p, c->ray_dir.p, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr,
-                      c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p);
+                      c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p,
+                      (uint32_t)c->tuning.fuse_units);
     hr::fill_u32((uint32_t*)c->ray_rgb.p, 0u, 3 * (size_t)n, nullptr, 1, c->stream);
     hr::RenderArgs ra{};
     ra.seed_lo = c->seed_lo;
@@ -1512,7 +1523,8 @@ hr_status hr_path_trace(hr_ctx* c, const float* org, const float* dir, uint32_t 
     HR_TRY(upload(c->ray_org.p, org, 3 * (size_t)n * sizeof(float), c->stream));
     HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
     hr::persist_batch(trace_scene(c), c->ray_org.p, c->ray_dir.p, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr,
-                      c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p);
+                      c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p,
+                      (uint32_t)c->tuning.fuse_units);
     HR_TRY(hr::ctx_path_trace(c, c->ray_hits.p, c->ray_prim.p, n, depth, c->ray_rgb.p, c->stream));
     HR_HIP(hipMemcpyAsync(rgb_out, c->ray_rgb.p, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HR_HIP(hipStreamSynchronize(c->stream));
@@ -1533,7 +1545,8 @@ hr_status hr_path_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, u
     HR_TRY(c->ray_prim.reserve(n));
     HR_TRY(c->ray_hits.reserve(7 * (size_t)n));
     hr::persist_batch(trace_scene(c), d_org, d_dir, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr, c->pool.p,
-                      (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p);
+                      (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p,
+                      (uint32_t)c->tuning.fuse_units);
     return hr::ctx_path_trace(c, c->ray_hits.p, c->ray_prim.p, n, depth, d_rgb_out, c->stream);
 }
 
@@ -1557,8 +1570,8 @@ hr_status hr_path_trace_dev_noalloc(hr_ctx* c, const float* d_org, const float* 
     uint32_t* prim = (uint32_t*)(t + kNoAllocMaxRays);
     float* hits = (float*)(prim + kNoAllocMaxRays);         // 7 planes of stride n
     hr::persist_batch(trace_scene(c), d_org, d_dir, n, 0, 0.f, t, prim, nullptr, pool, (uint32_t)c->tuning.refill,
-                      c->ray_count.p, c->work.p, c->stream, hits);
-    return path_trace_hits(c, hits, prim, n, depth, d_rgb_out, c->stream, default_mat);
+                      c->ray_count.p, c->work.p, c->stream, hits, (uint32_t)c->tuning.fuse_units);
+    return path_trace_hits(c, hits, prim, n, depth, d_rgb_out, c->stream, default_mat, pool);
 }
 
 hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
@@ -1612,6 +1625,10 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
     if (t->hier_refill < 1 || t->hier_refill > 64) return fail(HR_ERR_ARG, "hier_refill must be in [1, 64]");
     if (t->hier_bounds_blocks < 0 || t->hier_bounds_blocks > (int)hr::kObjBoundsBlocks)
         return fail(HR_ERR_ARG, "hier_bounds_blocks must be in [0, 2048]");
+    if (t->persistent_path != 0 && t->persistent_path != 1) return fail(HR_ERR_ARG, "persistent_path must be 0 or 1");
+    if (t->path_refill < 1 || t->path_refill > 64) return fail(HR_ERR_ARG, "path_refill must be in [1, 64]");
+    if (t->path_shade < 1 || t->path_shade > 64) return fail(HR_ERR_ARG, "path_shade must be in [1, 64]");
+    if (t->fuse_units != 0 && t->fuse_units != 1) return fail(HR_ERR_ARG, "fuse_units must be 0 or 1");
     c->tuning = *t;
     return HR_OK;
 }

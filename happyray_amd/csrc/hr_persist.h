@@ -6,18 +6,29 @@
 namespace hr {
 
 // `pool` is one device word (reset by the launch); `refill` is the number of
-// idle lanes of a wave that triggers fetching new rays (1..64).
+// idle lanes of a wave that triggers fetching new rays (1..64).  fuse 1: a
+// lane that steps to a new cell or leaf also tests its first candidate in the
+// same loop iteration; 0: one unit of work per lane and iteration.  Results
+// are the same either way.
 void persist_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
                      uint32_t* hit_prims, uint32_t* pool, uint32_t refill, unsigned long long* ray_count,
-                     hipStream_t s);
+                     hipStream_t s, uint32_t fuse = 0);
 void persist_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                 const uint32_t* hit_prims, uint8_t* occ, uint32_t* pool, uint32_t refill,
-                unsigned long long* ray_count, hipStream_t s);
+                unsigned long long* ray_count, hipStream_t s, uint32_t fuse = 0);
 // hits != nullptr (closest hit only): also the 7-plane hit buffer of the rays
 void persist_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
                    float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
                    unsigned long long* ray_count, unsigned long long* stats, hipStream_t s,
-                   float* hits = nullptr);
+                   float* hits = nullptr, uint32_t fuse = 0);
+
+// The path-tracing megakernel's paths (one per pixel of the hit buffer) in a
+// persistent launch: `refill` idle lanes fetch new pixels, `shade_min` lanes
+// waiting to shade trigger the wave's shading step (1..64).  Results equal
+// launch_pt's.
+void persist_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                const uint32_t* hit_prims, const Shading& sh, float* fb, uint32_t* pool, uint32_t refill,
+                uint32_t shade_min, unsigned long long* ray_count, hipStream_t s);
 
 // A scene of a batch as the traversal sees it: its grid (cells from the
 // scene's first cell; leaves, references and triangle records shared by the

@@ -101,8 +101,10 @@ __device__ __forceinline__ void uniform_load(const GridRegs& G, const TraceScene
     if (r.it < r.end) r.pn = S.prims[r.it];
 }
 
-// one ray/triangle test; true when an any-hit ray is finished
-template <bool kAny>
+// one ray/triangle test; true when an any-hit ray is finished.  kMixed: the
+// ray's kind is its own -- any hit when tterm > 0 (it stops once t < tterm),
+// closest hit when tterm = 0 -- for kernels that trace both kinds
+template <bool kAny, bool kMixed = false>
 __device__ __forceinline__ bool test_one(const TraceScene& S, RayState& r, Work& wk)
 {
     ++wk.tests;
@@ -124,17 +126,17 @@ __device__ __forceinline__ bool test_one(const TraceScene& S, RayState& r, Work&
     if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps && dist < r.t) {
         r.t = dist;
         r.best = p;
-        return kAny;
+        return kMixed ? r.t < r.tterm : kAny;
     }
     return false;
 }
 
 // the step after a cell's candidates are exhausted; false when the ray ends
-template <bool kAny>
+template <bool kAny, bool kMixed = false>
 __device__ __forceinline__ bool advance_uniform(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
 {
     const int d = min_dim(r.tx, r.ty, r.tz);
-    if (kAny && r.t < r.tterm) return false;
+    if ((kAny || kMixed) && r.t < r.tterm) return false;
     bool flag;
     if (d == 0) {
         flag = r.t > r.tx;
@@ -157,40 +159,44 @@ __device__ __forceinline__ bool advance_uniform(const GridRegs& G, const TraceSc
     return true;
 }
 
-template <bool kAny>
-__device__ __forceinline__ bool advance_two_level(const GridRegs& G, const TraceScene& S, RayState& r,
-                                                  Work& wk)
+// the leaf part of a two-level advance (TLGridTraverser.h:209-286): 0 = the
+// ray has ended, 1 = it moved to the next leaf of its top cell (loaded),
+// 2 = it left the top cell's leaves (the top-level step comes next)
+template <bool kAny, bool kMixed = false>
+__device__ __forceinline__ int leaf_step(const TraceScene& S, RayState& r, Work& wk)
 {
-    if (r.in_leaf) {
-        // end of a leaf's candidates (TLGridTraverser.h:209-286)
-        if (kAny && r.t < r.tterm) return false;
-        const int d = min_dim(r.ux, r.uy, r.uz);
-        const int r0 = (int)(r.cell & 0xFFu), r1 = (int)((r.cell >> 8) & 0xFFu),
-                  r2 = (int)((r.cell >> 16) & 0xFFu);
-        bool second;
-        if (d == 0) {
-            if (r.t < r.ux) return false;
-            r.lx += r.rcp.x > 0.f ? 1 : -1;
-            r.ux += r.ddx;
-            second = r.lx != r0 && r.lx != -1;
-        } else if (d == 1) {
-            if (r.t < r.uy) return false;
-            r.ly += r.rcp.y > 0.f ? 1 : -1;
-            r.uy += r.ddy;
-            second = r.ly != r1 && r.ly != -1;
-        } else {
-            if (r.t < r.uz) return false;
-            r.lz += r.rcp.z > 0.f ? 1 : -1;
-            r.uz += r.ddz;
-            second = r.lz != r2 && r.lz != -1;
-        }
-        if (second) {
-            leaf_load(S, r, wk);
-            return true;
-        }
-        r.in_leaf = false;
+    if ((kAny || kMixed) && r.t < r.tterm) return 0;
+    const int d = min_dim(r.ux, r.uy, r.uz);
+    const int r0 = (int)(r.cell & 0xFFu), r1 = (int)((r.cell >> 8) & 0xFFu), r2 = (int)((r.cell >> 16) & 0xFFu);
+    bool second;
+    if (d == 0) {
+        if (r.t < r.ux) return 0;
+        r.lx += r.rcp.x > 0.f ? 1 : -1;
+        r.ux += r.ddx;
+        second = r.lx != r0 && r.lx != -1;
+    } else if (d == 1) {
+        if (r.t < r.uy) return 0;
+        r.ly += r.rcp.y > 0.f ? 1 : -1;
+        r.uy += r.ddy;
+        second = r.ly != r1 && r.ly != -1;
+    } else {
+        if (r.t < r.uz) return 0;
+        r.lz += r.rcp.z > 0.f ? 1 : -1;
+        r.uz += r.ddz;
+        second = r.lz != r2 && r.lz != -1;
     }
-    // next top cell (TLGridTraverser.h:292-347)
+    if (second) {
+        leaf_load(S, r, wk);
+        return 1;
+    }
+    r.in_leaf = false;
+    return 2;
+}
+
+// the top-level step (TLGridTraverser.h:292-347) and the visit of the next
+// top cell; false when the ray leaves the grid or its hit lies before the cell
+__device__ __forceinline__ bool top_step(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+{
     const int d = min_dim(r.tx, r.ty, r.tz);
     bool flag;
     if (d == 0) {
@@ -217,14 +223,25 @@ __device__ __forceinline__ bool advance_two_level(const GridRegs& G, const Trace
     return true;
 }
 
+template <bool kAny, bool kMixed = false>
+__device__ __forceinline__ bool advance_two_level(const GridRegs& G, const TraceScene& S, RayState& r,
+                                                  Work& wk)
+{
+    if (r.in_leaf) {
+        const int k = leaf_step<kAny, kMixed>(S, r, wk);
+        if (k != 2) return k == 1;
+    }
+    return top_step(G, S, r, wk);
+}
+
 // set up the DDA of a generated ray; false when it does not enter the grid
-template <bool kTwo, bool kAny>
+template <bool kTwo, bool kAny, bool kMixed = false>
 __device__ __forceinline__ bool ray_begin(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
 {
     r.best = S.nt;
     r.it = r.end = 0u;
     r.in_leaf = false;
-    bool flag = !(kAny && r.t < r.tterm);
+    bool flag = !((kAny || kMixed) && r.t < r.tterm);
     flag = dda_setup<!kTwo>(G, r.org, r.rcp, r.tentry, r.cx, r.cy, r.cz, r.tx, r.ty, r.tz) && flag;
     if (!flag) return false;
     if (kTwo) top_visit(G, S, r, wk);
@@ -365,8 +382,13 @@ struct GenBatch {
 // ---------------------------------------------------------------------------
 
 // at most 64 VGPRs, so eight waves per SIMD hide the gathers' latency
-// (AO 3.19 -> 3.12 ms against the compiler's 70 VGPRs and seven waves)
-template <bool kTwo, bool kAny, class Gen>
+// (AO 3.19 -> 3.12 ms against the compiler's 70 VGPRs and seven waves).
+// kFuse: a lane whose cell or leaf has no candidates left advances and, when
+// the new one has candidates, tests the first of them in the same iteration
+// (one or two units per iteration instead of one: AO 3.08 -> 3.01 ms on the
+// 1M-triangle frame).  Every ray runs the same units in the same order
+// either way.
+template <bool kTwo, bool kAny, class Gen, bool kFuse = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_persist(
     TraceScene S, Gen gen, uint32_t nrays, uint32_t* __restrict__ pool, uint32_t refill,
     unsigned long long* ray_count, unsigned long long* stats)
@@ -401,9 +423,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
             continue;
         }
         if (live) {
-            bool done;
-            if (r.it < r.end) done = test_one<kAny>(S, r, wk);
-            else done = kTwo ? !advance_two_level<kAny>(G, S, r, wk) : !advance_uniform<kAny>(G, S, r, wk);
+            bool done = false;
+            if (kFuse) {
+                if (r.it >= r.end)
+                    done = kTwo ? !advance_two_level<kAny>(G, S, r, wk) : !advance_uniform<kAny>(G, S, r, wk);
+                if (!done && r.it < r.end) done = test_one<kAny>(S, r, wk);
+            } else {
+                if (r.it < r.end) done = test_one<kAny>(S, r, wk);
+                else done = kTwo ? !advance_two_level<kAny>(G, S, r, wk) : !advance_uniform<kAny>(G, S, r, wk);
+            }
             if (done) {
                 live = false;
                 gen.finish(j, r, true, S);
@@ -513,6 +541,208 @@ __global__ __launch_bounds__(kBlock) void k_persist_multi(const SceneDesc* __res
     add_work(stats, started, wk);
 }
 
+// ---------------------------------------------------------------------------
+// persistent path tracer
+// ---------------------------------------------------------------------------
+//
+// The megakernel's paths (k_pt, PathTracer.h:41-372) with the traversal of
+// k_persist: each lane holds one path and runs, per loop iteration, one unit
+// of its current ray (a triangle test or a DDA advance).  A lane whose ray has
+// finished waits in the shading state; the wave runs the shading code (path
+// vertex, light sample, bounce, the next ray's set-up) once at least
+// `shade_min` of its lanes wait or none is tracing, and fetches new pixels
+// once `refill` of its lanes are idle.  Per path the operations and their
+// order are k_pt's, except that the bounce of an opaque vertex is computed
+// before its shadow ray is traced: the bounce reads nothing the shadow ray
+// changes, and the light sample is added as k_pt adds it (diffuse, then
+// specular part: after the vertex the last bounce is never specular), so
+// the radiance is bit-identical.
+
+constexpr uint32_t kPathIdle = 0, kPathShade = 1, kPathTrace = 2;
+
+// the path's state between shading steps; the path's position and direction
+// live in the lane's ray (org, dir) while it shades
+struct PathLane {
+    f3 att, rad;
+    f3 pdir;             // extension direction waiting for the shadow ray
+    f3 add_d, add_s;     // light sample added when the shadow ray is unoccluded
+    uint32_t pix;
+    int depth;
+    bool last_spec;
+    bool after_shadow;   // the finished ray was the vertex's shadow ray
+    bool end_after;      // the path ends after that shadow ray
+};
+
+__device__ __forceinline__ void pt_store(const RenderArgs& ra, float* fb, uint32_t pix, f3 rad)
+{
+    const float nw = 1.f / (float)(image_of(ra) + 1);
+    const float ow = 1.f - nw;
+    float* o = fb + 3 * (size_t)pix;
+    o[0] = o[0] * ow + rad.x * nw;
+    o[1] = o[1] * ow + rad.y * nw;
+    o[2] = o[2] * ow + rad.z * nw;
+}
+
+// a ray of the path: shadow rays stop at the first hit closer than 0.9999,
+// extension rays find the closest hit (tterm 0)
+template <bool kTwo>
+__device__ __forceinline__ bool pt_begin_ray(const GridRegs& G, const TraceScene& S, RayState& r, f3 org, f3 d,
+                                             bool shadow, Work& wk)
+{
+    set_ray(r, org, d);
+    r.t = shadow ? 0.9999f : kFltMax;
+    r.tterm = shadow ? 0.9999f : 0.f;
+    return ray_begin<kTwo, false, true>(G, S, r, wk);
+}
+
+// a finished extension ray: the next vertex, or the end of the path
+__device__ __forceinline__ bool pt_extension_done(RayState& r, PathLane& L, const TraceScene& S)
+{
+    if (!(r.t < kFltMax)) return false;
+    r.org = r.org + (r.t - kEps) * r.dir;
+    ++L.depth;
+    return true;
+}
+
+template <bool kTwo>
+__global__ __launch_bounds__(kBlock) void k_persist_pt(TraceScene S, RenderArgs ra, uint32_t npix,
+                                                       const float* __restrict__ hits,
+                                                       const uint32_t* __restrict__ hit_prims, Shading sh,
+                                                       float* __restrict__ fb, uint32_t* __restrict__ pool,
+                                                       uint32_t refill, uint32_t shade_min,
+                                                       unsigned long long* ray_count)
+{
+    const GridRegs G = load_grid(S.gp);
+    const uint64_t lt = __lanemask_lt();
+    RayState r;
+    PathLane L;
+    Work wk;
+    uint32_t state = kPathIdle, traced = 0;
+    bool exhausted = false;
+    while (true) {
+        // new pixels for idle lanes
+        const uint64_t idle = __ballot(state == kPathIdle);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill || idle == ~0ull)) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(pool, nidle);
+            base = __shfl(base, 0, kWave);
+            if (base + nidle >= npix) exhausted = true;
+            if (state == kPathIdle) {
+                const uint32_t pix = base + (uint32_t)__popcll(idle & lt);
+                if (pix < npix) {
+                    const float th = hits[6 * (size_t)npix + pix];
+                    const uint32_t prim = hit_prims[pix];
+                    L.pix = pix;
+                    if (th < kFltMax && prim < S.nt) {
+                        r.org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+                        r.dir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix],
+                                   hits[5 * (size_t)npix + pix]);
+                        r.t = th;
+                        r.best = prim;
+                        L.att = rep(1.f);
+                        L.rad = rep(0.f);
+                        L.depth = 0;
+                        L.last_spec = true;
+                        L.after_shadow = false;
+                        state = kPathShade;
+                    } else {
+                        pt_store(ra, fb, pix, rep(0.f));
+                    }
+                }
+            }
+        }
+        const uint64_t shading = __ballot(state == kPathShade);
+        const uint64_t tracing = __ballot(state == kPathTrace);
+        if (!shading && !tracing) {
+            if (exhausted) break;
+            continue;
+        }
+        if (shading && ((uint32_t)__popcll(shading) >= shade_min || !tracing) && state == kPathShade) {
+            bool done = false;      // the path has ended
+            bool shadow = false;    // the next ray is a shadow ray
+            f3 ndir;
+            if (L.after_shadow) {
+                // PathTracer.h:244-258 with the shadow ray's outcome
+                L.after_shadow = false;
+                if (!(r.t < 0.9999f)) {
+                    L.rad = L.rad + L.add_d;
+                    L.rad = L.rad + L.add_s;
+                }
+                done = L.end_after;
+                ndir = L.pdir;
+            } else {
+                PathState P;
+                P.org = r.org;
+                P.dir = r.dir;
+                P.att = L.att;
+                P.radiance = L.rad;
+                P.ray_t = r.t;
+                P.prim = r.best;
+                P.depth = L.depth;
+                P.last_spec = L.last_spec;
+                PathVertex V;
+                pt_vertex(S, sh, ra, L.pix, P, V);
+                const bool last = P.depth + 1 >= ra.pt_depth;
+                if (V.opaque && !last) pt_bounce(P, V);
+                const bool end = V.end || last;
+                L.att = P.att;
+                L.rad = P.radiance;
+                L.last_spec = P.last_spec;
+                r.org = P.org;
+                ndir = P.dir;
+                if (V.opaque && V.shadow) {
+                    L.add_d = V.add_d;
+                    L.add_s = V.add_s;
+                    L.pdir = P.dir;
+                    L.end_after = end;
+                    L.after_shadow = true;
+                    shadow = true;
+                    ndir = V.sdir;
+                } else {
+                    done = end;
+                }
+            }
+            if (done) {
+                pt_store(ra, fb, L.pix, L.rad);
+                state = kPathIdle;
+            } else {
+                ++traced;
+                if (pt_begin_ray<kTwo>(G, S, r, r.org, ndir, shadow, wk)) {
+                    state = kPathTrace;
+                } else if (!shadow) {
+                    // the extension ray misses the grid: the path ends
+                    pt_store(ra, fb, L.pix, L.rad);
+                    state = kPathIdle;
+                }
+                // a shadow ray outside the grid is unoccluded: shade again
+            }
+        }
+        if (state == kPathTrace) {
+            bool fin;
+            if (r.it < r.end) fin = test_one<false, true>(S, r, wk);
+            else fin = kTwo ? !advance_two_level<false, true>(G, S, r, wk) : !advance_uniform<false, true>(G, S, r, wk);
+            if (fin) {
+                if (L.after_shadow) {
+                    state = kPathShade;
+                } else if (pt_extension_done(r, L, S)) {
+                    state = kPathShade;
+                } else {
+                    pt_store(ra, fb, L.pix, L.rad);
+                    state = kPathIdle;
+                }
+            }
+        }
+    }
+    if (ray_count) {
+        unsigned long long c = traced;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if (__lane_id() == 0 && c) atomicAdd(ray_count, c);
+    }
+    add_work(ra.stats, traced, wk);
+}
+
 template <class K>
 uint32_t persistent_blocks(K kernel)
 {
@@ -530,51 +760,72 @@ uint32_t persistent_blocks(K kernel)
 
 template <bool kTwo, bool kAny, class Gen>
 void launch(const TraceScene& S, const Gen& gen, uint32_t nrays, uint32_t* pool, uint32_t refill,
-            unsigned long long* ray_count, unsigned long long* stats, hipStream_t s)
+            unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, uint32_t fuse)
 {
     if (nrays == 0) return;
     fill_u32(pool, 0u, 1, nullptr, 1, s);
-    auto kern = k_persist<kTwo, kAny, Gen>;
-    uint32_t blocks = persistent_blocks(kern);
     const uint32_t need = (nrays + kBlock - 1) / kBlock;
+    auto kern = fuse ? k_persist<kTwo, kAny, Gen, true> : k_persist<kTwo, kAny, Gen, false>;
+    uint32_t blocks = fuse ? persistent_blocks(k_persist<kTwo, kAny, Gen, true>)
+                           : persistent_blocks(k_persist<kTwo, kAny, Gen, false>);
     if (blocks > need) blocks = need;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, S, gen, nrays, pool, refill, ray_count,
-                       stats);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, S, gen, nrays, pool, refill, ray_count, stats);
 }
 
 }  // namespace
 
 void persist_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
                      uint32_t* hit_prims, uint32_t* pool, uint32_t refill, unsigned long long* ray_count,
-                     hipStream_t s)
+                     hipStream_t s, uint32_t fuse)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
     GenPrimary g{ cam, ra, npix, hits, hit_prims };
-    if (S.two_level) launch<true, false>(S, g, npix, pool, refill, ray_count, ra.stats, s);
-    else launch<false, false>(S, g, npix, pool, refill, ray_count, ra.stats, s);
+    if (S.two_level) launch<true, false>(S, g, npix, pool, refill, ray_count, ra.stats, s, fuse);
+    else launch<false, false>(S, g, npix, pool, refill, ray_count, ra.stats, s, fuse);
 }
 
 void persist_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                 const uint32_t* hit_prims, uint8_t* occ, uint32_t* pool, uint32_t refill,
-                unsigned long long* ray_count, hipStream_t s)
+                unsigned long long* ray_count, hipStream_t s, uint32_t fuse)
 {
     GenAO g{ ra, npix, hits, hit_prims, occ };
     const uint32_t n = npix * (uint32_t)ra.ao_samples;
-    if (S.two_level) launch<true, true>(S, g, n, pool, refill, ray_count, ra.stats, s);
-    else launch<false, true>(S, g, n, pool, refill, ray_count, ra.stats, s);
+    if (S.two_level) launch<true, true>(S, g, n, pool, refill, ray_count, ra.stats, s, fuse);
+    else launch<false, true>(S, g, n, pool, refill, ray_count, ra.stats, s, fuse);
 }
 
 void persist_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
                    float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
-                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, float* hits)
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, float* hits,
+                   uint32_t fuse)
 {
     GenBatch g{ org, dir, n, tmax, anyhit, t_out, prim_out, occ_out, anyhit ? nullptr : hits };
     if (S.two_level) {
-        if (anyhit) launch<true, true>(S, g, n, pool, refill, ray_count, stats, s);
-        else launch<true, false>(S, g, n, pool, refill, ray_count, stats, s);
+        if (anyhit) launch<true, true>(S, g, n, pool, refill, ray_count, stats, s, fuse);
+        else launch<true, false>(S, g, n, pool, refill, ray_count, stats, s, fuse);
     } else {
-        if (anyhit) launch<false, true>(S, g, n, pool, refill, ray_count, stats, s);
-        else launch<false, false>(S, g, n, pool, refill, ray_count, stats, s);
+        if (anyhit) launch<false, true>(S, g, n, pool, refill, ray_count, stats, s, fuse);
+        else launch<false, false>(S, g, n, pool, refill, ray_count, stats, s, fuse);
+    }
+}
+
+void persist_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
+                const uint32_t* hit_prims, const Shading& sh, float* fb, uint32_t* pool, uint32_t refill,
+                uint32_t shade_min, unsigned long long* ray_count, hipStream_t s)
+{
+    if (npix == 0) return;
+    fill_u32(pool, 0u, 1, nullptr, 1, s);
+    const uint32_t need = (npix + kBlock - 1) / kBlock;
+    if (S.two_level) {
+        uint32_t blocks = persistent_blocks(k_persist_pt<true>);
+        if (blocks > need) blocks = need;
+        hipLaunchKernelGGL(k_persist_pt<true>, dim3(blocks), dim3(kBlock), 0, s, S, ra, npix, hits, hit_prims, sh,
+                           fb, pool, refill, shade_min, ray_count);
+    } else {
+        uint32_t blocks = persistent_blocks(k_persist_pt<false>);
+        if (blocks > need) blocks = need;
+        hipLaunchKernelGGL(k_persist_pt<false>, dim3(blocks), dim3(kBlock), 0, s, S, ra, npix, hits, hit_prims, sh,
+                           fb, pool, refill, shade_min, ray_count);
     }
 }
 

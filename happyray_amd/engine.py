@@ -353,9 +353,17 @@ class Engine:
         return t
 
     def set_tuning(self, persistent_primary=0, persistent_secondary=1, refill=48, hier_refill=12,
-                   hier_bounds_blocks=0):
+                   hier_bounds_blocks=0, persistent_path=None, path_refill=None, path_shade=None,
+                   fuse_units=None):
+        """Kernel choices (hr_tuning).  The path and fuse fields keep their
+        current values when not given."""
+        cur = self.tuning()
         t = _lib.Tuning(int(persistent_primary), int(persistent_secondary), int(refill), int(hier_refill),
-                        int(hier_bounds_blocks))
+                        int(hier_bounds_blocks),
+                        int(cur.persistent_path if persistent_path is None else persistent_path),
+                        int(cur.path_refill if path_refill is None else path_refill),
+                        int(cur.path_shade if path_shade is None else path_shade),
+                        int(cur.fuse_units if fuse_units is None else fuse_units))
         check(self._L.hr_set_tuning(self._h, C.byref(t)), "hr_set_tuning")
 
     def tuning(self):

@@ -297,6 +297,17 @@ typedef struct {
                                   0 (default) one triangle per thread on up to 2048
                                   workgroups; 1..2048 caps the count, so that each
                                   thread walks a run of several triangles              */
+    int persistent_path;       /* path tracing (mode 2, paths from given rays): 0 one
+                                  path per lane, 1 persistent paths with dynamic pixel
+                                  fetch and batched shading (same results)             */
+    int path_refill;           /* idle lanes of a wave that trigger fetching new
+                                  pixels in persistent paths, 1..64                    */
+    int path_shade;            /* lanes waiting to shade that trigger a wave's shading
+                                  step in persistent paths, 1..64                      */
+    int fuse_units;            /* persistent traversal: 1 (default) a lane that steps to a
+                                  new cell or leaf also tests its first candidate in the
+                                  same loop iteration; 0 one unit of work (a triangle test
+                                  or a grid step) per lane and iteration (same results) */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

@@ -201,14 +201,15 @@ def test_keyframe_interpolation_parity():
 def test_persistent_matches_one_ray_per_lane(cluster, kind, lt, ll):
     """The persistent kernels (dynamic ray fetch) and the one-ray-per-lane
     kernels apply the same operations to every ray: bit-identical outputs and
-    identical work counters, for several refill thresholds."""
+    identical work counters, for several refill thresholds, with one unit of
+    work per loop iteration or a step and a test fused."""
     print(f"persistent vs one-ray-per-lane kind={kind}", flush=True)
     e = gpu_engine(cluster, kind, lt, ll)
     e.set_camera_dict(cluster.camera)
     org, d = random_rays(cluster, 30000, seed=17)
     out = {}
-    for pers, refill in ((0, 16), (1, 1), (1, 16), (1, 64)):
-        e.set_tuning(pers, pers, refill)
+    for pers, refill, fuse in ((0, 16, 1), (1, 1, 0), (1, 16, 1), (1, 64, 0), (1, 48, 1)):
+        e.set_tuning(pers, pers, refill, fuse_units=fuse)
         e.work_stats()
         img = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0, collect_stats=True)
         ws = e.work_stats()
@@ -216,10 +217,10 @@ def test_persistent_matches_one_ray_per_lane(cluster, kind, lt, ll):
         planes, prims = e.primary_hits()
         t, p = e.trace(org, d)
         o = e.occluded(org, d, 1.5)
-        out[(pers, refill)] = (img, occ, planes, prims, t, p, o,
+        out[(pers, refill, fuse)] = (img, occ, planes, prims, t, p, o,
                                (ws.primary.rays, ws.primary.steps, ws.primary.tests,
                                 ws.secondary.rays, ws.secondary.steps, ws.secondary.tests))
-    ref = out[(0, 16)]
+    ref = out[(0, 16, 1)]
     for key, val in out.items():
         for a, b in zip(val[:7], ref[:7]):
             np.testing.assert_array_equal(a, b, err_msg=str(key))
@@ -266,4 +267,41 @@ def test_device_pointer_entry_points(cluster):
     np.testing.assert_array_equal(occ_d.to_host(), e.occluded(org, d, 2.5))
     for a in (o_d, d_d, t_d, p_d, occ_d):
         a.free()
+    e.close()
+
+
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)])
+def test_persistent_paths_match_one_path_per_lane(kind, lt, ll):
+    """The persistent path tracer (dynamic pixel fetch, batched shading) and
+    the one-path-per-lane megakernel apply the same operations to every path:
+    bit-identical images over diffuse, glossy and glass materials, identical
+    work counters, for several refill / shading thresholds, and the oracle's
+    image; paths from given rays (hr_path_trace) likewise."""
+    sc = mixed_material_cluster(res=96)
+    print(f"persistent paths vs one path per lane kind={kind}", flush=True)
+    e = gpu_engine(sc, kind, lt, ll)
+    e.set_camera_dict(sc.camera)
+    e.set_lights(sc.lights)
+    org, d = random_rays(sc, 20000, seed=41)
+    out = {}
+    for pers, refill, shade in ((0, 32, 16), (1, 1, 1), (1, 32, 16), (1, 64, 64), (1, 48, 8)):
+        e.set_tuning(persistent_path=pers, path_refill=refill, path_shade=shade)
+        e.work_stats()
+        imgs = [e.render_frame(i, MODE_PATH, jitter=1, pt_depth=6, collect_stats=True) for i in (0, 1)]
+        ws = e.work_stats()
+        rgb = e.path_trace(org, d, depth=5)
+        out[(pers, refill, shade)] = (imgs[0], imgs[1], rgb, (ws.secondary.rays, ws.secondary.steps,
+                                                              ws.secondary.tests))
+    ref = out[(0, 32, 16)]
+    for key, val in out.items():
+        for a, b in zip(val[:3], ref[:3]):
+            np.testing.assert_array_equal(a, b, err_msg=str(key))
+        assert val[3] == ref[3], (key, val[3], ref[3])
+    og = oracle_grid(sc, kind, lt, ll)
+    cam = O.camera(**{k: sc.camera[k] for k in ("pos", "dir", "up", "fov")},
+                   res_x=sc.camera["res"][0], res_y=sc.camera["res"][1])
+    fb_o, _ = O.render_pt(og, cam, O.params(seed=SEED, jitter=1, pt_depth=6, nthreads=8), sc.materials,
+                          sc.mat_idx, sc.lights)
+    assert fb_o.max() > 0
+    assert_image_match(ref[0], fb_o)
     e.close()
