@@ -119,19 +119,31 @@ def roofline_ao(ws, npix_hit, ao_rays, kernel_ms):
     return bytes_alg, flops, gbs, tflops
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_soup_in_room_ao_1024.md")
+COUNTER_FILE = os.path.join(ROOT, "profiles", "counters_soup_in_room_ao_1024.md")
 
 
-def committed_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed two-counter
-    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+def committed_counter(key):
+    """A figure of the committed rocprofv3 counter summary of this workload
+    (`key = value` lines), or None."""
     import re
     try:
-        text = open(TRAFFIC_FILE).read()
+        text = open(COUNTER_FILE).read()
     except OSError:
         return None
-    m = re.search(r"^traffic_bytes_per_launch " + re.escape(kernel) + r" = (\d+)$", text, re.M)
-    return int(m.group(1)) if m else None
+    m = re.search(r"^" + re.escape(key) + r" = ([0-9.eE+-]+)$", text, re.M)
+    return float(m.group(1)) if m else None
+
+
+def build_bytes(nt, nv, info):
+    """Algorithmic bytes of one two-level rebuild: each phase reads its inputs
+    and writes its outputs once, sort = one read and one write of keys and
+    values (interpolation 36 V; top level 24 N + 24 V + 40 P_t + 8 C_t; leaf
+    resolution and begin scan 20 C_t; leaf pair count and write
+    2 (8 P_t + 8 C_t + 12 N + 12 V); leaf pairs, sort, ranges 36 P_l + 8 L)."""
+    V, N = nv, nt
+    Ct, Pt, L, Pl = info.num_cells, info.num_top_pairs, info.num_leaves, info.num_refs
+    return 36 * V + (24 * N + 24 * V + 40 * Pt + 8 * Ct) + 20 * Ct + 2 * (8 * Pt + 8 * Ct + 12 * N + 12 * V) \
+        + 36 * Pl + 8 * L
 
 
 def cpu_baseline(scene, cam, ao_dist, threads):
@@ -661,7 +673,13 @@ def main():
     render_ms = np.array(prim_ms) + np.array(ao_ms) + np.array(shade_ms)
     info = e.grid_info()
     tun = e.tuning()
-    ao_kernel = "k_persist<true, true, GenAO>" if tun.persistent_secondary else "k_ao_rays<true>"
+    ao_kernel = (f"k_persist<true, true, GenAO, {'true' if tun.fuse_units else 'false'}>"
+                 if tun.persistent_secondary else "k_ao_rays<true>")
+    rebuild_ms = float(np.mean(build_ms))
+    bb = build_bytes(scene.nt, scene.nv, info)
+    bcnt = committed_counter("build_counter_bytes")
+    bbusy = committed_counter("build_counter_busy_us")
+    tr = committed_counter(f"traffic_bytes_per_launch {ao_kernel}")
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1,
@@ -675,21 +693,34 @@ def main():
                    "triangles": scene.nt, "rays_per_step": int(rays // args.steps),
                    "top_res": list(info.res), "leaves": info.num_leaves,
                    "leaf_refs": info.num_refs, "ao_dist": round(ao_dist, 4)},
-        "rebuild_ms": round(float(np.mean(build_ms)), 4),
+        "rebuild_ms": round(rebuild_ms, 4),
         "render_ms": round(float(np.mean(render_ms)), 4),
         "render_mrays_s": round(rays / args.steps / (float(np.mean(render_ms)) * 1e-3) / 1e6, 3),
         "kernel_ms": {"primary": round(float(np.mean(prim_ms)), 4), "ao_rays": round(ao_avg, 4),
                       "ao_shade": round(float(np.mean(shade_ms)), 4)},
         "roofline": {"kernel": ao_kernel, "bound": "hbm", "achieved": round(gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
-                     "traffic": committed_traffic(ao_kernel),
-                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
+                     "traffic": None if tr is None else int(tr),
+                     "traffic_source": os.path.relpath(COUNTER_FILE, ROOT),
                      "bytes_per_launch": int(bytes_alg),
+                     "bytes": "algorithmic gathers (8 B per DDA step, 40 B per triangle test, 32 B per hit "
+                              "pixel, 1 B per AO ray), served mostly by the L2 and the last-level cache",
+                     "limiter": "VALU issue under divergence: the VALU is busy in most cycles while an "
+                                "instruction computes for about a fifth of a wave's lanes",
+                     "counters": {"valu_busy": committed_counter(f"counter {ao_kernel} valu_busy"),
+                                  "lanes_active": committed_counter(f"counter {ao_kernel} lanes_active"),
+                                  "l2_hit": committed_counter(f"counter {ao_kernel} l2_hit")},
                      "valu": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 5),
                               "flops_per_launch": int(flops)},
                      "work": {"ao_rays": int(ao_rays), "steps": int(ws.secondary.steps),
                               "tests": int(ws.secondary.tests)}},
+        "build_roofline": {"bound": "hbm", "algorithmic_bytes": int(bb),
+                           "achieved": round(bb / (rebuild_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(bb / (rebuild_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "counter_bytes": None if bcnt is None else int(bcnt),
+                           "counter_frac": None if bcnt is None or not bbusy else
+                           round(bcnt / (bbusy * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
     }
     e.close()
     if args.copy_calibration:
