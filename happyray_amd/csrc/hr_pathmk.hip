@@ -89,6 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_path(TraceScene S, RenderArgs 
                                                        const uint32_t* __restrict__ hit_prims, Shading sh,
                                                        PathBuffers pb, unsigned long long* ray_count)
 {
+    __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = pix < npix;   // every lane stays for add_work
     const uint32_t image = (uint32_t)image_of(ra);
@@ -111,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_path(TraceScene S, RenderArgs 
     Mat m;
     uint32_t traced = 0;
     Work wk;
-    const GridRegs G = load_grid(S.gp);
+    GridRegs G = load_grid(S.gp);
+    if (kTwo) sub_table(G, sub);
     if (alive) {
         m = material(sh, prim);
         if (!(m.ior > 0.f)) {
@@ -262,12 +264,15 @@ template <bool kTwo>
 __global__ __launch_bounds__(kBlock) void k_path_shadow(TraceScene S, RenderArgs ra, uint32_t npix, Shading sh,
                                                         PathBuffers pb, unsigned long long* ray_count)
 {
+    __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = pix < npix;   // every lane stays for add_work
     const float t = in ? pb.end[6 * (size_t)npix + pix] : kFltMax;
     f3 ls = rep(0.f), dl = rep(0.f);
     bool traced = false;
     Work wk;
+    GridRegs G = load_grid(S.gp);
+    if (kTwo) sub_table(G, sub);
     if (t < kFltMax) {
         float u[4];
         rand4(ra.seed_lo, ra.seed_hi, pix, (uint32_t)image_of(ra), kStreamPathLight, u);
@@ -292,7 +297,6 @@ __global__ __launch_bounds__(kBlock) void k_path_shadow(TraceScene S, RenderArgs
             const f3 dn = dir * sqrtf(a);
             const float cos_ln = dot(lnrm, -dn);
             ls = lint * len(cross(le1, le2)) * a * fmaxf(0.f, cos_ln) * 0.5f;
-            const GridRegs G = load_grid(S.gp);
             float st = 0.9999f;
             f3 d2;
             trace_ray<kTwo, true>(G, S, org, dir, st, 0.9999f, d2, wk);

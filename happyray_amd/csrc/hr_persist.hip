@@ -382,7 +382,11 @@ struct GenBatch {
 // ---------------------------------------------------------------------------
 
 // at most 64 VGPRs, so eight waves per SIMD hide the gathers' latency
-// (AO 3.19 -> 3.12 ms against the compiler's 70 VGPRs and seven waves).
+// (AO 3.19 -> 3.12 ms against the compiler's 70 VGPRs and seven waves).  The
+// leaf cell sizes of a top-cell visit are divided here, not read from the
+// LDS table of the one-ray-per-lane kernels: under the 64-register cap the
+// table's address spills (8 -> 40..120 bytes of scratch per lane, AO
+// 3.02 -> 3.06..4.08 ms).
 // kFuse: a lane whose cell or leaf has no candidates left advances and, when
 // the new one has candidates, tests the first of them in the same iteration
 // (one or two units per iteration instead of one: AO 3.08 -> 3.01 ms on the
@@ -721,7 +725,8 @@ __global__ __launch_bounds__(kBlock) void k_persist_pt(TraceScene S, RenderArgs 
         if (state == kPathTrace) {
             bool fin;
             if (r.it < r.end) fin = test_one<false, true>(S, r, wk);
-            else fin = kTwo ? !advance_two_level<false, true>(G, S, r, wk) : !advance_uniform<false, true>(G, S, r, wk);
+            else fin = kTwo ? !advance_two_level<false, true>(G, S, r, wk)
+                            : !advance_uniform<false, true>(G, S, r, wk);
             if (fin) {
                 if (L.after_shadow) {
                     state = kPathShade;

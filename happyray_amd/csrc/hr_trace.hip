@@ -46,12 +46,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     TraceScene S, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
     uint32_t* __restrict__ hit_prims, unsigned long long* ray_count)
 {
+    __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = pix < npix;
     count_rays(ray_count, active);
     Work wk;
+    GridRegs G = load_grid(S.gp);
+    if (kTwo) sub_table(G, sub);
     if (active) {
-        const GridRegs G = load_grid(S.gp);
         const f3 d0 = primary_direction(cam, ra, pix, npix);
         const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
         float t = kFltMax;
@@ -93,8 +95,10 @@ __global__ __launch_bounds__(kBlock) void k_ao_rays(TraceScene S, RenderArgs ra,
     count_rays(ray_count, active);
     Work wk;
     if (in && !active) occ[j] = 1u;
+    __shared__ float sub[kSubTable];
+    GridRegs G = load_grid(S.gp);
+    if (kTwo) sub_table(G, sub);
     if (active) {
-    const GridRegs G = load_grid(S.gp);
     // AmbientOcclusionRayGenerator::operator() (RayGenerators.h:281-343)
     f3 org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
     const f3 pdir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix],
@@ -183,9 +187,11 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
                                                const uint32_t* __restrict__ hit_prims, Shading sh,
                                                float* __restrict__ fb, unsigned long long* ray_count)
 {
+    __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = pix < npix;
-    const GridRegs G = load_grid(S.gp);
+    GridRegs G = load_grid(S.gp);
+    if (kTwo) sub_table(G, sub);
     PathState P;
     P.radiance = rep(0.f);
     P.ray_t = in ? hits[6 * (size_t)npix + pix] : kFltMax;
@@ -385,11 +391,13 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(TraceScene S, const floa
                                                         unsigned long long* ray_count,
                                                         unsigned long long* stats)
 {
+    __shared__ float sub[kSubTable];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     count_rays(ray_count, i < n);
     Work wk;
+    GridRegs G = load_grid(S.gp);
+    if (kTwo) sub_table(G, sub);
     if (i < n) {
-        const GridRegs G = load_grid(S.gp);
         const f3 o = mk(org[i], org[(size_t)n + i], org[2 * (size_t)n + i]);
         const f3 d = mk(dir[i], dir[(size_t)n + i], dir[2 * (size_t)n + i]);
         float t = kAny ? tmax : kFltMax;

@@ -13,14 +13,46 @@
 namespace hr {
 namespace trav {
 
+// a pointer into LDS (the workgroup's shared memory)
+typedef __attribute__((address_space(3))) const float* lds_cfloat_ptr;
+
 struct GridRegs {
     f3 bmin, bmax, cs, csr;
     int rx, ry, rz;
+    lds_cfloat_ptr sub;   // nullptr, or the workgroup's table of leaf cell sizes (sub_table)
 };
+
+// Leaf cell size of a top cell per axis: 1 / (cell_size_rcp * leaf
+// resolution), the three divisions of a top-cell visit (TLGridTraverser.h:
+// 160-172).  The resolution is an integer in [1, 255], so a workgroup keeps
+// the values of one grid in a table of 3 x 256 entries in LDS, computed by the
+// same expression, and a visit reads them instead of dividing.
+constexpr int kSubTable = 3 * 256;
+
+// every thread of the workgroup must call this (it synchronizes)
+__device__ __forceinline__ void sub_table(GridRegs& g, float* tab)
+{
+    for (int i = (int)threadIdx.x; i < kSubTable; i += (int)blockDim.x) {
+        const int a = i >> 8;
+        const float c = a == 0 ? g.csr.x : (a == 1 ? g.csr.y : g.csr.z);
+        tab[i] = 1.f / (c * (float)(i & 255));
+    }
+    __syncthreads();
+    g.sub = (lds_cfloat_ptr)tab;
+}
+
+// kTab: from the table (G.sub must be set), else by division
+template <bool kTab = true>
+__device__ __forceinline__ f3 leaf_cell_size(const GridRegs& G, f3 srcp, int r0, int r1, int r2)
+{
+    if (kTab) return mk(G.sub[r0], G.sub[256 + r1], G.sub[512 + r2]);
+    return mk(1.f / srcp.x, 1.f / srcp.y, 1.f / srcp.z);
+}
 
 __device__ __forceinline__ GridRegs load_grid(const GridParams* gp)
 {
     GridRegs g;
+    g.sub = nullptr;
     g.bmin = mk(gp->bmin[0], gp->bmin[1], gp->bmin[2]);
     g.bmax = mk(gp->bmax[0], gp->bmax[1], gp->bmax[2]);
     g.cs = mk(gp->cs[0], gp->cs[1], gp->cs[2]);
@@ -202,7 +234,8 @@ __device__ void trav_two_level(const GridRegs& G, const TraceScene& S, f3 org, f
             cf.x = floorf((ep - mb).x * srcp.x);
             cf.y = floorf((ep - mb).y * srcp.y);
             cf.z = floorf((ep - mb).z * srcp.z);
-            const float scx = 1.f / srcp.x, scy = 1.f / srcp.y, scz = 1.f / srcp.z;
+            const f3 sc = leaf_cell_size(G, srcp, r0, r1, r2);
+            const float scx = sc.x, scy = sc.y, scz = sc.z;
             cf = vmin(cres - rep(1.f), vmax(cf, rep(0.f)));
             float ux = ((cf.x + (rcp.x > 0.f ? 1.f : 0.f)) * scx + mb.x - org.x) * rcp.x;
             float uy = ((cf.y + (rcp.y > 0.f ? 1.f : 0.f)) * scy + mb.y - org.y) * rcp.y;
