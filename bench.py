@@ -673,8 +673,13 @@ def main():
     render_ms = np.array(prim_ms) + np.array(ao_ms) + np.array(shade_ms)
     info = e.grid_info()
     tun = e.tuning()
-    ao_kernel = (f"k_persist<true, true, GenAO, {'true' if tun.fuse_units else 'false'}>"
-                 if tun.persistent_secondary else "k_ao_rays<true>")
+    if not tun.persistent_secondary:
+        ao_kernel = "k_ao_rays<true>"
+    elif tun.test_batch:
+        ao_kernel = "k_persist_bt<true, true, GenAO, 16>"
+    else:
+        ao_kernel = f"k_persist<true, true, GenAO, {'true' if tun.fuse_units else 'false'}>"
+
     rebuild_ms = float(np.mean(build_ms))
     bb = build_bytes(scene.nt, scene.nv, info)
     bcnt = committed_counter("build_counter_bytes")

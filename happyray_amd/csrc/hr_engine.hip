@@ -137,7 +137,7 @@ struct hr_ctx {
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
-    hr_tuning tuning{ 0, 1, 48, 12, 0, 0, 32, 16, 1 };
+    hr_tuning tuning{ 0, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2 };
 
     // captured frames: a rebuild graph and a render graph, replayed per frame
     hipGraphExec_t g_build = nullptr, g_render = nullptr;
@@ -282,6 +282,15 @@ hr_status reserve_build(hr_ctx* c)
         if (c->sort_chain.cap != before) hr::fill_u32(c->sort_chain.p, 0u, c->sort_chain.cap, nullptr, 1, c->stream);
     }
     return HR_OK;
+}
+
+hr::PersistOpts persist_opts(const hr_ctx* c)
+{
+    hr::PersistOpts o;
+    o.fuse = (uint32_t)c->tuning.fuse_units;
+    o.batch = (uint32_t)c->tuning.test_batch;
+    o.steps = (uint32_t)c->tuning.advance_steps;
+    return o;
 }
 
 hr::BuildBuffers build_buffers(hr_ctx* c)
@@ -695,7 +704,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     const uint32_t refill = (uint32_t)c->tuning.refill;
     if (c->tuning.persistent_primary)
         hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
-                            c->stream, (uint32_t)c->tuning.fuse_units);
+                            c->stream, persist_opts(c));
     else
         hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
     if (events) HR_HIP(hipEventRecord(c->ev_p1, c->stream));
@@ -703,7 +712,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     if (mode == HR_MODE_AO) {
         if (pers)
             hr::persist_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->pool.p, refill, c->ray_count.p,
-                           c->stream, (uint32_t)c->tuning.fuse_units);
+                           c->stream, persist_opts(c));
         else
             hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
@@ -1376,7 +1385,7 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
                                   c->ray_count.p, c->stream);
         else if (pers)
             hr::persist_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->pool.p, refill,
-                              c->ray_count.p, c->work.p, c->stream, nullptr, (uint32_t)c->tuning.fuse_units);
+                              c->ray_count.p, c->work.p, c->stream, nullptr, persist_opts(c));
         else
             hr::launch_trace_batch(S, org, dir, n, anyhit, tmax, t_out, prim_out, occ_out, c->ray_count.p,
                                    c->work.p, c->stream);
@@ -1401,7 +1410,7 @@ static hr_status trace_common(hr_ctx* c, const float* org, const float* dir, uin
     else if (pers)
         hr::persist_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
                           c->ray_occ.p, c->pool.p, refill, c->ray_count.p, c->work.p, c->stream, nullptr,
-                          (uint32_t)c->tuning.fuse_units);
+                          persist_opts(c));
     else
         hr::launch_trace_batch(S, c->ray_org.p, c->ray_dir.p, n, anyhit, tmax, c->ray_t.p, c->ray_prim.p,
                                c->ray_occ.p, c->ray_count.p, c->work.p, c->stream);
@@ -1460,7 +1469,7 @@ hr_status hr_render_rays(hr_ctx* c, const float* rays, uint32_t n, float* rgb_ou
     // SimpleRayTraverser: trace, then simpleShade at image 0
     hr::persist_batch(trace_scene(c), c->ray_org.p, c->ray_dir.p, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr,
                       c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p,
-                      (uint32_t)c->tuning.fuse_units);
+                      persist_opts(c));
     hr::fill_u32((uint32_t*)c->ray_rgb.p, 0u, 3 * (size_t)n, nullptr, 1, c->stream);
     hr::RenderArgs ra{};
     ra.seed_lo = c->seed_lo;
@@ -1524,7 +1533,7 @@ hr_status hr_path_trace(hr_ctx* c, const float* org, const float* dir, uint32_t 
     HR_TRY(upload(c->ray_dir.p, dir, 3 * (size_t)n * sizeof(float), c->stream));
     hr::persist_batch(trace_scene(c), c->ray_org.p, c->ray_dir.p, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr,
                       c->pool.p, (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p,
-                      (uint32_t)c->tuning.fuse_units);
+                      persist_opts(c));
     HR_TRY(hr::ctx_path_trace(c, c->ray_hits.p, c->ray_prim.p, n, depth, c->ray_rgb.p, c->stream));
     HR_HIP(hipMemcpyAsync(rgb_out, c->ray_rgb.p, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HR_HIP(hipStreamSynchronize(c->stream));
@@ -1546,7 +1555,7 @@ hr_status hr_path_trace_dev(hr_ctx* c, const float* d_org, const float* d_dir, u
     HR_TRY(c->ray_hits.reserve(7 * (size_t)n));
     hr::persist_batch(trace_scene(c), d_org, d_dir, n, 0, 0.f, c->ray_t.p, c->ray_prim.p, nullptr, c->pool.p,
                       (uint32_t)c->tuning.refill, c->ray_count.p, c->work.p, c->stream, c->ray_hits.p,
-                      (uint32_t)c->tuning.fuse_units);
+                      persist_opts(c));
     return hr::ctx_path_trace(c, c->ray_hits.p, c->ray_prim.p, n, depth, d_rgb_out, c->stream);
 }
 
@@ -1570,7 +1579,7 @@ hr_status hr_path_trace_dev_noalloc(hr_ctx* c, const float* d_org, const float* 
     uint32_t* prim = (uint32_t*)(t + kNoAllocMaxRays);
     float* hits = (float*)(prim + kNoAllocMaxRays);         // 7 planes of stride n
     hr::persist_batch(trace_scene(c), d_org, d_dir, n, 0, 0.f, t, prim, nullptr, pool, (uint32_t)c->tuning.refill,
-                      c->ray_count.p, c->work.p, c->stream, hits, (uint32_t)c->tuning.fuse_units);
+                      c->ray_count.p, c->work.p, c->stream, hits, persist_opts(c));
     return path_trace_hits(c, hits, prim, n, depth, d_rgb_out, c->stream, default_mat, pool);
 }
 
@@ -1629,6 +1638,8 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
     if (t->path_refill < 1 || t->path_refill > 64) return fail(HR_ERR_ARG, "path_refill must be in [1, 64]");
     if (t->path_shade < 1 || t->path_shade > 64) return fail(HR_ERR_ARG, "path_shade must be in [1, 64]");
     if (t->fuse_units != 0 && t->fuse_units != 1) return fail(HR_ERR_ARG, "fuse_units must be 0 or 1");
+    if (t->test_batch != 0 && t->test_batch != 1) return fail(HR_ERR_ARG, "test_batch must be 0 or 1");
+    if (t->advance_steps < 1 || t->advance_steps > 4) return fail(HR_ERR_ARG, "advance_steps must be in [1, 4]");
     c->tuning = *t;
     return HR_OK;
 }

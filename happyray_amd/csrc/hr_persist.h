@@ -5,22 +5,33 @@
 
 namespace hr {
 
+// Kernel choice of a persistent launch; results are the same for every choice.
+//   batch 0: each lane tests its own candidates, one per loop iteration, and
+//     fuse 1 lets a lane that steps to a new cell or leaf test its first
+//     candidate in the same iteration (fuse 0: one unit of work per lane and
+//     iteration);
+//   batch 1: each lane hands up to 16 candidates per iteration to its wave,
+//     which tests them all at full width, and takes up to `steps` grid steps
+//     (1..4) per iteration to reach its next candidates.
+struct PersistOpts {
+    uint32_t fuse = 1;
+    uint32_t batch = 1;
+    uint32_t steps = 2;
+};
+
 // `pool` is one device word (reset by the launch); `refill` is the number of
-// idle lanes of a wave that triggers fetching new rays (1..64).  fuse 1: a
-// lane that steps to a new cell or leaf also tests its first candidate in the
-// same loop iteration; 0: one unit of work per lane and iteration.  Results
-// are the same either way.
+// idle lanes of a wave that triggers fetching new rays (1..64).
 void persist_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
                      uint32_t* hit_prims, uint32_t* pool, uint32_t refill, unsigned long long* ray_count,
-                     hipStream_t s, uint32_t fuse = 0);
+                     hipStream_t s, const PersistOpts& o = PersistOpts{});
 void persist_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                 const uint32_t* hit_prims, uint8_t* occ, uint32_t* pool, uint32_t refill,
-                unsigned long long* ray_count, hipStream_t s, uint32_t fuse = 0);
+                unsigned long long* ray_count, hipStream_t s, const PersistOpts& o = PersistOpts{});
 // hits != nullptr (closest hit only): also the 7-plane hit buffer of the rays
 void persist_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
                    float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
                    unsigned long long* ray_count, unsigned long long* stats, hipStream_t s,
-                   float* hits = nullptr, uint32_t fuse = 0);
+                   float* hits = nullptr, const PersistOpts& o = PersistOpts{});
 
 // The path-tracing megakernel's paths (one per pixel of the hit buffer) in a
 // persistent launch: `refill` idle lanes fetch new pixels, `shade_min` lanes

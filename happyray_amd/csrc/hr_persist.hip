@@ -22,6 +22,7 @@ namespace {
 
 using namespace trav;
 constexpr int kBlock = 256;
+constexpr float kInf = __builtin_huge_valf();
 
 struct RayState {
     f3 org, dir, rcp;
@@ -56,7 +57,7 @@ __device__ __forceinline__ void leaf_load(const TraceScene& S, RayState& r, Work
 
 // visit the current top cell: non-empty cells start their leaf DDA at
 // tentry (TLGridTraverser.h:147-197) and load the first leaf range
-__device__ __forceinline__ void top_visit(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+__device__ __forceinline__ void top_visit(const GridRegs& G, const TraceScene& S, RayState& r, f3 org, Work& wk)
 {
     ++wk.steps;
     const uint2 cell = S.cells[cell_index(G, r.cx, r.cy, r.cz)];
@@ -64,7 +65,7 @@ __device__ __forceinline__ void top_visit(const GridRegs& G, const TraceScene& S
     r.in_leaf = false;
     if (!(cell.y & 0x40000000u)) return;
     const int r0 = (int)(cell.y & 0xFFu), r1 = (int)((cell.y >> 8) & 0xFFu), r2 = (int)((cell.y >> 16) & 0xFFu);
-    const f3 ep = r.org + rep(r.tentry) / r.rcp;
+    const f3 ep = org + rep(r.tentry) / r.rcp;
     const f3 cres = mk((float)r0, (float)r1, (float)r2);
     const f3 srcp = G.csr * cres;
     f3 mb;
@@ -77,9 +78,9 @@ __device__ __forceinline__ void top_visit(const GridRegs& G, const TraceScene& S
     cf.z = floorf((ep - mb).z * srcp.z);
     const float scx = 1.f / srcp.x, scy = 1.f / srcp.y, scz = 1.f / srcp.z;
     cf = vmin(cres - rep(1.f), vmax(cf, rep(0.f)));
-    r.ux = ((cf.x + (r.rcp.x > 0.f ? 1.f : 0.f)) * scx + mb.x - r.org.x) * r.rcp.x;
-    r.uy = ((cf.y + (r.rcp.y > 0.f ? 1.f : 0.f)) * scy + mb.y - r.org.y) * r.rcp.y;
-    r.uz = ((cf.z + (r.rcp.z > 0.f ? 1.f : 0.f)) * scz + mb.z - r.org.z) * r.rcp.z;
+    r.ux = ((cf.x + (r.rcp.x > 0.f ? 1.f : 0.f)) * scx + mb.x - org.x) * r.rcp.x;
+    r.uy = ((cf.y + (r.rcp.y > 0.f ? 1.f : 0.f)) * scy + mb.y - org.y) * r.rcp.y;
+    r.uz = ((cf.z + (r.rcp.z > 0.f ? 1.f : 0.f)) * scz + mb.z - org.z) * r.rcp.z;
     r.lx = (int)cf.x;
     r.ly = (int)cf.y;
     r.lz = (int)cf.z;
@@ -92,6 +93,11 @@ __device__ __forceinline__ void top_visit(const GridRegs& G, const TraceScene& S
     leaf_load(S, r, wk);
 }
 
+__device__ __forceinline__ void top_visit(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+{
+    top_visit(G, S, r, r.org, wk);
+}
+
 __device__ __forceinline__ void uniform_load(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
 {
     ++wk.steps;
@@ -99,6 +105,23 @@ __device__ __forceinline__ void uniform_load(const GridRegs& G, const TraceScene
     r.it = cell.x;
     r.end = cell.y;
     if (r.it < r.end) r.pn = S.prims[r.it];
+}
+
+// MollerTrumboreIntersectionTest (RayTriangleIntersector.h:52-90): the hit
+// distance, or +inf when the triangle is missed (barycentrics outside, or
+// not beyond the intersection epsilon); a NaN distance compares false too
+__device__ __forceinline__ float mt_dist(f3 org, f3 dir, f3 v0, f3 v1, f3 v2)
+{
+    const f3 e1 = v1 - v0;
+    const f3 e2 = v2 - v0;
+    const f3 pv = cross(dir, e2);
+    const float det_rcp = 1.f / dot(e1, pv);
+    f3 tv = org - v0;
+    const float alpha = det_rcp * dot(tv, pv);
+    tv = cross(tv, e1);
+    const float beta = det_rcp * dot(tv, dir);
+    const float dist = det_rcp * dot(e2, tv);
+    return (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps) ? dist : kInf;
 }
 
 // one ray/triangle test; true when an any-hit ray is finished.  kMixed: the
@@ -114,16 +137,8 @@ __device__ __forceinline__ bool test_one(const TraceScene& S, RayState& r, Work&
     if (r.it < r.end) r.pn = S.prims[r.it];
     f3 v0, v1, v2;
     load_tri(S.rec, p, v0, v1, v2);
-    const f3 e1 = v1 - v0;
-    const f3 e2 = v2 - v0;
-    const f3 pv = cross(r.dir, e2);
-    const float det_rcp = 1.f / dot(e1, pv);
-    f3 tv = r.org - v0;
-    const float alpha = det_rcp * dot(tv, pv);
-    tv = cross(tv, e1);
-    const float beta = det_rcp * dot(tv, r.dir);
-    const float dist = det_rcp * dot(e2, tv);
-    if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps && dist < r.t) {
+    const float dist = mt_dist(r.org, r.dir, v0, v1, v2);
+    if (dist < r.t) {
         r.t = dist;
         r.best = p;
         return kMixed ? r.t < r.tterm : kAny;
@@ -195,7 +210,7 @@ __device__ __forceinline__ int leaf_step(const TraceScene& S, RayState& r, Work&
 
 // the top-level step (TLGridTraverser.h:292-347) and the visit of the next
 // top cell; false when the ray leaves the grid or its hit lies before the cell
-__device__ __forceinline__ bool top_step(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+__device__ __forceinline__ bool top_step(const GridRegs& G, const TraceScene& S, RayState& r, f3 org, Work& wk)
 {
     const int d = min_dim(r.tx, r.ty, r.tz);
     bool flag;
@@ -219,8 +234,13 @@ __device__ __forceinline__ bool top_step(const GridRegs& G, const TraceScene& S,
         flag = flag && r.cz != G.rz && r.cz != -1;
     }
     if (!flag) return false;
-    top_visit(G, S, r, wk);
+    top_visit(G, S, r, org, wk);
     return true;
+}
+
+__device__ __forceinline__ bool top_step(const GridRegs& G, const TraceScene& S, RayState& r, Work& wk)
+{
+    return top_step(G, S, r, r.org, wk);
 }
 
 template <bool kAny, bool kMixed = false>
@@ -244,7 +264,7 @@ __device__ __forceinline__ bool ray_begin(const GridRegs& G, const TraceScene& S
     bool flag = !((kAny || kMixed) && r.t < r.tterm);
     flag = dda_setup<!kTwo>(G, r.org, r.rcp, r.tentry, r.cx, r.cy, r.cz, r.tx, r.ty, r.tz) && flag;
     if (!flag) return false;
-    if (kTwo) top_visit(G, S, r, wk);
+    if (kTwo) top_visit(G, S, r, r.org, wk);
     else uniform_load(G, S, r, wk);
     return true;
 }
@@ -447,6 +467,173 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 #pragma unroll
         for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
         if (ray_count && __lane_id() == 0 && c) atomicAdd(ray_count, c);
+    }
+    add_work(stats, started, wk);
+}
+
+// Wave-batched candidate tests.  In k_persist a lane tests its own candidates
+// one per iteration, so the test code runs for the lanes that happen to have
+// one -- about a third of the wave.  Here each lane hands up to kK candidates
+// of its current cell or leaf to the wave per iteration; the wave tests all of
+// them at full width (item k of the wave's list on lane k mod 64), each item
+// reading its ray from LDS, and every hit is folded into its ray's 64-bit key
+// in LDS by an atomic minimum:
+//   closest hit: key = (distance bits, candidate index) -- the smallest
+//     distance, the first candidate in leaf order among equal distances,
+//     which is what the sequential test with `dist < t` keeps;
+//   any hit: key = (candidate index, distance bits) -- the first hit in
+//     leaf order, where the sequential test stops.
+// Candidates are consumed in leaf order, kK at a time, so every ray ends with
+// the t and primitive of the one-ray-per-lane traversal.  The ray origins and
+// directions live in LDS only (the lane keeps the DDA state), which pays for
+// the registers of the batched test.  An any-hit ray may test up to kK - 1
+// candidates past its first hit; the work statistics count the tests of the
+// sequential traversal (up to and including that hit), as the reference's
+// traversal would make them.
+//
+// A lane also takes up to `steps_max` grid steps per iteration when the leaves
+// or cells it steps into are empty, so that more lanes reach candidates by the
+// wave's test batch.  The workgroup uses about 30 KB of LDS; five waves per
+// SIMD leave each lane 96 VGPRs, which the batch needs without spilling.
+constexpr int kBatchK = 16;   // candidates a lane hands to its wave per iteration
+
+template <int kK>
+struct WaveTests {
+    float ox[kWave], oy[kWave], oz[kWave];
+    float dx[kWave], dy[kWave], dz[kWave];
+    float t[kWave];
+    unsigned long long key[kWave];
+    uint32_t cand[kWave * kK];
+    uint8_t own[kWave * kK];
+};
+
+template <bool kTwo, bool kAny, class Gen, int kK>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_persist_bt(
+    TraceScene S, Gen gen, uint32_t nrays, uint32_t* __restrict__ pool, uint32_t refill, uint32_t steps_max,
+    unsigned long long* ray_count, unsigned long long* stats)
+{
+    __shared__ WaveTests<kK> lds[kBlock / kWave];
+    WaveTests<kK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
+    const GridRegs G = load_grid(S.gp);
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = __lanemask_lt();
+    RayState r;
+    Work wk;
+    uint32_t j = 0, started = 0;
+    bool live = false;
+    bool exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill || idle == ~0ull)) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(pool, nidle);
+            base = __shfl(base, 0, kWave);
+            if (base + nidle >= nrays) exhausted = true;
+            if (!live) {
+                j = base + (uint32_t)__popcll(idle & lt);
+                if (j < nrays) {
+                    const bool ok = gen.start(j, r, S);
+                    started += ok ? 1u : 0u;
+                    live = ok && ray_begin<kTwo, kAny>(G, S, r, wk);
+                    if (live) {
+                        W.ox[lane] = r.org.x;
+                        W.oy[lane] = r.org.y;
+                        W.oz[lane] = r.org.z;
+                        W.dx[lane] = r.dir.x;
+                        W.dy[lane] = r.dir.y;
+                        W.dz[lane] = r.dir.z;
+                    } else {
+                        gen.finish(j, r, ok, S);
+                    }
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        bool done = false;
+        if (live && r.it >= r.end) {
+            for (uint32_t m = 0; m < steps_max; ++m) {
+                if (kTwo) {
+                    bool top = true;
+                    if (r.in_leaf) {
+                        const int k = leaf_step<kAny>(S, r, wk);
+                        if (k != 2) {
+                            top = false;
+                            done = k != 1;
+                        }
+                    }
+                    if (top) done = !top_step(G, S, r, mk(W.ox[lane], W.oy[lane], W.oz[lane]), wk);
+                } else {
+                    done = !advance_uniform<kAny>(G, S, r, wk);
+                }
+                if (done || r.it < r.end) break;
+            }
+        }
+        uint32_t cnt = 0;
+        if (live && !done && r.it < r.end) cnt = r.end - r.it < (uint32_t)kK ? r.end - r.it : (uint32_t)kK;
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        if (total) {
+            const uint32_t ex = incl - cnt;
+#pragma unroll
+            for (int q = 0; q < kK; ++q) {
+                if ((uint32_t)q < cnt) {
+                    W.cand[ex + q] = r.it + (uint32_t)q;
+                    W.own[ex + q] = (uint8_t)lane;
+                }
+            }
+            if (cnt) {
+                W.t[lane] = r.t;
+                W.key[lane] = ~0ull;
+            }
+            wave_sync();
+            for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
+                const uint32_t k = b0 + lane;
+                if (k < total) {
+                    const uint32_t o = W.own[k];
+                    const uint32_t ci = W.cand[k];
+                    f3 v0, v1, v2;
+                    load_tri(S.rec, S.prims[ci], v0, v1, v2);
+                    const float dist = mt_dist(mk(W.ox[o], W.oy[o], W.oz[o]), mk(W.dx[o], W.dy[o], W.dz[o]),
+                                               v0, v1, v2);
+                    if (dist < W.t[o]) {
+                        const unsigned long long db = __float_as_uint(dist);
+                        atomicMin(&W.key[o], kAny ? ((unsigned long long)ci << 32 | db) : (db << 32 | ci));
+                    }
+                }
+            }
+            wave_sync();
+            if (cnt) {
+                const unsigned long long key = W.key[lane];
+                uint32_t tested = cnt;
+                if (key != ~0ull) {
+                    const uint32_t ci = kAny ? (uint32_t)(key >> 32) : (uint32_t)key;
+                    r.t = __uint_as_float(kAny ? (uint32_t)key : (uint32_t)(key >> 32));
+                    r.best = S.prims[ci];
+                    if (kAny) {
+                        done = true;
+                        tested = ci - r.it + 1;   // the sequential traversal stops here
+                    }
+                }
+                wk.tests += tested;
+                r.it += cnt;
+            }
+        }
+        if (live && done) {
+            live = false;
+            r.org = mk(W.ox[lane], W.oy[lane], W.oz[lane]);
+            r.dir = mk(W.dx[lane], W.dy[lane], W.dz[lane]);
+            gen.finish(j, r, true, S);
+        }
+    }
+    if (ray_count) {
+        unsigned long long c = started;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if (lane == 0 && c) atomicAdd(ray_count, c);
     }
     add_work(stats, started, wk);
 }
@@ -765,14 +952,22 @@ uint32_t persistent_blocks(K kernel)
 
 template <bool kTwo, bool kAny, class Gen>
 void launch(const TraceScene& S, const Gen& gen, uint32_t nrays, uint32_t* pool, uint32_t refill,
-            unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, uint32_t fuse)
+            unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, const PersistOpts& o)
 {
     if (nrays == 0) return;
     fill_u32(pool, 0u, 1, nullptr, 1, s);
     const uint32_t need = (nrays + kBlock - 1) / kBlock;
-    auto kern = fuse ? k_persist<kTwo, kAny, Gen, true> : k_persist<kTwo, kAny, Gen, false>;
-    uint32_t blocks = fuse ? persistent_blocks(k_persist<kTwo, kAny, Gen, true>)
-                           : persistent_blocks(k_persist<kTwo, kAny, Gen, false>);
+    if (o.batch) {
+        auto kern = k_persist_bt<kTwo, kAny, Gen, kBatchK>;
+        uint32_t blocks = persistent_blocks(kern);
+        if (blocks > need) blocks = need;
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, S, gen, nrays, pool, refill, o.steps, ray_count,
+                           stats);
+        return;
+    }
+    auto kern = o.fuse ? k_persist<kTwo, kAny, Gen, true> : k_persist<kTwo, kAny, Gen, false>;
+    uint32_t blocks = o.fuse ? persistent_blocks(k_persist<kTwo, kAny, Gen, true>)
+                             : persistent_blocks(k_persist<kTwo, kAny, Gen, false>);
     if (blocks > need) blocks = need;
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, S, gen, nrays, pool, refill, ray_count, stats);
 }
@@ -781,36 +976,36 @@ void launch(const TraceScene& S, const Gen& gen, uint32_t nrays, uint32_t* pool,
 
 void persist_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
                      uint32_t* hit_prims, uint32_t* pool, uint32_t refill, unsigned long long* ray_count,
-                     hipStream_t s, uint32_t fuse)
+                     hipStream_t s, const PersistOpts& o)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
     GenPrimary g{ cam, ra, npix, hits, hit_prims };
-    if (S.two_level) launch<true, false>(S, g, npix, pool, refill, ray_count, ra.stats, s, fuse);
-    else launch<false, false>(S, g, npix, pool, refill, ray_count, ra.stats, s, fuse);
+    if (S.two_level) launch<true, false>(S, g, npix, pool, refill, ray_count, ra.stats, s, o);
+    else launch<false, false>(S, g, npix, pool, refill, ray_count, ra.stats, s, o);
 }
 
 void persist_ao(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                 const uint32_t* hit_prims, uint8_t* occ, uint32_t* pool, uint32_t refill,
-                unsigned long long* ray_count, hipStream_t s, uint32_t fuse)
+                unsigned long long* ray_count, hipStream_t s, const PersistOpts& o)
 {
     GenAO g{ ra, npix, hits, hit_prims, occ };
     const uint32_t n = npix * (uint32_t)ra.ao_samples;
-    if (S.two_level) launch<true, true>(S, g, n, pool, refill, ray_count, ra.stats, s, fuse);
-    else launch<false, true>(S, g, n, pool, refill, ray_count, ra.stats, s, fuse);
+    if (S.two_level) launch<true, true>(S, g, n, pool, refill, ray_count, ra.stats, s, o);
+    else launch<false, true>(S, g, n, pool, refill, ray_count, ra.stats, s, o);
 }
 
 void persist_batch(const TraceScene& S, const float* org, const float* dir, uint32_t n, int anyhit, float tmax,
                    float* t_out, uint32_t* prim_out, uint8_t* occ_out, uint32_t* pool, uint32_t refill,
                    unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, float* hits,
-                   uint32_t fuse)
+                   const PersistOpts& o)
 {
     GenBatch g{ org, dir, n, tmax, anyhit, t_out, prim_out, occ_out, anyhit ? nullptr : hits };
     if (S.two_level) {
-        if (anyhit) launch<true, true>(S, g, n, pool, refill, ray_count, stats, s, fuse);
-        else launch<true, false>(S, g, n, pool, refill, ray_count, stats, s, fuse);
+        if (anyhit) launch<true, true>(S, g, n, pool, refill, ray_count, stats, s, o);
+        else launch<true, false>(S, g, n, pool, refill, ray_count, stats, s, o);
     } else {
-        if (anyhit) launch<false, true>(S, g, n, pool, refill, ray_count, stats, s, fuse);
-        else launch<false, false>(S, g, n, pool, refill, ray_count, stats, s, fuse);
+        if (anyhit) launch<false, true>(S, g, n, pool, refill, ray_count, stats, s, o);
+        else launch<false, false>(S, g, n, pool, refill, ray_count, stats, s, o);
     }
 }
 

@@ -352,18 +352,21 @@ class Engine:
         check(self._L.hr_get_timing(self._h, C.byref(t)), "hr_get_timing")
         return t
 
-    def set_tuning(self, persistent_primary=0, persistent_secondary=1, refill=48, hier_refill=12,
-                   hier_bounds_blocks=0, persistent_path=None, path_refill=None, path_shade=None,
-                   fuse_units=None):
-        """Kernel choices (hr_tuning).  The path and fuse fields keep their
-        current values when not given."""
+    def set_tuning(self, persistent_primary=0, persistent_secondary=1, refill=None, hier_refill=None,
+                   hier_bounds_blocks=None, persistent_path=None, path_refill=None, path_shade=None,
+                   fuse_units=None, test_batch=None, advance_steps=None):
+        """Kernel choices (hr_tuning).  Every field but the two persistent flags
+        keeps its current value when not given."""
         cur = self.tuning()
-        t = _lib.Tuning(int(persistent_primary), int(persistent_secondary), int(refill), int(hier_refill),
-                        int(hier_bounds_blocks),
-                        int(cur.persistent_path if persistent_path is None else persistent_path),
-                        int(cur.path_refill if path_refill is None else path_refill),
-                        int(cur.path_shade if path_shade is None else path_shade),
-                        int(cur.fuse_units if fuse_units is None else fuse_units))
+
+        def pick(v, name):
+            return int(getattr(cur, name) if v is None else v)
+
+        t = _lib.Tuning(int(persistent_primary), int(persistent_secondary), pick(refill, "refill"),
+                        pick(hier_refill, "hier_refill"), pick(hier_bounds_blocks, "hier_bounds_blocks"),
+                        pick(persistent_path, "persistent_path"), pick(path_refill, "path_refill"),
+                        pick(path_shade, "path_shade"), pick(fuse_units, "fuse_units"),
+                        pick(test_batch, "test_batch"), pick(advance_steps, "advance_steps"))
         check(self._L.hr_set_tuning(self._h, C.byref(t)), "hr_set_tuning")
 
     def tuning(self):

@@ -290,7 +290,7 @@ typedef struct {
                                   traversal with dynamic ray fetch                       */
     int persistent_secondary;  /* AO rays and ray batches: 1 (default) persistent, 0 not */
     int refill;                /* idle lanes of a 64-lane wave that trigger a fetch,
-                                  1..64 (default 48)                                     */
+                                  1..64 (default 56)                                     */
     int hier_refill;           /* the same for the AO rays of the instanced hierarchy,
                                   whose lengths vary more: 1..64 (default 12)            */
     int hier_bounds_blocks;    /* workgroups of the hierarchy build's object-bounds pass:
@@ -304,10 +304,20 @@ typedef struct {
                                   pixels in persistent paths, 1..64                    */
     int path_shade;            /* lanes waiting to shade that trigger a wave's shading
                                   step in persistent paths, 1..64                      */
-    int fuse_units;            /* persistent traversal: 1 (default) a lane that steps to a
-                                  new cell or leaf also tests its first candidate in the
-                                  same loop iteration; 0 one unit of work (a triangle test
-                                  or a grid step) per lane and iteration (same results) */
+    int fuse_units;            /* persistent traversal with per-lane tests: 1 (default) a
+                                  lane that steps to a new cell or leaf also tests its first
+                                  candidate in the same loop iteration; 0 one unit of work
+                                  (a triangle test or a grid step) per lane and iteration
+                                  (same results)                                         */
+    int test_batch;            /* persistent traversal: 1 (default) each lane hands up to 16
+                                  candidates of its cell or leaf per loop iteration to its
+                                  wave, which tests them all at full width; 0 each lane
+                                  tests its own candidates (fuse_units applies).  Same
+                                  results; with 1 an any-hit ray may test up to 15
+                                  candidates past its first hit (counted in the work
+                                  statistics)                                            */
+    int advance_steps;         /* test_batch 1: grid steps a lane may take per loop
+                                  iteration to reach its next candidates, 1..4 (default 2) */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

@@ -201,15 +201,18 @@ def test_keyframe_interpolation_parity():
 def test_persistent_matches_one_ray_per_lane(cluster, kind, lt, ll):
     """The persistent kernels (dynamic ray fetch) and the one-ray-per-lane
     kernels apply the same operations to every ray: bit-identical outputs and
-    identical work counters, for several refill thresholds, with one unit of
-    work per loop iteration or a step and a test fused."""
+    identical work counters, for several refill thresholds, with per-lane tests
+    (one unit of work per loop iteration, or a step and a test fused) and with
+    wave-batched tests at 1 to 4 grid steps per iteration."""
     print(f"persistent vs one-ray-per-lane kind={kind}", flush=True)
     e = gpu_engine(cluster, kind, lt, ll)
     e.set_camera_dict(cluster.camera)
     org, d = random_rays(cluster, 30000, seed=17)
     out = {}
-    for pers, refill, fuse in ((0, 16, 1), (1, 1, 0), (1, 16, 1), (1, 64, 0), (1, 48, 1)):
-        e.set_tuning(pers, pers, refill, fuse_units=fuse)
+    for pers, refill, fuse, batch, steps in ((0, 16, 1, 0, 1), (1, 1, 0, 0, 1), (1, 16, 1, 0, 1), (1, 64, 0, 0, 1),
+                                            (1, 48, 1, 0, 1), (1, 56, 1, 1, 1), (1, 56, 1, 1, 2), (1, 1, 1, 1, 4),
+                                            (1, 64, 1, 1, 3), (1, 16, 0, 1, 2)):
+        e.set_tuning(pers, pers, refill, fuse_units=fuse, test_batch=batch, advance_steps=steps)
         e.work_stats()
         img = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0, collect_stats=True)
         ws = e.work_stats()
@@ -217,10 +220,10 @@ def test_persistent_matches_one_ray_per_lane(cluster, kind, lt, ll):
         planes, prims = e.primary_hits()
         t, p = e.trace(org, d)
         o = e.occluded(org, d, 1.5)
-        out[(pers, refill, fuse)] = (img, occ, planes, prims, t, p, o,
-                               (ws.primary.rays, ws.primary.steps, ws.primary.tests,
-                                ws.secondary.rays, ws.secondary.steps, ws.secondary.tests))
-    ref = out[(0, 16, 1)]
+        out[(pers, refill, fuse, batch, steps)] = (img, occ, planes, prims, t, p, o,
+                                                   (ws.primary.rays, ws.primary.steps, ws.primary.tests,
+                                                    ws.secondary.rays, ws.secondary.steps, ws.secondary.tests))
+    ref = out[(0, 16, 1, 0, 1)]
     for key, val in out.items():
         for a, b in zip(val[:7], ref[:7]):
             np.testing.assert_array_equal(a, b, err_msg=str(key))
