@@ -710,8 +710,9 @@ def main():
                      "bytes_per_launch": int(bytes_alg),
                      "bytes": "algorithmic gathers (8 B per DDA step, 40 B per triangle test, 32 B per hit "
                               "pixel, 1 B per AO ray), served mostly by the L2 and the last-level cache",
-                     "limiter": "VALU issue under divergence: the VALU is busy in most cycles while an "
-                                "instruction computes for about a fifth of a wave's lanes",
+                     "limiter": "VALU issue under divergence: the VALU is busy in about three quarters of the "
+                                "cycles while an instruction computes for about two fifths of a wave's lanes "
+                                "(the batched triangle tests run at full width, the grid steps diverge)",
                      "counters": {"valu_busy": committed_counter(f"counter {ao_kernel} valu_busy"),
                                   "lanes_active": committed_counter(f"counter {ao_kernel} lanes_active"),
                                   "l2_hit": committed_counter(f"counter {ao_kernel} l2_hit")},
