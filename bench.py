@@ -676,7 +676,7 @@ def main():
     if not tun.persistent_secondary:
         ao_kernel = "k_ao_rays<true>"
     elif tun.test_batch:
-        ao_kernel = "k_persist_bt<true, true, GenAO, 16>"
+        ao_kernel = "k_persist_bt<true, true, GenAO, 12>"
     else:
         ao_kernel = f"k_persist<true, true, GenAO, {'true' if tun.fuse_units else 'false'}>"
 
@@ -710,7 +710,7 @@ def main():
                      "bytes_per_launch": int(bytes_alg),
                      "bytes": "algorithmic gathers (8 B per DDA step, 40 B per triangle test, 32 B per hit "
                               "pixel, 1 B per AO ray), served mostly by the L2 and the last-level cache",
-                     "limiter": "VALU issue under divergence: the VALU is busy in about three quarters of the "
+                     "limiter": "VALU issue under divergence: the VALU is busy in about four fifths of the "
                                 "cycles while an instruction computes for about two fifths of a wave's lanes "
                                 "(the batched triangle tests run at full width, the grid steps diverge)",
                      "counters": {"valu_busy": committed_counter(f"counter {ao_kernel} valu_busy"),

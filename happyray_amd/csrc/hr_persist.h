@@ -10,7 +10,7 @@ namespace hr {
 //     fuse 1 lets a lane that steps to a new cell or leaf test its first
 //     candidate in the same iteration (fuse 0: one unit of work per lane and
 //     iteration);
-//   batch 1: each lane hands up to 16 candidates per iteration to its wave,
+//   batch 1: each lane hands up to 12 candidates per iteration to its wave,
 //     which tests them all at full width, and takes up to `steps` grid steps
 //     (1..4) per iteration to reach its next candidates.
 struct PersistOpts {

@@ -493,9 +493,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 //
 // A lane also takes up to `steps_max` grid steps per iteration when the leaves
 // or cells it steps into are empty, so that more lanes reach candidates by the
-// wave's test batch.  The workgroup uses about 30 KB of LDS; five waves per
-// SIMD leave each lane 96 VGPRs, which the batch needs without spilling.
-constexpr int kBatchK = 16;   // candidates a lane hands to its wave per iteration
+// wave's test batch.  With kK = 12 the workgroup uses 24 KB of LDS, and six
+// waves per SIMD (80 VGPRs per lane, a few values spilled) beat five waves
+// without spills and 16 candidates per lane (AO 2.29 against 2.40 ms on the
+// 1M-triangle frame): the gathers need the extra wave.
+constexpr int kBatchK = 12;   // candidates a lane hands to its wave per iteration
 
 template <int kK>
 struct WaveTests {
@@ -508,7 +510,7 @@ struct WaveTests {
 };
 
 template <bool kTwo, bool kAny, class Gen, int kK>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_persist_bt(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_persist_bt(
     TraceScene S, Gen gen, uint32_t nrays, uint32_t* __restrict__ pool, uint32_t refill, uint32_t steps_max,
     unsigned long long* ray_count, unsigned long long* stats)
 {

@@ -309,13 +309,13 @@ typedef struct {
                                   candidate in the same loop iteration; 0 one unit of work
                                   (a triangle test or a grid step) per lane and iteration
                                   (same results)                                         */
-    int test_batch;            /* persistent traversal: 1 (default) each lane hands up to 16
+    int test_batch;            /* persistent traversal: 1 (default) each lane hands up to 12
                                   candidates of its cell or leaf per loop iteration to its
                                   wave, which tests them all at full width; 0 each lane
                                   tests its own candidates (fuse_units applies).  Same
-                                  results; with 1 an any-hit ray may test up to 15
-                                  candidates past its first hit (counted in the work
-                                  statistics)                                            */
+                                  results and work statistics; with 1 an any-hit ray may
+                                  test up to 11 candidates past its first hit, which the
+                                  statistics leave out                                  */
     int advance_steps;         /* test_batch 1: grid steps a lane may take per loop
                                   iteration to reach its next candidates, 1..4 (default 2) */
 } hr_tuning;
