@@ -137,7 +137,14 @@ struct hr_ctx {
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
-    hr_tuning tuning{ 0, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2 };
+    hr_tuning tuning{ 2, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2 };
+    // persistent_primary 2: the camera-ray kernel is chosen by timing one frame
+    // of each (0 one ray per lane, 1 persistent) after the scene, grid or camera
+    // changes; results are the same either way
+    float pp_ms[2] = { 0.f, 0.f };
+    int pp_measured = 0;                 // trial frames timed (0, 1, 2)
+    int pp_pending = -1;                 // kernel of an eager frame whose time is unread
+    int pp_choice = -1;                  // the faster kernel once both are timed
 
     // captured frames: a rebuild graph and a render graph, replayed per frame
     hipGraphExec_t g_build = nullptr, g_render = nullptr;
@@ -643,6 +650,43 @@ void enqueue_normals(hr_ctx* c, const hr::FrameSel* fs)
     }
 }
 
+void reset_primary_choice(hr_ctx* c)
+{
+    c->pp_measured = 0;
+    c->pp_pending = -1;
+    c->pp_choice = -1;
+}
+
+bool primary_decided(const hr_ctx* c) { return c->tuning.persistent_primary != 2 || c->pp_choice >= 0; }
+
+// reads the camera-ray time of the last trial frame (its events must have
+// been recorded by that frame; waits for it)
+void resolve_primary_trial(hr_ctx* c)
+{
+    if (c->pp_pending < 0) return;
+    float ms = 0.f;
+    if (hipEventSynchronize(c->ev_p1) == hipSuccess && hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess) {
+        c->pp_ms[c->pp_pending] = ms;
+        c->pp_measured = c->pp_pending + 1;
+    }
+    c->pp_pending = -1;
+    if (c->pp_measured == 2) c->pp_choice = c->pp_ms[1] < c->pp_ms[0] ? 1 : 0;
+}
+
+// the camera-ray kernel of the next frame: 0 one ray per lane, 1 persistent.
+// Under persistent_primary 2 the first timed frame after a reset runs kernel
+// 0 and the second kernel 1, each timed by the frame's events; the faster one
+// is kept.  timed = false (a frame recorded into a graph) starts no trial.
+int primary_kernel(hr_ctx* c, bool timed)
+{
+    if (c->tuning.persistent_primary != 2) return c->tuning.persistent_primary;
+    if (timed) resolve_primary_trial(c);
+    if (c->pp_choice >= 0) return c->pp_choice;
+    if (!timed) return 0;
+    c->pp_pending = c->pp_measured;
+    return c->pp_pending;
+}
+
 // Primary rays and the integrator.  fs != nullptr: the image id is read on the
 // device (captured frames); events = false while a graph is being recorded.
 hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_params* p,
@@ -680,7 +724,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     if (c->kind == HR_GRID_HIERARCHY) {
         // TLGHAOIntegrator::integrate (TLGridHierarchyAOIntegrator.h:421-573)
         hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
-                                  c->fb.p, c->tuning.persistent_primary != 0, c->tuning.persistent_secondary != 0,
+                                  c->fb.p, c->tuning.persistent_primary == 1, c->tuning.persistent_secondary != 0,
                                   c->pool.p, (uint32_t)c->tuning.hier_refill, c->ray_count.p, c->stream);
         if (events) {
             HR_HIP(hipEventRecord(c->ev_p1, c->stream));
@@ -702,7 +746,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     }
     const bool pers = c->tuning.persistent_secondary != 0;
     const uint32_t refill = (uint32_t)c->tuning.refill;
-    if (c->tuning.persistent_primary)
+    if (primary_kernel(c, events))
         hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
                             c->stream, persist_opts(c));
     else
@@ -789,6 +833,7 @@ std::vector<unsigned char> graph_signature(hr_ctx* c, hr_mode mode, const hr_ren
     put(v, c->nn);
     put(v, c->nframes_n);
     put(v, c->tuning);
+    put(v, c->pp_choice);
     return v;
 }
 
@@ -1043,6 +1088,7 @@ hr_status hr_upload_mesh(hr_ctx* c, const float* xyz1, const float* xyz2, float 
                          const uint32_t* idx, uint32_t nt)
 {
     HR_TRY(check_ctx(c));
+    reset_primary_choice(c);
     if (!xyz1 || nv == 0) return fail(HR_ERR_ARG, "a mesh needs at least one vertex");
     if (nt > 0 && !idx) return fail(HR_ERR_ARG, "null index buffer");
     if (nt > 0x3FFFFFFFu) return fail(HR_ERR_ARG, "too many triangles");
@@ -1101,6 +1147,7 @@ hr_status hr_set_keyframes(hr_ctx* c, const float* xyz, uint32_t nframes, uint32
                            uint32_t nt)
 {
     HR_TRY(check_ctx(c));
+    reset_primary_choice(c);
     if (!xyz || nframes == 0 || nv == 0) return fail(HR_ERR_ARG, "empty key frames");
     if (nt > 0 && !idx) return fail(HR_ERR_ARG, "null index buffer");
     HR_TRY(validate_indices(idx, nt, nv));
@@ -1166,6 +1213,7 @@ hr_status hr_set_normals(hr_ctx* c, const float* normals, uint32_t nframes, uint
 hr_status hr_set_grid(hr_ctx* c, hr_grid_kind kind, float lambda_top, float lambda_leaf)
 {
     HR_TRY(check_ctx(c));
+    reset_primary_choice(c);
     if (kind != HR_GRID_UNIFORM && kind != HR_GRID_UNIFORM_EXACT && kind != HR_GRID_TWO_LEVEL &&
         kind != HR_GRID_HIERARCHY)
         return fail(HR_ERR_ARG, "unknown grid kind");
@@ -1219,6 +1267,7 @@ hr_status hr_set_camera(hr_ctx* c, const float pos[3], const float dir[3], const
                         int res_x, int res_y)
 {
     HR_TRY(check_ctx(c));
+    reset_primary_choice(c);
     if (!pos || !dir || !up) return fail(HR_ERR_ARG, "null camera vector");
     if (res_x <= 0 || res_y <= 0 || (int64_t)res_x * res_y > (1ll << 28))
         return fail(HR_ERR_ARG, "invalid resolution");
@@ -1291,7 +1340,8 @@ hr_status hr_render_frame_graph(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff
     c->normals_dirty = true;
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
     const size_t fb_bytes = 3 * (size_t)npix * sizeof(float);
-    if (c->g_build && c->g_render && npix == c->last_npix && c->graph_sig == graph_signature(c, mode, p)) {
+    if (c->g_build && c->g_render && npix == c->last_npix && primary_decided(c) &&
+        c->graph_sig == graph_signature(c, mode, p)) {
         *c->h_frame = hr::FrameSel{ k1, k2, coeff, image_id };
         HR_HIP(hipEventRecord(c->ev_b0, c->stream));
         HR_HIP(hipGraphLaunch(c->g_build, c->stream));
@@ -1328,6 +1378,7 @@ hr_status hr_render_frame_graph(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff
     c->graph_timed = false;
     if (rgb_out) HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, fb_bytes, hipMemcpyDeviceToHost, c->stream));
     HR_HIP(hipStreamSynchronize(c->stream));
+    resolve_primary_trial(c);   // before the capture, which records no trial
     return record_graphs(c, mode, p);
 }
 
@@ -1627,9 +1678,10 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
 {
     HR_TRY(check_ctx(c));
     if (!t) return fail(HR_ERR_ARG, "null tuning");
-    if ((t->persistent_primary != 0 && t->persistent_primary != 1) ||
-        (t->persistent_secondary != 0 && t->persistent_secondary != 1))
-        return fail(HR_ERR_ARG, "persistent flags must be 0 or 1");
+    if (t->persistent_primary < 0 || t->persistent_primary > 2)
+        return fail(HR_ERR_ARG, "persistent_primary must be 0, 1 or 2");
+    if (t->persistent_secondary != 0 && t->persistent_secondary != 1)
+        return fail(HR_ERR_ARG, "persistent_secondary must be 0 or 1");
     if (t->refill < 1 || t->refill > 64) return fail(HR_ERR_ARG, "refill must be in [1, 64]");
     if (t->hier_refill < 1 || t->hier_refill > 64) return fail(HR_ERR_ARG, "hier_refill must be in [1, 64]");
     if (t->hier_bounds_blocks < 0 || t->hier_bounds_blocks > (int)hr::kObjBoundsBlocks)
@@ -1639,8 +1691,9 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
     if (t->path_shade < 1 || t->path_shade > 64) return fail(HR_ERR_ARG, "path_shade must be in [1, 64]");
     if (t->fuse_units != 0 && t->fuse_units != 1) return fail(HR_ERR_ARG, "fuse_units must be 0 or 1");
     if (t->test_batch != 0 && t->test_batch != 1) return fail(HR_ERR_ARG, "test_batch must be 0 or 1");
-    if (t->advance_steps < 1 || t->advance_steps > 4) return fail(HR_ERR_ARG, "advance_steps must be in [1, 4]");
+    if (t->advance_steps < 1 || t->advance_steps > 16) return fail(HR_ERR_ARG, "advance_steps must be in [1, 16]");
     c->tuning = *t;
+    reset_primary_choice(c);
     return HR_OK;
 }
 

@@ -352,17 +352,18 @@ class Engine:
         check(self._L.hr_get_timing(self._h, C.byref(t)), "hr_get_timing")
         return t
 
-    def set_tuning(self, persistent_primary=0, persistent_secondary=1, refill=None, hier_refill=None,
+    def set_tuning(self, persistent_primary=None, persistent_secondary=None, refill=None, hier_refill=None,
                    hier_bounds_blocks=None, persistent_path=None, path_refill=None, path_shade=None,
                    fuse_units=None, test_batch=None, advance_steps=None):
-        """Kernel choices (hr_tuning).  Every field but the two persistent flags
-        keeps its current value when not given."""
+        """Kernel choices (hr_tuning).  Every field keeps its current value
+        when not given."""
         cur = self.tuning()
 
         def pick(v, name):
             return int(getattr(cur, name) if v is None else v)
 
-        t = _lib.Tuning(int(persistent_primary), int(persistent_secondary), pick(refill, "refill"),
+        t = _lib.Tuning(pick(persistent_primary, "persistent_primary"),
+                        pick(persistent_secondary, "persistent_secondary"), pick(refill, "refill"),
                         pick(hier_refill, "hier_refill"), pick(hier_bounds_blocks, "hier_bounds_blocks"),
                         pick(persistent_path, "persistent_path"), pick(path_refill, "path_refill"),
                         pick(path_shade, "path_shade"), pick(fuse_units, "fuse_units"),

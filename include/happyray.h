@@ -286,8 +286,13 @@ hr_status hr_path_trace_dev_noalloc(hr_ctx* ctx, const float* d_org, const float
 /* Kernel choices of a context (no reference counterpart: the reference fixes
  * its launch shapes at compile time, src/RT/Algorithm/RayTracingKernels.h:34-43). */
 typedef struct {
-    int persistent_primary;    /* camera rays: 0 (default) one ray per lane, 1 persistent
-                                  traversal with dynamic ray fetch                       */
+    int persistent_primary;    /* camera rays: 0 one ray per lane, 1 persistent traversal
+                                  with dynamic ray fetch, 2 (default) whichever of the two
+                                  traced the camera rays faster: the first two timed frames
+                                  after the mesh, key frames, grid, camera or tuning change
+                                  run one each (the first waits for the second to finish),
+                                  later frames the faster.  Same results either way; the
+                                  instanced hierarchy takes 2 as 0                       */
     int persistent_secondary;  /* AO rays and ray batches: 1 (default) persistent, 0 not */
     int refill;                /* idle lanes of a 64-lane wave that trigger a fetch,
                                   1..64 (default 56)                                     */

@@ -231,6 +231,34 @@ def test_persistent_matches_one_ray_per_lane(cluster, kind, lt, ll):
     e.close()
 
 
+def test_camera_kernel_choice(cluster):
+    """persistent_primary 2 (the default) runs each camera-ray kernel on one
+    of its first two frames and keeps the faster; every frame equals both
+    fixed choices bit for bit, before and after the choice and after a
+    camera change restarts it."""
+    print("automatic camera-ray kernel", flush=True)
+    e = gpu_engine(cluster, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    e.set_camera_dict(cluster.camera)
+    assert e.tuning().persistent_primary == 2
+    ref = {}
+    for pp in (0, 1):
+        e.set_tuning(persistent_primary=pp)
+        img = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0)
+        ref[pp] = (img, e.ao_occlusion(4)) + e.primary_hits()
+    for a, b in zip(ref[0], ref[1]):
+        np.testing.assert_array_equal(a, b)
+    e.set_tuning(persistent_primary=2)
+    for i in range(5):
+        if i == 3:
+            e.set_camera_dict(cluster.camera)
+        img = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0)
+        got = (img, e.ao_occlusion(4)) + e.primary_hits()
+        for a, b in zip(got, ref[0]):
+            np.testing.assert_array_equal(a, b, err_msg=f"frame {i}")
+    assert e.tuning().persistent_primary == 2
+    e.close()
+
+
 def test_work_counters_match_oracle(cluster):
     """DDA steps and triangle tests counted by the kernels equal the oracle's."""
     print("work counters vs oracle", flush=True)
