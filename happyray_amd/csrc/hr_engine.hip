@@ -300,6 +300,13 @@ hr::PersistOpts persist_opts(const hr_ctx* c)
     return o;
 }
 
+// the path tracer's kernel: wave-cooperative with test_batch (its grid steps
+// per batch), else one path per lane
+uint32_t pt_wave_steps(const hr_ctx* c)
+{
+    return c->tuning.test_batch ? (uint32_t)c->tuning.advance_steps : 0u;
+}
+
 hr::BuildBuffers build_buffers(hr_ctx* c)
 {
     hr::BuildBuffers b;
@@ -772,7 +779,8 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
                            (uint32_t)c->tuning.path_refill, (uint32_t)c->tuning.path_shade, c->ray_count.p,
                            c->stream);
         else
-            hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream);
+            hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream,
+                          pt_wave_steps(c));
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
     } else if (mode == HR_MODE_SIMPLE) {
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
@@ -954,7 +962,7 @@ hr_status path_trace_hits(hr_ctx* c, const float* hits, const uint32_t* prims, u
         hr::persist_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, pool ? pool : c->pool.p,
                        (uint32_t)c->tuning.path_refill, (uint32_t)c->tuning.path_shade, c->ray_count.p, s);
     else
-        hr::launch_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, c->ray_count.p, s);
+        hr::launch_pt(trace_scene(c), ra, n, hits, prims, sh, rgb, c->ray_count.p, s, pt_wave_steps(c));
     HR_HIP(hipGetLastError());
     return HR_OK;
 }

@@ -57,9 +57,12 @@ void launch_ao_rays(const TraceScene& S, const RenderArgs& ra, uint32_t npix, co
 void launch_ao_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                      const uint32_t* hit_prims, const uint8_t* occ, const Shading& sh, float* fb,
                      hipStream_t s);
+// wave_steps > 0: the wave-cooperative kernel (batched candidate tests, up to
+// wave_steps grid steps per lane and batch); 0: one path per lane throughout.
+// The same image either way.
 void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                const uint32_t* hit_prims, const Shading& sh, float* fb,
-               unsigned long long* ray_count, hipStream_t s);
+               unsigned long long* ray_count, hipStream_t s, uint32_t wave_steps = 0);
 // current-frame normals: a * (1 - 0) + b * 0 per component, as the reference
 // uploads normals with coefficient 0 (src/RT/RTEngine.cu:209-210); a, b are
 // SoA key-frame blocks of nn normals

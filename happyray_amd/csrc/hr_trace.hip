@@ -27,6 +27,7 @@
 // Kernel shape: one ray per lane, 256-lane workgroups; the grid parameters
 // are loaded once per lane from the device-resident GridParams block that
 // the builder wrote, so a trace needs no host knowledge of the grid.
+#include "hr_flat.h"
 #include "hr_traverse.h"
 
 namespace hr {
@@ -180,6 +181,105 @@ __global__ __launch_bounds__(kBlock) void k_ao_shade(TraceScene S, RenderArgs ra
 // ---------------------------------------------------------------------------
 // path tracing megakernel (PathTracer.h:41-372), one lane per pixel
 // ---------------------------------------------------------------------------
+
+// k_pt with the wave-cooperative trace: the same path loop, with the depth
+// iterations run in step by the whole wave (a finished path sits out the rest)
+// so that each shadow and extension ray goes through trace_wave, whose
+// candidate tests run in full-width batches.  The bounce of an opaque vertex
+// is computed before its shadow ray is traced (it reads nothing the shadow ray
+// decides, and the light term keeps the vertex's own last_spec), so only the
+// light term waits across the trace.  Per path the operations and their order
+// are k_pt's: the same image bit for bit.
+template <bool kTwo>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pt_wave(TraceScene S, RenderArgs ra, uint32_t npix,
+                                                    const float* __restrict__ hits,
+                                                    const uint32_t* __restrict__ hit_prims, Shading sh,
+                                                    float* __restrict__ fb, uint32_t steps_max,
+                                                    unsigned long long* ray_count)
+{
+    __shared__ WaveTests<kBatchK> lds[kBlock / kWave];
+    __shared__ float light_terms[6][kBlock];   // a vertex's light terms wait here across its shadow ray
+    WaveTests<kBatchK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = pix < npix;
+    const GridRegs G = load_grid(S.gp);
+    PathState P;
+    P.radiance = rep(0.f);
+    P.ray_t = in ? hits[6 * (size_t)npix + pix] : kFltMax;
+    P.prim = in ? hit_prims[pix] : S.nt;
+    uint32_t traced = 0;
+    Work wk;
+    bool alive = P.ray_t < kFltMax && P.prim < S.nt;
+    if (alive) {
+        P.org = mk(hits[pix], hits[(size_t)npix + pix], hits[2 * (size_t)npix + pix]);
+        P.dir = mk(hits[3 * (size_t)npix + pix], hits[4 * (size_t)npix + pix], hits[5 * (size_t)npix + pix]);
+        P.att = rep(1.f);
+        P.last_spec = true;
+    }
+    for (int depth = 0; depth < ra.pt_depth; ++depth) {
+        if (__ballot(alive) == 0ull) break;
+        PathVertex V;
+        bool shadow = false;
+        bool light_spec = true;
+        if (alive) {
+            P.depth = depth;
+            pt_vertex(S, sh, ra, pix, P, V);
+            if (V.opaque) {
+                shadow = V.shadow;
+                if (shadow) {
+                    light_terms[0][threadIdx.x] = V.add_d.x;
+                    light_terms[1][threadIdx.x] = V.add_d.y;
+                    light_terms[2][threadIdx.x] = V.add_d.z;
+                    light_terms[3][threadIdx.x] = V.add_s.x;
+                    light_terms[4][threadIdx.x] = V.add_s.y;
+                    light_terms[5][threadIdx.x] = V.add_s.z;
+                }
+                light_spec = P.last_spec;
+                if (depth + 1 >= ra.pt_depth) alive = false;
+                else pt_bounce(P, V);   // leaves P.org, the shadow ray's origin, as it is
+            }
+            if (alive && (V.end || depth + 1 >= ra.pt_depth)) alive = false;
+        }
+        // the shadow ray (PathTracer.h:236-258) and its light term
+        float st = 0.9999f;
+        f3 dd;
+        trace_wave<kTwo, true>(G, S, W, shadow, P.org, V.sdir, st, dd, steps_max, wk);
+        if (shadow) {
+            ++traced;
+            if (!(st < 0.9999f)) {
+                P.radiance = P.radiance + mk(light_terms[0][threadIdx.x], light_terms[1][threadIdx.x],
+                                             light_terms[2][threadIdx.x]);
+                if (!light_spec)
+                    P.radiance = P.radiance + mk(light_terms[3][threadIdx.x], light_terms[4][threadIdx.x],
+                                                 light_terms[5][threadIdx.x]);
+            }
+        }
+        // the extension ray (PathTracer.h:318-332)
+        float t = kFltMax;
+        f3 d2;
+        const uint32_t hit = trace_wave<kTwo, false>(G, S, W, alive, P.org, P.dir, t, d2, steps_max, wk);
+        if (alive) {
+            ++traced;
+            P.dir = d2;
+            if (!(t < kFltMax)) {
+                alive = false;
+            } else {
+                P.org = P.org + (t - kEps) * P.dir;
+                P.prim = hit;
+                P.ray_t = t;
+            }
+        }
+    }
+    if (ray_count && traced) atomicAdd(ray_count, (unsigned long long)traced);
+    add_work(ra.stats, traced, wk);
+    if (!in) return;
+    const float nw = 1.f / (float)(image_of(ra) + 1);
+    const float ow = 1.f - nw;
+    float* o = fb + 3 * (size_t)pix;
+    o[0] = o[0] * ow + P.radiance.x * nw;
+    o[1] = o[1] * ow + P.radiance.y * nw;
+    o[2] = o[2] * ow + P.radiance.z * nw;
+}
 
 template <bool kTwo>
 __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint32_t npix,
@@ -452,8 +552,17 @@ void launch_ao_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, c
 
 void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                const uint32_t* hit_prims, const Shading& sh, float* fb,
-               unsigned long long* ray_count, hipStream_t s)
+               unsigned long long* ray_count, hipStream_t s, uint32_t wave_steps)
 {
+    if (wave_steps) {
+        if (S.two_level)
+            hipLaunchKernelGGL(k_pt_wave<true>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits,
+                               hit_prims, sh, fb, wave_steps, ray_count);
+        else
+            hipLaunchKernelGGL(k_pt_wave<false>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits,
+                               hit_prims, sh, fb, wave_steps, ray_count);
+        return;
+    }
     if (S.two_level)
         hipLaunchKernelGGL(k_pt<true>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, ra, npix, hits,
                            hit_prims, sh, fb, ray_count);
