@@ -453,10 +453,25 @@ def main_served(args):
         lat[i] = p.latency_ms
     end = time.perf_counter()
     st = srv.stats()
-    srv.close()
     rays = sum(r[3] for r in reqs)
     served_wall = end - start
     served = rays / served_wall / 1e6
+
+    # capacity: the same requests submitted at once (arrival times ignored), so
+    # the server is never idle waiting for an arrival
+    cap_pending = []
+    cap_start = time.perf_counter()
+    for i, (t, kind, s, n, _, k1, k2, c) in enumerate(reqs):
+        if kind == "rebuild":
+            cap_pending.append(srv.submit_rebuild(s, k1, k2, c))
+        else:
+            o, d = ray_ptrs(s, kind, offsets[i])
+            cap_pending.append(srv.submit_device(kinds[kind], s, n, o, d, stride=P, tmax=dev[s][4], depth=4))
+    for p in cap_pending:
+        p.wait()
+    cap_wall = time.perf_counter() - cap_start
+    capacity = rays / cap_wall / 1e6
+    srv.close()
 
     # plain: sequential single-scene launches on the same resident pools
     engines = []
@@ -474,6 +489,7 @@ def main_served(args):
     t_s, p_s = DeviceArray((nmax,), np.float32), DeviceArray((nmax,), np.uint32)
     occ_s, rgb_s = DeviceArray((nmax,), np.uint8), DeviceArray((nmax, 3), np.float32)
     service = np.zeros(len(reqs))
+    by_kind = {"rebuild": 0.0, "closest": 0.0, "occluded": 0.0, "path": 0.0}
     for i, (t, kind, s, n, _, k1, k2, c) in enumerate(reqs):
         e = engines[s]
         t1 = time.perf_counter()
@@ -493,6 +509,7 @@ def main_served(args):
                 e.path_trace_dev(so.value, sd.value, n, 4, rgb_s.value)
             e.synchronize()
         service[i] = time.perf_counter() - t1
+        by_kind[kind] += service[i]
     arr = np.array([r[0] for r in reqs])
     done = np.zeros(len(reqs))
     clock = 0.0
@@ -521,6 +538,9 @@ def main_served(args):
         "plain_latency_ms": {"p50": round(float(np.percentile(plain_lat, 50)), 3),
                              "p99": round(float(np.percentile(plain_lat, 99)), 3)},
         "speedup_vs_plain": round(served / plain, 3),
+        "capacity_mrays_s": round(capacity, 3),
+        "capacity_speedup_vs_plain": round(capacity / (rays / service.sum() / 1e6), 3),
+        "plain_service_s": {k: round(v, 4) for k, v in by_kind.items()},
         "server": {"waves": st.waves - st0.waves, "builds": st.builds, "rebuilds": st.rebuilds,
                    "evictions": st.evictions, "peak_resident_gib": round(st.peak_resident_bytes / 2**30, 2),
                    "resident_scenes": st.resident_scenes},

@@ -732,7 +732,8 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         // TLGHAOIntegrator::integrate (TLGridHierarchyAOIntegrator.h:421-573)
         hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
                                   c->fb.p, c->tuning.persistent_primary == 1, c->tuning.persistent_secondary != 0,
-                                  c->pool.p, (uint32_t)c->tuning.hier_refill, c->ray_count.p, c->stream);
+                                  c->pool.p, (uint32_t)c->tuning.hier_refill, c->ray_count.p, c->stream,
+                                  c->tuning.test_batch != 0);
         if (events) {
             HR_HIP(hipEventRecord(c->ev_p1, c->stream));
             HR_HIP(hipEventRecord(c->ev_p2, c->stream));

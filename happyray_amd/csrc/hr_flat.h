@@ -306,17 +306,20 @@ struct WaveTests {
     uint8_t own[kWave * kK];
 };
 
-// The wave's test batch: each lane for which `want` holds and that has
-// candidates left hands up to kK of them (from r.it on) to the wave, the wave
-// tests them all and every such lane takes its ray's result.  Returns true on
-// a lane whose any-hit ray was hit.  Every lane of the wave calls it together.
+// The wave's test batch over candidate references [it, end) of S.prims: each
+// lane for which `want` holds and that has candidates left hands up to kK of
+// them to the wave, the wave tests them all (each against its owner's ray in
+// W) and every such lane takes its ray's result into t / best and moves it
+// past the candidates.  Returns true on a lane whose any-hit ray was hit.
+// Every lane of the wave calls it together.
 template <bool kAny, int kK>
-__device__ __forceinline__ bool wave_test_batch(const TraceScene& S, WaveTests<kK>& W, RayState& r, uint32_t lane,
-                                                bool want, Work& wk)
+__device__ __forceinline__ bool wave_test_range(const float4* __restrict__ rec, const uint32_t* __restrict__ prims,
+                                                WaveTests<kK>& W, uint32_t& it, uint32_t end, float& t,
+                                                uint32_t& best, uint32_t lane, bool want, Work& wk)
 {
     bool hit = false;
     uint32_t cnt = 0;
-    if (want && r.it < r.end) cnt = r.end - r.it < (uint32_t)kK ? r.end - r.it : (uint32_t)kK;
+    if (want && it < end) cnt = end - it < (uint32_t)kK ? end - it : (uint32_t)kK;
     const uint32_t incl = wave_incl_scan(cnt);
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
     if (total) {
@@ -324,12 +327,12 @@ __device__ __forceinline__ bool wave_test_batch(const TraceScene& S, WaveTests<k
 #pragma unroll
         for (int q = 0; q < kK; ++q) {
             if ((uint32_t)q < cnt) {
-                W.cand[ex + q] = r.it + (uint32_t)q;
+                W.cand[ex + q] = it + (uint32_t)q;
                 W.own[ex + q] = (uint8_t)lane;
             }
         }
         if (cnt) {
-            W.t[lane] = r.t;
+            W.t[lane] = t;
             W.key[lane] = ~0ull;
         }
         wave_sync();
@@ -339,7 +342,7 @@ __device__ __forceinline__ bool wave_test_batch(const TraceScene& S, WaveTests<k
                 const uint32_t o = W.own[k];
                 const uint32_t ci = W.cand[k];
                 f3 v0, v1, v2;
-                load_tri(S.rec, S.prims[ci], v0, v1, v2);
+                load_tri(rec, prims[ci], v0, v1, v2);
                 const float dist = mt_dist(mk(W.ox[o], W.oy[o], W.oz[o]), mk(W.dx[o], W.dy[o], W.dz[o]),
                                            v0, v1, v2);
                 if (dist < W.t[o]) {
@@ -354,18 +357,25 @@ __device__ __forceinline__ bool wave_test_batch(const TraceScene& S, WaveTests<k
             uint32_t tested = cnt;
             if (key != ~0ull) {
                 const uint32_t ci = kAny ? (uint32_t)(key >> 32) : (uint32_t)key;
-                r.t = __uint_as_float(kAny ? (uint32_t)key : (uint32_t)(key >> 32));
-                r.best = S.prims[ci];
+                t = __uint_as_float(kAny ? (uint32_t)key : (uint32_t)(key >> 32));
+                best = prims[ci];
                 if (kAny) {
                     hit = true;
-                    tested = ci - r.it + 1;   // the sequential traversal stops here
+                    tested = ci - it + 1;   // the sequential traversal stops here
                 }
             }
             wk.tests += tested;
-            r.it += cnt;
+            it += cnt;
         }
     }
     return hit;
+}
+
+template <bool kAny, int kK>
+__device__ __forceinline__ bool wave_test_batch(const TraceScene& S, WaveTests<kK>& W, RayState& r, uint32_t lane,
+                                                bool want, Work& wk)
+{
+    return wave_test_range<kAny, kK>(S.rec, S.prims, W, r.it, r.end, r.t, r.best, lane, want, wk);
 }
 
 // trace<> of one ray per active lane (RayTracingKernels.h:118-140: rcp = 1 / d,

@@ -105,10 +105,11 @@ void launch_trace_hier(const HierScene& H, const float* org, const float* dir, u
 // the AO shading of computeAOIlluminationTLGH
 // pers_primary / pers_ao: the camera / AO rays run in the persistent kernel,
 // which fetches rays through the device word *pool (reset here) whenever
-// `refill` lanes of a wave are idle
+// `refill` lanes of a wave are idle; batch: with the triangle tests batched
+// across the wave (the same outputs)
 void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const RenderArgs& ra, float* hits,
                            uint32_t* hit_prims, uint32_t* hit_inst, uint8_t* occ, const Shading& sh, float* fb,
                            bool pers_primary, bool pers_ao, uint32_t* pool, uint32_t refill,
-                           unsigned long long* ray_count, hipStream_t s);
+                           unsigned long long* ray_count, hipStream_t s, bool batch = false);
 
 }  // namespace hr

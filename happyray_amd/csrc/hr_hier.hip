@@ -10,6 +10,7 @@
 // An any-hit ray (AO) stops after the first top cell that leaves it with a
 // distance below 0.9999, as the reference's shadow rays do, and is occluded
 // when its final distance is below tmax.
+#include "hr_flat.h"
 #include "hr_hier.h"
 #include "hr_traverse.h"
 
@@ -473,30 +474,11 @@ __device__ __forceinline__ bool hp_begin(const HierScene& H, const GridRegs& G, 
     return true;
 }
 
-// one unit of work; true when the ray is finished
-__device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, HierRay& r)
+// one unit of work other than a triangle test (an object-grid step, an
+// instance set-up or a top-grid step); true when the ray is finished.
+// entered: an instance's object grid was entered (r.ot / r.dt set)
+__device__ __forceinline__ bool hp_unit(const HierScene& H, const GridRegs& G, HierRay& r, bool& entered)
 {
-    if (r.pit < r.pend) {
-        // one triangle of the object cell (MollerTrumboreIntersectionTest)
-        const uint32_t p = H.prims[r.pit];
-        ++r.pit;
-        f3 v0, v1, v2;
-        load_tri(H.rec, p, v0, v1, v2);
-        const f3 e1 = v1 - v0;
-        const f3 e2 = v2 - v0;
-        const f3 pv = cross(r.dt, e2);
-        const float det_rcp = 1.f / dot(e1, pv);
-        f3 tv = r.ot - v0;
-        const float alpha = det_rcp * dot(tv, pv);
-        tv = cross(tv, e1);
-        const float beta = det_rcp * dot(tv, r.dt);
-        const float dist = det_rcp * dot(e2, tv);
-        if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kIsectEps && dist < r.rayT) {
-            r.rayT = dist;
-            r.hit = p;
-        }
-        return false;
-    }
     if (r.in_inst) {
         // the object DDA's step after a cell
         const int d = min_dim(r.otx, r.oty, r.otz);
@@ -586,6 +568,7 @@ __device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, H
         r.rayT = r.t;
         r.hit = H.nt;
         r.in_inst = true;
+        entered = true;
         hp_obj_cell(H, r);
         return false;
     }
@@ -612,6 +595,26 @@ __device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, H
     if (!flag) return true;
     hp_top_cell(H, G, r);
     return false;
+}
+
+// one unit of work; true when the ray is finished
+__device__ __forceinline__ bool hp_step(const HierScene& H, const GridRegs& G, HierRay& r)
+{
+    if (r.pit < r.pend) {
+        // one triangle of the object cell (MollerTrumboreIntersectionTest)
+        const uint32_t p = H.prims[r.pit];
+        ++r.pit;
+        f3 v0, v1, v2;
+        load_tri(H.rec, p, v0, v1, v2);
+        const float dist = mt_dist(r.ot, r.dt, v0, v1, v2);
+        if (dist < r.rayT) {
+            r.rayT = dist;
+            r.hit = p;
+        }
+        return false;
+    }
+    bool entered = false;
+    return hp_unit(H, G, r, entered);
 }
 
 // AmbientOcclusionRayGenerator of the hierarchy (TLGridHierarchyAOIntegrator.h:296-380)
@@ -754,23 +757,100 @@ __global__ __launch_bounds__(kBlock) void k_hier_persist(HierScene H, Gen g, uin
     }
 }
 
+// k_hier_persist with the triangle tests batched across the wave (the
+// candidate references of each lane's current object cell, up to kBatchK per
+// lane and iteration, tested at full width against the lanes' object-space
+// rays kept in LDS; see wave_test_range).  A lane whose cell has no candidate
+// left takes one other unit per iteration.  Every ray's units and their order
+// are trace_hier's, so the outputs are the same.  Four waves per SIMD (at most
+// 128 VGPRs): render 25.6 -> 24.6 ms against the compiler's 144 VGPRs and three
+// waves, 31.6 ms with per-lane tests, on 10,000 instances at 1024^2 x 4 AO.
+template <class Gen>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_hier_persist_bt(HierScene H, Gen g, uint32_t* __restrict__ pool,
+                                                            uint32_t refill, unsigned long long* ray_count)
+{
+    __shared__ WaveTests<kBatchK> lds[kBlock / kWave];
+    WaveTests<kBatchK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
+    const GridRegs G = load_grid(H.top);
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = __lanemask_lt();
+    const uint32_t nrays = g.count();
+    HierRay r;
+    Work wk;
+    uint32_t j = 0, started = 0;
+    bool live = false, exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill || idle == ~0ull)) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(pool, nidle);
+            base = __shfl(base, 0, kWave);
+            if (base + nidle >= nrays) exhausted = true;
+            if (!live) {
+                j = base + (uint32_t)__popcll(idle & lt);
+                if (j < nrays) {
+                    bool traced = false;
+                    live = g.gen(H, G, j, r, traced);
+                    started += traced ? 1u : 0u;
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        bool done = false;
+        if (live && r.pit >= r.pend) {
+            bool entered = false;
+            done = hp_unit(H, G, r, entered);
+            if (entered) {
+                W.ox[lane] = r.ot.x;
+                W.oy[lane] = r.ot.y;
+                W.oz[lane] = r.ot.z;
+                W.dx[lane] = r.dt.x;
+                W.dy[lane] = r.dt.y;
+                W.dz[lane] = r.dt.z;
+            }
+        }
+        wave_test_range<false, kBatchK>(H.rec, H.prims, W, r.pit, r.pend, r.rayT, r.hit, lane, live && !done, wk);
+        if (live && done) {
+            live = false;
+            g.finish(H, j, r);
+        }
+    }
+    if (ray_count) {
+        unsigned long long c = started;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if (lane == 0 && c) atomicAdd(ray_count, c);
+    }
+}
+
+template <class K>
+uint32_t hier_blocks(K kernel)
+{
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0);
+    return (uint32_t)((cus > 0 ? cus : 256) * (per > 0 ? per : 1));
+}
+
 template <class Gen>
 void launch_hier_persist(const HierScene& H, const Gen& g, uint32_t nrays, uint32_t* pool, uint32_t refill,
-                         unsigned long long* ray_count, hipStream_t s)
+                         unsigned long long* ray_count, hipStream_t s, bool batch)
 {
-    static uint32_t blocks_max = 0;
-    if (!blocks_max) {
-        int dev = 0, cus = 0, per = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_hier_persist<Gen>, kBlock, 0);
-        blocks_max = (uint32_t)((cus > 0 ? cus : 256) * (per > 0 ? per : 1));
-    }
+    static uint32_t blocks_max[2] = { 0, 0 };
+    if (!blocks_max[batch]) blocks_max[batch] = batch ? hier_blocks(k_hier_persist_bt<Gen>) : hier_blocks(k_hier_persist<Gen>);
     uint32_t blocks = cdiv_u(nrays, kBlock);
-    if (blocks > blocks_max) blocks = blocks_max;
+    if (blocks > blocks_max[batch]) blocks = blocks_max[batch];
     if (!blocks) return;
     fill_u32(pool, 0u, 1, nullptr, 1, s);
-    hipLaunchKernelGGL(k_hier_persist<Gen>, dim3(blocks), dim3(kBlock), 0, s, H, g, pool, refill, ray_count);
+    if (batch)
+        hipLaunchKernelGGL(k_hier_persist_bt<Gen>, dim3(blocks), dim3(kBlock), 0, s, H, g, pool, refill, ray_count);
+    else
+        hipLaunchKernelGGL(k_hier_persist<Gen>, dim3(blocks), dim3(kBlock), 0, s, H, g, pool, refill, ray_count);
 }
 
 }  // namespace
@@ -796,12 +876,12 @@ void launch_trace_hier(const HierScene& H, const float* org, const float* dir, u
 void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const RenderArgs& ra, float* hits,
                            uint32_t* hit_prims, uint32_t* hit_inst, uint8_t* occ, const Shading& sh, float* fb,
                            bool pers_primary, bool pers_ao, uint32_t* pool, uint32_t refill,
-                           unsigned long long* ray_count, hipStream_t s)
+                           unsigned long long* ray_count, hipStream_t s, bool batch)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
     if (pers_primary) {
         const GenPrimaryHier g{ cam, ra, npix, hits, hit_prims, hit_inst };
-        launch_hier_persist(H, g, npix, pool, refill, ray_count, s);
+        launch_hier_persist(H, g, npix, pool, refill, ray_count, s, batch);
     } else {
         hipLaunchKernelGGL(k_primary_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, cam, ra, npix, hits,
                            hit_prims, hit_inst, ray_count);
@@ -809,7 +889,7 @@ void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const Rende
     const size_t nrays = (size_t)npix * (size_t)ra.ao_samples;
     if (pers_ao) {
         const GenAOHier g{ ra, npix, hits, hit_prims, hit_inst, occ };
-        launch_hier_persist(H, g, (uint32_t)nrays, pool, refill, ray_count, s);
+        launch_hier_persist(H, g, (uint32_t)nrays, pool, refill, ray_count, s, batch);
     } else {
         hipLaunchKernelGGL(k_ao_rays_hier, dim3(cdiv_u(nrays, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits,
                            hit_prims, hit_inst, occ, ray_count);
