@@ -306,16 +306,21 @@ struct WaveTests {
     uint8_t own[kWave * kK];
 };
 
-// The wave's test batch over candidate references [it, end) of S.prims: each
+// The wave's test batch over candidate references [it, end) of prims: each
 // lane for which `want` holds and that has candidates left hands up to kK of
 // them to the wave, the wave tests them all (each against its owner's ray in
 // W) and every such lane takes its ray's result into t / best and moves it
 // past the candidates.  Returns true on a lane whose any-hit ray was hit.
-// Every lane of the wave calls it together.
-template <bool kAny, int kK>
+// Every lane of the wave calls it together.  kOwn: the lanes' rays belong to
+// scenes with references and records of their own; an item reads its owner's
+// from own_rec / own_prims (the wave's 64 entries in LDS), and rec / prims are
+// the calling lane's.
+template <bool kAny, int kK, bool kOwn = false>
 __device__ __forceinline__ bool wave_test_range(const float4* __restrict__ rec, const uint32_t* __restrict__ prims,
                                                 WaveTests<kK>& W, uint32_t& it, uint32_t end, float& t,
-                                                uint32_t& best, uint32_t lane, bool want, Work& wk)
+                                                uint32_t& best, uint32_t lane, bool want, Work& wk,
+                                                const float4* const* own_rec = nullptr,
+                                                const uint32_t* const* own_prims = nullptr)
 {
     bool hit = false;
     uint32_t cnt = 0;
@@ -342,7 +347,8 @@ __device__ __forceinline__ bool wave_test_range(const float4* __restrict__ rec, 
                 const uint32_t o = W.own[k];
                 const uint32_t ci = W.cand[k];
                 f3 v0, v1, v2;
-                load_tri(rec, prims[ci], v0, v1, v2);
+                if (kOwn) load_tri(own_rec[o], own_prims[o][ci], v0, v1, v2);
+                else load_tri(rec, prims[ci], v0, v1, v2);
                 const float dist = mt_dist(mk(W.ox[o], W.oy[o], W.oz[o]), mk(W.dx[o], W.dy[o], W.dz[o]),
                                            v0, v1, v2);
                 if (dist < W.t[o]) {

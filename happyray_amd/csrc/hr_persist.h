@@ -71,6 +71,7 @@ struct RayBatchDev {
 // b; lanes refill from a global counter and may hold rays of different scenes.
 void persist_multi(const SceneDesc* scenes, const RayBatchDev* batches, const uint32_t* prefix, uint32_t nb,
                    uint32_t nrays, int anyhit, uint32_t* pool, uint32_t refill, uint32_t blocks_per_cu,
-                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s);
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s,
+                   const PersistOpts& o = PersistOpts{});
 
 }  // namespace hr

@@ -393,6 +393,131 @@ __global__ __launch_bounds__(kBlock) void k_persist_multi(const SceneDesc* __res
     add_work(stats, started, wk);
 }
 
+// k_persist_multi with the candidate tests batched across the wave (see
+// k_persist_bt): a lane's ray may belong to any scene, so each lane also keeps
+// its scene's triangle records and references in LDS for the items it hands
+// out.  Up to steps_max grid steps per lane and iteration.
+template <bool kAny>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_persist_multi_bt(
+    const SceneDesc* __restrict__ scenes, const RayBatchDev* __restrict__ batches, const uint32_t* __restrict__ prefix,
+    uint32_t nb, uint32_t nrays, uint32_t* __restrict__ pool, uint32_t refill, uint32_t steps_max,
+    unsigned long long* ray_count, unsigned long long* stats)
+{
+    __shared__ WaveTests<kBatchK> lds[kBlock / kWave];
+    __shared__ const float4* recs[kBlock];
+    __shared__ const uint32_t* prims[kBlock];
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave;
+    WaveTests<kBatchK>& W = lds[wave];
+    const float4* const* own_rec = recs + wave * kWave;
+    const uint32_t* const* own_prims = prims + wave * kWave;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = __lanemask_lt();
+    RayState r;
+    Work wk;
+    TraceScene S = scenes[0].S;
+    GridRegs G = load_grid(S.gp);
+    uint32_t j = 0, bi = 0, started = 0;
+    bool live = false;
+    bool exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill || idle == ~0ull)) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(pool, nidle);
+            base = __shfl(base, 0, kWave);
+            if (base + nidle >= nrays) exhausted = true;
+            if (!live) {
+                j = base + (uint32_t)__popcll(idle & lt);
+                if (j < nrays) {
+                    uint32_t lo = 0, hi = nb;            // prefix[lo] <= j < prefix[hi]
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (prefix[mid] <= j) lo = mid;
+                        else hi = mid;
+                    }
+                    bi = lo;
+                    const RayBatchDev& B = batches[bi];
+                    const uint32_t jl = j - prefix[bi];
+                    S = scenes[B.scene].S;
+                    G = load_grid(S.gp);
+                    const size_t st = B.stride ? B.stride : B.n;
+                    set_ray(r, mk(B.org[jl], B.org[st + jl], B.org[2 * st + jl]),
+                            mk(B.dir[jl], B.dir[st + jl], B.dir[2 * st + jl]));
+                    r.t = kAny ? B.tmax : kFltMax;
+                    r.tterm = B.tmax;
+                    ++started;
+                    live = ray_begin<true, kAny>(G, S, r, wk);
+                    j = jl;
+                    if (live) {
+                        W.ox[lane] = r.org.x;
+                        W.oy[lane] = r.org.y;
+                        W.oz[lane] = r.org.z;
+                        W.dx[lane] = r.dir.x;
+                        W.dy[lane] = r.dir.y;
+                        W.dz[lane] = r.dir.z;
+                        recs[threadIdx.x] = S.rec;
+                        prims[threadIdx.x] = S.prims;
+                    } else if (kAny) {
+                        B.occ_out[jl] = r.t < B.tmax ? 1u : 0u;
+                        if (B.t_out) B.t_out[jl] = r.t;
+                    } else {
+                        B.t_out[jl] = r.t;
+                        B.prim_out[jl] = scenes[B.scene].nt;
+                        if (B.hits) store_hit(B.hits, B.n, jl, r);
+                    }
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        bool done = false;
+        if (live && r.it >= r.end) {
+            for (uint32_t m = 0; m < steps_max; ++m) {
+                bool top = true;
+                if (r.in_leaf) {
+                    const int k = leaf_step<kAny>(S, r, wk);
+                    if (k != 2) {
+                        top = false;
+                        done = k != 1;
+                    }
+                }
+                if (top) done = !top_step(G, S, r, mk(W.ox[lane], W.oy[lane], W.oz[lane]), wk);
+                if (done || r.it < r.end) break;
+            }
+        }
+        if (wave_test_range<kAny, kBatchK, true>(S.rec, S.prims, W, r.it, r.end, r.t, r.best, lane, live && !done,
+                                                 wk, own_rec, own_prims))
+            done = true;
+        if (live && done) {
+            live = false;
+            const RayBatchDev& B = batches[bi];
+            if (kAny) {
+                B.occ_out[j] = r.t < B.tmax ? 1u : 0u;
+                if (B.t_out) B.t_out[j] = r.t;
+            } else {
+                const SceneDesc& D = scenes[B.scene];
+                B.t_out[j] = r.t;
+                B.prim_out[j] = r.t < kFltMax ? r.best - D.tri_base : D.nt;
+                if (B.hits) {
+                    r.org = mk(W.ox[lane], W.oy[lane], W.oz[lane]);
+                    r.dir = mk(W.dx[lane], W.dy[lane], W.dz[lane]);
+                    store_hit(B.hits, B.n, j, r);
+                }
+            }
+        }
+    }
+    if (ray_count) {
+        unsigned long long c = started;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if (lane == 0 && c) atomicAdd(ray_count, c);
+    }
+    add_work(stats, started, wk);
+}
+
 // ---------------------------------------------------------------------------
 // persistent path tracer
 // ---------------------------------------------------------------------------
@@ -692,7 +817,7 @@ void persist_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const 
 
 void persist_multi(const SceneDesc* scenes, const RayBatchDev* batches, const uint32_t* prefix, uint32_t nb,
                    uint32_t nrays, int anyhit, uint32_t* pool, uint32_t refill, uint32_t blocks_per_cu,
-                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s)
+                   unsigned long long* ray_count, unsigned long long* stats, hipStream_t s, const PersistOpts& o)
 {
     if (nrays == 0 || nb == 0) return;
     fill_u32(pool, 0u, 1, nullptr, 1, s);
@@ -701,10 +826,22 @@ void persist_multi(const SceneDesc* scenes, const RayBatchDev* batches, const ui
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (cus <= 0) cus = 256;
     uint32_t blocks = (uint32_t)cus * (blocks_per_cu ? blocks_per_cu : 1u);
+    const uint32_t need = (nrays + kBlock - 1) / kBlock;
+    if (o.batch) {
+        if (blocks_per_cu == 0)
+            blocks = anyhit ? persistent_blocks(k_persist_multi_bt<true>) : persistent_blocks(k_persist_multi_bt<false>);
+        if (blocks > need) blocks = need;
+        if (anyhit)
+            hipLaunchKernelGGL(k_persist_multi_bt<true>, dim3(blocks), dim3(kBlock), 0, s, scenes, batches, prefix, nb,
+                               nrays, pool, refill, o.steps, ray_count, stats);
+        else
+            hipLaunchKernelGGL(k_persist_multi_bt<false>, dim3(blocks), dim3(kBlock), 0, s, scenes, batches, prefix,
+                               nb, nrays, pool, refill, o.steps, ray_count, stats);
+        return;
+    }
     if (blocks_per_cu == 0) {
         blocks = anyhit ? persistent_blocks(k_persist_multi<true>) : persistent_blocks(k_persist_multi<false>);
     }
-    const uint32_t need = (nrays + kBlock - 1) / kBlock;
     if (blocks > need) blocks = need;
     if (anyhit)
         hipLaunchKernelGGL(k_persist_multi<true>, dim3(blocks), dim3(kBlock), 0, s, scenes, batches, prefix, nb, nrays,
