@@ -59,7 +59,8 @@ class RenderParams(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("render_ms", C.c_double), ("primary_ms", C.c_double),
-                ("secondary_ms", C.c_double), ("shade_ms", C.c_double), ("rays_traced", C.c_uint64)]
+                ("secondary_ms", C.c_double), ("shade_ms", C.c_double), ("rays_traced", C.c_uint64),
+                ("primary_kernel", C.c_int)]
 
 
 class Tuning(C.Structure):

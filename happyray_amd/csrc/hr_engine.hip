@@ -138,13 +138,14 @@ struct hr_ctx {
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
     hr_tuning tuning{ 2, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2 };
-    // persistent_primary 2: the camera-ray kernel is chosen by timing one frame
-    // of each (0 one ray per lane, 1 persistent) after the scene, grid or camera
-    // changes; results are the same either way
-    float pp_ms[2] = { 0.f, 0.f };
-    int pp_measured = 0;                 // trial frames timed (0, 1, 2)
+    // persistent_primary 2: the camera-ray kernel is chosen by timing two frames
+    // of each (0 one ray per lane, 1 persistent, 3 wave-batched tests) after the
+    // scene, grid or camera changes; results are the same either way
+    float pp_ms[3] = { 0.f, 0.f, 0.f };  // least time of each kernel's trials
+    int pp_measured = 0;                 // trial frames timed (0 .. 6)
     int pp_pending = -1;                 // kernel of an eager frame whose time is unread
-    int pp_choice = -1;                  // the faster kernel once both are timed
+    int pp_choice = -1;                  // the fastest kernel once all are timed
+    int last_pk = -1;                    // camera-ray kernel of the last render
 
     // captured frames: a rebuild graph and a render graph, replayed per frame
     hipGraphExec_t g_build = nullptr, g_render = nullptr;
@@ -668,30 +669,44 @@ bool primary_decided(const hr_ctx* c) { return c->tuning.persistent_primary != 2
 
 // reads the camera-ray time of the last trial frame (its events must have
 // been recorded by that frame; waits for it)
+// the camera-ray kernels that persistent_primary 2 chooses from, in trial order:
+// one ray per lane, persistent, one ray per lane with wave-batched tests
+constexpr int kPrimaryKernels[3] = { 0, 1, 3 };
+// two rounds of trials: a kernel's first launch in a process, and the first
+// frame after a change, can take more than twice its steady time
+constexpr int kPrimaryTrials = 6;
+
 void resolve_primary_trial(hr_ctx* c)
 {
     if (c->pp_pending < 0) return;
     float ms = 0.f;
     if (hipEventSynchronize(c->ev_p1) == hipSuccess && hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess) {
-        c->pp_ms[c->pp_pending] = ms;
+        const int k = c->pp_pending % 3;
+        c->pp_ms[k] = c->pp_pending < 3 ? ms : std::min(c->pp_ms[k], ms);
         c->pp_measured = c->pp_pending + 1;
     }
     c->pp_pending = -1;
-    if (c->pp_measured == 2) c->pp_choice = c->pp_ms[1] < c->pp_ms[0] ? 1 : 0;
+    if (c->pp_measured == kPrimaryTrials) {
+        int b = 0;
+        for (int k = 1; k < 3; ++k)
+            if (c->pp_ms[k] < c->pp_ms[b]) b = k;
+        c->pp_choice = kPrimaryKernels[b];
+    }
 }
 
-// the camera-ray kernel of the next frame: 0 one ray per lane, 1 persistent.
-// Under persistent_primary 2 the first timed frame after a reset runs kernel
-// 0 and the second kernel 1, each timed by the frame's events; the faster one
-// is kept.  timed = false (a frame recorded into a graph) starts no trial.
+// the camera-ray kernel of the next frame: 0 one ray per lane, 1 persistent,
+// 3 one ray per lane with wave-batched tests.  Under persistent_primary 2 the
+// first six timed frames after a reset run kernels 0, 1, 3, 0, 1, 3, each timed
+// by the frame's events (read by resolve_primary_trial at the start of the next
+// frame), and the kernel with the least time is kept.  timed = false (a frame
+// recorded into a graph) starts no trial.
 int primary_kernel(hr_ctx* c, bool timed)
 {
     if (c->tuning.persistent_primary != 2) return c->tuning.persistent_primary;
-    if (timed) resolve_primary_trial(c);
     if (c->pp_choice >= 0) return c->pp_choice;
     if (!timed) return 0;
     c->pp_pending = c->pp_measured;
-    return c->pp_pending;
+    return kPrimaryKernels[c->pp_pending % 3];
 }
 
 // Primary rays and the integrator.  fs != nullptr: the image id is read on the
@@ -724,12 +739,15 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         sh.normals = c->normals.p;
         sh.nidx = c->nidx.p;
     }
+    // the last trial frame's camera-ray time, read before its events are recorded again
+    if (events) resolve_primary_trial(c);
     if (events) HR_HIP(hipEventRecord(c->ev_r0, c->stream));
     c->hier_npix = c->kind == HR_GRID_HIERARCHY ? npix : 0u;
     c->ao_dump_ok = false;
     c->occ_npix = mode == HR_MODE_AO ? npix : 0u;
     if (c->kind == HR_GRID_HIERARCHY) {
         // TLGHAOIntegrator::integrate (TLGridHierarchyAOIntegrator.h:421-573)
+        c->last_pk = c->tuning.persistent_primary == 1 ? 1 : 0;
         hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
                                   c->fb.p, c->tuning.persistent_primary == 1, c->tuning.persistent_secondary != 0,
                                   c->pool.p, (uint32_t)c->tuning.hier_refill, c->ray_count.p, c->stream,
@@ -754,11 +772,14 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     }
     const bool pers = c->tuning.persistent_secondary != 0;
     const uint32_t refill = (uint32_t)c->tuning.refill;
-    if (primary_kernel(c, events))
+    const int pk = primary_kernel(c, events);
+    c->last_pk = pk;
+    if (pk == 1)
         hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
                             c->stream, persist_opts(c));
     else
-        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream);
+        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream,
+                           pk == 3 ? (uint32_t)c->tuning.advance_steps : 0u);
     if (events) HR_HIP(hipEventRecord(c->ev_p1, c->stream));
     if (ra.stats) ra.stats = c->work.p + 3;
     if (mode == HR_MODE_AO) {
@@ -1662,6 +1683,7 @@ hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
     unsigned long long rays = 0;
     HR_HIP(hipMemcpy(&rays, c->ray_count.p, sizeof rays, hipMemcpyDeviceToHost));
     out->rays_traced = rays;
+    out->primary_kernel = c->last_pk;
     return HR_OK;
 }
 
@@ -1687,8 +1709,8 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
 {
     HR_TRY(check_ctx(c));
     if (!t) return fail(HR_ERR_ARG, "null tuning");
-    if (t->persistent_primary < 0 || t->persistent_primary > 2)
-        return fail(HR_ERR_ARG, "persistent_primary must be 0, 1 or 2");
+    if (t->persistent_primary < 0 || t->persistent_primary > 3)
+        return fail(HR_ERR_ARG, "persistent_primary must be 0, 1, 2 or 3");
     if (t->persistent_secondary != 0 && t->persistent_secondary != 1)
         return fail(HR_ERR_ARG, "persistent_secondary must be 0 or 1");
     if (t->refill < 1 || t->refill > 64) return fail(HR_ERR_ARG, "refill must be in [1, 64]");

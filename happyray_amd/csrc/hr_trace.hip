@@ -74,6 +74,42 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     add_work(ra.stats, active ? 1u : 0u, wk);
 }
 
+// camera rays through trace_wave: one ray per lane as in k_primary, with the
+// wave's candidate tests batched (a lane whose ray has ended lends its slot to
+// the others' tests); the same hits as k_primary
+template <bool kTwo>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_primary_wave(
+    TraceScene S, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
+    uint32_t* __restrict__ hit_prims, uint32_t steps_max, unsigned long long* ray_count)
+{
+    __shared__ WaveTests<kBatchK> lds[kBlock / kWave];
+    WaveTests<kBatchK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = pix < npix;
+    count_rays(ray_count, active);
+    Work wk;
+    const GridRegs G = load_grid(S.gp);
+    f3 d0 = rep(0.f);
+    if (active) d0 = primary_direction(cam, ra, pix, npix);
+    const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
+    float t = kFltMax;
+    f3 dir;
+    const uint32_t best = trace_wave<kTwo, false>(G, S, W, active, org, d0, t, dir, steps_max, wk);
+    if (active) {
+        const uint32_t prim = t < kFltMax ? best : S.nt;
+        const f3 hp = org + (t - kEps) * dir;
+        hits[pix] = hp.x;
+        hits[(size_t)npix + pix] = hp.y;
+        hits[2 * (size_t)npix + pix] = hp.z;
+        hits[3 * (size_t)npix + pix] = dir.x;
+        hits[4 * (size_t)npix + pix] = dir.y;
+        hits[5 * (size_t)npix + pix] = dir.z;
+        hits[6 * (size_t)npix + pix] = t;
+        hit_prims[pix] = prim;
+    }
+    add_work(ra.stats, active ? 1u : 0u, wk);
+}
+
 // ---------------------------------------------------------------------------
 // ambient occlusion
 // ---------------------------------------------------------------------------
@@ -519,9 +555,18 @@ inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlo
 }  // namespace
 
 void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
-                    uint32_t* hit_prims, unsigned long long* ray_count, hipStream_t s)
+                    uint32_t* hit_prims, unsigned long long* ray_count, hipStream_t s, uint32_t wave_steps)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
+    if (wave_steps) {
+        if (S.two_level)
+            hipLaunchKernelGGL(k_primary_wave<true>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, cam, ra, npix,
+                               hits, hit_prims, wave_steps, ray_count);
+        else
+            hipLaunchKernelGGL(k_primary_wave<false>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, cam, ra, npix,
+                               hits, hit_prims, wave_steps, ray_count);
+        return;
+    }
     if (S.two_level)
         hipLaunchKernelGGL(k_primary<true>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, cam, ra, npix,
                            hits, hit_prims, ray_count);

@@ -20,6 +20,11 @@ from . import _lib
 from ._lib import (GRID_HIERARCHY, GRID_TWO_LEVEL, GRID_UNIFORM, GRID_UNIFORM_EXACT, MODE_AO, MODE_PATH, MODE_PATH_MULTI,
                    MODE_SIMPLE, check)
 
+# timed frames that persistent_primary 2 spends choosing the camera-ray kernel
+# after the mesh, key frames, grid, camera or tuning change (kernels 0, 1, 3,
+# twice each); a caller that times frames renders these first
+CAMERA_KERNEL_TRIALS = 6
+
 _f32p = C.POINTER(C.c_float)
 _u32p = C.POINTER(C.c_uint32)
 _u8p = C.POINTER(C.c_uint8)

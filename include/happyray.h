@@ -107,6 +107,9 @@ typedef struct {
     double secondary_ms;     /* AO-ray kernel or path-tracing kernel             */
     double shade_ms;         /* AO shading kernel                                */
     uint64_t rays_traced;    /* rays traversed since the context was created     */
+    int primary_kernel;      /* camera-ray kernel of the last render: 0 one ray per
+                                lane, 1 persistent, 3 one ray per lane with
+                                wave-batched tests (-1 before the first render)   */
 } hr_timing;
 
 /* Work counters of the renders run with collect_stats = 1 and of all ray
@@ -287,12 +290,15 @@ hr_status hr_path_trace_dev_noalloc(hr_ctx* ctx, const float* d_org, const float
  * its launch shapes at compile time, src/RT/Algorithm/RayTracingKernels.h:34-43). */
 typedef struct {
     int persistent_primary;    /* camera rays: 0 one ray per lane, 1 persistent traversal
-                                  with dynamic ray fetch, 2 (default) whichever of the two
-                                  traced the camera rays faster: the first two timed frames
-                                  after the mesh, key frames, grid, camera or tuning change
-                                  run one each (the first waits for the second to finish),
-                                  later frames the faster.  Same results either way; the
-                                  instanced hierarchy takes 2 as 0                       */
+                                  with dynamic ray fetch, 3 one ray per lane with the
+                                  wave's candidate tests batched, 2 (default) whichever of
+                                  0, 1 and 3 traced the camera rays fastest: the first
+                                  six timed frames after the mesh, key frames, grid,
+                                  camera or tuning change run 0, 1, 3, 0, 1, 3 (each waits
+                                  for the one before it to finish), later frames the one
+                                  with the least time
+                                  (hr_timing.primary_kernel shows which ran).  Same results
+                                  either way; the instanced hierarchy takes 2 and 3 as 0 */
     int persistent_secondary;  /* AO rays and ray batches: 1 (default) persistent, 0 not */
     int refill;                /* idle lanes of a 64-lane wave that trigger a fetch,
                                   1..64 (default 56)                                     */

@@ -232,29 +232,36 @@ def test_persistent_matches_one_ray_per_lane(cluster, kind, lt, ll):
 
 
 def test_camera_kernel_choice(cluster):
-    """persistent_primary 2 (the default) runs each camera-ray kernel on one
-    of its first two frames and keeps the faster; every frame equals both
-    fixed choices bit for bit, before and after the choice and after a
-    camera change restarts it."""
+    """persistent_primary 2 (the default) runs the camera-ray kernels 0, 1, 3,
+    0, 1, 3 on its first six timed frames and then keeps the fastest; a camera
+    change starts the trials again.  Every frame equals every fixed choice bit
+    for bit."""
     print("automatic camera-ray kernel", flush=True)
     e = gpu_engine(cluster, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
     e.set_camera_dict(cluster.camera)
     assert e.tuning().persistent_primary == 2
-    ref = {}
-    for pp in (0, 1):
+    ref = None
+    for pp in (0, 1, 3):
         e.set_tuning(persistent_primary=pp)
         img = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0)
-        ref[pp] = (img, e.ao_occlusion(4)) + e.primary_hits()
-    for a, b in zip(ref[0], ref[1]):
-        np.testing.assert_array_equal(a, b)
+        assert e.timing().primary_kernel == pp
+        got = (img, e.ao_occlusion(4)) + e.primary_hits()
+        if ref is None:
+            ref = got
+        for a, b in zip(got, ref):
+            np.testing.assert_array_equal(a, b, err_msg=f"kernel {pp}")
     e.set_tuning(persistent_primary=2)
-    for i in range(5):
-        if i == 3:
+    used = []
+    for i in range(14):
+        if i == 8:
             e.set_camera_dict(cluster.camera)
         img = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=2.0)
+        used.append(e.timing().primary_kernel)
         got = (img, e.ao_occlusion(4)) + e.primary_hits()
-        for a, b in zip(got, ref[0]):
+        for a, b in zip(got, ref):
             np.testing.assert_array_equal(a, b, err_msg=f"frame {i}")
+    assert used[:6] == [0, 1, 3] * 2 and used[8:14] == [0, 1, 3] * 2, used
+    assert used[6] in (0, 1, 3) and used[7] == used[6], used
     assert e.tuning().persistent_primary == 2
     e.close()
 
