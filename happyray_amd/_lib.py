@@ -177,6 +177,9 @@ SIGNATURES = {
     "hr_synchronize": (C.c_int, [_vp]),
     "hr_render_frame_graph": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int,
                                          C.POINTER(RenderParams), _f32p]),
+    "hr_submit_frame_graph": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int,
+                                         C.POINTER(RenderParams)]),
+    "hr_get_framebuffer": (C.c_int, [_vp, _f32p]),
     "hr_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(Options)]),
     "hr_batch_destroy": (C.c_int, [_vp]),
     "hr_batch_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),

@@ -147,12 +147,33 @@ struct hr_ctx {
     int pp_choice = -1;                  // the fastest kernel once all are timed
     int last_pk = -1;                    // camera-ray kernel of the last render
 
-    // captured frames: a rebuild graph and a render graph, replayed per frame
-    hipGraphExec_t g_build = nullptr, g_render = nullptr;
+    // captured frames: a rebuild graph per slot of pinned read-back memory and
+    // one render graph, replayed per frame.  Up to two submitted frames are in
+    // flight, in alternate slots; pending[0] is the older.
+    struct FramePin {                        // pinned, one per slot
+        hr::FrameSel sel;                    // source of the frame selection
+        hr::BuildStatus status;              // build status after the rebuild
+        hr::GridParams params;               // grid parameters after the rebuild
+        uint32_t halt;                       // the halt word after the rebuild
+    };
+    struct Submitted {
+        uint32_t k1, k2;
+        float coeff;
+        int image;
+        int slot;
+    };
+    hipGraphExec_t g_build[2] = { nullptr, nullptr };
+    hipGraphExec_t g_render = nullptr;
     std::vector<unsigned char> graph_sig;    // state the graphs were recorded with
+    hr_mode graph_mode = HR_MODE_AO;         // the graphs' mode and render parameters
+    hr_render_params graph_params{};
     DevBuf<hr::FrameSel> frame_sel;
-    hr::FrameSel* h_frame = nullptr;         // pinned: source of the frame selection
-    hr::GridParams* h_params_pin = nullptr;  // pinned: grid parameters after a replay
+    DevBuf<uint32_t> halt;                   // sticky on the device: a replayed rebuild overflowed
+    FramePin* h_pin = nullptr;
+    hipEvent_t ev_g[2][3] = {};              // per slot: rebuild start, rebuild end, render end
+    Submitted pending[2] = {};
+    int npending = 0, next_slot = 0;
+    float graph_build_ms = 0.f, graph_render_ms = 0.f;   // the last completed replay
     bool graph_timed = false;
 
     // this device's copy of the allocation-free entry point's scratch array
@@ -212,10 +233,15 @@ hr_status set_device(hr_ctx* c)
     return HR_OK;
 }
 
+hr_status settle_frames(hr_ctx* c);
+
+// every entry point but hr_submit_frame_graph: the context is valid, its
+// device current, and the captured frames in flight completed
 hr_status check_ctx(hr_ctx* c)
 {
     if (!c) return fail(HR_ERR_ARG, "null context");
-    return set_device(c);
+    HR_TRY(set_device(c));
+    return settle_frames(c);
 }
 
 uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -448,9 +474,12 @@ void set_info(hr_ctx* c, const hr::BuildStatus& st)
 
 void drop_graphs(hr_ctx* c)
 {
-    if (c->g_build) (void)hipGraphExecDestroy(c->g_build);
+    for (hipGraphExec_t& g : c->g_build) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
     if (c->g_render) (void)hipGraphExecDestroy(c->g_render);
-    c->g_build = c->g_render = nullptr;
+    c->g_render = nullptr;
     c->graph_sig.clear();
 }
 
@@ -880,39 +909,173 @@ hr_status end_capture(hr_ctx* c, hipGraphExec_t* out)
     return HR_OK;
 }
 
-// Record the rebuild graph (frame selection upload, interpolation, build,
-// normals, status read-back into pinned memory) and the render graph
-// (first-image clear, primary rays, integrator).  No memset node is recorded.
+// Record the rebuild graph of each slot (frame selection upload from the
+// slot's pinned memory, interpolation, build, normals, the frame gate, and
+// read-back of the status, grid parameters and halt word into the slot) and
+// the render graph (first-image clear, primary rays, integrator).  No memset
+// node is recorded.
 hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 {
     drop_graphs(c);
     hr::FrameSel* fs = c->frame_sel.p;
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
-    HR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-    hipError_t e = hipMemcpyAsync(fs, c->h_frame, sizeof(hr::FrameSel), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) {
-        enqueue_build(c, fs);
-        if (c->has_normals) enqueue_normals(c, fs);
-        e = hipMemcpyAsync(c->h_status, c->status.p, sizeof(hr::BuildStatus), hipMemcpyDeviceToHost, c->stream);
-    }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(c->h_params_pin, c->params.p, sizeof(hr::GridParams), hipMemcpyDeviceToHost,
-                           c->stream);
-    hr_status st = end_capture(c, &c->g_build);
-    if (e != hipSuccess) st = fail(HR_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
-    if (st != HR_OK) {
-        drop_graphs(c);
-        return st;
+    for (int s = 0; s < 2; ++s) {
+        hr_ctx::FramePin* pin = c->h_pin + s;
+        HR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        hipError_t e = hipMemcpyAsync(fs, &pin->sel, sizeof(hr::FrameSel), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            enqueue_build(c, fs);
+            if (c->has_normals) enqueue_normals(c, fs);
+            hr::launch_frame_gate(c->status.p, c->halt.p, fs, c->stream);
+            e = hipMemcpyAsync(&pin->status, c->status.p, sizeof(hr::BuildStatus), hipMemcpyDeviceToHost,
+                               c->stream);
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&pin->params, c->params.p, sizeof(hr::GridParams), hipMemcpyDeviceToHost,
+                               c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&pin->halt, c->halt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+        hr_status st = end_capture(c, &c->g_build[s]);
+        if (e != hipSuccess) st = fail(HR_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+        if (st != HR_OK) {
+            drop_graphs(c);
+            return st;
+        }
     }
     HR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     hr::launch_clear_first(c->fb.p, 3 * (size_t)npix, fs, c->stream);
     const hr_status rs = enqueue_render(c, mode, 0, p, fs, false);
-    st = end_capture(c, &c->g_render);
+    const hr_status st = end_capture(c, &c->g_render);
     if (rs != HR_OK || st != HR_OK) {
         drop_graphs(c);
         return rs != HR_OK ? rs : st;
     }
     c->graph_sig = graph_signature(c, mode, p);
+    c->graph_mode = mode;
+    c->graph_params = *p;
+    return HR_OK;
+}
+
+// ---- captured frames in flight ----
+
+bool graphs_ready(hr_ctx* c, hr_mode mode, const hr_render_params* p)
+{
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    return c->g_build[0] && c->g_build[1] && c->g_render && npix == c->last_npix && primary_decided(c) &&
+           c->graph_sig == graph_signature(c, mode, p);
+}
+
+void select_frame(hr_ctx* c, const hr_ctx::Submitted& f)
+{
+    c->src1 = c->keyframes.p + (size_t)f.k1 * 3 * c->nv;
+    c->src2 = c->keyframes.p + (size_t)f.k2 * 3 * c->nv;
+    c->coeff = f.coeff;
+    c->k1 = f.k1;
+    c->k2 = f.k2;
+    c->normals_dirty = true;
+}
+
+// one frame without graphs (the build grows its buffers as needed); waits for
+// the build, not for the render
+hr_status frame_uncaptured(hr_ctx* c, const hr_ctx::Submitted& f, hr_mode mode, const hr_render_params* p)
+{
+    select_frame(c, f);
+    c->built = false;
+    HR_TRY(do_build(c));
+    HR_TRY(prepare_render(c, mode, p));
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    if (f.image == 0 || npix != c->last_npix) {
+        hr::fill_u32((uint32_t*)c->fb.p, 0u, 3 * (size_t)npix, nullptr, 1, c->stream);
+        c->last_npix = npix;
+    }
+    if (uses_normals(mode) && c->has_normals) {
+        enqueue_normals(c, nullptr);
+        c->normals_dirty = false;
+    }
+    HR_TRY(enqueue_render(c, mode, f.image, p, nullptr, true));
+    c->render_timed = true;
+    c->graph_timed = false;
+    return HR_OK;
+}
+
+// A replayed rebuild outgrew its buffers: that frame and every frame queued
+// behind it were skipped on the device (their render kernels returned at
+// entry).  Render them again in order without graphs, which grows the
+// buffers, and record the graphs anew.
+hr_status recover_frames(hr_ctx* c)
+{
+    HR_HIP(hipStreamSynchronize(c->stream));
+    hr_ctx::Submitted redo[2];
+    const int n = c->npending;
+    for (int i = 0; i < n; ++i) redo[i] = c->pending[i];
+    c->npending = 0;
+    drop_graphs(c);
+    hr::fill_u32(c->halt.p, 0u, 1, nullptr, 1, c->stream);
+    const hr_mode mode = c->graph_mode;
+    const hr_render_params p = c->graph_params;
+    for (int i = 0; i < n; ++i) HR_TRY(frame_uncaptured(c, redo[i], mode, &p));
+    HR_HIP(hipStreamSynchronize(c->stream));
+    resolve_primary_trial(c);
+    return record_graphs(c, mode, &p);
+}
+
+// waits for the older frame in flight and takes its results (grid info,
+// timing), or recovers it and the frames behind it
+hr_status complete_oldest(hr_ctx* c)
+{
+    const hr_ctx::Submitted f = c->pending[0];
+    HR_HIP(hipEventSynchronize(c->ev_g[f.slot][2]));
+    const hr_ctx::FramePin& pin = c->h_pin[f.slot];
+    if (pin.halt != 0u) return recover_frames(c);
+    c->h_params = pin.params;
+    set_info(c, pin.status);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_g[f.slot][0], c->ev_g[f.slot][1]) == hipSuccess) c->graph_build_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev_g[f.slot][1], c->ev_g[f.slot][2]) == hipSuccess) c->graph_render_ms = ms;
+    c->build_timed = c->render_timed = c->graph_timed = true;
+    c->built = true;
+    c->normals_dirty = false;
+    if (c->graph_mode == HR_MODE_AO) c->last_ao_samples = c->graph_params.ao_samples;
+    c->pending[0] = c->pending[1];
+    --c->npending;
+    return HR_OK;
+}
+
+// completes every frame in flight (called by every entry point but the
+// frame submission itself)
+hr_status settle_frames(hr_ctx* c)
+{
+    while (c->npending > 0) HR_TRY(complete_oldest(c));
+    return HR_OK;
+}
+
+// Queues one captured frame.  With the graphs recorded for this state the
+// frame is replayed into the free slot, after the frame two submissions back
+// has completed, and the call returns without waiting for it; otherwise the
+// frames in flight complete, this frame is rendered without graphs and the
+// graphs are recorded.
+hr_status submit_frame(hr_ctx* c, const hr_ctx::Submitted& f, hr_mode mode, const hr_render_params* p)
+{
+    if (c->npending == 2) HR_TRY(complete_oldest(c));
+    if (!graphs_ready(c, mode, p)) {
+        HR_TRY(settle_frames(c));
+        HR_TRY(frame_uncaptured(c, f, mode, p));
+        HR_HIP(hipStreamSynchronize(c->stream));
+        resolve_primary_trial(c);   // before the capture, which records no trial
+        return record_graphs(c, mode, p);
+    }
+    const int s = c->next_slot;
+    select_frame(c, f);
+    c->h_pin[s].sel = hr::FrameSel{ f.k1, f.k2, f.coeff, f.image, 0u };
+    HR_HIP(hipEventRecord(c->ev_g[s][0], c->stream));
+    HR_HIP(hipGraphLaunch(c->g_build[s], c->stream));
+    HR_HIP(hipEventRecord(c->ev_g[s][1], c->stream));
+    HR_HIP(hipGraphLaunch(c->g_render, c->stream));
+    HR_HIP(hipEventRecord(c->ev_g[s][2], c->stream));
+    hr_ctx::Submitted q = f;
+    q.slot = s;
+    c->pending[c->npending++] = q;
+    c->next_slot = s ^ 1;
     return HR_OK;
 }
 
@@ -1000,6 +1163,17 @@ hr_status hr::ctx_path_trace(hr_ctx* c, const float* hits, const uint32_t* prims
 // C ABI
 // ---------------------------------------------------------------------------
 
+namespace {
+hr_status check_frame_args(hr_ctx* c, uint32_t k1, uint32_t k2, hr_mode mode, int image_id,
+                           const hr_render_params* p)
+{
+    HR_TRY(no_hierarchy(c, "captured frames"));
+    if (c->nframes == 0) return fail(HR_ERR_STATE, "captured frames need resident key frames");
+    if (k1 >= c->nframes || k2 >= c->nframes) return fail(HR_ERR_ARG, "key frame index out of range");
+    return check_render_args(c, mode, image_id, p);
+}
+}  // namespace
+
 extern "C" {
 
 const char* hr_last_error(void) { return g_last_error.c_str(); }
@@ -1032,11 +1206,17 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         if ((s = c->pool.reserve(1)) != HR_OK) break;
         e = hipHostMalloc((void**)&c->h_status, sizeof(hr::BuildStatus), hipHostMallocDefault);
         if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
-        e = hipHostMalloc((void**)&c->h_frame, sizeof(hr::FrameSel), hipHostMallocDefault);
+        e = hipHostMalloc((void**)&c->h_pin, 2 * sizeof(hr_ctx::FramePin), hipHostMallocDefault);
         if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
-        e = hipHostMalloc((void**)&c->h_params_pin, sizeof(hr::GridParams), hipHostMallocDefault);
-        if (e != hipSuccess) { s = fail(HR_ERR_OOM, hipGetErrorString(e)); break; }
+        std::memset((void*)c->h_pin, 0, 2 * sizeof(hr_ctx::FramePin));
         if ((s = c->frame_sel.reserve(1)) != HR_OK) break;
+        if ((s = c->halt.reserve(1)) != HR_OK) break;
+        for (auto& slot : c->ev_g)
+            for (hipEvent_t& ev : slot) {
+                e = hipEventCreate(&ev);
+                if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+            }
+        if (s != HR_OK) break;
         if ((s = noalloc_scratch(c->device, &c->noalloc)) != HR_OK) break;
         hipEvent_t* evs[6] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1, &c->ev_p1, &c->ev_p2 };
         for (hipEvent_t* ev : evs) {
@@ -1047,6 +1227,8 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         e = hipMemsetAsync(c->ray_count.p, 0, sizeof(unsigned long long), c->stream);
         if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
         e = hipMemsetAsync(c->work.p, 0, 6 * sizeof(unsigned long long), c->stream);
+        if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+        e = hipMemsetAsync(c->halt.p, 0, sizeof(uint32_t), c->stream);
         if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
     } while (0);
     if (s != HR_OK) {
@@ -1095,9 +1277,12 @@ hr_status hr_destroy(hr_ctx* c)
     c->pool.release();
     drop_graphs(c);
     c->frame_sel.release();
+    c->halt.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
-    if (c->h_frame) (void)hipHostFree(c->h_frame);
-    if (c->h_params_pin) (void)hipHostFree(c->h_params_pin);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
+    for (auto& slot : c->ev_g)
+        for (hipEvent_t ev : slot)
+            if (ev) (void)hipEventDestroy(ev);
     hipEvent_t evs[6] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1, c->ev_p1, c->ev_p2 };
     for (hipEvent_t ev : evs)
         if (ev) (void)hipEventDestroy(ev);
@@ -1358,58 +1543,36 @@ hr_status hr_render_frame_graph(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff
                                 const hr_render_params* p, float* rgb_out)
 {
     HR_TRY(check_ctx(c));
-    HR_TRY(no_hierarchy(c, "captured frames"));
-    if (c->nframes == 0) return fail(HR_ERR_STATE, "captured frames need resident key frames");
-    if (k1 >= c->nframes || k2 >= c->nframes) return fail(HR_ERR_ARG, "key frame index out of range");
-    HR_TRY(check_render_args(c, mode, image_id, p));
-    c->src1 = c->keyframes.p + (size_t)k1 * 3 * c->nv;
-    c->src2 = c->keyframes.p + (size_t)k2 * 3 * c->nv;
-    c->coeff = coeff;
-    c->k1 = k1;
-    c->k2 = k2;
-    c->normals_dirty = true;
-    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
-    const size_t fb_bytes = 3 * (size_t)npix * sizeof(float);
-    if (c->g_build && c->g_render && npix == c->last_npix && primary_decided(c) &&
-        c->graph_sig == graph_signature(c, mode, p)) {
-        *c->h_frame = hr::FrameSel{ k1, k2, coeff, image_id };
-        HR_HIP(hipEventRecord(c->ev_b0, c->stream));
-        HR_HIP(hipGraphLaunch(c->g_build, c->stream));
-        HR_HIP(hipEventRecord(c->ev_b1, c->stream));
-        HR_HIP(hipGraphLaunch(c->g_render, c->stream));
-        HR_HIP(hipEventRecord(c->ev_r1, c->stream));
-        if (rgb_out) HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, fb_bytes, hipMemcpyDeviceToHost, c->stream));
+    HR_TRY(check_frame_args(c, k1, k2, mode, image_id, p));
+    HR_TRY(submit_frame(c, hr_ctx::Submitted{ k1, k2, coeff, image_id, 0 }, mode, p));
+    HR_TRY(settle_frames(c));
+    if (rgb_out) {
+        const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+        HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, 3 * (size_t)npix * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
         HR_HIP(hipStreamSynchronize(c->stream));
-        if (c->h_status->overflow == 0u) {
-            c->h_params = *c->h_params_pin;
-            set_info(c, *c->h_status);
-            c->build_timed = c->render_timed = c->graph_timed = true;
-            c->normals_dirty = false;
-            if (mode == HR_MODE_AO) c->last_ao_samples = p->ao_samples;
-            return HR_OK;
-        }
-        // the rebuild outgrew its buffers: render this frame without the
-        // graphs (the build grows them) and record again
-        drop_graphs(c);
     }
-    c->built = false;
-    HR_TRY(do_build(c));
-    HR_TRY(prepare_render(c, mode, p));
-    if (image_id == 0 || npix != c->last_npix) {
-        hr::fill_u32((uint32_t*)c->fb.p, 0u, 3 * (size_t)npix, nullptr, 1, c->stream);
-        c->last_npix = npix;
-    }
-    if (uses_normals(mode) && c->has_normals) {
-        enqueue_normals(c, nullptr);
-        c->normals_dirty = false;
-    }
-    HR_TRY(enqueue_render(c, mode, image_id, p, nullptr, true));
-    c->render_timed = true;
-    c->graph_timed = false;
-    if (rgb_out) HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, fb_bytes, hipMemcpyDeviceToHost, c->stream));
+    return HR_OK;
+}
+
+hr_status hr_submit_frame_graph(hr_ctx* c, uint32_t k1, uint32_t k2, float coeff, hr_mode mode, int image_id,
+                                const hr_render_params* p)
+{
+    if (!c) return fail(HR_ERR_ARG, "null context");
+    HR_TRY(set_device(c));
+    HR_TRY(check_frame_args(c, k1, k2, mode, image_id, p));
+    return submit_frame(c, hr_ctx::Submitted{ k1, k2, coeff, image_id, 0 }, mode, p);
+}
+
+hr_status hr_get_framebuffer(hr_ctx* c, float* rgb_out)
+{
+    HR_TRY(check_ctx(c));
+    if (!c->last_npix) return fail(HR_ERR_STATE, "nothing rendered");
+    if (!rgb_out) return fail(HR_ERR_ARG, "null output pointer");
+    HR_HIP(hipMemcpyAsync(rgb_out, c->fb.p, 3 * (size_t)c->last_npix * sizeof(float), hipMemcpyDeviceToHost,
+                          c->stream));
     HR_HIP(hipStreamSynchronize(c->stream));
-    resolve_primary_trial(c);   // before the capture, which records no trial
-    return record_graphs(c, mode, p);
+    return HR_OK;
 }
 
 hr_status hr_get_primary_hits(hr_ctx* c, float* planes7, uint32_t* prims)
@@ -1671,14 +1834,18 @@ hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
     HR_HIP(hipStreamSynchronize(c->stream));
     out->build_ms = out->render_ms = out->primary_ms = out->secondary_ms = out->shade_ms = 0.0;
     float ms = 0.f;
-    if (c->build_timed && hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1) == hipSuccess) out->build_ms = ms;
     if (c->graph_timed) {
-        if (hipEventElapsedTime(&ms, c->ev_b1, c->ev_r1) == hipSuccess) out->render_ms = ms;
-    } else if (c->render_timed) {
-        if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_r1) == hipSuccess) out->render_ms = ms;
-        if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess) out->primary_ms = ms;
-        if (hipEventElapsedTime(&ms, c->ev_p1, c->ev_p2) == hipSuccess) out->secondary_ms = ms;
-        if (hipEventElapsedTime(&ms, c->ev_p2, c->ev_r1) == hipSuccess) out->shade_ms = ms;
+        // the last completed replay: rebuild graph, render graph
+        out->build_ms = c->graph_build_ms;
+        out->render_ms = c->graph_render_ms;
+    } else {
+        if (c->build_timed && hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1) == hipSuccess) out->build_ms = ms;
+        if (c->render_timed) {
+            if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_r1) == hipSuccess) out->render_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess) out->primary_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev_p1, c->ev_p2) == hipSuccess) out->secondary_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev_p2, c->ev_r1) == hipSuccess) out->shade_ms = ms;
+        }
     }
     unsigned long long rays = 0;
     HR_HIP(hipMemcpy(&rays, c->ray_count.p, sizeof rays, hipMemcpyDeviceToHost));

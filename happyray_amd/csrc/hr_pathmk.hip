@@ -89,6 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_path(TraceScene S, RenderArgs 
                                                        const uint32_t* __restrict__ hit_prims, Shading sh,
                                                        PathBuffers pb, unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = pix < npix;   // every lane stays for add_work
@@ -264,6 +265,7 @@ template <bool kTwo>
 __global__ __launch_bounds__(kBlock) void k_path_shadow(TraceScene S, RenderArgs ra, uint32_t npix, Shading sh,
                                                         PathBuffers pb, unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = pix < npix;   // every lane stays for add_work
@@ -317,6 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_path_shadow(TraceScene S, RenderArgs
 __global__ __launch_bounds__(kBlock) void k_path_direct(TraceScene S, RenderArgs ra, uint32_t npix, Shading sh,
                                                         PathBuffers pb)
 {
+    if (frame_skipped(ra)) return;
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     const float t = pb.end[6 * (size_t)npix + pix];
@@ -356,6 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_path_direct(TraceScene S, RenderArgs
 __global__ __launch_bounds__(kBlock) void k_path_blend(RenderArgs ra, uint32_t npix, PathBuffers pb,
                                                        const float* __restrict__ fb)
 {
+    if (frame_skipped(ra)) return;
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     const float nw = 1.f / (float)(image_of(ra) + 1);
@@ -372,6 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_path_blend(RenderArgs ra, uint32_t n
 __global__ __launch_bounds__(kBlock) void k_path_merge(RenderArgs ra, int resx, int resy, PathBuffers pb,
                                                        float* __restrict__ fb)
 {
+    if (frame_skipped(ra)) return;
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= (uint32_t)resx * (uint32_t)resy) return;
     const int x = (int)(pix % (uint32_t)resx), y = (int)(pix / (uint32_t)resx);

@@ -33,6 +33,8 @@ struct GenPrimary {
     float* hits;
     uint32_t* hit_prims;
 
+    __device__ bool skipped() const { return frame_skipped(ra); }
+
     __device__ bool start(uint32_t j, RayState& r, const TraceScene&) const
     {
         const f3 d0 = primary_direction(cam, ra, j, npix);
@@ -63,6 +65,8 @@ struct GenAO {
     const float* hits;
     const uint32_t* hit_prims;
     uint8_t* occ;
+
+    __device__ bool skipped() const { return frame_skipped(ra); }
 
     // AmbientOcclusionRayGenerator::operator() (RayGenerators.h:281-343)
     __device__ bool start(uint32_t j, RayState& r, const TraceScene& S) const
@@ -122,6 +126,7 @@ struct GenBatch {
     uint8_t* occ_out;
     float* hits;
 
+    __device__ bool skipped() const { return false; }
     __device__ bool start(uint32_t j, RayState& r, const TraceScene&) const
     {
         set_ray(r, mk(org[j], org[(size_t)n + j], org[2 * (size_t)n + j]),
@@ -163,6 +168,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     TraceScene S, Gen gen, uint32_t nrays, uint32_t* __restrict__ pool, uint32_t refill,
     unsigned long long* ray_count, unsigned long long* stats)
 {
+    if (gen.skipped()) return;
     const GridRegs G = load_grid(S.gp);
     const uint64_t lt = __lanemask_lt();
     RayState r;
@@ -222,6 +228,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     TraceScene S, Gen gen, uint32_t nrays, uint32_t* __restrict__ pool, uint32_t refill, uint32_t steps_max,
     unsigned long long* ray_count, unsigned long long* stats)
 {
+    if (gen.skipped()) return;
     __shared__ WaveTests<kK> lds[kBlock / kWave];
     WaveTests<kK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
     const GridRegs G = load_grid(S.gp);
@@ -589,6 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_persist_pt(TraceScene S, RenderArgs 
                                                        uint32_t refill, uint32_t shade_min,
                                                        unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     const GridRegs G = load_grid(S.gp);
     const uint64_t lt = __lanemask_lt();
     RayState r;

@@ -38,6 +38,11 @@ struct RenderArgs {
 };
 
 __device__ __forceinline__ int image_of(const RenderArgs& ra) { return ra.frame ? ra.frame->image : ra.image; }
+// a captured frame whose rebuild overflowed (or that follows one that did
+// before the host caught it) is skipped: each render kernel returns at entry,
+// the same for the whole grid, so the frame leaves the framebuffer, the hit and
+// path buffers and the ray and work counters as they were
+__device__ __forceinline__ bool frame_skipped(const RenderArgs& ra) { return ra.frame && ra.frame->skip; }
 
 struct Shading {
     const float* mats;          // 12 floats per PhongMaterial
@@ -70,6 +75,10 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
 // SoA key-frame blocks of nn normals
 // zero n floats of fb when fs->image == 0 (read on the device)
 void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s);
+// after a captured frame's rebuild: halt |= (the rebuild overflowed), and
+// fs->skip = halt.  halt is sticky until the host clears it, so every frame
+// queued behind an overflowed one is skipped too.
+void launch_frame_gate(const BuildStatus* st, uint32_t* halt, FrameSel* fs, hipStream_t s);
 // simpleShade (src/RT/Integrator/SimpleIntegrator.h:42-125): barycentric colour
 // times |cos| at each primary hit, background (0.15, 0.2, 0.3), running mean
 void launch_simple_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,

@@ -47,6 +47,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     TraceScene S, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
     uint32_t* __restrict__ hit_prims, unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = pix < npix;
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     TraceScene S, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
     uint32_t* __restrict__ hit_prims, uint32_t steps_max, unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     __shared__ WaveTests<kBatchK> lds[kBlock / kWave];
     WaveTests<kBatchK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,6 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_rays(TraceScene S, RenderArgs ra,
                                                     uint8_t* __restrict__ occ,
                                                     unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     const uint32_t S_ = (uint32_t)ra.ao_samples;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = j < npix * S_;
@@ -166,6 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_shade(TraceScene S, RenderArgs ra
                                                      const uint8_t* __restrict__ occ, Shading sh,
                                                      float* __restrict__ fb)
 {
+    if (frame_skipped(ra)) return;
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     const uint32_t S_ = (uint32_t)ra.ao_samples;
@@ -233,6 +237,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                     float* __restrict__ fb, uint32_t steps_max,
                                                     unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     __shared__ WaveTests<kBatchK> lds[kBlock / kWave];
     __shared__ float light_terms[6][kBlock];   // a vertex's light terms wait here across its shadow ray
     WaveTests<kBatchK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
@@ -323,6 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
                                                const uint32_t* __restrict__ hit_prims, Shading sh,
                                                float* __restrict__ fb, unsigned long long* ray_count)
 {
+    if (frame_skipped(ra)) return;
     __shared__ float sub[kSubTable];
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = pix < npix;
@@ -386,6 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_simple_shade(TraceScene S, RenderArg
                                                          const uint32_t* __restrict__ hit_prims,
                                                          float* __restrict__ fb)
 {
+    if (frame_skipped(ra)) return;
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     const float t = hits[6 * (size_t)npix + pix];
@@ -509,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_frame_normals(const float* __restric
 __global__ __launch_bounds__(kBlock) void k_clear_first(float* __restrict__ fb, size_t n,
                                                         const FrameSel* __restrict__ fs)
 {
-    if (fs->image != 0) return;
+    if (fs->image != 0 || fs->skip) return;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         fb[i] = 0.f;
 }
@@ -635,6 +642,20 @@ void launch_ao_dump(const TraceScene& S, const CameraDev& cam, const RenderArgs&
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
     const size_t n = (size_t)npix * ((size_t)ra.ao_samples + 1);
     hipLaunchKernelGGL(k_ao_dump, dim3(blocks_for(n)), dim3(kBlock), 0, s, S, cam, ra, npix, hits, hit_prims, out);
+}
+
+__global__ void k_frame_gate(const BuildStatus* __restrict__ st, uint32_t* __restrict__ halt,
+                             FrameSel* __restrict__ fs)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t h = *halt | (st->overflow != 0u ? 1u : 0u);
+    *halt = h;
+    fs->skip = h;
+}
+
+void launch_frame_gate(const BuildStatus* st, uint32_t* halt, FrameSel* fs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_frame_gate, dim3(1), dim3(64), 0, s, st, halt, fs);
 }
 
 void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s)

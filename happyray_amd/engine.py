@@ -259,6 +259,23 @@ class Engine:
               "hr_render_frame_graph")
         return out
 
+    def submit_frame_graph(self, k1, k2, coeff, image_id=0, mode=MODE_PATH, jitter=1, ao_samples=4,
+                           ao_dist=2.0, pt_depth=4, collect_stats=False):
+        """The same frame as render_frame_graph, queued without waiting for it:
+        returns once the frame submitted two calls earlier has finished.  Any
+        other call on the engine (framebuffer(), synchronize(), ...) first
+        completes the frames in flight."""
+        p = _lib.RenderParams(int(jitter), int(ao_samples), float(ao_dist), int(pt_depth),
+                              int(bool(collect_stats)))
+        check(self._L.hr_submit_frame_graph(self._h, int(k1), int(k2), float(coeff), mode, int(image_id),
+                                            C.byref(p)), "hr_submit_frame_graph")
+
+    def framebuffer(self):
+        """The framebuffer after the last render, (res_x*res_y, 3) float32."""
+        out = np.zeros((self._res[0] * self._res[1], 3), np.float32)
+        check(self._L.hr_get_framebuffer(self._h, _f(out)), "hr_get_framebuffer")
+        return out
+
     def primary_hits(self):
         n = self._res[0] * self._res[1]
         planes = np.zeros((7, n), np.float32)

@@ -238,12 +238,33 @@ hr_status hr_render(hr_ctx* ctx, hr_mode mode, int image_id, const hr_render_par
  * render) and replayed by later calls, the frame's (k1, k2, coeff, image_id)
  * reaching the kernels through device memory.  The graphs are recorded again when
  * the mode, render parameters, grid settings, camera, light count, geometry,
- * tuning or any buffer changes.  A replayed rebuild that outgrows its buffers is
- * detected after the replay; the frame is then rendered again without graphs
- * while the buffers grow.  hr_get_timing reports the rebuild graph as build_ms
- * and the render graph as render_ms. */
+ * tuning or any buffer changes.  A replayed rebuild that outgrows its buffers
+ * makes the device skip that frame's render (every render kernel returns at
+ * entry, so the framebuffer and counters stay as they were); the host finds it
+ * after the replay and renders the frame again without graphs while the
+ * buffers grow.  hr_get_timing reports the rebuild graph as build_ms and the
+ * render graph as render_ms. */
 hr_status hr_render_frame_graph(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coeff, hr_mode mode,
                                 int image_id, const hr_render_params* params, float* rgb_out);
+/* Captured frames without waiting (no reference counterpart: the reference's
+ * renderFrame returns after the frame, src/RT/RTEngine.cu:398-442).  The same
+ * frame as hr_render_frame_graph with rgb_out NULL, but a replay is only
+ * queued: the call returns once the frame submitted two calls earlier has
+ * finished, so the next frame's rebuild follows this frame's render on the GPU
+ * with no host round trip between them.  When the graphs must be recorded
+ * (first frame, changed state) the call renders without graphs and waits, as
+ * hr_render_frame_graph does.  Every other call on the context first completes
+ * the frames in flight, in order: it takes their grid info and timing, and a
+ * frame whose replayed rebuild overflowed (skipped on the device, together
+ * with any frame queued behind it) is rendered again without graphs then, so
+ * the running mean over images stays exactly that of hr_render_frame_graph;
+ * errors of that work are returned by that call.  hr_synchronize completes
+ * them all. */
+hr_status hr_submit_frame_graph(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coeff, hr_mode mode,
+                                int image_id, const hr_render_params* params);
+/* The framebuffer after the last render (3*res_x*res_y floats), as rgb_out of
+ * hr_render receives it; completes the captured frames in flight first. */
+hr_status hr_get_framebuffer(hr_ctx* ctx, float* rgb_out);
 /* Primary-ray hit buffer of the last render (SimpleRayBuffer, RayBuffers.h:34-195):
  * 7 SoA float planes (hit point xyz, direction xyz, t) and primitive ids. */
 hr_status hr_get_primary_hits(hr_ctx* ctx, float* planes7, uint32_t* prims);

@@ -63,10 +63,9 @@ def test_frame_graph_matches_uncaptured_frames(mode):
     e.close()
 
 
-def test_frame_graph_regrows_after_overflow():
-    """Key frame 1 turns small triangles into large ones: the replayed rebuild
-    overflows the pair buffers sized for key frame 0; the frame is rendered
-    again without graphs and the graphs are recorded anew."""
+def growing_soup():
+    """Key frame 1 turns the small triangles of key frame 0 into large ones, so
+    a rebuild of key frame 1 outgrows buffers sized for key frame 0."""
     rng = np.random.default_rng(21)
     n = 3000
     c = rng.uniform(0, 10, (n, 1, 3))
@@ -79,10 +78,22 @@ def test_frame_graph_regrows_after_overflow():
                                res=(48, 40)),
                           [scenes.area_light((2.0, 14.0, 2.0), (6.0, 0.0, 0.0), (0.0, 0.0, 6.0),
                                              (20.0, 20.0, 20.0))])
+    return sc, keys
+
+
+# frames of growing_soup: the third outgrows the buffers the graphs were recorded with
+GROWING_FRAMES = [(0, 0, 0.0), (0, 0, 0.0), (1, 1, 0.0), (1, 1, 0.0), (0, 1, 0.5), (0, 0, 0.0)]
+
+
+def test_frame_graph_regrows_after_overflow():
+    """The replayed rebuild of key frame 1 overflows the pair buffers sized
+    for key frame 0; the frame is rendered again without graphs and the graphs
+    are recorded anew."""
+    sc, keys = growing_soup()
     for kind, lt, ll in ((GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)):
         g = make_engine(sc, keys, kind, lt, ll)
         e = make_engine(sc, keys, kind, lt, ll)
-        for i, (k1, k2, cf) in enumerate([(0, 0, 0.0), (0, 0, 0.0), (1, 1, 0.0), (1, 1, 0.0), (0, 1, 0.5)]):
+        for i, (k1, k2, cf) in enumerate(GROWING_FRAMES):
             print(f"graph regrowth kind={kind} frame={i}", flush=True)
             fb_g = g.render_frame_graph(k1, k2, cf, image_id=0, mode=MODE_PATH, pt_depth=2)
             e.set_frame(k1, k2, cf)
@@ -90,5 +101,98 @@ def test_frame_graph_regrows_after_overflow():
             fb_e = e.render_frame(0, MODE_PATH, pt_depth=2)
             np.testing.assert_array_equal(fb_g, fb_e)
             grids_equal(g, e)
+        g.close()
+        e.close()
+
+
+@pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_PATH_MULTI])
+def test_overflowed_replay_keeps_running_mean(mode):
+    """A replay whose rebuild overflows is skipped on the device, so the
+    running mean over images 0..5 and the traced-ray count equal those of the
+    same frames run call by call."""
+    sc, keys = growing_soup()
+    kw = dict(jitter=1, pt_depth=2) if mode != MODE_AO else dict(jitter=1, ao_samples=4, ao_dist=3.0)
+    for kind, lt, ll in ((GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)):
+        g = make_engine(sc, keys, kind, lt, ll)
+        e = make_engine(sc, keys, kind, lt, ll)
+        for i, (k1, k2, cf) in enumerate(GROWING_FRAMES):
+            print(f"running mean across overflow mode={mode} kind={kind} image={i}", flush=True)
+            fb_g = g.render_frame_graph(k1, k2, cf, image_id=i, mode=mode, **kw)
+            e.set_frame(k1, k2, cf)
+            e.build_acc_struct()
+            fb_e = e.render_frame(i, mode, **kw)
+            np.testing.assert_array_equal(fb_g, fb_e)
+            grids_equal(g, e)
+        assert g.timing().rays_traced == e.timing().rays_traced
+        g.close()
+        e.close()
+
+
+def run_submitted(g, frames, mode, kw, check_at=()):
+    """Submits the frames without waiting; returns the framebuffers read at
+    the frame indices in check_at (each read completes the frames in flight)."""
+    got = {}
+    for i, (k1, k2, cf) in enumerate(frames):
+        g.submit_frame_graph(k1, k2, cf, image_id=i, mode=mode, **kw)
+        if i in check_at:
+            got[i] = g.framebuffer()
+    return got
+
+
+def run_called(e, frames, mode, kw):
+    fbs = []
+    for i, (k1, k2, cf) in enumerate(frames):
+        e.set_frame(k1, k2, cf)
+        e.build_acc_struct()
+        fbs.append(e.render_frame(i, mode, **kw))
+    return fbs
+
+
+@pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO])
+def test_submitted_frames_match_called_frames(mode):
+    """Frames queued by hr_submit_frame_graph (two in flight) give the same
+    running mean, grid and ray count as frames run call by call, read back in
+    the middle of the sequence and at its end."""
+    sc, keys = scenes.heightfield_keyframes(quads=64, frames=5, res=(64, 48))
+    kw = dict(jitter=1, pt_depth=3) if mode != MODE_AO else dict(jitter=1, ao_samples=4, ao_dist=5.0)
+    frames = [O.frame_ids(t, 0.4, 5) for t in (0.0, 0.4, 1.7, 3.9, 2.2, 0.1, 4.5, 1.1, 2.9, 3.3)]
+    g = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    e = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    print(f"submitted frames mode={mode}", flush=True)
+    got = run_submitted(g, frames, mode, kw, check_at=(4,))
+    g.synchronize()
+    last = g.framebuffer()
+    want = run_called(e, frames, mode, kw)
+    np.testing.assert_array_equal(got[4], want[4])
+    np.testing.assert_array_equal(last, want[-1])
+    grids_equal(g, e)
+    assert g.timing().rays_traced == e.timing().rays_traced
+    tm = g.timing()
+    assert tm.build_ms > 0 and tm.render_ms > 0
+    g.close()
+    e.close()
+
+
+def test_submitted_frames_recover_from_overflow():
+    """An overflowing replay with another frame queued behind it: both are
+    skipped on the device and rendered again when the host completes them
+    (in a read of the framebuffer, or in the next submission), so the running
+    mean equals that of the frames run call by call."""
+    sc, keys = growing_soup()
+    kw = dict(jitter=1, pt_depth=2)
+    frames = GROWING_FRAMES + [(1, 1, 0.0), (0, 1, 0.25)]
+    last = len(frames) - 1
+    cases = [(kind, lt, ll, at) for kind, lt, ll in ((GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0))
+             for at in ((3, last), (last,))]
+    for kind, lt, ll, at in cases:
+        print(f"submitted frames across overflow kind={kind} reads at {at}", flush=True)
+        g = make_engine(sc, keys, kind, lt, ll)
+        e = make_engine(sc, keys, kind, lt, ll)
+        got = run_submitted(g, frames, MODE_PATH, kw, check_at=at)
+        want = run_called(e, frames, MODE_PATH, kw)
+        for i in at:
+            np.testing.assert_array_equal(got[i], want[i])
+        grids_equal(g, e)
+        assert g.timing().rays_traced == e.timing().rays_traced
         g.close()
         e.close()
