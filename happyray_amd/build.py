@@ -28,6 +28,7 @@ FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-Werror=uninitialized", "-Werror=sometimes-uninitialized",
     "-I", INCLUDE, "-I", CSRC,
 ]
 
@@ -39,7 +40,8 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-I", INCLUDE, "-I", CSRC]
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Werror=uninitialized", "-I", INCLUDE,
+             "-I", CSRC]
 
 
 def _compile(src):

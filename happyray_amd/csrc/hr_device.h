@@ -40,15 +40,17 @@ struct BuildStatus {
 
 // The per-frame selection of a captured frame (HIP graph): key frames, the
 // interpolation coefficient and the image id, read by the kernels from device
-// memory so that one recorded graph serves every frame.  skip is set on the
-// device after the frame's rebuild when that rebuild, or the rebuild of an
-// earlier frame since the host last cleared the halt word, outgrew its
-// buffers: the frame's render kernels then return at entry.
+// memory so that one recorded graph serves every frame.  seq numbers the
+// context's captured frames.  skip is set on the device after the frame's
+// rebuild when that rebuild, or the rebuild of an earlier frame since the host
+// last reset the halt word, outgrew its buffers: the frame's render kernels
+// then return at entry.
 struct FrameSel {
     uint32_t k1, k2;
     float coeff;
     int image;
     uint32_t skip;
+    uint32_t seq;
 };
 
 constexpr uint32_t kOverflowCells = 1u;

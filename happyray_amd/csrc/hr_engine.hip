@@ -137,7 +137,7 @@ struct hr_ctx {
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
-    hr_tuning tuning{ 2, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2 };
+    hr_tuning tuning{ 2, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2, 2 };
     // persistent_primary 2: the camera-ray kernel is chosen by timing two frames
     // of each (0 one ray per lane, 1 persistent, 3 wave-batched tests) after the
     // scene, grid or camera changes; results are the same either way
@@ -147,8 +147,9 @@ struct hr_ctx {
     int pp_choice = -1;                  // the fastest kernel once all are timed
     int last_pk = -1;                    // camera-ray kernel of the last render
 
-    // captured frames: a rebuild graph per slot of pinned read-back memory and
-    // one render graph, replayed per frame.  Up to two submitted frames are in
+    // captured frames: per slot (0, 1) a rebuild graph, which reads the frame
+    // selection from and writes its status into the slot's pinned memory, and a
+    // render graph, replayed per frame.  Up to two submitted frames are in
     // flight, in alternate slots; pending[0] is the older.
     struct FramePin {                        // pinned, one per slot
         hr::FrameSel sel;                    // source of the frame selection
@@ -161,20 +162,59 @@ struct hr_ctx {
         float coeff;
         int image;
         int slot;
+        uint32_t seq;
     };
     hipGraphExec_t g_build[2] = { nullptr, nullptr };
-    hipGraphExec_t g_render = nullptr;
-    std::vector<unsigned char> graph_sig;    // state the graphs were recorded with
+    hipGraphExec_t g_render[2] = { nullptr, nullptr };
+    std::vector<unsigned char> graph_sig[2]; // per lane: state the graphs were recorded with
     hr_mode graph_mode = HR_MODE_AO;         // the graphs' mode and render parameters
     hr_render_params graph_params{};
-    DevBuf<hr::FrameSel> frame_sel;
-    DevBuf<uint32_t> halt;                   // sticky on the device: a replayed rebuild overflowed
+    int graph_lanes = 1;                     // lanes of the recorded graphs (slot s runs on lane s
+                                             // with two, on lane 0 with one)
+    DevBuf<uint32_t> halt;                   // least sequence number of an overflowed replay
     FramePin* h_pin = nullptr;
-    hipEvent_t ev_g[2][3] = {};              // per slot: rebuild start, rebuild end, render end
+    hipEvent_t ev_g[2][4] = {};              // per slot: rebuild start, rebuild end, render end,
+                                             // frame end (after its blend)
+    hipEvent_t ev_switch = nullptr;          // the lane left at the last lane switch: its work so far
     Submitted pending[2] = {};
     int npending = 0, next_slot = 0;
+    uint32_t frame_seq = 0;
     float graph_build_ms = 0.f, graph_render_ms = 0.f;   // the last completed replay
     bool graph_timed = false;
+
+    // Frame lanes: with two, captured frames alternate between two sets of
+    // per-frame buffers, each with its own stream, so that one frame's rebuild
+    // and render overlap the other's; each lane renders into its own buffer
+    // (rad) and k_frame_blend blends it into fb in frame order.  The fields
+    // above and below named in LaneFields hold lane `lane`; `stash` holds the
+    // other lane's.  Everything else (geometry, materials, lights, camera,
+    // framebuffer, counters) is shared.
+    DevBuf<hr::FrameSel> frame_sel;
+    DevBuf<float> rad;                       // a lane's frame values (3 floats per pixel)
+    DevBuf<unsigned long long> frame_rays;   // a lane's rays of its frame, added to ray_count by the blend
+    struct LaneFields {
+        hipStream_t stream = nullptr;
+        DevBuf<hr::GridParams> params;
+        DevBuf<hr::BuildStatus> status;
+        DevBuf<uint32_t> bounds_part;
+        DevBuf<float4> rec;
+        DevBuf<uint32_t> wave_counts, wave_offsets, keys, vals, keys_sorted, top_vals_sorted, prims, leaf_counts;
+        DevBuf<uint2> cells, leaves;
+        DevBuf<uint32_t> scan_chain, sort_chain, sort_keys_tmp, sort_vals_tmp, sort_hist;
+        uint64_t cells_cap = 0;
+        uint32_t pairs_cap = 0, leaves_cap = 0, leaf_pairs_cap = 0;
+        DevBuf<float> verts;
+        DevBuf<float4> normals;
+        DevBuf<float> hits;
+        DevBuf<uint32_t> hit_prims;
+        DevBuf<uint8_t> occ;
+        DevBuf<uint32_t> pool;
+        DevBuf<hr::FrameSel> frame_sel;
+        DevBuf<float> rad;
+        DevBuf<unsigned long long> frame_rays;
+    };
+    LaneFields stash;
+    int lane = 0;
 
     // this device's copy of the allocation-free entry point's scratch array
     unsigned char* noalloc = nullptr;
@@ -474,13 +514,94 @@ void set_info(hr_ctx* c, const hr::BuildStatus& st)
 
 void drop_graphs(hr_ctx* c)
 {
-    for (hipGraphExec_t& g : c->g_build) {
-        if (g) (void)hipGraphExecDestroy(g);
-        g = nullptr;
+    for (int s = 0; s < 2; ++s) {
+        if (c->g_build[s]) (void)hipGraphExecDestroy(c->g_build[s]);
+        if (c->g_render[s]) (void)hipGraphExecDestroy(c->g_render[s]);
+        c->g_build[s] = c->g_render[s] = nullptr;
+        c->graph_sig[s].clear();
     }
-    if (c->g_render) (void)hipGraphExecDestroy(c->g_render);
-    c->g_render = nullptr;
-    c->graph_sig.clear();
+}
+
+// exchanges the current lane's per-frame buffers and stream with the other's
+void swap_lane(hr_ctx* c)
+{
+    using std::swap;
+    hr_ctx::LaneFields& o = c->stash;
+    swap(c->stream, o.stream);
+    swap(c->params, o.params);
+    swap(c->status, o.status);
+    swap(c->bounds_part, o.bounds_part);
+    swap(c->rec, o.rec);
+    swap(c->wave_counts, o.wave_counts);
+    swap(c->wave_offsets, o.wave_offsets);
+    swap(c->keys, o.keys);
+    swap(c->vals, o.vals);
+    swap(c->keys_sorted, o.keys_sorted);
+    swap(c->top_vals_sorted, o.top_vals_sorted);
+    swap(c->prims, o.prims);
+    swap(c->leaf_counts, o.leaf_counts);
+    swap(c->cells, o.cells);
+    swap(c->leaves, o.leaves);
+    swap(c->scan_chain, o.scan_chain);
+    swap(c->sort_chain, o.sort_chain);
+    swap(c->sort_keys_tmp, o.sort_keys_tmp);
+    swap(c->sort_vals_tmp, o.sort_vals_tmp);
+    swap(c->sort_hist, o.sort_hist);
+    swap(c->cells_cap, o.cells_cap);
+    swap(c->pairs_cap, o.pairs_cap);
+    swap(c->leaves_cap, o.leaves_cap);
+    swap(c->leaf_pairs_cap, o.leaf_pairs_cap);
+    swap(c->verts, o.verts);
+    swap(c->normals, o.normals);
+    swap(c->hits, o.hits);
+    swap(c->hit_prims, o.hit_prims);
+    swap(c->occ, o.occ);
+    swap(c->pool, o.pool);
+    swap(c->frame_sel, o.frame_sel);
+    swap(c->rad, o.rad);
+    swap(c->frame_rays, o.frame_rays);
+    c->lane ^= 1;
+}
+
+hr_status use_lane(hr_ctx* c, int lane)
+{
+    if (c->lane == lane) return HR_OK;
+    swap_lane(c);
+    if (!c->stream) {
+        // first use of the second lane: its stream and fixed-size buffers
+        HR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HR_TRY(c->params.reserve(1));
+        HR_TRY(c->status.reserve(1));
+        HR_TRY(c->bounds_part.reserve(6 * (size_t)hr::kBoundsParts));
+        HR_TRY(c->pool.reserve(1));
+        HR_TRY(c->frame_sel.reserve(1));
+    }
+    return HR_OK;
+}
+
+void release_lane(hr_ctx::LaneFields& o)
+{
+    o.params.release();
+    o.status.release();
+    o.bounds_part.release();
+    o.rec.release();
+    DevBuf<uint32_t>* ubufs[] = { &o.wave_counts, &o.wave_offsets, &o.keys, &o.vals, &o.keys_sorted,
+                                  &o.top_vals_sorted, &o.prims, &o.leaf_counts, &o.scan_chain, &o.sort_chain,
+                                  &o.sort_keys_tmp, &o.sort_vals_tmp, &o.sort_hist, &o.hit_prims, &o.pool };
+    for (auto* b : ubufs) b->release();
+    o.cells.release();
+    o.leaves.release();
+    o.verts.release();
+    o.normals.release();
+    o.hits.release();
+    o.occ.release();
+    o.frame_sel.release();
+    o.rad.release();
+    o.frame_rays.release();
+    o.cells_cap = 0;
+    o.pairs_cap = o.leaves_cap = o.leaf_pairs_cap = 0;
+    if (o.stream) (void)hipStreamDestroy(o.stream);
+    o.stream = nullptr;
 }
 
 hr_status do_build(hr_ctx* c)
@@ -741,8 +862,12 @@ int primary_kernel(hr_ctx* c, bool timed)
 // Primary rays and the integrator.  fs != nullptr: the image id is read on the
 // device (captured frames); events = false while a graph is being recorded.
 hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_params* p,
-                         const hr::FrameSel* fs, bool events)
+                         const hr::FrameSel* fs, bool events, bool lane_out = false)
 {
+    // lane_out: a frame lane's render (captured), into the lane's own value
+    // buffer and ray counter, which k_frame_blend later blends in and adds
+    float* const fb = lane_out ? c->rad.p : c->fb.p;
+    unsigned long long* const rays = lane_out ? c->frame_rays.p : c->ray_count.p;
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
     hr::RenderArgs ra{};
     ra.seed_lo = c->seed_lo;
@@ -755,6 +880,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     ra.image = image_id;
     ra.stats = p->collect_stats ? c->work.p : nullptr;
     ra.frame = fs;
+    ra.direct = lane_out ? 1 : 0;
     const hr::TraceScene S = trace_scene(c);
     hr::Shading sh;
     sh.mats = c->mats.p;
@@ -778,8 +904,8 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         // TLGHAOIntegrator::integrate (TLGridHierarchyAOIntegrator.h:421-573)
         c->last_pk = c->tuning.persistent_primary == 1 ? 1 : 0;
         hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
-                                  c->fb.p, c->tuning.persistent_primary == 1, c->tuning.persistent_secondary != 0,
-                                  c->pool.p, (uint32_t)c->tuning.hier_refill, c->ray_count.p, c->stream,
+                                  fb, c->tuning.persistent_primary == 1, c->tuning.persistent_secondary != 0,
+                                  c->pool.p, (uint32_t)c->tuning.hier_refill, rays, c->stream,
                                   c->tuning.test_batch != 0);
         if (events) {
             HR_HIP(hipEventRecord(c->ev_p1, c->stream));
@@ -804,21 +930,21 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     const int pk = primary_kernel(c, events);
     c->last_pk = pk;
     if (pk == 1)
-        hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, c->ray_count.p,
+        hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, rays,
                             c->stream, persist_opts(c));
     else
-        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->ray_count.p, c->stream,
+        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, rays, c->stream,
                            pk == 3 ? (uint32_t)c->tuning.advance_steps : 0u);
     if (events) HR_HIP(hipEventRecord(c->ev_p1, c->stream));
     if (ra.stats) ra.stats = c->work.p + 3;
     if (mode == HR_MODE_AO) {
         if (pers)
-            hr::persist_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->pool.p, refill, c->ray_count.p,
+            hr::persist_ao(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->pool.p, refill, rays,
                            c->stream, persist_opts(c));
         else
-            hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, c->ray_count.p, c->stream);
+            hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, rays, c->stream);
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
-        hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, c->fb.p, c->stream);
+        hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, fb, c->stream);
         c->last_ao_samples = p->ao_samples;
         c->last_ao_ra = ra;
         c->last_ao_ra.stats = nullptr;
@@ -826,19 +952,19 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         c->ao_dump_ok = !fs;
     } else if (mode == HR_MODE_PATH) {
         if (c->tuning.persistent_path)
-            hr::persist_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->pool.p,
-                           (uint32_t)c->tuning.path_refill, (uint32_t)c->tuning.path_shade, c->ray_count.p,
+            hr::persist_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, fb, c->pool.p,
+                           (uint32_t)c->tuning.path_refill, (uint32_t)c->tuning.path_shade, rays,
                            c->stream);
         else
-            hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, c->fb.p, c->ray_count.p, c->stream,
+            hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, fb, rays, c->stream,
                           pt_wave_steps(c));
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
     } else if (mode == HR_MODE_SIMPLE) {
         if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
-        hr::launch_simple_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->fb.p, c->stream);
+        hr::launch_simple_shade(S, ra, npix, c->hits.p, c->hit_prims.p, fb, c->stream);
     } else {
-        hr::launch_path_multi(S, c->cam, ra, c->hits.p, c->hit_prims.p, sh, path_buffers(c), c->fb.p,
-                              c->ray_count.p, c->stream, events ? c->ev_p2 : nullptr);
+        hr::launch_path_multi(S, c->cam, ra, c->hits.p, c->hit_prims.p, sh, path_buffers(c), fb,
+                              rays, c->stream, events ? c->ev_p2 : nullptr);
     }
     if (events) HR_HIP(hipEventRecord(c->ev_r1, c->stream));
     HR_HIP(hipGetLastError());
@@ -876,7 +1002,8 @@ std::vector<unsigned char> graph_signature(hr_ctx* c, hr_mode mode, const hr_ren
                            c->sort_chain.p, c->sort_keys_tmp.p,
                            c->sort_vals_tmp.p, c->sort_hist.p, c->mats.p, c->mat_idx.p, c->lights.p,
                            c->light_cdf.p, c->normal_keys.p, c->normals.p, c->nidx.p, c->hits.p,
-                           c->hit_prims.p, c->fb.p, c->occ.p, c->pm.p, c->pm_prims.p };
+                           c->hit_prims.p, c->fb.p, c->occ.p, c->pm.p, c->pm_prims.p, c->pool.p,
+                           c->frame_sel.p, c->rad.p, c->frame_rays.p, c->status.p, c->params.p };
     for (const void* q : ptrs) put(v, q);
     put(v, c->cells_cap);
     put(v, c->pairs_cap);
@@ -909,24 +1036,67 @@ hr_status end_capture(hr_ctx* c, hipGraphExec_t* out)
     return HR_OK;
 }
 
-// Record the rebuild graph of each slot (frame selection upload from the
+// lanes for captured frames of this mode and these parameters: the
+// multi-kernel path tracer carries per-image state of its own, and work
+// statistics are not split per frame, so both keep to one lane
+int frame_lanes(const hr_ctx* c, hr_mode mode, const hr_render_params* p)
+{
+    return c->tuning.frame_lanes == 2 && mode != HR_MODE_PATH_MULTI && !p->collect_stats ? 2 : 1;
+}
+
+int slot_lane(const hr_ctx* c, int slot) { return c->graph_lanes == 2 ? slot : 0; }
+
+// Gives the current lane buffers for the current state by a build of the
+// current frame selection (no render): the capacities start from the other
+// lane's, so the build seldom regrows.
+hr_status prime_lane(hr_ctx* c, const hr_ctx::LaneFields& other, hr_mode mode, const hr_render_params* p)
+{
+    c->cells_cap = std::max(c->cells_cap, other.cells_cap);
+    c->pairs_cap = std::max(c->pairs_cap, other.pairs_cap);
+    c->leaves_cap = std::max(c->leaves_cap, other.leaves_cap);
+    c->leaf_pairs_cap = std::max(c->leaf_pairs_cap, other.leaf_pairs_cap);
+    HR_TRY(c->verts.reserve(3 * (size_t)c->nv));
+    if (c->has_normals) HR_TRY(c->normals.reserve(c->nn));
+    HR_TRY(do_build(c));
+    return prepare_render(c, mode, p);
+}
+
+// Record, for each slot, the rebuild graph (frame selection upload from the
 // slot's pinned memory, interpolation, build, normals, the frame gate, and
 // read-back of the status, grid parameters and halt word into the slot) and
-// the render graph (first-image clear, primary rays, integrator).  No memset
-// node is recorded.
+// the render graph (primary rays, integrator; with one lane also the
+// first-image clear).  With two lanes slot s is recorded on lane s, whose
+// buffers a build of the current frame selection sizes first; the render
+// writes the lane's value buffer.  No memset node is recorded.
 hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 {
     drop_graphs(c);
-    hr::FrameSel* fs = c->frame_sel.p;
+    const int lanes = frame_lanes(c, mode, p);
+    c->graph_lanes = lanes;
+    const int home = c->lane;
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
-    for (int s = 0; s < 2; ++s) {
+    hr_status st = HR_OK;
+    for (int s = 0; s < 2 && st == HR_OK; ++s) {
+        const int L = slot_lane(c, s);
+        if (L != c->lane) {
+            if ((st = use_lane(c, L)) != HR_OK) break;
+            if ((st = prime_lane(c, c->stash, mode, p)) != HR_OK) break;
+        }
+        if (lanes == 2) {
+            if ((st = c->rad.reserve(3 * (size_t)npix)) != HR_OK) break;
+            if ((st = c->frame_rays.reserve(1)) != HR_OK) break;
+        }
+        hr::FrameSel* fs = c->frame_sel.p;
         hr_ctx::FramePin* pin = c->h_pin + s;
-        HR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            st = fail(HR_ERR_DEVICE, "hipStreamBeginCapture failed");
+            break;
+        }
         hipError_t e = hipMemcpyAsync(fs, &pin->sel, sizeof(hr::FrameSel), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) {
             enqueue_build(c, fs);
             if (c->has_normals) enqueue_normals(c, fs);
-            hr::launch_frame_gate(c->status.p, c->halt.p, fs, c->stream);
+            hr::launch_frame_gate(c->status.p, c->halt.p, fs, lanes == 2 ? c->frame_rays.p : nullptr, c->stream);
             e = hipMemcpyAsync(&pin->status, c->status.p, sizeof(hr::BuildStatus), hipMemcpyDeviceToHost,
                                c->stream);
         }
@@ -935,22 +1105,26 @@ hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
                                c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(&pin->halt, c->halt.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
-        hr_status st = end_capture(c, &c->g_build[s]);
+        st = end_capture(c, &c->g_build[s]);
         if (e != hipSuccess) st = fail(HR_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
-        if (st != HR_OK) {
-            drop_graphs(c);
-            return st;
+        if (st != HR_OK) break;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            st = fail(HR_ERR_DEVICE, "hipStreamBeginCapture failed");
+            break;
         }
+        if (lanes == 1) hr::launch_clear_first(c->fb.p, 3 * (size_t)npix, fs, c->stream);
+        const hr_status rs = enqueue_render(c, mode, 0, p, fs, false, lanes == 2);
+        st = end_capture(c, &c->g_render[s]);
+        if (rs != HR_OK) st = rs;
+        if (st != HR_OK) break;
+        c->graph_sig[c->lane] = graph_signature(c, mode, p);
     }
-    HR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-    hr::launch_clear_first(c->fb.p, 3 * (size_t)npix, fs, c->stream);
-    const hr_status rs = enqueue_render(c, mode, 0, p, fs, false);
-    const hr_status st = end_capture(c, &c->g_render);
-    if (rs != HR_OK || st != HR_OK) {
+    const hr_status back = use_lane(c, home);
+    if (st == HR_OK) st = back;
+    if (st != HR_OK) {
         drop_graphs(c);
-        return rs != HR_OK ? rs : st;
+        return st;
     }
-    c->graph_sig = graph_signature(c, mode, p);
     c->graph_mode = mode;
     c->graph_params = *p;
     return HR_OK;
@@ -961,8 +1135,9 @@ hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 bool graphs_ready(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 {
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
-    return c->g_build[0] && c->g_build[1] && c->g_render && npix == c->last_npix && primary_decided(c) &&
-           c->graph_sig == graph_signature(c, mode, p);
+    return c->g_build[0] && c->g_build[1] && c->g_render[0] && c->g_render[1] && npix == c->last_npix &&
+           primary_decided(c) && c->graph_lanes == frame_lanes(c, mode, p) &&
+           c->graph_sig[c->lane] == graph_signature(c, mode, p);
 }
 
 void select_frame(hr_ctx* c, const hr_ctx::Submitted& f)
@@ -975,8 +1150,8 @@ void select_frame(hr_ctx* c, const hr_ctx::Submitted& f)
     c->normals_dirty = true;
 }
 
-// one frame without graphs (the build grows its buffers as needed); waits for
-// the build, not for the render
+// one frame without graphs on the current lane (the build grows its buffers
+// as needed); waits for the build, not for the render
 hr_status frame_uncaptured(hr_ctx* c, const hr_ctx::Submitted& f, hr_mode mode, const hr_render_params* p)
 {
     select_frame(c, f);
@@ -998,19 +1173,27 @@ hr_status frame_uncaptured(hr_ctx* c, const hr_ctx::Submitted& f, hr_mode mode, 
     return HR_OK;
 }
 
-// A replayed rebuild outgrew its buffers: that frame and every frame queued
-// behind it were skipped on the device (their render kernels returned at
-// entry).  Render them again in order without graphs, which grows the
-// buffers, and record the graphs anew.
-hr_status recover_frames(hr_ctx* c)
+hr_status sync_lanes(hr_ctx* c)
 {
     HR_HIP(hipStreamSynchronize(c->stream));
+    if (c->stash.stream) HR_HIP(hipStreamSynchronize(c->stash.stream));
+    return HR_OK;
+}
+
+// A replayed rebuild outgrew its buffers: that frame and every frame queued
+// behind it were skipped on the device (their render kernels returned at
+// entry, or their values were not blended in).  Render them again in order
+// without graphs on the current lane, which grows its buffers, and record
+// the graphs anew.
+hr_status recover_frames(hr_ctx* c)
+{
+    HR_TRY(sync_lanes(c));
     hr_ctx::Submitted redo[2];
     const int n = c->npending;
     for (int i = 0; i < n; ++i) redo[i] = c->pending[i];
     c->npending = 0;
     drop_graphs(c);
-    hr::fill_u32(c->halt.p, 0u, 1, nullptr, 1, c->stream);
+    hr::fill_u32(c->halt.p, 0xFFFFFFFFu, 1, nullptr, 1, c->stream);
     const hr_mode mode = c->graph_mode;
     const hr_render_params p = c->graph_params;
     for (int i = 0; i < n; ++i) HR_TRY(frame_uncaptured(c, redo[i], mode, &p));
@@ -1024,9 +1207,9 @@ hr_status recover_frames(hr_ctx* c)
 hr_status complete_oldest(hr_ctx* c)
 {
     const hr_ctx::Submitted f = c->pending[0];
-    HR_HIP(hipEventSynchronize(c->ev_g[f.slot][2]));
+    HR_HIP(hipEventSynchronize(c->ev_g[f.slot][3]));
     const hr_ctx::FramePin& pin = c->h_pin[f.slot];
-    if (pin.halt != 0u) return recover_frames(c);
+    if (pin.halt <= f.seq) return recover_frames(c);
     c->h_params = pin.params;
     set_info(c, pin.status);
     float ms = 0.f;
@@ -1042,7 +1225,8 @@ hr_status complete_oldest(hr_ctx* c)
 }
 
 // completes every frame in flight (called by every entry point but the
-// frame submission itself)
+// frame submission itself); the current lane then holds the last frame's
+// grid and hit buffers
 hr_status settle_frames(hr_ctx* c)
 {
     while (c->npending > 0) HR_TRY(complete_oldest(c));
@@ -1050,35 +1234,56 @@ hr_status settle_frames(hr_ctx* c)
 }
 
 // Queues one captured frame.  With the graphs recorded for this state the
-// frame is replayed into the free slot, after the frame two submissions back
-// has completed, and the call returns without waiting for it; otherwise the
-// frames in flight complete, this frame is rendered without graphs and the
-// graphs are recorded.
+// frame is replayed into the free slot (after the frame two submissions back
+// has completed) on that slot's lane, and the call returns without waiting
+// for it; otherwise the frames in flight complete, this frame is rendered
+// without graphs and the graphs are recorded.
 hr_status submit_frame(hr_ctx* c, const hr_ctx::Submitted& f, hr_mode mode, const hr_render_params* p)
 {
     if (c->npending == 2) HR_TRY(complete_oldest(c));
     if (!graphs_ready(c, mode, p)) {
         HR_TRY(settle_frames(c));
+        if (frame_lanes(c, mode, p) == 1 && c->lane != 0) {
+            // one lane runs on lane 0; the other lane's stream may still hold
+            // uncaptured work that this frame must follow
+            HR_TRY(sync_lanes(c));
+            HR_TRY(use_lane(c, 0));
+        }
         HR_TRY(frame_uncaptured(c, f, mode, p));
         HR_HIP(hipStreamSynchronize(c->stream));
         resolve_primary_trial(c);   // before the capture, which records no trial
         return record_graphs(c, mode, p);
     }
     const int s = c->next_slot;
+    const int L = slot_lane(c, s);
+    if (L != c->lane) {
+        // everything queued so far on the lane left behind (its last blend
+        // above all) comes before this frame's blend
+        HR_HIP(hipEventRecord(c->ev_switch, c->stream));
+        HR_TRY(use_lane(c, L));
+    }
     select_frame(c, f);
-    c->h_pin[s].sel = hr::FrameSel{ f.k1, f.k2, f.coeff, f.image, 0u };
+    hr_ctx::Submitted q = f;
+    q.slot = s;
+    q.seq = c->frame_seq++;
+    c->h_pin[s].sel = hr::FrameSel{ f.k1, f.k2, f.coeff, f.image, 0u, q.seq };
     HR_HIP(hipEventRecord(c->ev_g[s][0], c->stream));
     HR_HIP(hipGraphLaunch(c->g_build[s], c->stream));
     HR_HIP(hipEventRecord(c->ev_g[s][1], c->stream));
-    HR_HIP(hipGraphLaunch(c->g_render, c->stream));
+    HR_HIP(hipGraphLaunch(c->g_render[s], c->stream));
     HR_HIP(hipEventRecord(c->ev_g[s][2], c->stream));
-    hr_ctx::Submitted q = f;
-    q.slot = s;
+    if (c->graph_lanes == 2) {
+        HR_HIP(hipStreamWaitEvent(c->stream, c->ev_switch, 0));
+        const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+        hr::launch_frame_blend(c->frame_sel.p, c->halt.p, c->rad.p, c->fb.p, npix, c->frame_rays.p,
+                               c->ray_count.p, c->stream);
+        HR_HIP(hipGetLastError());
+    }
+    HR_HIP(hipEventRecord(c->ev_g[s][3], c->stream));
     c->pending[c->npending++] = q;
     c->next_slot = s ^ 1;
     return HR_OK;
 }
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1228,7 +1433,9 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
         if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
         e = hipMemsetAsync(c->work.p, 0, 6 * sizeof(unsigned long long), c->stream);
         if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
-        e = hipMemsetAsync(c->halt.p, 0, sizeof(uint32_t), c->stream);
+        e = hipMemsetAsync(c->halt.p, 0xFF, sizeof(uint32_t), c->stream);
+        if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+        e = hipEventCreateWithFlags(&c->ev_switch, hipEventDisableTiming);
         if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
     } while (0);
     if (s != HR_OK) {
@@ -1244,6 +1451,7 @@ hr_status hr_destroy(hr_ctx* c)
     if (!c) return HR_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stash.stream) (void)hipStreamSynchronize(c->stash.stream);
     c->normals.release();
     c->nidx.release();
     c->normal_keys.release();
@@ -1278,6 +1486,10 @@ hr_status hr_destroy(hr_ctx* c)
     drop_graphs(c);
     c->frame_sel.release();
     c->halt.release();
+    c->rad.release();
+    c->frame_rays.release();
+    release_lane(c->stash);
+    if (c->ev_switch) (void)hipEventDestroy(c->ev_switch);
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     for (auto& slot : c->ev_g)
@@ -1890,6 +2102,7 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
     if (t->fuse_units != 0 && t->fuse_units != 1) return fail(HR_ERR_ARG, "fuse_units must be 0 or 1");
     if (t->test_batch != 0 && t->test_batch != 1) return fail(HR_ERR_ARG, "test_batch must be 0 or 1");
     if (t->advance_steps < 1 || t->advance_steps > 16) return fail(HR_ERR_ARG, "advance_steps must be in [1, 16]");
+    if (t->frame_lanes < 1 || t->frame_lanes > 2) return fail(HR_ERR_ARG, "frame_lanes must be 1 or 2");
     c->tuning = *t;
     reset_primary_choice(c);
     return HR_OK;

@@ -559,12 +559,7 @@ struct PathLane {
 
 __device__ __forceinline__ void pt_store(const RenderArgs& ra, float* fb, uint32_t pix, f3 rad)
 {
-    const float nw = 1.f / (float)(image_of(ra) + 1);
-    const float ow = 1.f - nw;
-    float* o = fb + 3 * (size_t)pix;
-    o[0] = o[0] * ow + rad.x * nw;
-    o[1] = o[1] * ow + rad.y * nw;
-    o[2] = o[2] * ow + rad.z * nw;
+    fb_put(ra, fb + 3 * (size_t)pix, rad.x, rad.y, rad.z);
 }
 
 // a ray of the path: shadow rays stop at the first hit closer than 0.9999,

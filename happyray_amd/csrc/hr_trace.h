@@ -35,6 +35,9 @@ struct RenderArgs {
     unsigned long long* stats;   // nullptr, or {rays, steps, tests} counters
     const FrameSel* frame;       // nullptr, or the frame's selection in device memory
                                  // (its image id then replaces `image`)
+    int direct;                  // 1: each pixel's value is stored as it is (a frame lane's
+                                 // own buffer, which k_frame_blend then blends into the
+                                 // framebuffer); 0: blended in with weight 1/(image+1)
 };
 
 __device__ __forceinline__ int image_of(const RenderArgs& ra) { return ra.frame ? ra.frame->image : ra.image; }
@@ -43,6 +46,23 @@ __device__ __forceinline__ int image_of(const RenderArgs& ra) { return ra.frame 
 // the same for the whole grid, so the frame leaves the framebuffer, the hit and
 // path buffers and the ray and work counters as they were
 __device__ __forceinline__ bool frame_skipped(const RenderArgs& ra) { return ra.frame && ra.frame->skip; }
+
+// a pixel's value into the framebuffer: the running mean over images
+// (AOIntegrator.h:181-187), or stored as it is for a frame lane
+__device__ __forceinline__ void fb_put(const RenderArgs& ra, float* o, float x, float y, float z)
+{
+    if (ra.direct) {
+        o[0] = x;
+        o[1] = y;
+        o[2] = z;
+        return;
+    }
+    const float nw = 1.f / (float)(image_of(ra) + 1);
+    const float ow = 1.f - nw;
+    o[0] = o[0] * ow + x * nw;
+    o[1] = o[1] * ow + y * nw;
+    o[2] = o[2] * ow + z * nw;
+}
 
 struct Shading {
     const float* mats;          // 12 floats per PhongMaterial
@@ -75,10 +95,19 @@ void launch_pt(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const f
 // SoA key-frame blocks of nn normals
 // zero n floats of fb when fs->image == 0 (read on the device)
 void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s);
-// after a captured frame's rebuild: halt |= (the rebuild overflowed), and
-// fs->skip = halt.  halt is sticky until the host clears it, so every frame
-// queued behind an overflowed one is skipped too.
-void launch_frame_gate(const BuildStatus* st, uint32_t* halt, FrameSel* fs, hipStream_t s);
+// after a captured frame's rebuild: halt = min(halt, fs->seq) when the rebuild
+// overflowed, and fs->skip = (halt <= fs->seq).  halt holds the least sequence
+// number of an overflowed frame until the host resets it to 0xFFFFFFFF, so
+// every frame queued behind an overflowed one is skipped too.  frame_rays (a
+// frame lane's ray counter, or nullptr) is cleared for the frame.
+void launch_frame_gate(const BuildStatus* st, uint32_t* halt, FrameSel* fs, unsigned long long* frame_rays,
+                       hipStream_t s);
+// a frame lane's values into the framebuffer, in frame order: unless the frame
+// or an earlier one overflowed (halt <= fs->seq), fb = fb * (1 - w) + rad * w
+// with w = 1 / (image + 1) (fb taken as 0 for image 0, as after its clear), and
+// the frame's ray count is added to the context's
+void launch_frame_blend(const FrameSel* fs, const uint32_t* halt, const float* rad, float* fb, uint32_t npix,
+                        const unsigned long long* frame_rays, unsigned long long* ray_count, hipStream_t s);
 // simpleShade (src/RT/Integrator/SimpleIntegrator.h:42-125): barycentric colour
 // times |cos| at each primary hit, background (0.15, 0.2, 0.3), running mean
 void launch_simple_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,

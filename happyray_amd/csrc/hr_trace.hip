@@ -210,12 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_shade(TraceScene S, RenderArgs ra
         }
         rad = rad + (v * 1.f) / (float)S_;
     }
-    const float nw = 1.f / (float)(image_of(ra) + 1);
-    const float ow = 1.f - nw;
-    float* o = fb + 3 * (size_t)pix;
-    o[0] = o[0] * ow + rad.x * nw;
-    o[1] = o[1] * ow + rad.y * nw;
-    o[2] = o[2] * ow + rad.z * nw;
+    fb_put(ra, fb + 3 * (size_t)pix, rad.x, rad.y, rad.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,12 +309,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (ray_count && traced) atomicAdd(ray_count, (unsigned long long)traced);
     add_work(ra.stats, traced, wk);
     if (!in) return;
-    const float nw = 1.f / (float)(image_of(ra) + 1);
-    const float ow = 1.f - nw;
-    float* o = fb + 3 * (size_t)pix;
-    o[0] = o[0] * ow + P.radiance.x * nw;
-    o[1] = o[1] * ow + P.radiance.y * nw;
-    o[2] = o[2] * ow + P.radiance.z * nw;
+    fb_put(ra, fb + 3 * (size_t)pix, P.radiance.x, P.radiance.y, P.radiance.z);
 }
 
 template <bool kTwo>
@@ -375,12 +365,7 @@ __global__ __launch_bounds__(kBlock) void k_pt(TraceScene S, RenderArgs ra, uint
     if (ray_count && traced) atomicAdd(ray_count, (unsigned long long)traced);
     add_work(ra.stats, traced, wk);
     if (!in) return;
-    const float nw = 1.f / (float)(image_of(ra) + 1);
-    const float ow = 1.f - nw;
-    float* o = fb + 3 * (size_t)pix;
-    o[0] = o[0] * ow + P.radiance.x * nw;
-    o[1] = o[1] * ow + P.radiance.y * nw;
-    o[2] = o[2] * ow + P.radiance.z * nw;
+    fb_put(ra, fb + 3 * (size_t)pix, P.radiance.x, P.radiance.y, P.radiance.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -415,12 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_simple_shade(TraceScene S, RenderArg
     } else {
         rad = mk(0.15f, 0.2f, 0.3f);
     }
-    const float nw = 1.f / (float)(image_of(ra) + 1);
-    const float ow = 1.f - nw;
-    float* o = fb + 3 * (size_t)pix;
-    o[0] = o[0] * ow + rad.x * nw;
-    o[1] = o[1] * ow + rad.y * nw;
-    o[2] = o[2] * ow + rad.z * nw;
+    fb_put(ra, fb + 3 * (size_t)pix, rad.x, rad.y, rad.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -645,17 +625,48 @@ void launch_ao_dump(const TraceScene& S, const CameraDev& cam, const RenderArgs&
 }
 
 __global__ void k_frame_gate(const BuildStatus* __restrict__ st, uint32_t* __restrict__ halt,
-                             FrameSel* __restrict__ fs)
+                             FrameSel* __restrict__ fs, unsigned long long* __restrict__ frame_rays)
 {
     if (threadIdx.x != 0) return;
-    const uint32_t h = *halt | (st->overflow != 0u ? 1u : 0u);
-    *halt = h;
-    fs->skip = h;
+    const uint32_t seq = fs->seq;
+    const uint32_t h = st->overflow != 0u ? atomicMin(halt, seq) : atomicAdd(halt, 0u);
+    fs->skip = (h <= seq || st->overflow != 0u) ? 1u : 0u;
+    if (frame_rays) *frame_rays = 0ull;
 }
 
-void launch_frame_gate(const BuildStatus* st, uint32_t* halt, FrameSel* fs, hipStream_t s)
+void launch_frame_gate(const BuildStatus* st, uint32_t* halt, FrameSel* fs, unsigned long long* frame_rays,
+                       hipStream_t s)
 {
-    hipLaunchKernelGGL(k_frame_gate, dim3(1), dim3(64), 0, s, st, halt, fs);
+    hipLaunchKernelGGL(k_frame_gate, dim3(1), dim3(64), 0, s, st, halt, fs, frame_rays);
+}
+
+__global__ __launch_bounds__(kBlock) void k_frame_blend(const FrameSel* __restrict__ fs,
+                                                        const uint32_t* __restrict__ halt,
+                                                        const float* __restrict__ rad, float* __restrict__ fb,
+                                                        uint32_t npix, const unsigned long long* frame_rays,
+                                                        unsigned long long* ray_count)
+{
+    const uint32_t seq = fs->seq;
+    if (__hip_atomic_load(halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= seq) return;
+    const int image = fs->image;
+    const float nw = 1.f / (float)(image + 1);
+    const float ow = 1.f - nw;
+    const size_t n = 3 * (size_t)npix;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float f = image == 0 ? 0.f : fb[i];
+        fb[i] = f * ow + rad[i] * nw;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ray_count, *frame_rays);
+}
+
+void launch_frame_blend(const FrameSel* fs, const uint32_t* halt, const float* rad, float* fb, uint32_t npix,
+                        const unsigned long long* frame_rays, unsigned long long* ray_count, hipStream_t s)
+{
+    const size_t n = 3 * (size_t)npix;
+    const size_t want = (n + kBlock - 1) / kBlock;
+    const uint32_t blocks = (uint32_t)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(k_frame_blend, dim3(blocks), dim3(kBlock), 0, s, fs, halt, rad, fb, npix, frame_rays,
+                       ray_count);
 }
 
 void launch_clear_first(float* fb, size_t n, const FrameSel* fs, hipStream_t s)

@@ -130,7 +130,9 @@ typedef struct {
 const char* hr_last_error(void);
 hr_status hr_create(hr_ctx** out, const hr_options* opts);
 hr_status hr_destroy(hr_ctx* ctx);
-/* the context's HIP stream (hipStream_t), for callers that time or order work */
+/* the context's HIP stream (hipStream_t), for callers that time or order work:
+ * the stream of the lane that ran the last captured frame (hr_tuning.frame_lanes),
+ * on which the context's other work runs until the next captured frame */
 hr_status hr_get_stream(hr_ctx* ctx, void** stream);
 
 /* ---- geometry: RTEngine::upload(frame1, frame2, coeff) (src/RT/RTEngine.cu:194-215,
@@ -350,6 +352,13 @@ typedef struct {
                                   statistics leave out                                  */
     int advance_steps;         /* test_batch 1: grid steps a lane may take per loop
                                   iteration to reach its next candidates, 1..4 (default 2) */
+    int frame_lanes;           /* captured frames: 2 (default) two lanes of per-frame
+                                  buffers, each with its own stream, taken in turn, so
+                                  that a frame's rebuild and render overlap the previous
+                                  frame's render; each lane renders into a buffer of its
+                                  own, blended into the framebuffer in frame order.  1
+                                  one lane.  Same results; the multi-kernel path tracer
+                                  (mode 1) and renders with collect_stats use one lane */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

@@ -35,10 +35,13 @@ def grids_equal(a, b):
     np.testing.assert_array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("lanes", [2, 1])
 @pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_SIMPLE, MODE_PATH_MULTI])
-def test_frame_graph_matches_uncaptured_frames(mode):
+def test_frame_graph_matches_uncaptured_frames(mode, lanes):
     sc, keys = scenes.heightfield_keyframes(quads=64, frames=5, res=(64, 48))
     g = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    assert g.tuning().frame_lanes == 2
+    g.set_tuning(frame_lanes=lanes)
     e = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
     org, d = random_rays(sc, 4000, seed=9)
     kw = dict(jitter=1, pt_depth=3) if mode != MODE_AO else dict(jitter=1, ao_samples=4, ao_dist=5.0)
@@ -105,8 +108,9 @@ def test_frame_graph_regrows_after_overflow():
         e.close()
 
 
+@pytest.mark.parametrize("lanes", [2, 1])
 @pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_PATH_MULTI])
-def test_overflowed_replay_keeps_running_mean(mode):
+def test_overflowed_replay_keeps_running_mean(mode, lanes):
     """A replay whose rebuild overflows is skipped on the device, so the
     running mean over images 0..5 and the traced-ray count equal those of the
     same frames run call by call."""
@@ -114,9 +118,10 @@ def test_overflowed_replay_keeps_running_mean(mode):
     kw = dict(jitter=1, pt_depth=2) if mode != MODE_AO else dict(jitter=1, ao_samples=4, ao_dist=3.0)
     for kind, lt, ll in ((GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)):
         g = make_engine(sc, keys, kind, lt, ll)
+        g.set_tuning(frame_lanes=lanes)
         e = make_engine(sc, keys, kind, lt, ll)
         for i, (k1, k2, cf) in enumerate(GROWING_FRAMES):
-            print(f"running mean across overflow mode={mode} kind={kind} image={i}", flush=True)
+            print(f"running mean across overflow mode={mode} lanes={lanes} kind={kind} image={i}", flush=True)
             fb_g = g.render_frame_graph(k1, k2, cf, image_id=i, mode=mode, **kw)
             e.set_frame(k1, k2, cf)
             e.build_acc_struct()
@@ -148,8 +153,9 @@ def run_called(e, frames, mode, kw):
     return fbs
 
 
-@pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO])
-def test_submitted_frames_match_called_frames(mode):
+@pytest.mark.parametrize("lanes", [2, 1])
+@pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_SIMPLE])
+def test_submitted_frames_match_called_frames(mode, lanes):
     """Frames queued by hr_submit_frame_graph (two in flight) give the same
     running mean, grid and ray count as frames run call by call, read back in
     the middle of the sequence and at its end."""
@@ -157,8 +163,9 @@ def test_submitted_frames_match_called_frames(mode):
     kw = dict(jitter=1, pt_depth=3) if mode != MODE_AO else dict(jitter=1, ao_samples=4, ao_dist=5.0)
     frames = [O.frame_ids(t, 0.4, 5) for t in (0.0, 0.4, 1.7, 3.9, 2.2, 0.1, 4.5, 1.1, 2.9, 3.3)]
     g = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
+    g.set_tuning(frame_lanes=lanes)
     e = make_engine(sc, keys, GRID_TWO_LEVEL, 1.0 / 16, 3.0)
-    print(f"submitted frames mode={mode}", flush=True)
+    print(f"submitted frames mode={mode} lanes={lanes}", flush=True)
     got = run_submitted(g, frames, mode, kw, check_at=(4,))
     g.synchronize()
     last = g.framebuffer()
@@ -166,6 +173,11 @@ def test_submitted_frames_match_called_frames(mode):
     np.testing.assert_array_equal(got[4], want[4])
     np.testing.assert_array_equal(last, want[-1])
     grids_equal(g, e)
+    # the hit buffers are those of the last frame, whichever lane ran it
+    for a, b in zip(g.primary_hits(), e.primary_hits()):
+        np.testing.assert_array_equal(a, b)
+    if mode == MODE_AO:
+        np.testing.assert_array_equal(g.ao_occlusion(4), e.ao_occlusion(4))
     assert g.timing().rays_traced == e.timing().rays_traced
     tm = g.timing()
     assert tm.build_ms > 0 and tm.render_ms > 0
@@ -173,7 +185,8 @@ def test_submitted_frames_match_called_frames(mode):
     e.close()
 
 
-def test_submitted_frames_recover_from_overflow():
+@pytest.mark.parametrize("lanes", [2, 1])
+def test_submitted_frames_recover_from_overflow(lanes):
     """An overflowing replay with another frame queued behind it: both are
     skipped on the device and rendered again when the host completes them
     (in a read of the framebuffer, or in the next submission), so the running
@@ -185,8 +198,9 @@ def test_submitted_frames_recover_from_overflow():
     cases = [(kind, lt, ll, at) for kind, lt, ll in ((GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0))
              for at in ((3, last), (last,))]
     for kind, lt, ll, at in cases:
-        print(f"submitted frames across overflow kind={kind} reads at {at}", flush=True)
+        print(f"submitted frames across overflow lanes={lanes} kind={kind} reads at {at}", flush=True)
         g = make_engine(sc, keys, kind, lt, ll)
+        g.set_tuning(frame_lanes=lanes)
         e = make_engine(sc, keys, kind, lt, ll)
         got = run_submitted(g, frames, MODE_PATH, kw, check_at=at)
         want = run_called(e, frames, MODE_PATH, kw)
