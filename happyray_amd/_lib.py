@@ -60,7 +60,9 @@ class RenderParams(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("render_ms", C.c_double), ("primary_ms", C.c_double),
                 ("secondary_ms", C.c_double), ("shade_ms", C.c_double), ("rays_traced", C.c_uint64),
-                ("primary_kernel", C.c_int)]
+                ("primary_kernel", C.c_int), ("frames_replayed", C.c_uint64), ("replay_build_ms", C.c_double),
+                ("replay_render_ms", C.c_double), ("replay_primary_ms", C.c_double),
+                ("replay_secondary_ms", C.c_double), ("replay_shade_ms", C.c_double)]
 
 
 class Tuning(C.Structure):

@@ -148,9 +148,10 @@ struct hr_ctx {
     int last_pk = -1;                    // camera-ray kernel of the last render
 
     // captured frames: per slot (0, 1) a rebuild graph, which reads the frame
-    // selection from and writes its status into the slot's pinned memory, and a
-    // render graph, replayed per frame.  Up to two submitted frames are in
-    // flight, in alternate slots; pending[0] is the older.
+    // selection from and writes its status into the slot's pinned memory,
+    // replayed per frame and followed by the frame's render kernels, which
+    // read the same selection from device memory.  Up to two submitted frames
+    // are in flight, in alternate slots; pending[0] is the older.
     struct FramePin {                        // pinned, one per slot
         hr::FrameSel sel;                    // source of the frame selection
         hr::BuildStatus status;              // build status after the rebuild
@@ -165,7 +166,6 @@ struct hr_ctx {
         uint32_t seq;
     };
     hipGraphExec_t g_build[2] = { nullptr, nullptr };
-    hipGraphExec_t g_render[2] = { nullptr, nullptr };
     std::vector<unsigned char> graph_sig[2]; // per lane: state the graphs were recorded with
     hr_mode graph_mode = HR_MODE_AO;         // the graphs' mode and render parameters
     hr_render_params graph_params{};
@@ -175,11 +175,16 @@ struct hr_ctx {
     FramePin* h_pin = nullptr;
     hipEvent_t ev_g[2][4] = {};              // per slot: rebuild start, rebuild end, render end,
                                              // frame end (after its blend)
+    hipEvent_t ev_k[2][3] = {};              // per slot, in the render: after the camera rays,
+                                             // after the secondary rays, after the shading
     hipEvent_t ev_switch = nullptr;          // the lane left at the last lane switch: its work so far
     Submitted pending[2] = {};
     int npending = 0, next_slot = 0;
     uint32_t frame_seq = 0;
     float graph_build_ms = 0.f, graph_render_ms = 0.f;   // the last completed replay
+    float graph_primary_ms = 0.f, graph_secondary_ms = 0.f, graph_shade_ms = 0.f;
+    uint64_t replays = 0;                    // completed replays and the sums of their times
+    double replay_ms[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };   // build, render, primary, secondary, shade
     bool graph_timed = false;
 
     // Frame lanes: with two, captured frames alternate between two sets of
@@ -516,8 +521,7 @@ void drop_graphs(hr_ctx* c)
 {
     for (int s = 0; s < 2; ++s) {
         if (c->g_build[s]) (void)hipGraphExecDestroy(c->g_build[s]);
-        if (c->g_render[s]) (void)hipGraphExecDestroy(c->g_render[s]);
-        c->g_build[s] = c->g_render[s] = nullptr;
+        c->g_build[s] = nullptr;
         c->graph_sig[s].clear();
     }
 }
@@ -830,7 +834,10 @@ void resolve_primary_trial(hr_ctx* c)
 {
     if (c->pp_pending < 0) return;
     float ms = 0.f;
-    if (hipEventSynchronize(c->ev_p1) == hipSuccess && hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess) {
+    const bool timed =
+        hipEventSynchronize(c->ev_p1) == hipSuccess && hipEventElapsedTime(&ms, c->ev_r0, c->ev_p1) == hipSuccess;
+    if (!timed) (void)hipGetLastError();
+    if (timed) {
         const int k = c->pp_pending % 3;
         c->pp_ms[k] = c->pp_pending < 3 ? ms : std::min(c->pp_ms[k], ms);
         c->pp_measured = c->pp_pending + 1;
@@ -862,8 +869,15 @@ int primary_kernel(hr_ctx* c, bool timed)
 // Primary rays and the integrator.  fs != nullptr: the image id is read on the
 // device (captured frames); events = false while a graph is being recorded.
 hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_params* p,
-                         const hr::FrameSel* fs, bool events, bool lane_out = false)
+                         const hr::FrameSel* fs, bool events, bool lane_out = false,
+                         const hipEvent_t* marks = nullptr)
 {
+    // events after the camera rays, after the secondary rays and at the end:
+    // the context's (timed uncaptured frames) or `marks` (a captured frame's
+    // slot)
+    const hipEvent_t e_p1 = events ? c->ev_p1 : marks ? marks[0] : nullptr;
+    const hipEvent_t e_p2 = events ? c->ev_p2 : marks ? marks[1] : nullptr;
+    const hipEvent_t e_r1 = events ? c->ev_r1 : marks ? marks[2] : nullptr;
     // lane_out: a frame lane's render (captured), into the lane's own value
     // buffer and ray counter, which k_frame_blend later blends in and adds
     float* const fb = lane_out ? c->rad.p : c->fb.p;
@@ -907,22 +921,18 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
                                   fb, c->tuning.persistent_primary == 1, c->tuning.persistent_secondary != 0,
                                   c->pool.p, (uint32_t)c->tuning.hier_refill, rays, c->stream,
                                   c->tuning.test_batch != 0);
-        if (events) {
-            HR_HIP(hipEventRecord(c->ev_p1, c->stream));
-            HR_HIP(hipEventRecord(c->ev_p2, c->stream));
-            HR_HIP(hipEventRecord(c->ev_r1, c->stream));
-        }
+        if (e_p1) HR_HIP(hipEventRecord(e_p1, c->stream));
+        if (e_p2) HR_HIP(hipEventRecord(e_p2, c->stream));
+        if (e_r1) HR_HIP(hipEventRecord(e_r1, c->stream));
         c->last_ao_samples = p->ao_samples;
         HR_HIP(hipGetLastError());
         return HR_OK;
     }
     if (mode == HR_MODE_PATH_MULTI && c->nlights == 0) {
         // PathTracer::integrate returns at once without lights (PathTracer.h:1049-1053)
-        if (events) {
-            HR_HIP(hipEventRecord(c->ev_p1, c->stream));
-            HR_HIP(hipEventRecord(c->ev_p2, c->stream));
-            HR_HIP(hipEventRecord(c->ev_r1, c->stream));
-        }
+        if (e_p1) HR_HIP(hipEventRecord(e_p1, c->stream));
+        if (e_p2) HR_HIP(hipEventRecord(e_p2, c->stream));
+        if (e_r1) HR_HIP(hipEventRecord(e_r1, c->stream));
         return HR_OK;
     }
     const bool pers = c->tuning.persistent_secondary != 0;
@@ -935,7 +945,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     else
         hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, rays, c->stream,
                            pk == 3 ? (uint32_t)c->tuning.advance_steps : 0u);
-    if (events) HR_HIP(hipEventRecord(c->ev_p1, c->stream));
+    if (e_p1) HR_HIP(hipEventRecord(e_p1, c->stream));
     if (ra.stats) ra.stats = c->work.p + 3;
     if (mode == HR_MODE_AO) {
         if (pers)
@@ -943,7 +953,7 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
                            c->stream, persist_opts(c));
         else
             hr::launch_ao_rays(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, rays, c->stream);
-        if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+        if (e_p2) HR_HIP(hipEventRecord(e_p2, c->stream));
         hr::launch_ao_shade(S, ra, npix, c->hits.p, c->hit_prims.p, c->occ.p, sh, fb, c->stream);
         c->last_ao_samples = p->ao_samples;
         c->last_ao_ra = ra;
@@ -958,15 +968,15 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         else
             hr::launch_pt(S, ra, npix, c->hits.p, c->hit_prims.p, sh, fb, rays, c->stream,
                           pt_wave_steps(c));
-        if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+        if (e_p2) HR_HIP(hipEventRecord(e_p2, c->stream));
     } else if (mode == HR_MODE_SIMPLE) {
-        if (events) HR_HIP(hipEventRecord(c->ev_p2, c->stream));
+        if (e_p2) HR_HIP(hipEventRecord(e_p2, c->stream));
         hr::launch_simple_shade(S, ra, npix, c->hits.p, c->hit_prims.p, fb, c->stream);
     } else {
         hr::launch_path_multi(S, c->cam, ra, c->hits.p, c->hit_prims.p, sh, path_buffers(c), fb,
-                              rays, c->stream, events ? c->ev_p2 : nullptr);
+                              rays, c->stream, e_p2);
     }
-    if (events) HR_HIP(hipEventRecord(c->ev_r1, c->stream));
+    if (e_r1) HR_HIP(hipEventRecord(e_r1, c->stream));
     HR_HIP(hipGetLastError());
     return HR_OK;
 }
@@ -1061,13 +1071,12 @@ hr_status prime_lane(hr_ctx* c, const hr_ctx::LaneFields& other, hr_mode mode, c
     return prepare_render(c, mode, p);
 }
 
-// Record, for each slot, the rebuild graph (frame selection upload from the
+// Record, for each slot, the rebuild graph: frame selection upload from the
 // slot's pinned memory, interpolation, build, normals, the frame gate, and
-// read-back of the status, grid parameters and halt word into the slot) and
-// the render graph (primary rays, integrator; with one lane also the
-// first-image clear).  With two lanes slot s is recorded on lane s, whose
-// buffers a build of the current frame selection sizes first; the render
-// writes the lane's value buffer.  No memset node is recorded.
+// read-back of the status, grid parameters and halt word into the slot.
+// With two lanes slot s is recorded on lane s, whose buffers a build of the
+// current frame selection sizes first, and each lane gets its value buffer.
+// No memset node is recorded.
 hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 {
     drop_graphs(c);
@@ -1108,15 +1117,6 @@ hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
         st = end_capture(c, &c->g_build[s]);
         if (e != hipSuccess) st = fail(HR_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
         if (st != HR_OK) break;
-        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
-            st = fail(HR_ERR_DEVICE, "hipStreamBeginCapture failed");
-            break;
-        }
-        if (lanes == 1) hr::launch_clear_first(c->fb.p, 3 * (size_t)npix, fs, c->stream);
-        const hr_status rs = enqueue_render(c, mode, 0, p, fs, false, lanes == 2);
-        st = end_capture(c, &c->g_render[s]);
-        if (rs != HR_OK) st = rs;
-        if (st != HR_OK) break;
         c->graph_sig[c->lane] = graph_signature(c, mode, p);
     }
     const hr_status back = use_lane(c, home);
@@ -1135,7 +1135,7 @@ hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 bool graphs_ready(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 {
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
-    return c->g_build[0] && c->g_build[1] && c->g_render[0] && c->g_render[1] && npix == c->last_npix &&
+    return c->g_build[0] && c->g_build[1] && npix == c->last_npix &&
            primary_decided(c) && c->graph_lanes == frame_lanes(c, mode, p) &&
            c->graph_sig[c->lane] == graph_signature(c, mode, p);
 }
@@ -1212,9 +1212,31 @@ hr_status complete_oldest(hr_ctx* c)
     if (pin.halt <= f.seq) return recover_frames(c);
     c->h_params = pin.params;
     set_info(c, pin.status);
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->ev_g[f.slot][0], c->ev_g[f.slot][1]) == hipSuccess) c->graph_build_ms = ms;
-    if (hipEventElapsedTime(&ms, c->ev_g[f.slot][1], c->ev_g[f.slot][2]) == hipSuccess) c->graph_render_ms = ms;
+    const hipEvent_t* g = c->ev_g[f.slot];
+    const hipEvent_t* k = c->ev_k[f.slot];
+    float* outs[5] = { &c->graph_build_ms, &c->graph_render_ms, &c->graph_primary_ms, &c->graph_secondary_ms,
+                       &c->graph_shade_ms };
+    const hipEvent_t from[5] = { g[0], g[1], g[1], k[0], k[1] };
+    const hipEvent_t to[5] = { g[1], g[2], k[0], k[1], k[2] };
+    // a frame with an interval that cannot be read stays out of the sums
+    bool all = true;
+    for (int i = 0; i < 5; ++i) {
+        float ms = 0.f;
+        const hipError_t te = hipEventElapsedTime(&ms, from[i], to[i]);
+        if (te != hipSuccess) {
+            if (std::getenv("HR_DEBUG_TIMING"))
+                std::fprintf(stderr, "replay timing: seq %u slot %d interval %d: %s\n", f.seq, f.slot, i,
+                             hipGetErrorString(te));
+            (void)hipGetLastError();   // the frame's results stand; leave no error behind
+            ms = 0.f;
+            all = false;
+        }
+        *outs[i] = ms;
+    }
+    if (all) {
+        for (int i = 0; i < 5; ++i) c->replay_ms[i] += *outs[i];
+        ++c->replays;
+    }
     c->build_timed = c->render_timed = c->graph_timed = true;
     c->built = true;
     c->normals_dirty = false;
@@ -1270,11 +1292,14 @@ hr_status submit_frame(hr_ctx* c, const hr_ctx::Submitted& f, hr_mode mode, cons
     HR_HIP(hipEventRecord(c->ev_g[s][0], c->stream));
     HR_HIP(hipGraphLaunch(c->g_build[s], c->stream));
     HR_HIP(hipEventRecord(c->ev_g[s][1], c->stream));
-    HR_HIP(hipGraphLaunch(c->g_render[s], c->stream));
+    // the render: launched directly (its few launches cost the host little, and
+    // frames run ahead of the host), timed by ordinary events
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    if (c->graph_lanes == 1) hr::launch_clear_first(c->fb.p, 3 * (size_t)npix, c->frame_sel.p, c->stream);
+    HR_TRY(enqueue_render(c, mode, 0, p, c->frame_sel.p, false, c->graph_lanes == 2, c->ev_k[s]));
     HR_HIP(hipEventRecord(c->ev_g[s][2], c->stream));
     if (c->graph_lanes == 2) {
         HR_HIP(hipStreamWaitEvent(c->stream, c->ev_switch, 0));
-        const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
         hr::launch_frame_blend(c->frame_sel.p, c->halt.p, c->rad.p, c->fb.p, npix, c->frame_rays.p,
                                c->ray_count.p, c->stream);
         HR_HIP(hipGetLastError());
@@ -1421,6 +1446,11 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
                 e = hipEventCreate(&ev);
                 if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
             }
+        for (auto& slot : c->ev_k)
+            for (hipEvent_t& ev : slot) {
+                e = hipEventCreate(&ev);
+                if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+            }
         if (s != HR_OK) break;
         if ((s = noalloc_scratch(c->device, &c->noalloc)) != HR_OK) break;
         hipEvent_t* evs[6] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1, &c->ev_p1, &c->ev_p2 };
@@ -1493,6 +1523,9 @@ hr_status hr_destroy(hr_ctx* c)
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     for (auto& slot : c->ev_g)
+        for (hipEvent_t ev : slot)
+            if (ev) (void)hipEventDestroy(ev);
+    for (auto& slot : c->ev_k)
         for (hipEvent_t ev : slot)
             if (ev) (void)hipEventDestroy(ev);
     hipEvent_t evs[6] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1, c->ev_p1, c->ev_p2 };
@@ -2047,9 +2080,12 @@ hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
     out->build_ms = out->render_ms = out->primary_ms = out->secondary_ms = out->shade_ms = 0.0;
     float ms = 0.f;
     if (c->graph_timed) {
-        // the last completed replay: rebuild graph, render graph
+        // the last completed replay: rebuild graph, render and its parts
         out->build_ms = c->graph_build_ms;
         out->render_ms = c->graph_render_ms;
+        out->primary_ms = c->graph_primary_ms;
+        out->secondary_ms = c->graph_secondary_ms;
+        out->shade_ms = c->graph_shade_ms;
     } else {
         if (c->build_timed && hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1) == hipSuccess) out->build_ms = ms;
         if (c->render_timed) {
@@ -2058,11 +2094,18 @@ hr_status hr_get_timing(hr_ctx* c, hr_timing* out)
             if (hipEventElapsedTime(&ms, c->ev_p1, c->ev_p2) == hipSuccess) out->secondary_ms = ms;
             if (hipEventElapsedTime(&ms, c->ev_p2, c->ev_r1) == hipSuccess) out->shade_ms = ms;
         }
+        (void)hipGetLastError();   // an interval not recorded reads as 0; leave no error behind
     }
     unsigned long long rays = 0;
     HR_HIP(hipMemcpy(&rays, c->ray_count.p, sizeof rays, hipMemcpyDeviceToHost));
     out->rays_traced = rays;
     out->primary_kernel = c->last_pk;
+    out->frames_replayed = c->replays;
+    out->replay_build_ms = c->replay_ms[0];
+    out->replay_render_ms = c->replay_ms[1];
+    out->replay_primary_ms = c->replay_ms[2];
+    out->replay_secondary_ms = c->replay_ms[3];
+    out->replay_shade_ms = c->replay_ms[4];
     return HR_OK;
 }
 

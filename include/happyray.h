@@ -110,6 +110,12 @@ typedef struct {
     int primary_kernel;      /* camera-ray kernel of the last render: 0 one ray per
                                 lane, 1 persistent, 3 one ray per lane with
                                 wave-batched tests (-1 before the first render)   */
+    uint64_t frames_replayed;  /* captured frames replayed and completed since the
+                                  context was created, and the sums of their times
+                                  (rebuild graph, render, and the render's camera-
+                                  ray, secondary and shading parts); a replay whose
+                                  events cannot be read is not counted             */
+    double replay_build_ms, replay_render_ms, replay_primary_ms, replay_secondary_ms, replay_shade_ms;
 } hr_timing;
 
 /* Work counters of the renders run with collect_stats = 1 and of all ray
@@ -236,16 +242,16 @@ hr_status hr_render(hr_ctx* ctx, hr_mode mode, int image_id, const hr_render_par
  * render as synchronous phases, src/RT/RTEngine.cu:194-453).  One call renders a
  * frame of the resident key frames: interpolation of k1, k2 with coeff, a full
  * rebuild, primary rays and the integrator, with results equal to hr_set_frame +
- * hr_build + hr_render.  The work is recorded once as two HIP graphs (rebuild;
- * render) and replayed by later calls, the frame's (k1, k2, coeff, image_id)
- * reaching the kernels through device memory.  The graphs are recorded again when
- * the mode, render parameters, grid settings, camera, light count, geometry,
- * tuning or any buffer changes.  A replayed rebuild that outgrows its buffers
- * makes the device skip that frame's render (every render kernel returns at
- * entry, so the framebuffer and counters stay as they were); the host finds it
- * after the replay and renders the frame again without graphs while the
- * buffers grow.  hr_get_timing reports the rebuild graph as build_ms and the
- * render graph as render_ms. */
+ * hr_build + hr_render.  The rebuild is recorded once as a HIP graph (one per
+ * frame slot) and replayed by later calls, followed by the render kernels; the
+ * frame's (k1, k2, coeff, image_id) reaches the kernels through device memory.
+ * The graphs are recorded again when the mode, render parameters, grid settings,
+ * camera, light count, geometry, tuning or any buffer changes.  A replayed
+ * rebuild that outgrows its buffers makes the device skip that frame's render
+ * (every render kernel returns at entry, so the framebuffer and counters stay as
+ * they were); the host finds it after the replay and renders the frame again
+ * without graphs while the buffers grow.  hr_get_timing reports the rebuild graph
+ * as build_ms and the render as render_ms, primary_ms, secondary_ms, shade_ms. */
 hr_status hr_render_frame_graph(hr_ctx* ctx, uint32_t k1, uint32_t k2, float coeff, hr_mode mode,
                                 int image_id, const hr_render_params* params, float* rgb_out);
 /* Captured frames without waiting (no reference counterpart: the reference's

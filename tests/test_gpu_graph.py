@@ -181,6 +181,9 @@ def test_submitted_frames_match_called_frames(mode, lanes):
     assert g.timing().rays_traced == e.timing().rays_traced
     tm = g.timing()
     assert tm.build_ms > 0 and tm.render_ms > 0
+    # replays after each slot's first (and after any re-recording) are timed inside their graphs
+    assert 1 <= tm.frames_replayed <= len(frames) - 1
+    assert tm.replay_build_ms > 0 and tm.replay_primary_ms > 0 and tm.replay_render_ms >= tm.replay_primary_ms
     g.close()
     e.close()
 
