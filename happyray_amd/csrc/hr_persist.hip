@@ -68,7 +68,8 @@ struct GenAO {
 
     __device__ bool skipped() const { return frame_skipped(ra); }
 
-    // AmbientOcclusionRayGenerator::operator() (RayGenerators.h:281-343)
+    // AmbientOcclusionRayGenerator::operator() (RayGenerators.h:281-343);
+    // rays in pixel row order (the tiled order measured slower here)
     __device__ bool start(uint32_t j, RayState& r, const TraceScene& S) const
     {
         const uint32_t ns = (uint32_t)ra.ao_samples;

@@ -49,8 +49,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 {
     if (frame_skipped(ra)) return;
     __shared__ float sub[kSubTable];
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = pix < npix;
+    uint32_t pix;
+    const bool active = tiled_pixel(blockIdx.x * blockDim.x + threadIdx.x, (uint32_t)cam.res_x,
+                                    (uint32_t)cam.res_y, pix);
     count_rays(ray_count, active);
     Work wk;
     GridRegs G = load_grid(S.gp);
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     if (frame_skipped(ra)) return;
     __shared__ WaveTests<kBatchK> lds[kBlock / kWave];
     WaveTests<kBatchK>& W = lds[__builtin_amdgcn_readfirstlane((int)threadIdx.x) / kWave];
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;   // row order: tiles measured slower here
     const bool active = pix < npix;
     count_rays(ray_count, active);
     Work wk;
@@ -554,11 +555,12 @@ void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs&
                                hits, hit_prims, wave_steps, ray_count);
         return;
     }
+    const uint32_t tile_blocks = blocks_for(tiled_slots((uint32_t)cam.res_x, (uint32_t)cam.res_y));
     if (S.two_level)
-        hipLaunchKernelGGL(k_primary<true>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, cam, ra, npix,
+        hipLaunchKernelGGL(k_primary<true>, dim3(tile_blocks), dim3(kBlock), 0, s, S, cam, ra, npix,
                            hits, hit_prims, ray_count);
     else
-        hipLaunchKernelGGL(k_primary<false>, dim3(blocks_for(npix)), dim3(kBlock), 0, s, S, cam, ra, npix,
+        hipLaunchKernelGGL(k_primary<false>, dim3(tile_blocks), dim3(kBlock), 0, s, S, cam, ra, npix,
                            hits, hit_prims, ray_count);
 }
 

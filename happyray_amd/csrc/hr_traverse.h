@@ -324,6 +324,29 @@ __device__ __forceinline__ void add_work(unsigned long long* stats, uint32_t ray
 }
 
 
+// Tiled pixel order for the one-ray-per-lane camera kernel: 8x8-pixel tiles
+// in row-major order, the pixels of a tile in Z order, so that a wave's 64
+// camera rays cover one tile and visit nearby cells (camera rays of the 1M-
+// triangle soup 0.679 -> 0.640 ms; the wave-batched camera kernel and the AO
+// rays measured slower in this order and keep row order).  Slots past the
+// image in its last row and column of tiles map to no pixel (false).  Only
+// the assignment of rays to lanes changes.
+__host__ __device__ __forceinline__ uint32_t tiled_slots(uint32_t res_x, uint32_t res_y)
+{
+    return ((res_x + 7u) >> 3) * ((res_y + 7u) >> 3) * 64u;
+}
+
+__device__ __forceinline__ bool tiled_pixel(uint32_t q, uint32_t res_x, uint32_t res_y, uint32_t& pix)
+{
+    const uint32_t tiles_x = (res_x + 7u) >> 3;
+    const uint32_t t = q >> 6, w = q & 63u;
+    const uint32_t lx = (w & 1u) | ((w >> 1) & 2u) | ((w >> 2) & 4u);
+    const uint32_t ly = ((w >> 1) & 1u) | ((w >> 2) & 2u) | ((w >> 3) & 4u);
+    const uint32_t x = (t % tiles_x) * 8u + lx, y = (t / tiles_x) * 8u + ly;
+    pix = y * res_x + x;
+    return x < res_x && y < res_y;
+}
+
 // RegularPixelSampler<1,1> / GaussianPixelSampler and Camera::getDirection
 // (src/RT/Primitive/Camera.h:68-131)
 __device__ __forceinline__ f3 primary_direction(const CameraDev& cam, const RenderArgs& ra,

@@ -124,6 +124,32 @@ def test_primary_and_ao_parity(cluster, kind, lt, ll, jitter):
     e.close()
 
 
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 1.0 / 16, 3.0)])
+def test_odd_resolution_parity(cluster, kind, lt, ll):
+    """An image that is not a whole number of 8x8 tiles (camera and AO rays
+    are handed to lanes in tiled order): every camera-ray kernel and both AO
+    kernels equal the oracle."""
+    cam_d = dict(cluster.camera, res=(70, 45))
+    og = oracle_grid(cluster, kind, lt, ll)
+    cam = O.camera(**{k: cam_d[k] for k in ("pos", "dir", "up", "fov")}, res_x=70, res_y=45)
+    p = O.params(seed=SEED, jitter=1, ao_samples=3, ao_dist=2.0, nthreads=8)
+    planes_o, prims_o, _ = O.primary(og, cam, p)
+    fb_o, occ_o, _ = O.render_ao(og, cam, p, cluster.materials, cluster.mat_idx)
+    hit = prims_o < cluster.nt
+    e = gpu_engine(cluster, kind, lt, ll)
+    e.set_camera_dict(cam_d)
+    for pp, pers in ((0, 1), (1, 1), (3, 1), (0, 0)):
+        print(f"odd resolution kind={kind} camera kernel={pp} persistent AO={pers}", flush=True)
+        e.set_tuning(persistent_primary=pp, persistent_secondary=pers)
+        fb_g = e.render_frame(0, MODE_AO, jitter=1, ao_samples=3, ao_dist=2.0)
+        planes_g, prims_g = e.primary_hits()
+        np.testing.assert_array_equal(prims_g, prims_o)
+        np.testing.assert_array_equal(planes_g[:, hit], planes_o[:, hit])
+        np.testing.assert_array_equal(e.ao_occlusion(3), occ_o)
+        assert_image_match(fb_g, fb_o)
+    e.close()
+
+
 @pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)])
 def test_path_tracing_parity(cluster, kind, lt, ll):
     print(f"PT parity kind={kind}", flush=True)
