@@ -106,10 +106,12 @@ void launch_trace_hier(const HierScene& H, const float* org, const float* dir, u
 // pers_primary / pers_ao: the camera / AO rays run in the persistent kernel,
 // which fetches rays through the device word *pool (reset here) whenever
 // `refill` lanes of a wave are idle; batch: with the triangle tests batched
-// across the wave (the same outputs)
+// across the wave (the same outputs); after_primary / after_ao (or nullptr):
+// events recorded after the camera rays and after the AO rays
 void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const RenderArgs& ra, float* hits,
                            uint32_t* hit_prims, uint32_t* hit_inst, uint8_t* occ, const Shading& sh, float* fb,
                            bool pers_primary, bool pers_ao, uint32_t* pool, uint32_t refill,
-                           unsigned long long* ray_count, hipStream_t s, bool batch = false);
+                           unsigned long long* ray_count, hipStream_t s, bool batch = false,
+                           hipEvent_t after_primary = nullptr, hipEvent_t after_ao = nullptr);
 
 }  // namespace hr

@@ -269,9 +269,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     HierScene H, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
     uint32_t* __restrict__ hit_prims, uint32_t* __restrict__ hit_inst, unsigned long long* ray_count)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    count_rays(ray_count, pix < npix);
-    if (pix >= npix) return;
+    // pixels in 8x8 tiles: a wave's camera rays enter the same instances
+    uint32_t pix;
+    const bool active = tiled_pixel(blockIdx.x * blockDim.x + threadIdx.x, (uint32_t)cam.res_x,
+                                    (uint32_t)cam.res_y, pix);
+    count_rays(ray_count, active);
+    if (!active) return;
     Work wk;
     const f3 d0 = primary_direction(cam, ra, pix, npix);
     const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
@@ -876,16 +879,19 @@ void launch_trace_hier(const HierScene& H, const float* org, const float* dir, u
 void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const RenderArgs& ra, float* hits,
                            uint32_t* hit_prims, uint32_t* hit_inst, uint8_t* occ, const Shading& sh, float* fb,
                            bool pers_primary, bool pers_ao, uint32_t* pool, uint32_t refill,
-                           unsigned long long* ray_count, hipStream_t s, bool batch)
+                           unsigned long long* ray_count, hipStream_t s, bool batch, hipEvent_t after_primary,
+                           hipEvent_t after_ao)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
     if (pers_primary) {
         const GenPrimaryHier g{ cam, ra, npix, hits, hit_prims, hit_inst };
         launch_hier_persist(H, g, npix, pool, refill, ray_count, s, batch);
     } else {
-        hipLaunchKernelGGL(k_primary_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, cam, ra, npix, hits,
+        const uint32_t slots = tiled_slots((uint32_t)cam.res_x, (uint32_t)cam.res_y);
+        hipLaunchKernelGGL(k_primary_hier, dim3(cdiv_u(slots, kBlock)), dim3(kBlock), 0, s, H, cam, ra, npix, hits,
                            hit_prims, hit_inst, ray_count);
     }
+    if (after_primary) (void)hipEventRecord(after_primary, s);
     const size_t nrays = (size_t)npix * (size_t)ra.ao_samples;
     if (pers_ao) {
         const GenAOHier g{ ra, npix, hits, hit_prims, hit_inst, occ };
@@ -894,6 +900,7 @@ void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const Rende
         hipLaunchKernelGGL(k_ao_rays_hier, dim3(cdiv_u(nrays, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits,
                            hit_prims, hit_inst, occ, ray_count);
     }
+    if (after_ao) (void)hipEventRecord(after_ao, s);
     hipLaunchKernelGGL(k_ao_shade_hier, dim3(cdiv_u(npix, kBlock)), dim3(kBlock), 0, s, H, ra, npix, hits, hit_prims,
                        hit_inst, occ, sh, fb);
 }
