@@ -746,10 +746,12 @@ hr_status check_render_args(hr_ctx* c, hr_mode mode, int image_id, const hr_rend
         return fail(HR_ERR_STATE, "the instanced hierarchy renders ambient occlusion only");
     if (mode == HR_MODE_AO && (p->ao_samples < 1 || p->ao_samples > 256))
         return fail(HR_ERR_ARG, "ao_samples must be in [1, 256]");
-    // AO ray ids are 32-bit (pixel * samples + sample), with room for the ray
-    // pool's overshoot
+    // AO ray ids are 32-bit (pixel * samples + sample, over the image rounded
+    // up to whole 8x8 tiles where rays are taken in tiled order), with room
+    // for the ray pool's overshoot
     if (mode == HR_MODE_AO &&
-        (uint64_t)c->cam.res_x * (uint64_t)c->cam.res_y * (uint64_t)p->ao_samples > 0xFFFFFFFFull - (1ull << 20))
+        (uint64_t)((c->cam.res_x + 7) / 8 * 8) * (uint64_t)((c->cam.res_y + 7) / 8 * 8) *
+                (uint64_t)p->ao_samples > 0xFFFFFFFFull - (1ull << 20))
         return fail(HR_ERR_ARG, "pixels * ao_samples exceeds 2^32 - 2^20 AO rays (32-bit ray ids)");
     if (mode == HR_MODE_PATH && (p->pt_depth < 1 || p->pt_depth > 64))
         return fail(HR_ERR_ARG, "pt_depth must be in [1, 64]");
