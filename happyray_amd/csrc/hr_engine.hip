@@ -1082,6 +1082,8 @@ hr_status prime_lane(hr_ctx* c, const hr_ctx::LaneFields& other, hr_mode mode, c
 hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
 {
     drop_graphs(c);
+    // nothing is in flight here and the halt word is clear: frames number from 0 again
+    c->frame_seq = 0;
     const int lanes = frame_lanes(c, mode, p);
     c->graph_lanes = lanes;
     const int home = c->lane;
@@ -1334,7 +1336,15 @@ size_t hr::ctx_device_bytes(hr_ctx* c)
     b += c->ray_wire.bytes() + c->tri_obj.bytes() + c->obj_first.bytes() + c->obj_bounds_enc.bytes();
     b += c->obj_cell_counts.bytes() + c->obj_cell_base.bytes() + c->inst_counts.bytes() + c->inst_offsets.bytes();
     b += c->inst.bytes() + c->inst_grid.bytes() + c->obj_gp.bytes() + c->hit_inst.bytes() + c->ray_inst.bytes() + c->obj_part.bytes();
-    b += c->bounds_part.bytes();
+    b += c->bounds_part.bytes() + c->frame_sel.bytes() + c->rad.bytes() + c->frame_rays.bytes();
+    // the other frame lane's buffers
+    const hr_ctx::LaneFields& o = c->stash;
+    b += o.params.bytes() + o.status.bytes() + o.bounds_part.bytes() + o.rec.bytes() + o.wave_counts.bytes();
+    b += o.wave_offsets.bytes() + o.keys.bytes() + o.vals.bytes() + o.keys_sorted.bytes() + o.top_vals_sorted.bytes();
+    b += o.prims.bytes() + o.leaf_counts.bytes() + o.cells.bytes() + o.leaves.bytes() + o.scan_chain.bytes();
+    b += o.sort_chain.bytes() + o.sort_keys_tmp.bytes() + o.sort_vals_tmp.bytes() + o.sort_hist.bytes();
+    b += o.verts.bytes() + o.normals.bytes() + o.hits.bytes() + o.hit_prims.bytes() + o.occ.bytes() + o.pool.bytes();
+    b += o.frame_sel.bytes() + o.rad.bytes() + o.frame_rays.bytes();
     return b;
 }
 
