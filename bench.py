@@ -697,8 +697,8 @@ def soup_two_lanes(scene, ao_dist, S, steps, warmup):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
-                                   "note (DESIGN.md), not a benchmark", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU only; sharding over several GPUs is "
+                                   "described, not benchmarked", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "keyframes_pt":
         return main_keyframes(args)

@@ -1,7 +1,7 @@
 /*
  * hr_oracle.h -- scalar CPU restatement of happyray's per-frame hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: it is linked or
  * loaded by tests/, by __graft_entry__.smoke() and by bench.py's
  * cpu_baseline leg, and by nothing else.  The product (happyray_amd/,
  * include/happyray.h) never calls it and never falls back to it.

@@ -1,4 +1,4 @@
-"""The reference-side lights binding of INTEGRATION.md compiles against
+"""The reference-side lights binding of the integration notes compiles against
 include/happyray.h and turns an AreaLightSourceCollection into hr_area_light
 records and weights as SceneLoader::createLightSources fills it
 (src/Application/SceneLoader.cpp:94-210: weight = area x |intensity|,
@@ -68,7 +68,7 @@ int main()
 def binding_source():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     m = re.search(r"// \[lights-binding\].*?\n(.*?)// \[/lights-binding\]", text, re.S)
-    assert m, "INTEGRATION.md has no lights-binding block"
+    assert m, "the integration notes have no lights-binding block"
     return m.group(1)
 
 
