@@ -175,8 +175,12 @@ struct hr_ctx {
     FramePin* h_pin = nullptr;
     hipEvent_t ev_g[2][4] = {};              // per slot: rebuild start, rebuild end, render end,
                                              // frame end (after its blend)
-    hipEvent_t ev_k[2][3] = {};              // per slot, in the render: after the camera rays,
-                                             // after the secondary rays, after the shading
+    hipEvent_t ev_k[2][5] = {};              // per slot, in the render: camera rays start / end,
+                                             // secondary rays start / end, shading end
+    hipEvent_t ev_built[2] = {};             // per lane: its last rebuild done (ordered lanes)
+    hipEvent_t ev_blend = nullptr;           // the last blend into the framebuffer (ordered lanes)
+    bool graph_ordered = false;              // two lanes, each frame's secondary rays alone
+    bool deferred = false;                   // the newest frame in flight waits for its secondary part
     hipEvent_t ev_switch = nullptr;          // the lane left at the last lane switch: its work so far
     Submitted pending[2] = {};
     int npending = 0, next_slot = 0;
@@ -871,16 +875,22 @@ int primary_kernel(hr_ctx* c, bool timed)
 
 // Primary rays and the integrator.  fs != nullptr: the image id is read on the
 // device (captured frames); events = false while a graph is being recorded.
+// phases: 1 the camera rays, 2 the secondary rays and the shading, 3 both
+// (the instanced hierarchy and the path-multi mode take 3 only)
+constexpr int kPhaseCamera = 1, kPhaseSecondary = 2, kPhaseAll = 3;
+
 hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_params* p,
                          const hr::FrameSel* fs, bool events, bool lane_out = false,
-                         const hipEvent_t* marks = nullptr)
+                         const hipEvent_t* marks = nullptr, int phases = kPhaseAll)
 {
-    // events after the camera rays, after the secondary rays and at the end:
-    // the context's (timed uncaptured frames) or `marks` (a captured frame's
-    // slot)
-    const hipEvent_t e_p1 = events ? c->ev_p1 : marks ? marks[0] : nullptr;
-    const hipEvent_t e_p2 = events ? c->ev_p2 : marks ? marks[1] : nullptr;
-    const hipEvent_t e_r1 = events ? c->ev_r1 : marks ? marks[2] : nullptr;
+    // events: the context's (timed uncaptured frames: after the camera rays,
+    // after the secondary rays, at the end) or a captured frame's slot marks
+    // (camera start, camera end, secondary start, secondary end, shading end)
+    const hipEvent_t e_c0 = marks ? marks[0] : nullptr;
+    const hipEvent_t e_p1 = events ? c->ev_p1 : marks ? marks[1] : nullptr;
+    const hipEvent_t e_c1 = marks ? marks[2] : nullptr;
+    const hipEvent_t e_p2 = events ? c->ev_p2 : marks ? marks[3] : nullptr;
+    const hipEvent_t e_r1 = events ? c->ev_r1 : marks ? marks[4] : nullptr;
     // lane_out: a frame lane's render (captured), into the lane's own value
     // buffer and ray counter, which k_frame_blend later blends in and adds
     float* const fb = lane_out ? c->rad.p : c->fb.p;
@@ -932,22 +942,29 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
     }
     if (mode == HR_MODE_PATH_MULTI && c->nlights == 0) {
         // PathTracer::integrate returns at once without lights (PathTracer.h:1049-1053)
-        if (e_p1) HR_HIP(hipEventRecord(e_p1, c->stream));
-        if (e_p2) HR_HIP(hipEventRecord(e_p2, c->stream));
-        if (e_r1) HR_HIP(hipEventRecord(e_r1, c->stream));
+        for (hipEvent_t e : { e_c0, e_p1, e_c1, e_p2, e_r1 })
+            if (e) HR_HIP(hipEventRecord(e, c->stream));
         return HR_OK;
     }
     const bool pers = c->tuning.persistent_secondary != 0;
     const uint32_t refill = (uint32_t)c->tuning.refill;
-    const int pk = primary_kernel(c, events);
-    c->last_pk = pk;
-    if (pk == 1)
-        hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, rays,
-                            c->stream, persist_opts(c));
-    else
-        hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, rays, c->stream,
-                           pk == 3 ? (uint32_t)c->tuning.advance_steps : 0u);
-    if (e_p1) HR_HIP(hipEventRecord(e_p1, c->stream));
+    if (phases & kPhaseCamera) {
+        const int pk = primary_kernel(c, events);
+        c->last_pk = pk;
+        if (e_c0) HR_HIP(hipEventRecord(e_c0, c->stream));
+        if (pk == 1)
+            hr::persist_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, c->pool.p, refill, rays,
+                                c->stream, persist_opts(c));
+        else
+            hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, rays, c->stream,
+                               pk == 3 ? (uint32_t)c->tuning.advance_steps : 0u);
+        if (e_p1) HR_HIP(hipEventRecord(e_p1, c->stream));
+    }
+    if (!(phases & kPhaseSecondary)) {
+        HR_HIP(hipGetLastError());
+        return HR_OK;
+    }
+    if (e_c1) HR_HIP(hipEventRecord(e_c1, c->stream));
     if (ra.stats) ra.stats = c->work.p + 3;
     if (mode == HR_MODE_AO) {
         if (pers)
@@ -1053,7 +1070,7 @@ hr_status end_capture(hr_ctx* c, hipGraphExec_t* out)
 // statistics are not split per frame, so both keep to one lane
 int frame_lanes(const hr_ctx* c, hr_mode mode, const hr_render_params* p)
 {
-    return c->tuning.frame_lanes == 2 && mode != HR_MODE_PATH_MULTI && !p->collect_stats ? 2 : 1;
+    return c->tuning.frame_lanes >= 2 && mode != HR_MODE_PATH_MULTI && !p->collect_stats ? 2 : 1;
 }
 
 int slot_lane(const hr_ctx* c, int slot) { return c->graph_lanes == 2 ? slot : 0; }
@@ -1086,6 +1103,7 @@ hr_status record_graphs(hr_ctx* c, hr_mode mode, const hr_render_params* p)
     c->frame_seq = 0;
     const int lanes = frame_lanes(c, mode, p);
     c->graph_lanes = lanes;
+    c->graph_ordered = lanes == 2 && c->tuning.frame_lanes == 3;
     const int home = c->lane;
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
     hr_status st = HR_OK;
@@ -1141,6 +1159,7 @@ bool graphs_ready(hr_ctx* c, hr_mode mode, const hr_render_params* p)
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
     return c->g_build[0] && c->g_build[1] && npix == c->last_npix &&
            primary_decided(c) && c->graph_lanes == frame_lanes(c, mode, p) &&
+           c->graph_ordered == (c->graph_lanes == 2 && c->tuning.frame_lanes == 3) &&
            c->graph_sig[c->lane] == graph_signature(c, mode, p);
 }
 
@@ -1196,6 +1215,7 @@ hr_status recover_frames(hr_ctx* c)
     const int n = c->npending;
     for (int i = 0; i < n; ++i) redo[i] = c->pending[i];
     c->npending = 0;
+    c->deferred = false;
     drop_graphs(c);
     hr::fill_u32(c->halt.p, 0xFFFFFFFFu, 1, nullptr, 1, c->stream);
     const hr_mode mode = c->graph_mode;
@@ -1208,8 +1228,11 @@ hr_status recover_frames(hr_ctx* c)
 
 // waits for the older frame in flight and takes its results (grid info,
 // timing), or recovers it and the frames behind it
+hr_status finish_deferred(hr_ctx* c, hipEvent_t after);
+
 hr_status complete_oldest(hr_ctx* c)
 {
+    if (c->deferred && c->npending == 1) HR_TRY(finish_deferred(c, nullptr));
     const hr_ctx::Submitted f = c->pending[0];
     HR_HIP(hipEventSynchronize(c->ev_g[f.slot][3]));
     const hr_ctx::FramePin& pin = c->h_pin[f.slot];
@@ -1220,8 +1243,8 @@ hr_status complete_oldest(hr_ctx* c)
     const hipEvent_t* k = c->ev_k[f.slot];
     float* outs[5] = { &c->graph_build_ms, &c->graph_render_ms, &c->graph_primary_ms, &c->graph_secondary_ms,
                        &c->graph_shade_ms };
-    const hipEvent_t from[5] = { g[0], g[1], g[1], k[0], k[1] };
-    const hipEvent_t to[5] = { g[1], g[2], k[0], k[1], k[2] };
+    const hipEvent_t from[5] = { g[0], k[0], k[0], k[2], k[3] };
+    const hipEvent_t to[5] = { g[1], k[4], k[1], k[3], k[4] };
     // a frame with an interval that cannot be read stays out of the sums
     bool all = true;
     for (int i = 0; i < 5; ++i) {
@@ -1255,8 +1278,33 @@ hr_status complete_oldest(hr_ctx* c)
 // grid and hit buffers
 hr_status settle_frames(hr_ctx* c)
 {
+    if (c->deferred) HR_TRY(finish_deferred(c, nullptr));
     while (c->npending > 0) HR_TRY(complete_oldest(c));
     return HR_OK;
+}
+
+// Ordered lanes: the secondary rays, shading and blend of the newest frame in
+// flight, on its own lane, after `after` (the next frame's rebuild, so that
+// the rebuild runs beside this frame's camera rays and not beside its
+// secondary rays), and its blend after the previous frame's
+hr_status finish_deferred(hr_ctx* c, hipEvent_t after)
+{
+    const hr_ctx::Submitted d = c->pending[c->npending - 1];
+    const int back = c->lane;
+    HR_TRY(use_lane(c, slot_lane(c, d.slot)));
+    if (after) HR_HIP(hipStreamWaitEvent(c->stream, after, 0));
+    HR_TRY(enqueue_render(c, c->graph_mode, 0, &c->graph_params, c->frame_sel.p, false, true, c->ev_k[d.slot],
+                          kPhaseSecondary));
+    HR_HIP(hipEventRecord(c->ev_g[d.slot][2], c->stream));
+    HR_HIP(hipStreamWaitEvent(c->stream, c->ev_blend, 0));
+    const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
+    hr::launch_frame_blend(c->frame_sel.p, c->halt.p, c->rad.p, c->fb.p, npix, c->frame_rays.p, c->ray_count.p,
+                           c->stream);
+    HR_HIP(hipGetLastError());
+    HR_HIP(hipEventRecord(c->ev_blend, c->stream));
+    HR_HIP(hipEventRecord(c->ev_g[d.slot][3], c->stream));
+    c->deferred = false;
+    return use_lane(c, back);
 }
 
 // Queues one captured frame.  With the graphs recorded for this state the
@@ -1296,6 +1344,22 @@ hr_status submit_frame(hr_ctx* c, const hr_ctx::Submitted& f, hr_mode mode, cons
     HR_HIP(hipEventRecord(c->ev_g[s][0], c->stream));
     HR_HIP(hipGraphLaunch(c->g_build[s], c->stream));
     HR_HIP(hipEventRecord(c->ev_g[s][1], c->stream));
+    if (c->graph_ordered) {
+        // this rebuild runs beside the previous frame's camera rays; that
+        // frame's secondary rays follow it, and this frame's camera rays follow
+        // those, so the secondary rays have the GPU to themselves
+        HR_HIP(hipEventRecord(c->ev_built[L], c->stream));
+        if (c->deferred) {
+            const hr_ctx::Submitted prev = c->pending[c->npending - 1];
+            HR_TRY(finish_deferred(c, c->ev_built[L]));
+            HR_HIP(hipStreamWaitEvent(c->stream, c->ev_k[prev.slot][3], 0));
+        }
+        HR_TRY(enqueue_render(c, mode, 0, p, c->frame_sel.p, false, true, c->ev_k[s], kPhaseCamera));
+        c->pending[c->npending++] = q;
+        c->deferred = true;
+        c->next_slot = s ^ 1;
+        return HR_OK;
+    }
     // the render: launched directly (its few launches cost the host little, and
     // frames run ahead of the host), timed by ordinary events
     const uint32_t npix = (uint32_t)c->cam.res_x * (uint32_t)c->cam.res_y;
@@ -1463,6 +1527,10 @@ hr_status hr_create(hr_ctx** out, const hr_options* opts)
                 e = hipEventCreate(&ev);
                 if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
             }
+        for (hipEvent_t* ev : { &c->ev_built[0], &c->ev_built[1], &c->ev_blend }) {
+            e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+            if (e != hipSuccess) { s = fail(HR_ERR_DEVICE, hipGetErrorString(e)); break; }
+        }
         if (s != HR_OK) break;
         if ((s = noalloc_scratch(c->device, &c->noalloc)) != HR_OK) break;
         hipEvent_t* evs[6] = { &c->ev_b0, &c->ev_b1, &c->ev_r0, &c->ev_r1, &c->ev_p1, &c->ev_p2 };
@@ -1540,6 +1608,8 @@ hr_status hr_destroy(hr_ctx* c)
     for (auto& slot : c->ev_k)
         for (hipEvent_t ev : slot)
             if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : { c->ev_built[0], c->ev_built[1], c->ev_blend })
+        if (ev) (void)hipEventDestroy(ev);
     hipEvent_t evs[6] = { c->ev_b0, c->ev_b1, c->ev_r0, c->ev_r1, c->ev_p1, c->ev_p2 };
     for (hipEvent_t ev : evs)
         if (ev) (void)hipEventDestroy(ev);
@@ -1706,6 +1776,7 @@ hr_status hr_set_grid(hr_ctx* c, hr_grid_kind kind, float lambda_top, float lamb
 hr_status hr_build(hr_ctx* c)
 {
     HR_TRY(check_ctx(c));
+    c->graph_timed = false;   // hr_get_timing reports this build, not the last replay
     return do_build(c);
 }
 
@@ -2157,7 +2228,7 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
     if (t->fuse_units != 0 && t->fuse_units != 1) return fail(HR_ERR_ARG, "fuse_units must be 0 or 1");
     if (t->test_batch != 0 && t->test_batch != 1) return fail(HR_ERR_ARG, "test_batch must be 0 or 1");
     if (t->advance_steps < 1 || t->advance_steps > 16) return fail(HR_ERR_ARG, "advance_steps must be in [1, 16]");
-    if (t->frame_lanes < 1 || t->frame_lanes > 2) return fail(HR_ERR_ARG, "frame_lanes must be 1 or 2");
+    if (t->frame_lanes < 1 || t->frame_lanes > 3) return fail(HR_ERR_ARG, "frame_lanes must be 1, 2 or 3");
     c->tuning = *t;
     reset_primary_choice(c);
     return HR_OK;

@@ -362,9 +362,12 @@ typedef struct {
                                   buffers, each with its own stream, taken in turn, so
                                   that a frame's rebuild and render overlap the previous
                                   frame's render; each lane renders into a buffer of its
-                                  own, blended into the framebuffer in frame order.  1
-                                  one lane.  Same results; the multi-kernel path tracer
-                                  (mode 1) and renders with collect_stats use one lane */
+                                  own, blended into the framebuffer in frame order.  3
+                                  two lanes where a frame's rebuild runs beside the
+                                  previous frame's camera rays and each frame's
+                                  secondary rays run alone.  1 one lane.  Same results;
+                                  the multi-kernel path tracer (mode 1) and renders with
+                                  collect_stats use one lane                            */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

@@ -35,7 +35,7 @@ def grids_equal(a, b):
     np.testing.assert_array_equal(pa, pb)
 
 
-@pytest.mark.parametrize("lanes", [2, 1])
+@pytest.mark.parametrize("lanes", [2, 3, 1])
 @pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_SIMPLE, MODE_PATH_MULTI])
 def test_frame_graph_matches_uncaptured_frames(mode, lanes):
     sc, keys = scenes.heightfield_keyframes(quads=64, frames=5, res=(64, 48))
@@ -108,7 +108,7 @@ def test_frame_graph_regrows_after_overflow():
         e.close()
 
 
-@pytest.mark.parametrize("lanes", [2, 1])
+@pytest.mark.parametrize("lanes", [2, 3, 1])
 @pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_PATH_MULTI])
 def test_overflowed_replay_keeps_running_mean(mode, lanes):
     """A replay whose rebuild overflows is skipped on the device, so the
@@ -153,7 +153,7 @@ def run_called(e, frames, mode, kw):
     return fbs
 
 
-@pytest.mark.parametrize("lanes", [2, 1])
+@pytest.mark.parametrize("lanes", [2, 3, 1])
 @pytest.mark.parametrize("mode", [MODE_PATH, MODE_AO, MODE_SIMPLE])
 def test_submitted_frames_match_called_frames(mode, lanes):
     """Frames queued by hr_submit_frame_graph (two in flight) give the same
@@ -188,7 +188,7 @@ def test_submitted_frames_match_called_frames(mode, lanes):
     e.close()
 
 
-@pytest.mark.parametrize("lanes", [2, 1])
+@pytest.mark.parametrize("lanes", [2, 3, 1])
 def test_submitted_frames_recover_from_overflow(lanes):
     """An overflowing replay with another frame queued behind it: both are
     skipped on the device and rendered again when the host completes them
