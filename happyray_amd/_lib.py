@@ -70,7 +70,7 @@ class Tuning(C.Structure):
                 ("refill", C.c_int), ("hier_refill", C.c_int), ("hier_bounds_blocks", C.c_int),
                 ("persistent_path", C.c_int), ("path_refill", C.c_int), ("path_shade", C.c_int),
                 ("fuse_units", C.c_int), ("test_batch", C.c_int), ("advance_steps", C.c_int),
-                ("frame_lanes", C.c_int)]
+                ("frame_lanes", C.c_int), ("camera_blocks", C.c_int)]
 
 
 class WorkCounts(C.Structure):

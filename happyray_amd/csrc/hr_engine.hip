@@ -137,7 +137,7 @@ struct hr_ctx {
     DevBuf<unsigned long long> ray_count;
     DevBuf<unsigned long long> work;     // {rays, steps, tests} x {primary, secondary}
     DevBuf<uint32_t> pool;               // persistent kernels' ray counter
-    hr_tuning tuning{ 2, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2, 2 };
+    hr_tuning tuning{ 2, 1, 56, 12, 0, 0, 32, 16, 1, 1, 2, 2, 4 };
     // persistent_primary 2: the camera-ray kernel is chosen by timing two frames
     // of each (0 one ray per lane, 1 persistent, 3 wave-batched tests) after the
     // scene, grid or camera changes; results are the same either way
@@ -957,7 +957,8 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
                                 c->stream, persist_opts(c));
         else
             hr::launch_primary(S, c->cam, ra, c->hits.p, c->hit_prims.p, rays, c->stream,
-                               pk == 3 ? (uint32_t)c->tuning.advance_steps : 0u);
+                               pk == 3 ? (uint32_t)c->tuning.advance_steps : 0u,
+                               (uint32_t)c->tuning.camera_blocks, c->pool.p);
         if (e_p1) HR_HIP(hipEventRecord(e_p1, c->stream));
     }
     if (!(phases & kPhaseSecondary)) {
@@ -2229,6 +2230,7 @@ hr_status hr_set_tuning(hr_ctx* c, const hr_tuning* t)
     if (t->test_batch != 0 && t->test_batch != 1) return fail(HR_ERR_ARG, "test_batch must be 0 or 1");
     if (t->advance_steps < 1 || t->advance_steps > 16) return fail(HR_ERR_ARG, "advance_steps must be in [1, 16]");
     if (t->frame_lanes < 1 || t->frame_lanes > 3) return fail(HR_ERR_ARG, "frame_lanes must be 1, 2 or 3");
+    if (t->camera_blocks < 0 || t->camera_blocks > 16) return fail(HR_ERR_ARG, "camera_blocks must be in [0, 16]");
     c->tuning = *t;
     reset_primary_choice(c);
     return HR_OK;

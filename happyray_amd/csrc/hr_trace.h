@@ -75,10 +75,13 @@ struct Shading {
 };
 
 // hit buffer: 7 SoA planes of npix floats (hit point xyz, dir xyz, t) + prims
-// wave_steps > 0: k_primary_wave (batched candidate tests); the same hits
+// wave_steps > 0: k_primary_wave (batched candidate tests); the same hits.
+// Otherwise k_primary: blocks_per_cu > 0 (with next, a device word) runs that
+// many workgroups per CU taking 8x8 pixel tiles from a counter, 0 one
+// workgroup per 256 pixels; the same hits either way
 void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
                     uint32_t* hit_prims, unsigned long long* ray_count, hipStream_t s,
-                    uint32_t wave_steps = 0);
+                    uint32_t wave_steps = 0, uint32_t blocks_per_cu = 0, uint32_t* next = nullptr);
 void launch_ao_rays(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,
                     const uint32_t* hit_prims, uint8_t* occ, unsigned long long* ray_count, hipStream_t s);
 void launch_ao_shade(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,

@@ -41,39 +41,66 @@ using namespace trav;
 // primary rays
 // ---------------------------------------------------------------------------
 
-// six waves per SIMD (at most 80 VGPRs): 0.695 -> 0.682 ms for 1024x1024 camera rays
+// one camera ray per lane: slot q of the 8x8 pixel tiles; false for a slot
+// past the image's edge
+template <bool kTwo>
+__device__ __forceinline__ bool camera_ray(const TraceScene& S, const GridRegs& G, const CameraDev& cam,
+                                           const RenderArgs& ra, uint32_t npix, uint32_t q, float* __restrict__ hits,
+                                           uint32_t* __restrict__ hit_prims, unsigned long long* ray_count, Work& wk)
+{
+    uint32_t pix;
+    const bool active = tiled_pixel(q, (uint32_t)cam.res_x, (uint32_t)cam.res_y, pix);
+    count_rays(ray_count, active);
+    if (!active) return false;
+    const f3 d0 = primary_direction(cam, ra, pix, npix);
+    const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
+    float t = kFltMax;
+    f3 dir;
+    const uint32_t best = trace_ray<kTwo, false>(G, S, org, d0, t, 0.f, dir, wk);
+    const uint32_t prim = t < kFltMax ? best : S.nt;
+    const f3 hp = org + (t - kEps) * dir;
+    hits[pix] = hp.x;
+    hits[(size_t)npix + pix] = hp.y;
+    hits[2 * (size_t)npix + pix] = hp.z;
+    hits[3 * (size_t)npix + pix] = dir.x;
+    hits[4 * (size_t)npix + pix] = dir.y;
+    hits[5 * (size_t)npix + pix] = dir.z;
+    hits[6 * (size_t)npix + pix] = t;
+    hit_prims[pix] = prim;
+    return true;
+}
+
+// Camera rays, one per lane, in 8x8 pixel tiles (a wave's 64 rays cover one
+// tile).  tiles == 0: workgroup b takes the four tiles of slots 256 b ..;
+// otherwise the grid is a few workgroups per CU whose waves take tiles from
+// the counter *next (zero at launch) until all `tiles` are taken, so the
+// kernel leaves wave slots to work on other streams (a neighbouring frame's
+// rebuild or secondary rays).  Six waves per SIMD (at most 80 VGPRs).
 template <bool kTwo>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_primary(
     TraceScene S, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
-    uint32_t* __restrict__ hit_prims, unsigned long long* ray_count)
+    uint32_t* __restrict__ hit_prims, unsigned long long* ray_count, uint32_t tiles, uint32_t* __restrict__ next)
 {
     if (frame_skipped(ra)) return;
     __shared__ float sub[kSubTable];
-    uint32_t pix;
-    const bool active = tiled_pixel(blockIdx.x * blockDim.x + threadIdx.x, (uint32_t)cam.res_x,
-                                    (uint32_t)cam.res_y, pix);
-    count_rays(ray_count, active);
     Work wk;
     GridRegs G = load_grid(S.gp);
     if (kTwo) sub_table(G, sub);
-    if (active) {
-        const f3 d0 = primary_direction(cam, ra, pix, npix);
-        const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
-        float t = kFltMax;
-        f3 dir;
-        const uint32_t best = trace_ray<kTwo, false>(G, S, org, d0, t, 0.f, dir, wk);
-        const uint32_t prim = t < kFltMax ? best : S.nt;
-        const f3 hp = org + (t - kEps) * dir;
-        hits[pix] = hp.x;
-        hits[(size_t)npix + pix] = hp.y;
-        hits[2 * (size_t)npix + pix] = hp.z;
-        hits[3 * (size_t)npix + pix] = dir.x;
-        hits[4 * (size_t)npix + pix] = dir.y;
-        hits[5 * (size_t)npix + pix] = dir.z;
-        hits[6 * (size_t)npix + pix] = t;
-        hit_prims[pix] = prim;
+    uint32_t traced = 0;
+    if (tiles == 0) {
+        traced = camera_ray<kTwo>(S, G, cam, ra, npix, blockIdx.x * blockDim.x + threadIdx.x, hits, hit_prims,
+                                  ray_count, wk) ? 1u : 0u;
+    } else {
+        while (true) {
+            uint32_t tile = 0;
+            if (__lane_id() == 0) tile = atomicAdd(next, 1u);
+            tile = __shfl(tile, 0, kWave);
+            if (tile >= tiles) break;
+            traced += camera_ray<kTwo>(S, G, cam, ra, npix, tile * kWave + __lane_id(), hits, hit_prims, ray_count,
+                                       wk) ? 1u : 0u;
+        }
     }
-    add_work(ra.stats, active ? 1u : 0u, wk);
+    add_work(ra.stats, traced, wk);
 }
 
 // camera rays through trace_wave: one ray per lane as in k_primary, with the
@@ -540,10 +567,23 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(TraceScene S, const floa
 
 inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
+uint32_t device_cus()
+{
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cached = cus > 0 ? cus : 256;
+    }
+    return (uint32_t)cached;
+}
+
 }  // namespace
 
 void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs& ra, float* hits,
-                    uint32_t* hit_prims, unsigned long long* ray_count, hipStream_t s, uint32_t wave_steps)
+                    uint32_t* hit_prims, unsigned long long* ray_count, hipStream_t s, uint32_t wave_steps,
+                    uint32_t blocks_per_cu, uint32_t* next)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
     if (wave_steps) {
@@ -555,13 +595,19 @@ void launch_primary(const TraceScene& S, const CameraDev& cam, const RenderArgs&
                                hits, hit_prims, wave_steps, ray_count);
         return;
     }
-    const uint32_t tile_blocks = blocks_for(tiled_slots((uint32_t)cam.res_x, (uint32_t)cam.res_y));
+    const uint32_t slots = tiled_slots((uint32_t)cam.res_x, (uint32_t)cam.res_y);
+    uint32_t blocks = blocks_for(slots), tiles = 0;
+    if (blocks_per_cu > 0 && next && (size_t)blocks_per_cu * device_cus() < blocks) {
+        blocks = blocks_per_cu * device_cus();
+        tiles = slots / kWave;
+        fill_u32(next, 0u, 1, nullptr, 1, s);
+    }
     if (S.two_level)
-        hipLaunchKernelGGL(k_primary<true>, dim3(tile_blocks), dim3(kBlock), 0, s, S, cam, ra, npix,
-                           hits, hit_prims, ray_count);
+        hipLaunchKernelGGL(k_primary<true>, dim3(blocks), dim3(kBlock), 0, s, S, cam, ra, npix,
+                           hits, hit_prims, ray_count, tiles, next);
     else
-        hipLaunchKernelGGL(k_primary<false>, dim3(tile_blocks), dim3(kBlock), 0, s, S, cam, ra, npix,
-                           hits, hit_prims, ray_count);
+        hipLaunchKernelGGL(k_primary<false>, dim3(blocks), dim3(kBlock), 0, s, S, cam, ra, npix,
+                           hits, hit_prims, ray_count, tiles, next);
 }
 
 void launch_ao_rays(const TraceScene& S, const RenderArgs& ra, uint32_t npix, const float* hits,

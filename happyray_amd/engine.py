@@ -376,7 +376,8 @@ class Engine:
 
     def set_tuning(self, persistent_primary=None, persistent_secondary=None, refill=None, hier_refill=None,
                    hier_bounds_blocks=None, persistent_path=None, path_refill=None, path_shade=None,
-                   fuse_units=None, test_batch=None, advance_steps=None, frame_lanes=None):
+                   fuse_units=None, test_batch=None, advance_steps=None, frame_lanes=None,
+                   camera_blocks=None):
         """Kernel choices (hr_tuning).  Every field keeps its current value
         when not given."""
         cur = self.tuning()
@@ -390,7 +391,7 @@ class Engine:
                         pick(persistent_path, "persistent_path"), pick(path_refill, "path_refill"),
                         pick(path_shade, "path_shade"), pick(fuse_units, "fuse_units"),
                         pick(test_batch, "test_batch"), pick(advance_steps, "advance_steps"),
-                        pick(frame_lanes, "frame_lanes"))
+                        pick(frame_lanes, "frame_lanes"), pick(camera_blocks, "camera_blocks"))
         check(self._L.hr_set_tuning(self._h, C.byref(t)), "hr_set_tuning")
 
     def tuning(self):

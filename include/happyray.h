@@ -368,6 +368,12 @@ typedef struct {
                                   secondary rays run alone.  1 one lane.  Same results;
                                   the multi-kernel path tracer (mode 1) and renders with
                                   collect_stats use one lane                            */
+    int camera_blocks;         /* camera rays one per lane (persistent_primary 0, or
+                                  that kernel chosen under 2):
+                                  k in 1..16 (default 4) runs k workgroups per CU whose
+                                  waves take 8x8 pixel tiles from a counter, leaving
+                                  wave slots to a neighbouring frame's work; 0 one
+                                  workgroup per 256 pixels.  Same results              */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

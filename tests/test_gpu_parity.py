@@ -150,6 +150,34 @@ def test_odd_resolution_parity(cluster, kind, lt, ll):
     e.close()
 
 
+@pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 1.0 / 16, 3.0)])
+def test_camera_tiles_from_counter_parity(cluster, kind, lt, ll):
+    """Camera rays whose 8x8 tiles are taken from a counter by one workgroup
+    per CU (camera_blocks 1: 256 workgroups for the 1,344 tiles of a 333x251
+    image) equal the oracle's, as do those of one workgroup per 256 pixels,
+    and both count the same rays."""
+    cam_d = dict(cluster.camera, res=(333, 251))
+    og = oracle_grid(cluster, kind, lt, ll)
+    cam = O.camera(**{k: cam_d[k] for k in ("pos", "dir", "up", "fov")}, res_x=333, res_y=251)
+    p = O.params(seed=SEED, jitter=1, ao_samples=1, ao_dist=2.0, nthreads=8)
+    planes_o, prims_o, _ = O.primary(og, cam, p)
+    hit = prims_o < cluster.nt
+    e = gpu_engine(cluster, kind, lt, ll)
+    e.set_camera_dict(cam_d)
+    counts = []
+    for blocks in (1, 0):
+        print(f"camera tiles kind={kind} camera_blocks={blocks}", flush=True)
+        e.set_tuning(persistent_primary=0, camera_blocks=blocks)
+        r0 = e.timing().rays_traced
+        e.render_frame(0, MODE_AO, jitter=1, ao_samples=1, ao_dist=2.0, download=False)
+        counts.append(e.timing().rays_traced - r0)
+        planes_g, prims_g = e.primary_hits()
+        np.testing.assert_array_equal(prims_g, prims_o)
+        np.testing.assert_array_equal(planes_g[:, hit], planes_o[:, hit])
+    assert counts[0] == counts[1] >= 333 * 251
+    e.close()
+
+
 @pytest.mark.parametrize("kind,lt,ll", [(GRID_UNIFORM_EXACT, 5.0, 1.0), (GRID_TWO_LEVEL, 0.5, 3.0)])
 def test_path_tracing_parity(cluster, kind, lt, ll):
     print(f"PT parity kind={kind}", flush=True)
