@@ -933,7 +933,8 @@ hr_status enqueue_render(hr_ctx* c, hr_mode mode, int image_id, const hr_render_
         hr::launch_render_ao_hier(hier_scene(c), c->cam, ra, c->hits.p, c->hit_prims.p, c->hit_inst.p, c->occ.p, sh,
                                   fb, c->tuning.persistent_primary == 1, c->tuning.persistent_secondary != 0,
                                   c->pool.p, (uint32_t)c->tuning.hier_refill, rays, c->stream,
-                                  c->tuning.test_batch != 0, e_p1, e_p2);
+                                  c->tuning.test_batch != 0, e_p1, e_p2,
+                                  c->tuning.camera_blocks > 0 ? hr::kHierCameraBlocks : 0u);
         HR_HIP(hipGetLastError());
         if (e_r1) HR_HIP(hipEventRecord(e_r1, c->stream));
         c->last_ao_samples = p->ao_samples;

@@ -23,6 +23,18 @@ using namespace trav;
 
 inline uint32_t cdiv_u(size_t a, size_t b) { return (uint32_t)((a + b - 1) / b); }
 
+uint32_t device_cus()
+{
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cached = cus > 0 ? cus : 256;
+    }
+    return (uint32_t)cached;
+}
+
 // ---------------------------------------------------------------------------
 // traversal
 // ---------------------------------------------------------------------------
@@ -264,17 +276,17 @@ __global__ __launch_bounds__(kBlock) void k_trace_hier(HierScene H, const float*
 // TLGHAOIntegrator
 // ---------------------------------------------------------------------------
 
-// six waves per SIMD (at most 80 VGPRs): render 32.1 -> 31.6 ms on 10,000 instances
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_primary_hier(
-    HierScene H, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
-    uint32_t* __restrict__ hit_prims, uint32_t* __restrict__ hit_inst, unsigned long long* ray_count)
+// the camera ray of slot q of the 8x8 pixel tiles (a wave's rays enter the
+// same instances); false for a slot past the image's edge
+__device__ __forceinline__ bool hier_camera_ray(const HierScene& H, const CameraDev& cam, const RenderArgs& ra,
+                                                uint32_t npix, uint32_t q, float* __restrict__ hits,
+                                                uint32_t* __restrict__ hit_prims, uint32_t* __restrict__ hit_inst,
+                                                unsigned long long* ray_count)
 {
-    // pixels in 8x8 tiles: a wave's camera rays enter the same instances
     uint32_t pix;
-    const bool active = tiled_pixel(blockIdx.x * blockDim.x + threadIdx.x, (uint32_t)cam.res_x,
-                                    (uint32_t)cam.res_y, pix);
+    const bool active = tiled_pixel(q, (uint32_t)cam.res_x, (uint32_t)cam.res_y, pix);
     count_rays(ray_count, active);
-    if (!active) return;
+    if (!active) return false;
     Work wk;
     const f3 d0 = primary_direction(cam, ra, pix, npix);
     const f3 org = mk(cam.center[0], cam.center[1], cam.center[2]);
@@ -292,6 +304,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     hits[6 * (size_t)npix + pix] = t;
     hit_prims[pix] = t < kFltMax ? best : H.nt;
     hit_inst[pix] = t < kFltMax ? inst : 0xFFFFFFFFu;
+    return true;
+}
+
+// camera rays one per lane.  tiles == 0: workgroup b takes slots 256 b ..;
+// otherwise a few workgroups per CU whose waves take 8x8 tiles from the
+// counter *next (zero at launch) until all `tiles` are taken.  Six waves per
+// SIMD (at most 80 VGPRs): render 32.1 -> 31.6 ms on 10,000 instances
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_primary_hier(
+    HierScene H, CameraDev cam, RenderArgs ra, uint32_t npix, float* __restrict__ hits,
+    uint32_t* __restrict__ hit_prims, uint32_t* __restrict__ hit_inst, unsigned long long* ray_count,
+    uint32_t tiles, uint32_t* __restrict__ next)
+{
+    if (tiles == 0) {
+        (void)hier_camera_ray(H, cam, ra, npix, blockIdx.x * blockDim.x + threadIdx.x, hits, hit_prims, hit_inst,
+                              ray_count);
+        return;
+    }
+    while (true) {
+        uint32_t tile = 0;
+        if (__lane_id() == 0) tile = atomicAdd(next, 1u);
+        tile = __shfl(tile, 0, kWave);
+        if (tile >= tiles) break;
+        (void)hier_camera_ray(H, cam, ra, npix, tile * kWave + __lane_id(), hits, hit_prims, hit_inst, ray_count);
+    }
 }
 
 // the hit's face normal in world coordinates (TLGridHierarchyAOIntegrator.h:316-357)
@@ -880,7 +916,7 @@ void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const Rende
                            uint32_t* hit_prims, uint32_t* hit_inst, uint8_t* occ, const Shading& sh, float* fb,
                            bool pers_primary, bool pers_ao, uint32_t* pool, uint32_t refill,
                            unsigned long long* ray_count, hipStream_t s, bool batch, hipEvent_t after_primary,
-                           hipEvent_t after_ao)
+                           hipEvent_t after_ao, uint32_t camera_blocks)
 {
     const uint32_t npix = (uint32_t)cam.res_x * (uint32_t)cam.res_y;
     if (pers_primary) {
@@ -888,8 +924,14 @@ void launch_render_ao_hier(const HierScene& H, const CameraDev& cam, const Rende
         launch_hier_persist(H, g, npix, pool, refill, ray_count, s, batch);
     } else {
         const uint32_t slots = tiled_slots((uint32_t)cam.res_x, (uint32_t)cam.res_y);
-        hipLaunchKernelGGL(k_primary_hier, dim3(cdiv_u(slots, kBlock)), dim3(kBlock), 0, s, H, cam, ra, npix, hits,
-                           hit_prims, hit_inst, ray_count);
+        uint32_t blocks = cdiv_u(slots, kBlock), tiles = 0;
+        if (camera_blocks > 0 && (size_t)camera_blocks * device_cus() < blocks) {
+            blocks = camera_blocks * device_cus();
+            tiles = slots / kWave;
+            fill_u32(pool, 0u, 1, nullptr, 1, s);
+        }
+        hipLaunchKernelGGL(k_primary_hier, dim3(blocks), dim3(kBlock), 0, s, H, cam, ra, npix, hits, hit_prims,
+                           hit_inst, ray_count, tiles, pool);
     }
     if (after_primary) (void)hipEventRecord(after_primary, s);
     const size_t nrays = (size_t)npix * (size_t)ra.ao_samples;

@@ -372,8 +372,10 @@ typedef struct {
                                   that kernel chosen under 2):
                                   k in 1..16 (default 4) runs k workgroups per CU whose
                                   waves take 8x8 pixel tiles from a counter, leaving
-                                  wave slots to a neighbouring frame's work; 0 one
-                                  workgroup per 256 pixels.  Same results              */
+                                  wave slots to a neighbouring frame's work (the
+                                  instanced hierarchy's camera kernel takes its
+                                  occupancy, six, for any k); 0 one workgroup per 256
+                                  pixels.  Same results                                  */
 } hr_tuning;
 hr_status hr_set_tuning(hr_ctx* ctx, const hr_tuning* t);
 hr_status hr_get_tuning(hr_ctx* ctx, hr_tuning* t);

@@ -217,12 +217,15 @@ def test_hier_scene_file(tmp_path):
 
 def test_hier_full_size_kernels_agree():
     """The benchmark's instanced scene (10,000 instances, 121k-triangle mesh)
-    at 256x256 with 4 AO rays per hit: the persistent kernels and the
-    one-ray-per-lane kernels give identical primary hits, instances and
-    occlusion bits, and every AO ray whose pixel has a hit is counted."""
+    at 640x640 with 4 AO rays per hit: the persistent kernels, the
+    one-ray-per-lane kernels and the camera kernel whose waves take 8x8 tiles
+    from a counter (six workgroups per CU for 6,400 tiles) give identical
+    primary hits, instances and occlusion bits, and every AO ray whose pixel
+    has a hit is counted."""
     print("hierarchy full size: persistent vs one ray per lane", flush=True)
+    rx, ry = 640, 640
     sc, ins = scenes.instanced_scene(seed=12, n_inst=10000, subdiv=5, n_soup=100000, extent=100.0,
-                                     res=(256, 256), soup_size=0.02)
+                                     res=(rx, ry), soup_size=0.02)
     e = Engine(seed=SEED)
     e.upload_scene(sc)
     e.set_instances(ins.obj_first, ins.objects, ins.matrices, ins.aabb_min, ins.aabb_max)
@@ -231,16 +234,18 @@ def test_hier_full_size_kernels_agree():
     e.build()
     ao = 0.05 * e.get_bounding_box_diagonal_length()
     out = []
-    for pp, pers, refill in ((0, 0, 12), (0, 1, 12), (1, 1, 4), (0, 1, 64)):
-        e.set_tuning(pp, pers, hier_refill=refill)
+    for pp, pers, refill, cb in ((0, 0, 12, 0), (0, 1, 12, 0), (1, 1, 4, 0), (0, 1, 64, 0), (0, 1, 12, 4)):
+        print(f"hierarchy full size: camera kernel {pp} camera_blocks {cb} persistent AO {pers} refill {refill}",
+              flush=True)
+        e.set_tuning(pp, pers, hier_refill=refill, camera_blocks=cb)
         r0 = e.timing().rays_traced
         fb = e.render_frame(0, MODE_AO, jitter=1, ao_samples=4, ao_dist=ao)
         rays = e.timing().rays_traced - r0
         planes, prims = e.primary_hits()
         out.append((np.asarray(fb), planes, prims, e.primary_instances(), e.ao_occlusion(4), rays))
     hits = int((out[0][2] < sc.nt).sum())
-    assert 0 < hits <= 256 * 256
-    assert out[0][5] == 256 * 256 + 4 * hits
+    assert 0 < hits <= rx * ry
+    assert out[0][5] == rx * ry + 4 * hits
     for o in out[1:]:
         for a, b in zip(o[:5], out[0][:5]):
             np.testing.assert_array_equal(a, b)
