@@ -287,3 +287,56 @@ def test_two_level_structure(oracle):
     assert begin == g.c.num_leaves
     assert g.c.num_top_pairs == top.c.num_refs
     assert missing <= 0.001 * total
+
+
+def test_leaf_assignment_brute_force(oracle):
+    """Each (top cell, primitive) pair lands in the leaves of the top cell's
+    sub-grid that its bounding box meets (TLGridBuildKernels.h:536-663), checked
+    against the leaf boxes in double precision: every leaf the box meets by
+    more than a margin lists the primitive, and no leaf farther than the
+    margin does.  A transposed sub-grid index or a wrong leaf origin fails."""
+    sc = scenes.random_soup(13, 4000, extent=6.0, size=0.3)
+    lt, ll = 0.25, 3.0
+    g = build(sc, O.GRID_TWO_LEVEL, lt, ll)
+    top = build(sc, O.GRID_UNIFORM, lt)
+    tcells, tprims = top.cells(), top.prims()
+    cells, leaves, prims = g.cells(), g.leaves(), g.prims()
+    p = g.params()
+    bmin = p["bmin"].astype(np.float64)
+    cs = p["cell_size"].astype(np.float64)
+    gx, gy, _ = (int(v) for v in p["res"])
+    v = sc.verts.astype(np.float64)
+    lo_all = v[:, sc.idx].min(axis=2).T          # (nt, 3) box per triangle
+    hi_all = v[:, sc.idx].max(axis=2).T
+    m = 1e-4
+    pairs = undecided = 0
+    subdivided = 0
+    for c in range(len(cells)):
+        lb, y = int(cells[c, 0]), int(cells[c, 1])
+        rx, ry, rz, _, _ = unpack(y)
+        tl = tprims[tcells[c, 0]:tcells[c, 1]]
+        if len(tl) == 0:
+            continue
+        subdivided += rx * ry * rz > 1
+        origin = bmin + np.array([c % gx, (c // gx) % gy, c // (gx * gy)], np.float64) * cs
+        ls = cs / np.array([rx, ry, rz], np.float64)
+        # leaf boxes in sub-grid order x + y*rx + z*rx*ry
+        zz, yy, xx = np.meshgrid(np.arange(rz), np.arange(ry), np.arange(rx), indexing="ij")
+        idx3 = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.float64)
+        c0 = origin + idx3 * ls
+        c1 = c0 + ls
+        listed = {}
+        for li in range(rx * ry * rz):
+            b, e = leaves[lb + li]
+            for q in prims[b:e].tolist():
+                listed.setdefault(q, set()).add(li)
+        for q in tl.tolist():
+            lo, hi = lo_all[q], hi_all[q]
+            inner = set(np.nonzero(np.all(hi - m >= c0, 1) & np.all(lo + m <= c1, 1))[0].tolist())
+            outer = set(np.nonzero(np.all(hi + m >= c0, 1) & np.all(lo - m <= c1, 1))[0].tolist())
+            got = listed.get(q, set())
+            assert inner <= got <= outer, (c, q)
+            undecided += len(outer - inner)
+            pairs += 1
+    assert subdivided > 20 and pairs > 4000
+    assert undecided < 0.05 * pairs
